@@ -1,0 +1,456 @@
+"""Host-side mirror of the reference interface for the form-factor / horizon /
+radiosity / thermal path, on top of the C ABI (include/illum_hip.h).
+
+Reference names kept:
+  IllumContext.compute_F            illum_context::compute_F       (illum.hpp:18-20)
+  IllumContext.make_horizons        illum_context::make_horizons   (illum.hpp:22-23)
+  IllumContext.save/load_horizons   (illum.hpp:25-27)
+  IllumContext.get_direct_radiosity (illum.hpp:29-30)
+  IllumContext.get_num_faces        (illum.hpp:32-34)
+  IllumContext.set_horizon_obj_file (illum.hpp:36-38)
+  fib_spiral                        (illum.hpp:50)
+  ThermalModel(nfaces, depths, ti, rhoc, T0).step(dt, Q) / get_radiosity()
+                                    thermal_model (thermal.hpp:17-105)
+  conductionQ                       conduction.hpp:16-28
+  gauss_seidel-style solve          radiosity_solve (illum_cli.cpp:425-440)
+
+Arrays are numpy (host) by default; torch CUDA tensors are passed as device
+pointers (no host staging).  Every compute call runs on the GPU through
+libillum_hip; nothing here computes results on the CPU.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import call
+
+_PP = C.c_void_p
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return C.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _is_dev(a):
+    return a is not None and hasattr(a, "is_cuda") and a.is_cuda
+
+
+class Context:
+    """One GPU + one HIP stream (illum_ctx)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        call("illum_ctx_create", device, C.byref(h))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            call("illum_ctx_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        call("illum_ctx_synchronize", self.h)
+
+    def set_profiling(self, on):
+        call("illum_ctx_set_profiling", self.h, 1 if on else 0)
+
+    def reset_profile(self):
+        call("illum_ctx_reset_profile", self.h)
+
+    def profile(self):
+        """{stage: (total_ms, launches)} of hipEvent-timed stages."""
+        cap = 64
+        ms = np.zeros(cap)
+        cnt = np.zeros(cap, dtype=np.int64)
+        names = C.create_string_buffer(8192)
+        n = C.c_int()
+        call("illum_ctx_get_profile", self.h, cap, _ptr(ms), _ptr(cnt), names, 8192, C.byref(n))
+        keys = names.raw.split(b"\0")[: n.value]
+        return {k.decode(): (float(ms[i]), int(cnt[i])) for i, k in enumerate(keys)}
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def synth_ellipsoid(subdiv, axes=(13.0, 11.0, 9.0), ncraters=40, depth=(0.05, 0.3),
+                    width=(0.2, 1.5), seed=0):
+    """Seeded cratered triaxial ellipsoid: returns (V [nv,3] f64, F [nf,3] i32)."""
+    nv = C.c_int64()
+    nf = C.c_int64()
+    call("illum_synth_sizes", subdiv, C.byref(nv), C.byref(nf))
+    V = np.zeros((nv.value, 3))
+    F = np.zeros((nf.value, 3), dtype=np.int32)
+    call("illum_synth_ellipsoid", subdiv, axes[0], axes[1], axes[2], ncraters, depth[0],
+         depth[1], width[0], width[1], seed, _ptr(V), _ptr(F))
+    return V, F
+
+
+class Mesh:
+    def __init__(self, ctx, V=None, F=None, VN=None, vertex_divisor=1.0, obj_path=None):
+        self.ctx = ctx
+        h = C.c_void_p()
+        if obj_path is not None:
+            call("illum_mesh_from_obj", ctx.h, str(obj_path).encode(), vertex_divisor, C.byref(h))
+        else:
+            V = np.ascontiguousarray(V, dtype=np.float64)
+            F = np.ascontiguousarray(F, dtype=np.int32)
+            VN = None if VN is None else np.ascontiguousarray(VN, dtype=np.float64)
+            call("illum_mesh_create", ctx.h, _ptr(V), V.shape[0], _ptr(F), F.shape[0], _ptr(VN),
+                 vertex_divisor, C.byref(h))
+        self.h = h
+        n = C.c_int64()
+        call("illum_mesh_num_faces", h, C.byref(n))
+        self.nf = n.value
+
+    def close(self):
+        if self.h:
+            call("illum_mesh_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def geometry(self):
+        nf = self.nf
+        cen = np.zeros((nf, 3), np.float32)
+        nrm = np.zeros((nf, 3), np.float32)
+        rad = np.zeros(nf, np.float32)
+        area = np.zeros(nf)
+        frame = np.zeros((nf, 9))
+        call("illum_mesh_geometry", self.h, _ptr(cen), _ptr(nrm), _ptr(rad), _ptr(area), _ptr(frame))
+        return dict(centroid=cen, normal=nrm, radius=rad, area=area, frame=frame)
+
+    def triangles(self):
+        t = np.zeros((self.nf, 9), np.float32)
+        call("illum_mesh_triangles", self.h, _ptr(t))
+        return t
+
+
+class BVH:
+    def __init__(self, mesh, leaf_size=4):
+        self.mesh = mesh
+        h = C.c_void_p()
+        call("illum_bvh_build", mesh.h, leaf_size, C.byref(h))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            call("illum_bvh_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        n = C.c_int64()
+        d = C.c_int()
+        l = C.c_int()
+        call("illum_bvh_info", self.h, C.byref(n), C.byref(d), C.byref(l))
+        return dict(nodes=n.value, depth=d.value, leaf_size=l.value)
+
+    def closest_hit(self, orig, dirs):
+        orig = np.ascontiguousarray(orig, np.float32)
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        n = orig.shape[0]
+        hit = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        call("illum_bvh_closest_hit", self.h, _ptr(orig), _ptr(dirs), n, _ptr(hit), _ptr(t))
+        return hit, t
+
+
+class SparseMatrix:
+    """N x N form-factor matrix held on the GPU in CSC and CSR form."""
+
+    def __init__(self, ctx, h):
+        self.ctx = ctx
+        self.h = h
+        n = C.c_int64()
+        nnz = C.c_int64()
+        call("illum_sparse_info", h, C.byref(n), C.byref(nnz))
+        self.n = n.value
+        self.nnz = nnz.value
+
+    @classmethod
+    def from_csr(cls, ctx, n, rowptr, colidx, val):
+        rowptr = np.ascontiguousarray(rowptr, np.int64)
+        colidx = np.ascontiguousarray(colidx, np.int32)
+        val = np.ascontiguousarray(val, np.float64)
+        h = C.c_void_p()
+        call("illum_sparse_create_csr", ctx.h, n, int(rowptr[-1]), _ptr(rowptr), _ptr(colidx),
+             _ptr(val), C.byref(h))
+        return cls(ctx, h)
+
+    def close(self):
+        if self.h:
+            call("illum_sparse_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def csc(self):
+        cp = np.zeros(self.n + 1, np.int64)
+        ri = np.zeros(self.nnz, np.int32)
+        v = np.zeros(self.nnz)
+        call("illum_sparse_copy_csc", self.h, _ptr(cp), _ptr(ri), _ptr(v))
+        return cp, ri, v
+
+    def csr(self):
+        rp = np.zeros(self.n + 1, np.int64)
+        ci = np.zeros(self.nnz, np.int32)
+        v = np.zeros(self.nnz)
+        call("illum_sparse_copy_csr", self.h, _ptr(rp), _ptr(ci), _ptr(v))
+        return rp, ci, v
+
+    def visibility(self):
+        """Raw visibility bit rows (n x words uint32) before the zero drop."""
+        w = C.c_int64()
+        call("illum_sparse_visibility_words", self.h, C.byref(w))
+        bits = np.zeros((self.n, w.value), np.uint32)
+        call("illum_sparse_copy_visibility", self.h, _ptr(bits))
+        return bits
+
+    def to_scipy(self):
+        import scipy.sparse as sps
+        cp, ri, v = self.csc()
+        return sps.csc_matrix((v, ri, cp), shape=(self.n, self.n))
+
+
+def fib_spiral(n):
+    """fib_spiral(xy, n) (illum.cpp:468-483): n x 2 disk sample points."""
+    xy = np.zeros((2, n))  # column-major n x 2 == row-major 2 x n
+    call("illum_fib_spiral", _ptr(xy), n)
+    return xy.T.copy()
+
+
+class IllumContext:
+    """Mirror of illum_context (illum.hpp:6-48) over the GPU library."""
+
+    def __init__(self, obj_path=None, V=None, F=None, VN=None, vertex_divisor=None,
+                 leaf_size=4, ctx=None):
+        self.ctx = ctx or default_context()
+        if obj_path is not None:
+            div = 100.0 if vertex_divisor is None else vertex_divisor
+            self.mesh = Mesh(self.ctx, obj_path=obj_path, vertex_divisor=div)
+        else:
+            div = 1.0 if vertex_divisor is None else vertex_divisor
+            self.mesh = Mesh(self.ctx, V=V, F=F, VN=VN, vertex_divisor=div)
+        self.bvh = BVH(self.mesh, leaf_size)
+        self.num_faces = self.mesh.nf
+        self.horizons = None
+        self._horizon_ctx = None
+        self.leaf_size = leaf_size
+
+    def get_num_faces(self):
+        return self.num_faces
+
+    def set_horizon_obj_file(self, path, vertex_divisor=100.0):
+        occ = Mesh(self.ctx, obj_path=path, vertex_divisor=vertex_divisor)
+        self._horizon_ctx = (occ, BVH(occ, self.leaf_size))
+
+    def compute_F(self, albedo, offset=1e-5):
+        """albedo: scalar, per-facet array, or a path to an Armadillo vector."""
+        if isinstance(albedo, str):
+            from .armaio import load_mat
+            albedo = load_mat(albedo).ravel()
+        h = C.c_void_p()
+        if np.isscalar(albedo):
+            call("illum_form_factors", self.bvh.h, float(albedo), None, offset, C.byref(h))
+        else:
+            a = np.ascontiguousarray(albedo, np.float64)
+            if a.shape != (self.num_faces,):
+                raise ValueError("albedo vector must have one entry per facet")
+            call("illum_form_factors", self.bvh.h, 0.0, _ptr(a), offset, C.byref(h))
+        return SparseMatrix(self.ctx, h)
+
+    def make_horizons(self, nphi=361, theta_eps=1e-3, offset=1e-5, out=None):
+        occ = self._horizon_ctx[1].h if self._horizon_ctx else None
+        if out is not None and _is_dev(out):
+            call("illum_horizons", self.bvh.h, occ, nphi, theta_eps, offset, _ptr(out),
+                 _lib.ILLUM_OUT_ON_DEVICE)
+            self.horizons = out
+            return out
+        hz = np.zeros((self.num_faces, nphi))  # row f = column f of the nphi x N matrix
+        call("illum_horizons", self.bvh.h, occ, nphi, theta_eps, offset, _ptr(hz), 0)
+        self.horizons = hz.T
+        return self.horizons
+
+    def save_horizons(self, path):
+        from .armaio import save_mat
+        save_mat(np.asfortranarray(self.horizons), path)
+
+    def load_horizons(self, path):
+        from .armaio import load_mat
+        self.horizons = load_mat(path)
+
+    def get_direct_radiosity(self, sun_position, disk_xy):
+        """sun_position: (3,) or (3, S) km; returns E of shape (N,) or (N, S)."""
+        sun = np.asarray(sun_position, np.float64)
+        single = sun.ndim == 1
+        sun = np.ascontiguousarray(sun.reshape(3, -1).T)  # S x 3 row-major == 3 x S col-major
+        S = sun.shape[0]
+        disk = np.asfortranarray(np.asarray(disk_xy, np.float64))
+        hz = np.asfortranarray(self.horizons)
+        nphi = hz.shape[0]
+        E = np.zeros((S, self.num_faces))
+        call("illum_direct_radiosity", self.mesh.h, hz.ctypes.data_as(C.c_void_p), nphi,
+             _ptr(sun), S, disk.ctypes.data_as(C.c_void_p), disk.shape[0], _ptr(E), 0)
+        return E[0] if single else E.T
+
+
+def radiosity_solve(F, E, method="mixed", rel_tol=1e-8, max_iter=200, B=None):
+    """Solve (I - F) B = E for E of shape (N,) or (N, S).  Returns (B, resid, iters)."""
+    m = {"fp64": _lib.ILLUM_RAD_JACOBI_FP64, "mixed": _lib.ILLUM_RAD_MIXED}[method]
+    resid = np.zeros(max_iter + 1)
+    iters = C.c_int()
+    if _is_dev(E):
+        S = 1 if E.dim() == 1 else E.shape[1]
+        call("illum_radiosity_solve", F.h, _ptr(E), S, m, rel_tol, max_iter, _ptr(B),
+             _ptr(resid), C.byref(iters), _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_OUT_ON_DEVICE)
+        out = B
+    else:
+        E2 = np.asarray(E, np.float64)
+        single = E2.ndim == 1
+        Ecm = np.ascontiguousarray(E2.reshape(E2.shape[0], -1).T)  # S x N == N x S col-major
+        S = Ecm.shape[0]
+        Bcm = np.zeros_like(Ecm)
+        call("illum_radiosity_solve", F.h, _ptr(Ecm), S, m, rel_tol, max_iter, _ptr(Bcm),
+             _ptr(resid), C.byref(iters), 0)
+        out = Bcm[0] if single else Bcm.T
+    nres = min(len(resid), max(1, iters.value + 1))
+    return out, resid[:nres], iters.value
+
+
+class ThermalModel:
+    """Mirror of thermal_model (thermal.hpp:7-105) for many facets on the GPU.
+
+    depths: one grid (list) for all facets, or a list of grids with
+    grid_of_facet selecting one per facet.  thermal_inertia: scalar or
+    per-layer list (per grid); ti_scale: optional per-facet multiplier.
+    """
+
+    def __init__(self, nfaces, depths, thermal_inertia, rhoc, init_temp, emissivity=0.999,
+                 precision="fp64", grid_of_facet=None, ti_scale=None, ctx=None):
+        self.ctx = ctx or default_context()
+        grids = depths if (len(depths) and np.ndim(depths[0]) == 1) else [depths]
+        zs, tis, rcs, off = [], [], [], [0]
+        for g in grids:
+            g = np.asarray(g, np.float64)
+            zs.append(g)
+            ti = np.broadcast_to(np.asarray(thermal_inertia, np.float64), g.shape)
+            rc = np.broadcast_to(np.asarray(rhoc, np.float64), g.shape)
+            tis.append(ti)
+            rcs.append(rc)
+            off.append(off[-1] + len(g))
+        self.z = np.concatenate(zs)
+        self.goff = np.asarray(off, np.int64)
+        ti = np.ascontiguousarray(np.concatenate(tis))
+        rc = np.ascontiguousarray(np.concatenate(rcs))
+        self.nz_max = max(len(g) for g in zs)
+        self.nf = nfaces
+        gof = None if grid_of_facet is None else np.ascontiguousarray(grid_of_facet, np.int32)
+        tsc = None if ti_scale is None else np.ascontiguousarray(ti_scale, np.float64)
+        prec = {"fp64": _lib.ILLUM_THERM_FP64, "mixed": _lib.ILLUM_THERM_MIXED}[precision]
+        T0 = None
+        T0s = 0.0
+        if isinstance(init_temp, str):
+            from .armaio import load_mat
+            init_temp = load_mat(init_temp)
+        if np.isscalar(init_temp):
+            T0s = float(init_temp)
+        else:
+            T0 = np.asfortranarray(np.asarray(init_temp, np.float64))
+            if T0.shape != (self.nz_max + 1, nfaces):
+                raise _lib.IllumError(_lib.ILLUM_ERR_SHAPE,
+                                      "initial temperature profile is an incompatible size")
+        h = C.c_void_p()
+        call("illum_thermal_create", self.ctx.h, nfaces, len(grids), _ptr(self.goff),
+             _ptr(self.z), _ptr(ti), _ptr(rc), _ptr(gof), _ptr(tsc), emissivity, T0s,
+             None if T0 is None else T0.ctypes.data_as(C.c_void_p), prec, C.byref(h))
+        self.h = h
+        self.t = 0.0
+
+    def close(self):
+        if self.h:
+            call("illum_thermal_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, dt, Q):
+        if _is_dev(Q):
+            call("illum_thermal_step", self.h, dt, _ptr(Q), _lib.ILLUM_IN_ON_DEVICE)
+        else:
+            q = np.ascontiguousarray(Q, np.float64)
+            call("illum_thermal_step", self.h, dt, _ptr(q), 0)
+        self.t += dt
+
+    def run(self, dt, Qseries):
+        """nsteps consecutive steps; Qseries has shape (nsteps, N)."""
+        if _is_dev(Qseries):
+            n = Qseries.shape[0]
+            call("illum_thermal_run", self.h, dt, _ptr(Qseries), n, _lib.ILLUM_IN_ON_DEVICE)
+        else:
+            q = np.ascontiguousarray(Qseries, np.float64)
+            n = q.shape[0]
+            call("illum_thermal_run", self.h, dt, _ptr(q), n, 0)
+        self.t += dt * n
+
+    @property
+    def T(self):
+        T = np.zeros((self.nf, self.nz_max + 1))
+        call("illum_thermal_get", self.h, _ptr(T), None, None, 0)
+        return T.T
+
+    def surface(self):
+        Ts = np.zeros(self.nf)
+        Fs = np.zeros(self.nf)
+        call("illum_thermal_get", self.h, None, _ptr(Ts), _ptr(Fs), 0)
+        return Ts, Fs
+
+    def get_radiosity(self):
+        return -self.surface()[1]
+
+
+def conductionQ(nz, z, dt, Qn, Qnp1, T, ti, rhoc, emiss, Tsurf, Fgeotherm):
+    """conductionQ (conduction.hpp:16-28) on the GPU; returns (T, Tsurf, Fsurf)."""
+    Tc = np.array(T, np.float64)
+    ts = C.c_double(Tsurf)
+    fs = C.c_double(0.0)
+    z = np.ascontiguousarray(z, np.float64)
+    ti = np.ascontiguousarray(np.broadcast_to(ti, (nz,)), np.float64)
+    rhoc = np.ascontiguousarray(np.broadcast_to(rhoc, (nz,)), np.float64)
+    call("illum_conductionQ", nz, _ptr(z), dt, Qn, Qnp1, _ptr(Tc), _ptr(ti), _ptr(rhoc), emiss,
+         C.byref(ts), Fgeotherm, C.byref(fs))
+    return Tc, ts.value, fs.value

@@ -1,0 +1,215 @@
+// Host BVH build: binned SAH over triangle bounding boxes, flattened to a
+// threaded (stackless) layout in depth-first pre-order.  Replaces the Fast-BVH
+// constructor used by the reference (illum.cpp:25, illum.cpp:333-338).
+//
+// Layout (consumed by the traversal kernels):
+//   * inner node i: its first child is i + 1; its second child follows the
+//     whole first subtree; miss(i) = the node after the whole subtree of i;
+//   * leaf: triangles [first, first + count) of the reordered triangle array.
+// Boxes are padded by 4e-6 of the scene's largest coordinate magnitude so a
+// ray that the FP32 watertight predicate reports as hitting a triangle always
+// passes through every ancestor box with a wide margin against rounding.
+#include "internal.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+namespace illum {
+
+namespace {
+
+struct Box {
+  float lo[3], hi[3];
+  Box() { for (int c = 0; c < 3; ++c) { lo[c] = FLT_MAX; hi[c] = -FLT_MAX; } }
+  void grow(const Box &b) {
+    for (int c = 0; c < 3; ++c) { lo[c] = std::min(lo[c], b.lo[c]); hi[c] = std::max(hi[c], b.hi[c]); }
+  }
+  void grow(const float *p) {
+    for (int c = 0; c < 3; ++c) { lo[c] = std::min(lo[c], p[c]); hi[c] = std::max(hi[c], p[c]); }
+  }
+  double area() const {
+    double dx = (double)hi[0] - lo[0], dy = (double)hi[1] - lo[1], dz = (double)hi[2] - lo[2];
+    if (dx < 0 || dy < 0 || dz < 0) return 0.0;
+    return 2.0 * (dx * dy + dy * dz + dz * dx);
+  }
+};
+
+struct BuildNode {
+  Box box;
+  int left = -1, right = -1;  // children (build indices)
+  int first = 0, count = 0;   // leaf range in the permutation
+};
+
+struct Builder {
+  const HostGeometry &g;
+  int leaf_size;
+  std::vector<Box> tbox;
+  std::vector<float> tcen;   // centroid of each triangle's box
+  std::vector<int> perm;
+  std::vector<BuildNode> nodes;
+  int max_depth = 0;
+
+  Builder(const HostGeometry &g_, int ls) : g(g_), leaf_size(ls) {}
+
+  int build(int first, int count, int depth) {
+    max_depth = std::max(max_depth, depth);
+    int id = (int)nodes.size();
+    nodes.emplace_back();
+    Box b, cb;
+    for (int k = first; k < first + count; ++k) {
+      b.grow(tbox[perm[k]]);
+      cb.grow(&tcen[3 * perm[k]]);
+    }
+    nodes[id].box = b;
+    if (count <= leaf_size) {
+      nodes[id].first = first;
+      nodes[id].count = count;
+      return id;
+    }
+    // Binned SAH over the centroid box, 32 bins per axis.
+    const int NB = 32;
+    int best_axis = -1, best_split = -1;
+    double best_cost = DBL_MAX;
+    for (int axis = 0; axis < 3; ++axis) {
+      double lo = cb.lo[axis], hi = cb.hi[axis];
+      if (!(hi > lo)) continue;
+      Box bins[NB];
+      int cnt[NB] = {0};
+      double scale = NB / (hi - lo);
+      for (int k = first; k < first + count; ++k) {
+        int t = perm[k];
+        int bi = (int)((tcen[3 * t + axis] - lo) * scale);
+        bi = std::min(NB - 1, std::max(0, bi));
+        cnt[bi]++;
+        bins[bi].grow(tbox[t]);
+      }
+      double rarea[NB];
+      int rcnt[NB];
+      Box acc;
+      int ac = 0;
+      for (int s = NB - 1; s > 0; --s) {
+        acc.grow(bins[s]);
+        ac += cnt[s];
+        rarea[s] = acc.area();
+        rcnt[s] = ac;
+      }
+      Box lacc;
+      int lc = 0;
+      for (int s = 1; s < NB; ++s) {
+        lacc.grow(bins[s - 1]);
+        lc += cnt[s - 1];
+        if (lc == 0 || rcnt[s] == 0) continue;
+        double cost = lacc.area() * lc + rarea[s] * rcnt[s];
+        if (cost < best_cost) { best_cost = cost; best_axis = axis; best_split = s; }
+      }
+    }
+    int mid;
+    if (best_axis < 0) {
+      mid = first + count / 2;  // all centroids coincide: split by index
+    } else {
+      double lo = cb.lo[best_axis], hi = cb.hi[best_axis];
+      double scale = NB / (hi - lo);
+      auto pivot = std::stable_partition(perm.begin() + first, perm.begin() + first + count,
+                                         [&](int t) {
+                                           int bi = (int)((tcen[3 * t + best_axis] - lo) * scale);
+                                           bi = std::min(NB - 1, std::max(0, bi));
+                                           return bi < best_split;
+                                         });
+      mid = (int)(pivot - perm.begin());
+      if (mid == first || mid == first + count) mid = first + count / 2;
+    }
+    int l = build(first, mid - first, depth + 1);
+    int r = build(mid, first + count - mid, depth + 1);
+    nodes[id].left = l;
+    nodes[id].right = r;
+    return id;
+  }
+};
+
+inline float as_float(int32_t i) { float f; std::memcpy(&f, &i, 4); return f; }
+
+}  // namespace
+
+HostBVH build_bvh(const HostGeometry &g, int leaf_size) {
+  if (leaf_size < 1 || leaf_size > 64) throw Error(ILLUM_ERR_ARG, "leaf_size must be in [1, 64]");
+  const int64_t nf = g.nf;
+  Builder B(g, leaf_size);
+  B.tbox.resize(nf);
+  B.tcen.resize(3 * nf);
+  B.perm.resize(nf);
+  float scene_max = 0.0f;
+  for (int64_t t = 0; t < nf; ++t) {
+    Box b;
+    for (int k = 0; k < 3; ++k) b.grow(&g.tri[9 * t + 3 * k]);
+    B.tbox[t] = b;
+    for (int c = 0; c < 3; ++c) {
+      B.tcen[3 * t + c] = 0.5f * (b.lo[c] + b.hi[c]);
+      scene_max = std::max(scene_max, std::max(std::fabs(b.lo[c]), std::fabs(b.hi[c])));
+    }
+    B.perm[t] = (int)t;
+  }
+  B.nodes.reserve(2 * nf / std::max(1, leaf_size) + 16);
+  B.build(0, (int)nf, 0);
+  const float pad = scene_max * 4e-6f + FLT_MIN;
+
+  // Flatten in DFS pre-order, computing miss links.
+  HostBVH h;
+  h.leaf_size = leaf_size;
+  h.depth = B.max_depth;
+  const int nn = (int)B.nodes.size();
+  h.nnodes = nn;
+  h.nodes.resize(8 * (size_t)nn);
+  h.tris.resize(12 * (size_t)nf);
+  std::vector<int> order;  // build index in pre-order
+  order.reserve(nn);
+  std::vector<int> flat_of(nn, -1);
+  std::vector<int> stack = {0};
+  while (!stack.empty()) {
+    int id = stack.back();
+    stack.pop_back();
+    flat_of[id] = (int)order.size();
+    order.push_back(id);
+    if (B.nodes[id].left >= 0) {
+      stack.push_back(B.nodes[id].right);
+      stack.push_back(B.nodes[id].left);
+    }
+  }
+  // subtree size -> miss link = pre-order index + subtree size
+  std::vector<int> subtree(nn, 1);
+  for (int k = nn - 1; k >= 0; --k) {
+    int id = order[k];
+    if (B.nodes[id].left >= 0) subtree[id] = 1 + subtree[B.nodes[id].left] + subtree[B.nodes[id].right];
+  }
+  for (int k = 0; k < nn; ++k) {
+    const BuildNode &bn = B.nodes[order[k]];
+    int miss = k + subtree[order[k]];
+    if (miss >= nn) miss = -1;
+    int32_t info;
+    if (bn.left >= 0) {
+      info = -1;
+    } else {
+      if (bn.count >= 128) throw Error(ILLUM_ERR_CAPACITY, "leaf too large");
+      info = (bn.count << 24) | bn.first;
+    }
+    float *o = &h.nodes[8 * (size_t)k];
+    o[0] = bn.box.lo[0] - pad; o[1] = bn.box.lo[1] - pad; o[2] = bn.box.lo[2] - pad;
+    o[3] = as_float(miss);
+    o[4] = bn.box.hi[0] + pad; o[5] = bn.box.hi[1] + pad; o[6] = bn.box.hi[2] + pad;
+    o[7] = as_float(info);
+  }
+  for (int64_t k = 0; k < nf; ++k) {
+    int t = B.perm[k];
+    float *o = &h.tris[12 * (size_t)k];
+    for (int v = 0; v < 3; ++v) {
+      o[4 * v + 0] = g.tri[9 * t + 3 * v + 0];
+      o[4 * v + 1] = g.tri[9 * t + 3 * v + 1];
+      o[4 * v + 2] = g.tri[9 * t + 3 * v + 2];
+      o[4 * v + 3] = v == 0 ? as_float(t) : 0.0f;
+    }
+  }
+  return h;
+}
+
+}  // namespace illum

@@ -1,0 +1,572 @@
+// C ABI of libillum_hip: object lifetime, host<->device staging, error
+// translation.  Every entry point catches illum::Error / std::exception and
+// returns a status code with the message kept in a thread-local string.
+#include "hip_util.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+
+namespace illum {
+
+// Implemented in the kernel translation units.
+illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_vec,
+                           double offset);
+void bvh_closest_hit(illum_bvh *bvh, const float *orig, const float *dir, int64_t nray,
+                     int32_t *hit, float *t);
+void horizons(illum_bvh *bvh, illum_bvh *occl, int nphi, double theta_eps, double offset,
+              double *out, int flags);
+void direct_radiosity(illum_mesh *mesh, const double *hz, int nphi, const double *sun, int S,
+                      const double *disk, int ndisk, double *E, int flags);
+void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double rel_tol,
+                     int max_iter, double *B, double *resid, int *iters, int flags);
+illum_thermal *thermal_create(illum_ctx *ctx, int64_t nf, int ngrid, const int64_t *grid_off,
+                              const double *z, const double *ti, const double *rhoc,
+                              const int32_t *grid_of_facet, const double *ti_scale,
+                              double emissivity, double T0, const double *T0_full,
+                              int precision);
+void thermal_run(illum_thermal *th, double dt, const double *Q, int nsteps, int flags,
+                 double Fgeo, double *Tsurf_series);
+void thermal_get(illum_thermal *th, double *T, double *Tsurf, double *Fsurf, int flags);
+void thermal_free_impl(illum_thermal *th);
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+template <class Fn>
+static int guarded(Fn &&fn) {
+  try {
+    fn();
+    return ILLUM_OK;
+  } catch (const Error &e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_last_error("host allocation failed");
+    return ILLUM_ERR_OOM;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return ILLUM_ERR_ARG;
+  }
+}
+
+void collect_profile(illum_ctx *ctx) {
+  for (auto &p : ctx->pending) {
+    float ms = 0;
+    hipEvent_t e0 = (hipEvent_t)p.second.first, e1 = (hipEvent_t)p.second.second;
+    if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) {
+      auto &st = ctx->profile[p.first];
+      st.ms += ms;
+      st.launches += 1;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  ctx->pending.clear();
+}
+
+void sync(illum_ctx *ctx) {
+  ILLUM_HIP(hipStreamSynchronize(stream_of(ctx)));
+  if (!ctx->pending.empty()) collect_profile(ctx);
+}
+
+__global__ void scan_kernel(const int32_t *__restrict__ counts, int64_t n,
+                            int64_t *__restrict__ out) {
+  __shared__ int64_t s[1024];
+  const int tid = threadIdx.x;
+  const int64_t per = (n + 1023) / 1024;
+  const int64_t a = tid * per, b = a + per < n ? a + per : n;
+  int64_t sum = 0;
+  for (int64_t i = a; i < b; ++i) sum += counts[i];
+  s[tid] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int64_t v = tid >= off ? s[tid - off] : 0;
+    __syncthreads();
+    s[tid] += v;
+    __syncthreads();
+  }
+  int64_t run = s[tid] - sum;
+  for (int64_t i = a; i < b; ++i) {
+    out[i] = run;
+    run += counts[i];
+  }
+  if (tid == 1023) out[n] = s[1023];
+}
+
+void scan_counts(illum_ctx *ctx, const int32_t *counts, int64_t n, int64_t *offsets) {
+  scan_kernel<<<1, 1024, 0, stream_of(ctx)>>>(counts, n, offsets);
+  ILLUM_LAUNCH_CHECK();
+}
+
+static void set_device(illum_ctx *ctx) { ILLUM_HIP(hipSetDevice(ctx->device)); }
+
+static void destroy_mesh(illum_mesh *m) {
+  dfree(m->d_pr4); dfree(m->d_n4); dfree(m->d_area); dfree(m->d_frame);
+  delete m;
+}
+static void destroy_bvh(illum_bvh *b) {
+  dfree(b->d_nodes); dfree(b->d_tris);
+  delete b;
+}
+void destroy_sparse(illum_sparse *s) {
+  dfree(s->d_colptr); dfree(s->d_rowidx); dfree(s->d_cval);
+  dfree(s->d_rowptr); dfree(s->d_colidx); dfree(s->d_rval); dfree(s->d_rval32);
+  dfree(s->d_vis);
+  delete s;
+}
+void destroy_thermal(illum_thermal *t) {
+  dfree(t->d_nz); dfree(t->d_grid); dfree(t->d_s2); dfree(t->d_goff); dfree(t->d_T);
+  dfree(t->d_Qprev); dfree(t->d_F); dfree(t->d_work); dfree(t->d_work32); dfree(t->d_Qdev);
+  dfree(t->d_status);
+  thermal_free_impl(t);
+  delete t;
+}
+
+static illum_mesh *upload_mesh(illum_ctx *ctx, HostGeometry &&g) {
+  auto m = std::unique_ptr<illum_mesh, void (*)(illum_mesh *)>(new illum_mesh(), destroy_mesh);
+  m->ctx = ctx;
+  m->g = std::move(g);
+  const int64_t nf = m->g.nf;
+  std::vector<float> pr4(4 * nf), n4(4 * nf);
+  for (int64_t f = 0; f < nf; ++f) {
+    for (int c = 0; c < 3; ++c) {
+      pr4[4 * f + c] = m->g.cen[3 * f + c];
+      n4[4 * f + c] = m->g.nrm[3 * f + c];
+    }
+    pr4[4 * f + 3] = m->g.rad[f];
+    n4[4 * f + 3] = 0.0f;
+  }
+  hipStream_t s = stream_of(ctx);
+  m->d_pr4 = dalloc<float>(4 * nf);
+  m->d_n4 = dalloc<float>(4 * nf);
+  m->d_area = dalloc<double>(nf);
+  m->d_frame = dalloc<double>(9 * nf);
+  ILLUM_HIP(hipMemcpyAsync(m->d_pr4, pr4.data(), 16 * nf, hipMemcpyHostToDevice, s));
+  ILLUM_HIP(hipMemcpyAsync(m->d_n4, n4.data(), 16 * nf, hipMemcpyHostToDevice, s));
+  ILLUM_HIP(hipMemcpyAsync(m->d_area, m->g.area.data(), 8 * nf, hipMemcpyHostToDevice, s));
+  ILLUM_HIP(hipMemcpyAsync(m->d_frame, m->g.frame.data(), 72 * nf, hipMemcpyHostToDevice, s));
+  sync(ctx);
+  return m.release();
+}
+
+static illum_ctx *default_ctx() {
+  static std::mutex mu;
+  static illum_ctx *ctx = nullptr;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!ctx) {
+    auto c = std::unique_ptr<illum_ctx>(new illum_ctx());
+    c->device = 0;
+    ILLUM_HIP(hipSetDevice(0));
+    hipStream_t s;
+    ILLUM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    c->stream = s;
+    ctx = c.release();
+  }
+  return ctx;
+}
+
+}  // namespace illum
+
+using namespace illum;
+
+#define CHECK_ARG(cond, msg) \
+  do { if (!(cond)) throw Error(ILLUM_ERR_ARG, msg); } while (0)
+
+extern "C" {
+
+const char *illum_last_error(void) { return g_last_error.c_str(); }
+int illum_version(void) { return 1; }
+
+int illum_ctx_create(int device, illum_ctx **out) {
+  return guarded([&] {
+    CHECK_ARG(out, "null output");
+    int n = 0;
+    ILLUM_HIP(hipGetDeviceCount(&n));
+    CHECK_ARG(device >= 0 && device < n, "device index out of range");
+    auto c = std::unique_ptr<illum_ctx>(new illum_ctx());
+    c->device = device;
+    ILLUM_HIP(hipSetDevice(device));
+    hipStream_t s;
+    ILLUM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    c->stream = s;
+    *out = c.release();
+  });
+}
+
+int illum_ctx_destroy(illum_ctx *ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    (void)hipStreamSynchronize(stream_of(ctx));
+    collect_profile(ctx);
+    (void)hipStreamDestroy(stream_of(ctx));
+    delete ctx;
+  });
+}
+
+int illum_ctx_synchronize(illum_ctx *ctx) {
+  return guarded([&] { CHECK_ARG(ctx, "null context"); sync(ctx); });
+}
+
+int illum_ctx_set_profiling(illum_ctx *ctx, int enable) {
+  return guarded([&] { CHECK_ARG(ctx, "null context"); ctx->profiling = enable != 0; });
+}
+
+int illum_ctx_reset_profile(illum_ctx *ctx) {
+  return guarded([&] {
+    CHECK_ARG(ctx, "null context");
+    sync(ctx);
+    ctx->profile.clear();
+  });
+}
+
+int illum_ctx_get_profile(illum_ctx *ctx, int cap, double *total_ms, int64_t *launches,
+                          char *names, int names_cap, int *n) {
+  return guarded([&] {
+    CHECK_ARG(ctx && n, "null argument");
+    sync(ctx);
+    int k = 0, pos = 0;
+    for (auto &p : ctx->profile) {
+      if (k < cap) {
+        if (total_ms) total_ms[k] = p.second.ms;
+        if (launches) launches[k] = p.second.launches;
+        if (names) {
+          const int len = (int)p.first.size() + 1;
+          if (pos + len <= names_cap) {
+            std::memcpy(names + pos, p.first.c_str(), len);
+            pos += len;
+          }
+        }
+      }
+      ++k;
+    }
+    *n = k;
+  });
+}
+
+int illum_synth_sizes(int subdiv, int64_t *nv, int64_t *nf) {
+  return guarded([&] { synth_sizes(subdiv, nv, nf); });
+}
+
+int illum_synth_ellipsoid(int subdiv, double a, double b, double c, int ncraters, double dmin,
+                          double dmax, double wmin, double wmax, uint64_t seed, double *V,
+                          int32_t *F) {
+  return guarded([&] {
+    CHECK_ARG(V && F, "null output");
+    synth_ellipsoid(subdiv, a, b, c, ncraters, dmin, dmax, wmin, wmax, seed, V, F);
+  });
+}
+
+int illum_mesh_create(illum_ctx *ctx, const double *V, int64_t nv, const int32_t *F,
+                      int64_t nf, const double *VN, double vertex_divisor, illum_mesh **out) {
+  return guarded([&] {
+    CHECK_ARG(ctx && out, "null argument");
+    set_device(ctx);
+    *out = upload_mesh(ctx, make_geometry(V, nv, F, nf, VN, vertex_divisor));
+  });
+}
+
+int illum_mesh_from_obj(illum_ctx *ctx, const char *path, double vertex_divisor,
+                        illum_mesh **out) {
+  return guarded([&] {
+    CHECK_ARG(ctx && path && out, "null argument");
+    set_device(ctx);
+    std::vector<double> V, VN;
+    std::vector<int32_t> F;
+    read_obj(path, V, F, VN);
+    const int64_t nv = (int64_t)(V.size() / 3);
+    const double *vn = VN.size() == V.size() ? VN.data() : nullptr;
+    *out = upload_mesh(ctx, make_geometry(V.data(), nv, F.data(), (int64_t)(F.size() / 3), vn,
+                                          vertex_divisor));
+  });
+}
+
+int illum_mesh_destroy(illum_mesh *mesh) {
+  return guarded([&] { if (mesh) destroy_mesh(mesh); });
+}
+
+int illum_mesh_num_faces(const illum_mesh *mesh, int64_t *nf) {
+  return guarded([&] { CHECK_ARG(mesh && nf, "null argument"); *nf = mesh->g.nf; });
+}
+
+int illum_mesh_geometry(const illum_mesh *mesh, float *centroid, float *normal, float *radius,
+                        double *area, double *frame) {
+  return guarded([&] {
+    CHECK_ARG(mesh, "null mesh");
+    const auto &g = mesh->g;
+    if (centroid) std::memcpy(centroid, g.cen.data(), 4 * g.cen.size());
+    if (normal) std::memcpy(normal, g.nrm.data(), 4 * g.nrm.size());
+    if (radius) std::memcpy(radius, g.rad.data(), 4 * g.rad.size());
+    if (area) std::memcpy(area, g.area.data(), 8 * g.area.size());
+    if (frame) std::memcpy(frame, g.frame.data(), 8 * g.frame.size());
+  });
+}
+
+int illum_mesh_triangles(const illum_mesh *mesh, float *tri9) {
+  return guarded([&] {
+    CHECK_ARG(mesh && tri9, "null argument");
+    std::memcpy(tri9, mesh->g.tri.data(), 4 * mesh->g.tri.size());
+  });
+}
+
+int illum_bvh_build(illum_mesh *mesh, int leaf_size, illum_bvh **out) {
+  return guarded([&] {
+    CHECK_ARG(mesh && out, "null argument");
+    illum_ctx *ctx = mesh->ctx;
+    set_device(ctx);
+    auto b = std::unique_ptr<illum_bvh, void (*)(illum_bvh *)>(new illum_bvh(), destroy_bvh);
+    b->mesh = mesh;
+    b->h = build_bvh(mesh->g, leaf_size);
+    hipStream_t s = stream_of(ctx);
+    b->d_nodes = dalloc<float>(b->h.nodes.size());
+    b->d_tris = dalloc<float>(b->h.tris.size());
+    ILLUM_HIP(hipMemcpyAsync(b->d_nodes, b->h.nodes.data(), 4 * b->h.nodes.size(),
+                             hipMemcpyHostToDevice, s));
+    ILLUM_HIP(hipMemcpyAsync(b->d_tris, b->h.tris.data(), 4 * b->h.tris.size(),
+                             hipMemcpyHostToDevice, s));
+    sync(ctx);
+    *out = b.release();
+  });
+}
+
+int illum_bvh_destroy(illum_bvh *bvh) {
+  return guarded([&] { if (bvh) destroy_bvh(bvh); });
+}
+
+int illum_bvh_info(const illum_bvh *bvh, int64_t *nodes, int *depth, int *leaf_size) {
+  return guarded([&] {
+    CHECK_ARG(bvh, "null bvh");
+    if (nodes) *nodes = bvh->h.nnodes;
+    if (depth) *depth = bvh->h.depth;
+    if (leaf_size) *leaf_size = bvh->h.leaf_size;
+  });
+}
+
+int illum_bvh_closest_hit(illum_bvh *bvh, const float *orig, const float *dir, int64_t nray,
+                          int32_t *hit, float *t) {
+  return guarded([&] {
+    CHECK_ARG(bvh && orig && dir && hit && t && nray >= 0, "bad argument");
+    set_device(bvh->mesh->ctx);
+    bvh_closest_hit(bvh, orig, dir, nray, hit, t);
+  });
+}
+
+int illum_form_factors(illum_bvh *bvh, double albedo, const double *albedo_vec, double offset,
+                       illum_sparse **out) {
+  return guarded([&] {
+    CHECK_ARG(bvh && out, "null argument");
+    CHECK_ARG(std::isfinite(offset), "offset must be finite");
+    set_device(bvh->mesh->ctx);
+    *out = form_factors(bvh, albedo, albedo_vec, offset);
+  });
+}
+
+int illum_sparse_create_csr(illum_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rowptr,
+                            const int32_t *colidx, const double *val, illum_sparse **out) {
+  return guarded([&] {
+    CHECK_ARG(ctx && out && rowptr && n > 0 && nnz >= 0, "bad argument");
+    CHECK_ARG(nnz == 0 || (colidx && val), "null index/value arrays");
+    CHECK_ARG(rowptr[0] == 0 && rowptr[n] == nnz, "rowptr must start at 0 and end at nnz");
+    for (int64_t i = 0; i < n; ++i)
+      CHECK_ARG(rowptr[i + 1] >= rowptr[i], "rowptr must be non-decreasing");
+    for (int64_t k = 0; k < nnz; ++k)
+      CHECK_ARG(colidx[k] >= 0 && colidx[k] < n, "column index out of range");
+    set_device(ctx);
+    auto sp = std::unique_ptr<illum_sparse, void (*)(illum_sparse *)>(new illum_sparse(),
+                                                                      destroy_sparse);
+    sp->ctx = ctx;
+    sp->n = n;
+    sp->nnz = nnz;
+    hipStream_t s = stream_of(ctx);
+    sp->d_rowptr = dalloc<int64_t>(n + 1);
+    sp->d_colidx = dalloc<int32_t>(nnz);
+    sp->d_rval = dalloc<double>(nnz);
+    sp->d_rval32 = dalloc<float>(nnz);
+    std::vector<float> v32(nnz);
+    for (int64_t k = 0; k < nnz; ++k) v32[k] = (float)val[k];
+    ILLUM_HIP(hipMemcpyAsync(sp->d_rowptr, rowptr, 8 * (n + 1), hipMemcpyHostToDevice, s));
+    if (nnz) {
+      ILLUM_HIP(hipMemcpyAsync(sp->d_colidx, colidx, 4 * nnz, hipMemcpyHostToDevice, s));
+      ILLUM_HIP(hipMemcpyAsync(sp->d_rval, val, 8 * nnz, hipMemcpyHostToDevice, s));
+      ILLUM_HIP(hipMemcpyAsync(sp->d_rval32, v32.data(), 4 * nnz, hipMemcpyHostToDevice, s));
+    }
+    sync(ctx);
+    *out = sp.release();
+  });
+}
+
+int illum_sparse_destroy(illum_sparse *sp) {
+  return guarded([&] { if (sp) destroy_sparse(sp); });
+}
+
+int illum_sparse_info(const illum_sparse *sp, int64_t *n, int64_t *nnz) {
+  return guarded([&] {
+    CHECK_ARG(sp, "null matrix");
+    if (n) *n = sp->n;
+    if (nnz) *nnz = sp->nnz;
+  });
+}
+
+int illum_sparse_copy_csc(const illum_sparse *sp, int64_t *colptr, int32_t *rowidx,
+                          double *val) {
+  return guarded([&] {
+    CHECK_ARG(sp, "null matrix");
+    if (!sp->has_csc) throw Error(ILLUM_ERR_ARG, "matrix has no CSC form");
+    hipStream_t s = stream_of(sp->ctx);
+    if (colptr) ILLUM_HIP(hipMemcpyAsync(colptr, sp->d_colptr, 8 * (sp->n + 1), hipMemcpyDeviceToHost, s));
+    if (rowidx && sp->nnz) ILLUM_HIP(hipMemcpyAsync(rowidx, sp->d_rowidx, 4 * sp->nnz, hipMemcpyDeviceToHost, s));
+    if (val && sp->nnz) ILLUM_HIP(hipMemcpyAsync(val, sp->d_cval, 8 * sp->nnz, hipMemcpyDeviceToHost, s));
+    sync(sp->ctx);
+  });
+}
+
+int illum_sparse_copy_csr(const illum_sparse *sp, int64_t *rowptr, int32_t *colidx,
+                          double *val) {
+  return guarded([&] {
+    CHECK_ARG(sp, "null matrix");
+    hipStream_t s = stream_of(sp->ctx);
+    if (rowptr) ILLUM_HIP(hipMemcpyAsync(rowptr, sp->d_rowptr, 8 * (sp->n + 1), hipMemcpyDeviceToHost, s));
+    if (colidx && sp->nnz) ILLUM_HIP(hipMemcpyAsync(colidx, sp->d_colidx, 4 * sp->nnz, hipMemcpyDeviceToHost, s));
+    if (val && sp->nnz) ILLUM_HIP(hipMemcpyAsync(val, sp->d_rval, 8 * sp->nnz, hipMemcpyDeviceToHost, s));
+    sync(sp->ctx);
+  });
+}
+
+int illum_sparse_visibility_words(const illum_sparse *sp, int64_t *words_per_row) {
+  return guarded([&] {
+    CHECK_ARG(sp && words_per_row, "null argument");
+    if (!sp->d_vis) throw Error(ILLUM_ERR_ARG, "matrix carries no visibility rows");
+    *words_per_row = sp->vis_words;
+  });
+}
+
+int illum_sparse_copy_visibility(const illum_sparse *sp, uint32_t *bits) {
+  return guarded([&] {
+    CHECK_ARG(sp && bits, "null argument");
+    if (!sp->d_vis) throw Error(ILLUM_ERR_ARG, "matrix carries no visibility rows");
+    ILLUM_HIP(hipMemcpy(bits, sp->d_vis, 4 * (size_t)sp->n * sp->vis_words, hipMemcpyDeviceToHost));
+  });
+}
+
+int illum_horizons(illum_bvh *bvh, illum_bvh *occluder, int nphi, double theta_eps,
+                   double offset, double *out, int flags) {
+  return guarded([&] {
+    CHECK_ARG(bvh, "null bvh");
+    set_device(bvh->mesh->ctx);
+    horizons(bvh, occluder, nphi, theta_eps, offset, out, flags);
+  });
+}
+
+int illum_fib_spiral(double *xy, int n) {
+  return guarded([&] {
+    CHECK_ARG(xy && n > 0, "bad argument");
+    // illum.cpp:468-483; golden ratio as arma::datum::gratio.
+    const double TWO_PI = 2 * M_PI;
+    const double gr = 1.6180339887498948482;
+    double a = 0.0, da = (TWO_PI * (gr - 1)) / gr;
+    double r = 0.0, dr = 1.0 / (n + 1);
+    for (int i = 0; i < n; ++i) {
+      xy[i] = r * std::cos(a);
+      xy[n + i] = r * std::sin(a);
+      a = std::fmod(a + da, TWO_PI);
+      r += dr;
+    }
+  });
+}
+
+int illum_direct_radiosity(illum_mesh *mesh, const double *horizons_, int nphi,
+                           const double *sun_km, int S, const double *disk_xy, int ndisk,
+                           double *E, int flags) {
+  return guarded([&] {
+    CHECK_ARG(mesh, "null mesh");
+    set_device(mesh->ctx);
+    direct_radiosity(mesh, horizons_, nphi, sun_km, S, disk_xy, ndisk, E, flags);
+  });
+}
+
+int illum_radiosity_solve(illum_sparse *F, const double *E, int S, int method, double rel_tol,
+                          int max_iter, double *B, double *resid, int *iters, int flags) {
+  return guarded([&] {
+    CHECK_ARG(F, "null matrix");
+    set_device(F->ctx);
+    radiosity_solve(F, E, S, method, rel_tol, max_iter, B, resid, iters, flags);
+  });
+}
+
+int illum_thermal_create(illum_ctx *ctx, int64_t nf, int ngrid, const int64_t *grid_off,
+                         const double *z, const double *ti, const double *rhoc,
+                         const int32_t *grid_of_facet, const double *ti_scale,
+                         double emissivity, double T0, const double *T0_full, int precision,
+                         illum_thermal **out) {
+  return guarded([&] {
+    CHECK_ARG(ctx && out, "null argument");
+    set_device(ctx);
+    *out = thermal_create(ctx, nf, ngrid, grid_off, z, ti, rhoc, grid_of_facet, ti_scale,
+                          emissivity, T0, T0_full, precision);
+  });
+}
+
+int illum_thermal_destroy(illum_thermal *th) {
+  return guarded([&] { if (th) destroy_thermal(th); });
+}
+
+int illum_thermal_info(const illum_thermal *th, int64_t *nf, int *nzmax) {
+  return guarded([&] {
+    CHECK_ARG(th, "null thermal model");
+    if (nf) *nf = th->nf;
+    if (nzmax) *nzmax = th->nzmax;
+  });
+}
+
+int illum_thermal_step(illum_thermal *th, double dt, const double *Q, int flags) {
+  return guarded([&] {
+    CHECK_ARG(th, "null thermal model");
+    set_device(th->ctx);
+    thermal_run(th, dt, Q, 1, flags, 0.0, nullptr);
+  });
+}
+
+int illum_thermal_run(illum_thermal *th, double dt, const double *Qseries, int nsteps,
+                      int flags) {
+  return guarded([&] {
+    CHECK_ARG(th, "null thermal model");
+    set_device(th->ctx);
+    thermal_run(th, dt, Qseries, nsteps, flags, 0.0, nullptr);
+  });
+}
+
+int illum_thermal_get(illum_thermal *th, double *T, double *Tsurf, double *Fsurf, int flags) {
+  return guarded([&] {
+    CHECK_ARG(th, "null thermal model");
+    set_device(th->ctx);
+    thermal_get(th, T, Tsurf, Fsurf, flags);
+  });
+}
+
+int illum_conductionQ(int nz, const double *z, double dt, double Qn, double Qnp1, double *T,
+                      const double *ti, const double *rhoc, double emiss, double *Tsurf,
+                      double Fgeotherm, double *Fsurf) {
+  return guarded([&] {
+    CHECK_ARG(nz >= 2 && z && T && ti && rhoc && Tsurf && Fsurf, "bad argument");
+    illum_ctx *ctx = default_ctx();
+    set_device(ctx);
+    const int64_t off[2] = {0, nz};
+    std::vector<double> T0(nz + 1);
+    T0[0] = *Tsurf;
+    for (int i = 0; i < nz; ++i) T0[i + 1] = T[i];
+    illum_thermal *th = thermal_create(ctx, 1, 1, off, z, ti, rhoc, nullptr, nullptr, emiss,
+                                       0.0, T0.data(), ILLUM_THERM_FP64);
+    std::unique_ptr<illum_thermal, void (*)(illum_thermal *)> guard(th, destroy_thermal);
+    ILLUM_HIP(hipMemcpy(th->d_Qprev, &Qn, 8, hipMemcpyHostToDevice));
+    thermal_run(th, dt, &Qnp1, 1, 0, Fgeotherm, nullptr);
+    std::vector<double> Tout(nz + 1);
+    double F = 0;
+    thermal_get(th, Tout.data(), nullptr, &F, 0);
+    *Tsurf = Tout[0];
+    for (int i = 0; i < nz; ++i) T[i] = Tout[i + 1];
+    *Fsurf = F;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,372 @@
+// Form-factor assembly on the GPU (illum_context::compute_F, illum.cpp:31-237).
+//
+// Pipeline (all on the context stream):
+//   1. ff_visibility: one workgroup per source facet j.  The O(N) facing cull
+//      (illum.cpp:56-68) runs over all receivers with survivors compacted into
+//      an LDS ray queue (wave ballot + popcount prefix), so the closest-hit
+//      traversals (illum.cpp:72-94) run on full wavefronts.  Hit facets are
+//      deduplicated in an LDS bitset of N bits (replacing the sorted insert of
+//      illum.cpp:87-94).  The bitset row is written out raw (the visibility
+//      set) and filtered by the form-factor value (zero drop, illum.cpp:125-127)
+//      into the nonzero-pattern matrix M (row j = column j of F).
+//   2. prefix sums of per-row counts give the CSC column pointers; ff_fill
+//      walks each bit row in ascending order, so entries come out sorted by
+//      (j, i) exactly as the reference sort (illum.cpp:184-191) leaves them.
+//   3. a 512x512-bit tiled transpose of M yields the CSR pattern (row i =
+//      receiver), filled the same way.  Everything is deterministic: no
+//      floating-point atomics, no order-dependent accumulation.
+#include "hip_util.hpp"
+#include "ray.hpp"
+
+#include <cfloat>
+#include <cmath>
+
+namespace illum {
+
+constexpr int FF_BLOCK = 256;
+constexpr int FF_PER = 8;                          // candidates per thread per chunk
+constexpr int FF_QCAP = FF_BLOCK * (FF_PER + 1);   // queue capacity (ints)
+
+// Form factor F(i, j) for receiver i and source j (illum.cpp:100-119), in FP64
+// from the FP32 centroids/normals.  Returns 0 for r < DBL_EPSILON.
+__device__ __forceinline__ double ff_value(float4 pi, float4 ni, float4 pj, float4 nj,
+                                           double Aj) {
+  const double px = (double)pj.x - (double)pi.x;
+  const double py = (double)pj.y - (double)pi.y;
+  const double pz = (double)pj.z - (double)pi.z;
+  const double r = sqrt((px * px + py * py) + pz * pz);
+  if (r < DBL_EPSILON) return 0.0;
+  const double ux = px / r, uy = py / r, uz = pz / r;
+  const double mij = fmax(0.0, ((double)ni.x * ux + (double)ni.y * uy) + (double)ni.z * uz);
+  const double mji = fmax(0.0, -(((double)nj.x * ux + (double)nj.y * uy) + (double)nj.z * uz));
+  return ((mij * mji) * Aj) / ((M_PI * r) * r);
+}
+
+// Maps consecutive workgroup ids (dispatched round-robin over the 8 XCDs) to
+// contiguous source ranges per XCD, so each XCD's L2 serves spatially
+// coherent sources.  Bijective on [0, n).
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+  const int x = b & 7, k = b >> 3;
+  const int per = n >> 3, rem = n & 7;
+  return x * per + (x < rem ? x : rem) + k;
+}
+
+__global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
+    int nf, int nwords, const float4 *__restrict__ pr4, const float4 *__restrict__ n4,
+    const float4 *__restrict__ nodes, const float4 *__restrict__ tris, float offset,
+    const double *__restrict__ area, const double *__restrict__ albedo_vec, double albedo,
+    uint32_t *__restrict__ vis, uint32_t *__restrict__ mask, int32_t *__restrict__ counts,
+    int *__restrict__ status) {
+  extern __shared__ uint32_t smem[];
+  uint32_t *bits = smem;
+  int *queue = (int *)(smem + nwords);
+  __shared__ int s_qn;
+  __shared__ int s_red[FF_BLOCK / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int j = xcd_remap(blockIdx.x, nf);
+
+  for (int w = tid; w < nwords; w += FF_BLOCK) bits[w] = 0u;
+  if (tid == 0) s_qn = 0;
+  const float4 pj = pr4[j], nj = n4[j];
+  const float qx = pj.x + offset * nj.x, qy = pj.y + offset * nj.y, qz = pj.z + offset * nj.z;
+  __syncthreads();
+
+  const int chunk = FF_BLOCK * FF_PER;
+  for (int base = 0; base < nf; base += chunk) {
+#pragma unroll
+    for (int k = 0; k < FF_PER; ++k) {
+      const int i = base + k * FF_BLOCK + tid;
+      bool s = false;
+      if (i < nf && i != j) {
+        const float4 pi = pr4[i];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float dot = (dx * nj.x + dy * nj.y) + dz * nj.z;
+        s = !(dot <= -pi.w);
+      }
+      const uint64_t m = __ballot(s);
+      if (m) {
+        int slot = 0;
+        if (lane == 0) slot = atomicAdd(&s_qn, __popcll(m));
+        slot = __shfl(slot, 0);
+        if (s) queue[slot + __popcll(m & lanemask_lt())] = i;
+      }
+    }
+    __syncthreads();
+    const bool last = base + chunk >= nf;
+    int qrem = s_qn;
+    while (qrem >= FF_BLOCK || (last && qrem > 0)) {
+      const int take = qrem < FF_BLOCK ? qrem : FF_BLOCK;
+      if (tid < take) {
+        const int i = queue[qrem - take + tid];
+        const float4 pi = pr4[i];
+        const DRay r = make_ray(qx, qy, qz, pi.x - qx, pi.y - qy, pi.z - qz);
+        float t;
+        const int h = trace_closest(nodes, tris, r, &t);
+        if (h >= 0 && h != j) atomicOr(&bits[h >> 5], 1u << (h & 31));
+      }
+      qrem -= take;
+    }
+    __syncthreads();
+    if (tid == 0) s_qn = qrem;
+    __syncthreads();
+  }
+
+  // Write the raw visibility row and its nonzero-form-factor filter.
+  const double Aj = area[j];
+  int cnt = 0;
+  bool nan = false;
+  for (int w = tid; w < nwords; w += FF_BLOCK) {
+    const uint32_t word = bits[w];
+    uint32_t keep = 0u;
+    uint32_t rest = word;
+    while (rest) {
+      const int b = __ffs(rest) - 1;
+      rest &= rest - 1u;
+      const int i = 32 * w + b;
+      const double rho = albedo_vec ? albedo_vec[i] : albedo;
+      const double v = ff_value(pr4[i], n4[i], pj, nj, Aj) * rho;
+      if (v != v) nan = true;
+      if (v != 0.0) keep |= 1u << b;
+    }
+    vis[(size_t)j * nwords + w] = word;
+    mask[(size_t)j * nwords + w] = keep;
+    cnt += __popc(keep);
+  }
+  if (nan) atomicOr(status, 1);
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+  if (lane == 0) s_red[wid] = cnt;
+  __syncthreads();
+  if (tid == 0) {
+    int s = 0;
+    for (int w = 0; w < FF_BLOCK / 64; ++w) s += s_red[w];
+    counts[j] = s;
+  }
+}
+
+// Block-wide exclusive scan of one int per thread (blockDim == 256).
+__device__ __forceinline__ int block_exscan256(int v, int *s_wave, int *total) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  int x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_wave[wid] = x;
+  __syncthreads();
+  int off = 0, tot = 0;
+  for (int w = 0; w < 4; ++w) {
+    const int sw = s_wave[w];
+    if (w < wid) off += sw;
+    tot += sw;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + x - v;
+}
+
+// Fills one CSC column (csr == 0: row of M is source j, bits are receivers i)
+// or one CSR row (csr == 1: row of M^T is receiver i, bits are sources j).
+__global__ __launch_bounds__(256) void ff_fill_kernel(
+    int nrows, int nwords, const uint32_t *__restrict__ m, const int64_t *__restrict__ ptr,
+    int csr, const float4 *__restrict__ pr4, const float4 *__restrict__ n4,
+    const double *__restrict__ area, const double *__restrict__ albedo_vec, double albedo,
+    int32_t *__restrict__ idx_out, double *__restrict__ val_out, float *__restrict__ val32_out) {
+  __shared__ int s_wave[4];
+  const int row = blockIdx.x;
+  if (row >= nrows) return;
+  const int tid = threadIdx.x;
+  const int per = (nwords + 255) / 256;
+  const int w0 = tid * per, w1 = min(nwords, w0 + per);
+  const uint32_t *mr = m + (size_t)row * nwords;
+  int c = 0;
+  for (int w = w0; w < w1; ++w) c += __popc(mr[w]);
+  int total;
+  int off = block_exscan256(c, s_wave, &total);
+  int64_t pos = ptr[row] + off;
+  const float4 prow = pr4[row], nrow = n4[row];
+  for (int w = w0; w < w1; ++w) {
+    uint32_t rest = mr[w];
+    while (rest) {
+      const int b = __ffs(rest) - 1;
+      rest &= rest - 1u;
+      const int col = 32 * w + b;
+      double v;
+      if (csr) {  // row = receiver i, col = source j
+        const double rho = albedo_vec ? albedo_vec[row] : albedo;
+        v = ff_value(prow, nrow, pr4[col], n4[col], area[col]) * rho;
+      } else {    // row = source j, col = receiver i
+        const double rho = albedo_vec ? albedo_vec[col] : albedo;
+        v = ff_value(pr4[col], n4[col], prow, nrow, area[row]) * rho;
+      }
+      idx_out[pos] = col;
+      val_out[pos] = v;
+      if (val32_out) val32_out[pos] = (float)v;
+      ++pos;
+    }
+  }
+}
+
+// Transposes a (rows_p x 32*nwords)-bit matrix stored as rows of nwords words
+// (rows_p == 32*nwords, both multiples of 512) in 512x512-bit tiles.
+__global__ __launch_bounds__(256) void bit_transpose_kernel(const uint32_t *__restrict__ in,
+                                                            uint32_t *__restrict__ out,
+                                                            int nwords) {
+  __shared__ uint32_t tin[512][17];
+  __shared__ uint32_t tout[512][17];
+  const int tc = blockIdx.x, tr = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int k = 0; k < 32; ++k) {
+    const int idx = tid + 256 * k, r = idx >> 4, w = idx & 15;
+    tin[r][w] = in[(size_t)(512 * tr + r) * nwords + 16 * tc + w];
+  }
+  __syncthreads();
+  for (int combo = wv; combo < 128; combo += 4) {
+    const int w = combo >> 3, qp = combo & 7;
+    const int q = 2 * qp + (lane >> 5);
+    const uint32_t v = tin[32 * q + (lane & 31)][w];
+    for (int b = 0; b < 32; ++b) {
+      const uint64_t mb = __ballot((v >> b) & 1u);
+      if (lane == b) tout[32 * w + b][2 * qp] = (uint32_t)mb;
+      if (lane == 32 + b) tout[32 * w + b][2 * qp + 1] = (uint32_t)(mb >> 32);
+    }
+  }
+  __syncthreads();
+  for (int k = 0; k < 32; ++k) {
+    const int idx = tid + 256 * k, o = idx >> 4, q = idx & 15;
+    out[(size_t)(512 * tc + o) * nwords + 16 * tr + q] = tout[o][q];
+  }
+}
+
+__global__ void popcount_rows_kernel(int nrows, int nwords, const uint32_t *__restrict__ m,
+                                     int32_t *__restrict__ counts) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= nrows) return;
+  int c = 0;
+  for (int w = lane; w < nwords; w += 64) c += __popc(m[(size_t)row * nwords + w]);
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if (lane == 0) counts[row] = c;
+}
+
+__global__ void closest_hit_kernel(int64_t nray, const float *__restrict__ orig,
+                                   const float *__restrict__ dir,
+                                   const float4 *__restrict__ nodes,
+                                   const float4 *__restrict__ tris, int32_t *__restrict__ hit,
+                                   float *__restrict__ tout) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nray) return;
+  const DRay r = make_ray(orig[3 * k], orig[3 * k + 1], orig[3 * k + 2], dir[3 * k],
+                          dir[3 * k + 1], dir[3 * k + 2]);
+  float t;
+  hit[k] = trace_closest(nodes, tris, r, &t);
+  tout[k] = t;
+}
+
+// ---------------------------------------------------------------------------
+void bvh_closest_hit(illum_bvh *bvh, const float *orig, const float *dir, int64_t nray,
+                     int32_t *hit, float *t) {
+  illum_ctx *ctx = bvh->mesh->ctx;
+  if (nray <= 0) return;
+  DBuf<float> o(3 * nray), d(3 * nray), tt(nray);
+  DBuf<int32_t> h(nray);
+  hipStream_t s = stream_of(ctx);
+  ILLUM_HIP(hipMemcpyAsync(o.p, orig, 12 * nray, hipMemcpyHostToDevice, s));
+  ILLUM_HIP(hipMemcpyAsync(d.p, dir, 12 * nray, hipMemcpyHostToDevice, s));
+  const int blocks = (int)((nray + 255) / 256);
+  closest_hit_kernel<<<blocks, 256, 0, s>>>(nray, o.p, d.p, (const float4 *)bvh->d_nodes,
+                                            (const float4 *)bvh->d_tris, h.p, tt.p);
+  ILLUM_LAUNCH_CHECK();
+  ILLUM_HIP(hipMemcpyAsync(hit, h.p, 4 * nray, hipMemcpyDeviceToHost, s));
+  ILLUM_HIP(hipMemcpyAsync(t, tt.p, 4 * nray, hipMemcpyDeviceToHost, s));
+  sync(ctx);
+}
+
+illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_vec,
+                           double offset) {
+  illum_mesh *mesh = bvh->mesh;
+  illum_ctx *ctx = mesh->ctx;
+  hipStream_t s = stream_of(ctx);
+  const int64_t nf = mesh->g.nf;
+  const int nwords = (int)(((nf + 511) / 512) * 16);
+  const int64_t rows_p = (int64_t)nwords * 32;
+  const size_t lds = (size_t)(nwords + FF_QCAP) * 4;
+  if (lds > 160 * 1024)
+    throw Error(ILLUM_ERR_CAPACITY, "too many facets for the LDS visibility bitset (max ~1.2M)");
+  if (lds > 64 * 1024)
+    ILLUM_HIP(hipFuncSetAttribute((const void *)ff_visibility_kernel,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+
+  std::unique_ptr<illum_sparse, void (*)(illum_sparse *)> sp(new illum_sparse(), destroy_sparse);
+  sp->ctx = ctx;
+  sp->n = nf;
+  sp->vis_words = nwords;
+  sp->d_vis = dalloc<uint32_t>((size_t)nf * nwords);
+
+  DBuf<double> d_alb;
+  if (albedo_vec) {
+    d_alb.reset(nf);
+    ILLUM_HIP(hipMemcpyAsync(d_alb.p, albedo_vec, 8 * nf, hipMemcpyHostToDevice, s));
+  }
+  DBuf<uint32_t> M((size_t)rows_p * nwords), MT((size_t)rows_p * nwords);
+  DBuf<int32_t> counts(rows_p);
+  DBuf<int> status(1);
+  ILLUM_HIP(hipMemsetAsync(M.p, 0, (size_t)rows_p * nwords * 4, s));
+  ILLUM_HIP(hipMemsetAsync(status.p, 0, 4, s));
+
+  {
+    StageTimer st(ctx, "ff_visibility");
+    ff_visibility_kernel<<<(int)nf, FF_BLOCK, lds, s>>>(
+        (int)nf, nwords, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4,
+        (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_tris, (float)offset,
+        mesh->d_area, d_alb.p, albedo, sp->d_vis, M.p, counts.p, status.p);
+    ILLUM_LAUNCH_CHECK();
+  }
+  // CSC
+  sp->d_colptr = dalloc<int64_t>(nf + 1);
+  {
+    StageTimer st(ctx, "ff_csc");
+    scan_counts(ctx, counts.p, nf, sp->d_colptr);
+  }
+  int64_t nnz = 0;
+  int hstatus = 0;
+  ILLUM_HIP(hipMemcpyAsync(&nnz, sp->d_colptr + nf, 8, hipMemcpyDeviceToHost, s));
+  ILLUM_HIP(hipMemcpyAsync(&hstatus, status.p, 4, hipMemcpyDeviceToHost, s));
+  sync(ctx);
+  if (hstatus) throw Error(ILLUM_ERR_NAN, "computed a NaN form factor");
+  if (nnz >= ((int64_t)1 << 40)) throw Error(ILLUM_ERR_CAPACITY, "nnz out of range");
+  sp->nnz = nnz;
+  sp->has_csc = true;
+  sp->d_rowidx = dalloc<int32_t>(nnz);
+  sp->d_cval = dalloc<double>(nnz);
+  {
+    StageTimer st(ctx, "ff_csc");
+    ff_fill_kernel<<<(int)nf, 256, 0, s>>>((int)nf, nwords, M.p, sp->d_colptr, 0,
+                                           (const float4 *)mesh->d_pr4,
+                                           (const float4 *)mesh->d_n4, mesh->d_area, d_alb.p,
+                                           albedo, sp->d_rowidx, sp->d_cval, nullptr);
+    ILLUM_LAUNCH_CHECK();
+  }
+  // CSR via the bit transpose.
+  sp->d_rowptr = dalloc<int64_t>(nf + 1);
+  sp->d_colidx = dalloc<int32_t>(nnz);
+  sp->d_rval = dalloc<double>(nnz);
+  sp->d_rval32 = dalloc<float>(nnz);
+  {
+    StageTimer st(ctx, "ff_csr");
+    dim3 g(nwords / 16, (unsigned)(rows_p / 512));
+    bit_transpose_kernel<<<g, 256, 0, s>>>(M.p, MT.p, nwords);
+    ILLUM_LAUNCH_CHECK();
+    popcount_rows_kernel<<<(int)((nf + 3) / 4), 256, 0, s>>>((int)nf, nwords, MT.p, counts.p);
+    ILLUM_LAUNCH_CHECK();
+    scan_counts(ctx, counts.p, nf, sp->d_rowptr);
+    ff_fill_kernel<<<(int)nf, 256, 0, s>>>((int)nf, nwords, MT.p, sp->d_rowptr, 1,
+                                           (const float4 *)mesh->d_pr4,
+                                           (const float4 *)mesh->d_n4, mesh->d_area, d_alb.p,
+                                           albedo, sp->d_colidx, sp->d_rval, sp->d_rval32);
+    ILLUM_LAUNCH_CHECK();
+  }
+  sync(ctx);
+  return sp.release();
+}
+
+}  // namespace illum

@@ -1,0 +1,88 @@
+// HIP helpers shared by the kernel translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <memory>
+#include <string>
+
+#include "internal.hpp"
+
+#define ILLUM_HIP(call)                                                                   \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      throw ::illum::Error(e_ == hipErrorOutOfMemory ? ILLUM_ERR_OOM : ILLUM_ERR_HIP,     \
+                           std::string(#call) + ": " + hipGetErrorString(e_));           \
+  } while (0)
+
+#define ILLUM_LAUNCH_CHECK() ILLUM_HIP(hipGetLastError())
+
+namespace illum {
+
+inline hipStream_t stream_of(const illum_ctx *ctx) { return (hipStream_t)ctx->stream; }
+
+template <class T>
+T *dalloc(size_t n) {
+  if (n == 0) n = 1;
+  void *p = nullptr;
+  ILLUM_HIP(hipMalloc(&p, n * sizeof(T)));
+  return (T *)p;
+}
+
+template <class T>
+void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+// RAII device buffer for temporaries inside one call.
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() {}
+  explicit DBuf(size_t n_) : p(dalloc<T>(n_)), n(n_) {}
+  ~DBuf() { dfree(p); }
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  void reset(size_t n_) { dfree(p); p = dalloc<T>(n_); n = n_; }
+};
+
+// Records hipEvents around a named stage on the context stream when profiling
+// is enabled; durations are collected at the next synchronisation point.
+struct StageTimer {
+  illum_ctx *ctx;
+  std::string name;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  StageTimer(illum_ctx *c, const char *n) : ctx(c), name(n) {
+    if (ctx->profiling) {
+      ILLUM_HIP(hipEventCreate(&e0));
+      ILLUM_HIP(hipEventCreate(&e1));
+      ILLUM_HIP(hipEventRecord(e0, stream_of(ctx)));
+    }
+  }
+  ~StageTimer() {
+    if (ctx->profiling && e0) {
+      if (hipEventRecord(e1, stream_of(ctx)) == hipSuccess)
+        ctx->pending.push_back({name, {(void *)e0, (void *)e1}});
+    }
+  }
+};
+
+// Free every device buffer of a handle and the handle itself.
+void destroy_sparse(illum_sparse *s);
+void destroy_thermal(illum_thermal *t);
+
+// Drains pending stage events into the profile (after the stream is synced).
+void collect_profile(illum_ctx *ctx);
+void sync(illum_ctx *ctx);
+
+// Exclusive scan of n int32 counts into n+1 int64 offsets (device), on stream.
+void scan_counts(illum_ctx *ctx, const int32_t *counts, int64_t n, int64_t *offsets);
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const unsigned lane = __lane_id();
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+}  // namespace illum

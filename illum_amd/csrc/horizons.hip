@@ -1,0 +1,233 @@
+// Horizon maps (illum.cpp:274-363) and direct insolation (illum.cpp:382-483).
+//
+// Horizons: one lane per (facet, azimuth); a wavefront covers 64 consecutive
+// azimuths of one facet, so its rays share an origin and sweep neighbouring
+// directions (coherent traversal).  Each lane runs the reference bisection on
+// zenith angle (illum.cpp:304-316): start at pi/2, a hit moves the upper bound
+// down, a miss moves the lower bound up, stop when the midpoint moves by at
+// most theta_eps.  The midpoints form a fixed binary tree; it is tabulated on
+// the host (theta, sin theta, cos theta, stop flag per tree node) so the
+// kernel evaluates exactly the reference's directions with the host's libm
+// values and no device transcendental.  Only the boolean hit matters, so rays
+// use the any-hit traversal.
+#include "hip_util.hpp"
+#include "ray.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace illum {
+
+__global__ __launch_bounds__(256) void horizons_kernel(
+    int64_t nf, int nphi, const float4 *__restrict__ pr4, const float4 *__restrict__ n4,
+    const double *__restrict__ frame, const float4 *__restrict__ nodes,
+    const float4 *__restrict__ tris, float offset, const double *__restrict__ cphi,
+    const double *__restrict__ sphi, const double *__restrict__ th, int tree_nodes,
+    double *__restrict__ out) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nf * nphi) return;
+  const int64_t j = idx / nphi;
+  const int k = (int)(idx - j * nphi);
+  const float4 p = pr4[j], n = n4[j];
+  const float ox = p.x + offset * n.x, oy = p.y + offset * n.y, oz = p.z + offset * n.z;
+  const double *F = frame + 9 * j;
+  const double F00 = F[0], F10 = F[1], F20 = F[2];  // column b
+  const double F01 = F[3], F11 = F[4], F21 = F[5];  // column t
+  const double F02 = F[6], F12 = F[7], F22 = F[8];  // column n
+  const double cp = cphi[k], sp = sphi[k];
+  int m = 1;
+  for (;;) {
+    const double st = th[4 * m + 1], ct = th[4 * m + 2];
+    const double vx = cp * st, vy = sp * st, vz = ct;
+    const float dx = (float)((F00 * vx + F01 * vy) + F02 * vz);
+    const float dy = (float)((F10 * vx + F11 * vy) + F12 * vz);
+    const float dz = (float)((F20 * vx + F21 * vy) + F22 * vz);
+    const DRay r = make_ray(ox, oy, oz, dx, dy, dz);
+    const bool hit = trace_any(nodes, tris, r);
+    m = 2 * m + (hit ? 0 : 1);
+    if (m >= tree_nodes) { out[idx] = __builtin_nan(""); return; }
+    if (th[4 * m + 3] != 0.0) break;
+  }
+  out[idx] = th[4 * m];
+}
+
+// Tabulates the bisection tree: node m (root 1) holds theta (the midpoint
+// tested at m), sin, cos and whether the search stops on arriving at m.
+static std::vector<double> bisection_table(double theta_eps, int *tree_nodes) {
+  const int max_depth = 22;
+  std::vector<double> lo(2, 0.0), hi(2, M_PI), th(8, 0.0);
+  // node 1
+  std::vector<double> tab(4 * 2, 0.0);
+  tab[4 * 1 + 0] = M_PI / 2;
+  int depth = 0;
+  std::vector<int> frontier = {1};
+  std::vector<double> flo = {0.0}, fhi = {M_PI};
+  while (!frontier.empty()) {
+    if (depth >= max_depth) throw Error(ILLUM_ERR_ARG, "theta_eps too small (bisection deeper than 22 levels)");
+    std::vector<int> nf;
+    std::vector<double> nlo, nhi;
+    const size_t need = (size_t)4 << (depth + 2);
+    if (tab.size() < need) tab.resize(need, 0.0);
+    for (size_t q = 0; q < frontier.size(); ++q) {
+      const int m = frontier[q];
+      const double mid = tab[4 * m];
+      for (int c = 0; c < 2; ++c) {  // c = 0: hit (hi = mid), c = 1: miss (lo = mid)
+        const double l = c == 0 ? flo[q] : mid, h = c == 0 ? mid : fhi[q];
+        const double mid2 = (l + h) / 2;
+        const int child = 2 * m + c;
+        tab[4 * child] = mid2;
+        const bool stop = !(std::fabs(mid2 - mid) > theta_eps);
+        tab[4 * child + 3] = stop ? 1.0 : 0.0;
+        if (!stop) { nf.push_back(child); nlo.push_back(l); nhi.push_back(h); }
+      }
+    }
+    frontier.swap(nf); flo.swap(nlo); fhi.swap(nhi);
+    ++depth;
+  }
+  const int nodes = (int)(tab.size() / 4);
+  for (int m = 1; m < nodes; ++m) {
+    tab[4 * m + 1] = std::sin(tab[4 * m]);
+    tab[4 * m + 2] = std::cos(tab[4 * m]);
+  }
+  *tree_nodes = nodes;
+  return tab;
+}
+
+void horizons(illum_bvh *bvh, illum_bvh *occl, int nphi, double theta_eps, double offset,
+              double *out, int flags) {
+  illum_mesh *mesh = bvh->mesh;
+  illum_ctx *ctx = mesh->ctx;
+  hipStream_t s = stream_of(ctx);
+  if (nphi < 2) throw Error(ILLUM_ERR_ARG, "nphi must be >= 2");
+  if (!(theta_eps > 0)) throw Error(ILLUM_ERR_ARG, "theta_eps must be positive");
+  if (!out) throw Error(ILLUM_ERR_ARG, "null output");
+  illum_bvh *target = occl ? occl : bvh;
+  const int64_t nf = mesh->g.nf;
+  int tree_nodes = 0;
+  std::vector<double> tab = bisection_table(theta_eps, &tree_nodes);
+  // phis = linspace(0, 2 pi, nphi) with the endpoint exact (illum.cpp:329).
+  std::vector<double> cs(nphi), sn(nphi);
+  const double dphi = 2 * M_PI / (nphi - 1);
+  for (int k = 0; k < nphi; ++k) {
+    const double ph = k == nphi - 1 ? 2 * M_PI : k * dphi;
+    cs[k] = std::cos(ph);
+    sn[k] = std::sin(ph);
+  }
+  DBuf<double> d_tab(tab.size()), d_c(nphi), d_s(nphi);
+  ILLUM_HIP(hipMemcpyAsync(d_tab.p, tab.data(), 8 * tab.size(), hipMemcpyHostToDevice, s));
+  ILLUM_HIP(hipMemcpyAsync(d_c.p, cs.data(), 8 * nphi, hipMemcpyHostToDevice, s));
+  ILLUM_HIP(hipMemcpyAsync(d_s.p, sn.data(), 8 * nphi, hipMemcpyHostToDevice, s));
+  const int64_t total = nf * nphi;
+  DBuf<double> d_out;
+  double *dst = out;
+  if (!(flags & ILLUM_OUT_ON_DEVICE)) { d_out.reset(total); dst = d_out.p; }
+  {
+    StageTimer st(ctx, "horizons");
+    horizons_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(
+        nf, nphi, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4, mesh->d_frame,
+        (const float4 *)target->d_nodes, (const float4 *)target->d_tris, (float)offset, d_c.p,
+        d_s.p, d_tab.p, tree_nodes, dst);
+    ILLUM_LAUNCH_CHECK();
+  }
+  if (!(flags & ILLUM_OUT_ON_DEVICE))
+    ILLUM_HIP(hipMemcpyAsync(out, dst, 8 * total, hipMemcpyDeviceToHost, s));
+  sync(ctx);
+}
+
+// ---------------------------------------------------------------------------
+// Direct insolation.  One lane per (facet, sun direction); E is nf x S
+// column-major so consecutive lanes write consecutive facets.
+constexpr double ONE_AU_KM = 149.5978707e6;   // constants.hpp:7
+constexpr double SUN_RADIUS_KM = 695508;      // constants.hpp:8
+
+__global__ __launch_bounds__(256) void direct_kernel(
+    int64_t nf, int S, int nphi, const float4 *__restrict__ pr4,
+    const double *__restrict__ frame, const double *__restrict__ hz,
+    const double *__restrict__ sun, const double *__restrict__ disk, int ndisk,
+    double *__restrict__ E) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nf * S) return;
+  const int64_t f = idx % nf;
+  const int s = (int)(idx / nf);
+  const double TWO_PI = 2 * M_PI;
+  const double dphi = TWO_PI / (nphi - 1);
+  const float4 pf = pr4[f];
+  const double px = pf.x, py = pf.y, pz = pf.z;
+  const double sx = sun[3 * s], sy = sun[3 * s + 1], sz = sun[3 * s + 2];
+  double Nx = sx - px, Ny = sy - py, Nz = sz - pz;
+  const double nN = sqrt((Nx * Nx + Ny * Ny) + Nz * Nz);
+  Nx = Nx / nN; Ny = Ny / nN; Nz = Nz / nN;
+  // deterministic tangent: project the least-aligned coordinate axis
+  const double ax = fabs(Nx), ay = fabs(Ny), az = fabs(Nz);
+  int e = 0;
+  double mn = ax;
+  if (ay < mn) { e = 1; mn = ay; }
+  if (az < mn) { e = 2; }
+  const double Ne = e == 0 ? Nx : (e == 1 ? Ny : Nz);
+  double Tx = (e == 0 ? 1.0 : 0.0) - Nx * Ne;
+  double Ty = (e == 1 ? 1.0 : 0.0) - Ny * Ne;
+  double Tz = (e == 2 ? 1.0 : 0.0) - Nz * Ne;
+  const double nT = sqrt((Tx * Tx + Ty * Ty) + Tz * Tz);
+  Tx = Tx / nT; Ty = Ty / nT; Tz = Tz / nT;
+  const double Bx = Ty * Nz - Tz * Ny, By = Tz * Nx - Tx * Nz, Bz = Tx * Ny - Ty * Nx;
+  const double *F = frame + 9 * f;
+  const double *h = hz + f * nphi;
+  int count = 0;
+  for (int k = 0; k < ndisk - 1; ++k) {
+    const double xk = disk[k], yk = disk[ndisk + k];
+    const double dx = sx + SUN_RADIUS_KM * (xk * Tx + yk * Bx);
+    const double dy = sy + SUN_RADIUS_KM * (xk * Ty + yk * By);
+    const double dz = sz + SUN_RADIUS_KM * (xk * Tz + yk * Bz);
+    double wx = dx - px, wy = dy - py, wz = dz - pz;
+    const double nw = sqrt((wx * wx + wy * wy) + wz * wz);
+    wx = wx / nw; wy = wy / nw; wz = wz / nw;
+    const double d0 = (F[0] * wx + F[1] * wy) + F[2] * wz;
+    const double d1 = (F[3] * wx + F[4] * wy) + F[5] * wz;
+    const double d2 = (F[6] * wx + F[7] * wy) + F[8] * wz;
+    const double theta = acos(d2);
+    double phi = atan2(d1, d0);
+    if (phi < 0) phi = phi + TWO_PI;
+    int pi_ = (int)floor(phi / dphi);
+    pi_ = pi_ < 0 ? 0 : (pi_ > nphi - 1 ? nphi - 1 : pi_);
+    const double t = (phi - pi_ * dphi) / dphi;
+    if (theta < h[pi_] + t * dphi) ++count;
+  }
+  double E0 = (double)count / (nphi - 1);
+  E0 = E0 * fmax(0.0, (Nx * F[6] + Ny * F[7]) + Nz * F[8]);
+  const double ns = sqrt((sx * sx + sy * sy) + sz * sz);
+  const double q = ONE_AU_KM / ns;
+  E0 = E0 * (1372 * (q * q));
+  E[(int64_t)s * nf + f] = E0;
+}
+
+void direct_radiosity(illum_mesh *mesh, const double *hz, int nphi, const double *sun, int S,
+                      const double *disk, int ndisk, double *E, int flags) {
+  illum_ctx *ctx = mesh->ctx;
+  hipStream_t s = stream_of(ctx);
+  const int64_t nf = mesh->g.nf;
+  if (nphi < 2 || S < 1 || ndisk < 1) throw Error(ILLUM_ERR_ARG, "bad direct radiosity sizes");
+  if (!hz || !sun || !disk || !E) throw Error(ILLUM_ERR_ARG, "null pointer");
+  const bool in_dev = flags & ILLUM_IN_ON_DEVICE, out_dev = flags & ILLUM_OUT_ON_DEVICE;
+  DBuf<double> d_h, d_sun, d_disk, d_E;
+  const double *ph = hz, *psun = sun, *pdisk = disk;
+  if (!in_dev) {
+    d_h.reset(nf * nphi); d_sun.reset(3 * S); d_disk.reset(2 * ndisk);
+    ILLUM_HIP(hipMemcpyAsync(d_h.p, hz, 8 * nf * nphi, hipMemcpyHostToDevice, s));
+    ILLUM_HIP(hipMemcpyAsync(d_sun.p, sun, 8 * 3 * S, hipMemcpyHostToDevice, s));
+    ILLUM_HIP(hipMemcpyAsync(d_disk.p, disk, 8 * 2 * ndisk, hipMemcpyHostToDevice, s));
+    ph = d_h.p; psun = d_sun.p; pdisk = d_disk.p;
+  }
+  double *pE = E;
+  if (!out_dev) { d_E.reset(nf * S); pE = d_E.p; }
+  {
+    StageTimer st(ctx, "direct");
+    const int64_t total = nf * S;
+    direct_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(
+        nf, S, nphi, (const float4 *)mesh->d_pr4, mesh->d_frame, ph, psun, pdisk, ndisk, pE);
+    ILLUM_LAUNCH_CHECK();
+  }
+  if (!out_dev) ILLUM_HIP(hipMemcpyAsync(E, pE, 8 * nf * S, hipMemcpyDeviceToHost, s));
+  sync(ctx);
+}
+
+}  // namespace illum

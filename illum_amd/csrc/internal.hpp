@@ -1,0 +1,136 @@
+// Internal types shared by the host-side C++ and the HIP translation units.
+#pragma once
+
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <vector>
+#include <map>
+#include <stdexcept>
+
+#include "../../include/illum_hip.h"
+
+namespace illum {
+
+// Status-carrying exception used inside the library; every C entry point
+// catches it and converts it to (status, illum_last_error()).
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string &msg);
+
+// ---- host-side geometry (built once per mesh, uploaded) ------------------
+struct HostGeometry {
+  int64_t nf = 0;
+  std::vector<float> tri;      // nf x 9 : v0, v1, v2 (FP32, after divisor)
+  std::vector<float> cen;      // nf x 3 : centroid
+  std::vector<float> nrm;      // nf x 3 : unit normal (FP32)
+  std::vector<float> rad;      // nf     : bounding radius
+  std::vector<double> area;    // nf     : |(v1-v0)x(v2-v0)|/2 in FP64
+  std::vector<double> frame;   // nf x 9 : b, t, n columns (FP64)
+};
+
+// Builds HostGeometry from double vertices (rounded to float) and faces.
+HostGeometry make_geometry(const double *V, int64_t nv, const int32_t *F, int64_t nf,
+                           const double *VN, double vertex_divisor);
+
+// Synthetic cratered ellipsoid.
+void synth_sizes(int subdiv, int64_t *nv, int64_t *nf);
+void synth_ellipsoid(int subdiv, double a, double b, double c, int ncraters, double dmin,
+                     double dmax, double wmin, double wmax, uint64_t seed, double *V,
+                     int32_t *F);
+
+// Minimal Wavefront OBJ reader (v / vn / f, polygon faces fan-triangulated).
+void read_obj(const char *path, std::vector<double> &V, std::vector<int32_t> &F,
+              std::vector<double> &VN);
+
+// ---- flattened threaded BVH ----------------------------------------------
+// Node i occupies two float4 slots:
+//   lo = (bmin.x, bmin.y, bmin.z, as_float(miss))  miss = next node when the
+//                                                   box is missed or the leaf
+//                                                   is done (-1 = end)
+//   hi = (bmax.x, bmax.y, bmax.z, as_float(info))  info = -1 for an inner node
+//                                                   (hit link = i + 1), else
+//                                                   (count << 24) | first
+// Triangles are stored in leaf order as three float4 each:
+//   (v0.xyz, as_float(orig_index)), (v1.xyz, 0), (v2.xyz, 0).
+struct HostBVH {
+  std::vector<float> nodes;    // 8 floats per node
+  std::vector<float> tris;     // 12 floats per triangle
+  int64_t nnodes = 0;
+  int depth = 0;
+  int leaf_size = 0;
+};
+HostBVH build_bvh(const HostGeometry &g, int leaf_size);
+
+}  // namespace illum
+
+// ---- opaque handle definitions ------------------------------------------
+struct illum_ctx {
+  int device = 0;
+  void *stream = nullptr;              // hipStream_t
+  bool profiling = false;
+  struct Stage { double ms = 0; int64_t launches = 0; };
+  std::map<std::string, Stage> profile;
+  std::vector<std::pair<std::string, std::pair<void *, void *>>> pending;  // events
+};
+
+struct illum_mesh {
+  illum_ctx *ctx = nullptr;
+  illum::HostGeometry g;
+  // device copies
+  float *d_pr4 = nullptr;      // nf x float4 (centroid, radius)
+  float *d_n4 = nullptr;       // nf x float4 (normal, 0)
+  double *d_area = nullptr;    // nf
+  double *d_frame = nullptr;   // nf x 9
+};
+
+struct illum_bvh {
+  illum_mesh *mesh = nullptr;
+  illum::HostBVH h;
+  float *d_nodes = nullptr;
+  float *d_tris = nullptr;
+};
+
+struct illum_sparse {
+  illum_ctx *ctx = nullptr;
+  int64_t n = 0, nnz = 0;
+  bool has_csc = false;
+  int64_t *d_colptr = nullptr;
+  int32_t *d_rowidx = nullptr;
+  double *d_cval = nullptr;
+  int64_t *d_rowptr = nullptr;
+  int32_t *d_colidx = nullptr;
+  double *d_rval = nullptr;
+  float *d_rval32 = nullptr;
+  int64_t vis_words = 0;
+  uint32_t *d_vis = nullptr;   // raw visibility rows (union of first hits)
+};
+
+struct illum_thermal {
+  illum_ctx *ctx = nullptr;
+  int64_t nf = 0;
+  int64_t nfp = 0;             // padded facet stride of the layer-major arrays
+  int nzmax = 0;
+  int precision = ILLUM_THERM_FP64;
+  int ngrid = 0;
+  double emiss = 0.999;
+  int32_t *d_nz = nullptr;     // per facet layer count
+  int32_t *d_grid = nullptr;   // per facet grid id
+  double *d_s2 = nullptr;      // per facet ti_scale^2
+  void *impl = nullptr;        // host tables + per-dt device tables (thermal.hip)
+  int64_t *d_goff = nullptr;
+  double *d_T = nullptr;       // (nzmax+1) x nfp layer-major, row 0 = surface
+  double *d_Qprev = nullptr;   // nf
+  double *d_F = nullptr;       // nf surface flux
+  double *d_work = nullptr;    // scratch (nzmax x nfp)
+  float *d_work32 = nullptr;   // scratch (2 nzmax x nfp)
+  double *d_Qdev = nullptr;    // staging for host Q
+  int64_t qdev_cap = 0;
+  int *d_status = nullptr;
+  // grid tables on host (for validation / get)
+  std::vector<int64_t> goff;
+  std::vector<int32_t> nz_host;
+};

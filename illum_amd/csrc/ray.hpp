@@ -1,0 +1,168 @@
+// Device ray/triangle predicate and stackless BVH traversal.
+//
+// Visibility is defined by this FP32 predicate (the reference delegates it to
+// Fast-BVH's Tri::getIntersection, illum.cpp:73 and :301, whose source is not
+// available).  It is the watertight test of Woop, Benthin & Wald, "Watertight
+// Ray/Triangle Intersection", JCGT 2(1), 2013, two-sided, with the paper's
+// FP64 recomputation of an edge function that evaluates to exactly zero.
+// The reported distance is max(t, e), where e is the entry distance of the ray
+// into the triangle's own bounding box; because correctly rounded arithmetic
+// is monotone, every ancestor box then has an entry distance <= the reported
+// distance, so pruning boxes whose entry lies beyond the best hit never
+// changes the closest hit: a traversal returns exactly what a brute-force
+// loop over all triangles returns, whatever the tree.
+// Closest hit = smallest reported distance, ties to the smaller facet index.
+//
+// All arithmetic is written out operation by operation; the library is built
+// with -ffp-contract=off so no multiply-add is fused.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace illum {
+
+struct DRay {
+  float ox, oy, oz;   // origin
+  float ix, iy, iz;   // per-axis inverse direction for box slabs (finite)
+  float Sx, Sy, Sz;   // shear constants
+  int kx, ky, kz;     // axis permutation (kz = dominant axis)
+};
+
+__device__ __forceinline__ float sel3(float x, float y, float z, int k) {
+  return k == 0 ? x : (k == 1 ? y : z);
+}
+
+// Inverse of a direction component for the slab test.  A zero component is
+// replaced by +-1e-30 so every slab product stays finite and monotone.
+__device__ __forceinline__ float slab_inv(float d) {
+  float dd = d == 0.0f ? (__float_as_uint(d) >> 31 ? -1e-30f : 1e-30f) : d;
+  return 1.0f / dd;
+}
+
+__device__ __forceinline__ DRay make_ray(float ox, float oy, float oz, float dx, float dy,
+                                         float dz) {
+  DRay r;
+  r.ox = ox; r.oy = oy; r.oz = oz;
+  float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+  int kz = 0;
+  float m = ax;
+  if (ay > m) { kz = 1; m = ay; }
+  if (az > m) { kz = 2; }
+  int kx = kz + 1; if (kx == 3) kx = 0;
+  int ky = kx + 1; if (ky == 3) ky = 0;
+  float dkz = sel3(dx, dy, dz, kz);
+  if (dkz < 0.0f) { int t = kx; kx = ky; ky = t; }
+  r.kx = kx; r.ky = ky; r.kz = kz;
+  r.Sx = sel3(dx, dy, dz, kx) / dkz;
+  r.Sy = sel3(dx, dy, dz, ky) / dkz;
+  r.Sz = 1.0f / dkz;
+  r.ix = slab_inv(dx); r.iy = slab_inv(dy); r.iz = slab_inv(dz);
+  return r;
+}
+
+// Slab test against a node box. Returns the entry distance in *tn and
+// whether the box must be visited for a best distance tbest.
+__device__ __forceinline__ bool box_visit(const DRay &r, float4 lo, float4 hi, float tbest) {
+  float t0x = (lo.x - r.ox) * r.ix, t1x = (hi.x - r.ox) * r.ix;
+  float t0y = (lo.y - r.oy) * r.iy, t1y = (hi.y - r.oy) * r.iy;
+  float t0z = (lo.z - r.oz) * r.iz, t1z = (hi.z - r.oz) * r.iz;
+  float tn = fmaxf(fminf(t0x, t1x), fmaxf(fminf(t0y, t1y), fminf(t0z, t1z)));
+  float tf = fminf(fmaxf(t0x, t1x), fminf(fmaxf(t0y, t1y), fmaxf(t0z, t1z)));
+  return tn <= tf && tf > 0.0f && tn <= tbest;
+}
+
+// Watertight two-sided test.  On a hit (t > 0) writes the reported distance.
+__device__ __forceinline__ bool tri_hit(const DRay &r, float4 a, float4 b, float4 c,
+                                        float *tout) {
+  const float Ax0 = a.x - r.ox, Ay0 = a.y - r.oy, Az0 = a.z - r.oz;
+  const float Bx0 = b.x - r.ox, By0 = b.y - r.oy, Bz0 = b.z - r.oz;
+  const float Cx0 = c.x - r.ox, Cy0 = c.y - r.oy, Cz0 = c.z - r.oz;
+  const float Akx = sel3(Ax0, Ay0, Az0, r.kx), Aky = sel3(Ax0, Ay0, Az0, r.ky),
+              Akz = sel3(Ax0, Ay0, Az0, r.kz);
+  const float Bkx = sel3(Bx0, By0, Bz0, r.kx), Bky = sel3(Bx0, By0, Bz0, r.ky),
+              Bkz = sel3(Bx0, By0, Bz0, r.kz);
+  const float Ckx = sel3(Cx0, Cy0, Cz0, r.kx), Cky = sel3(Cx0, Cy0, Cz0, r.ky),
+              Ckz = sel3(Cx0, Cy0, Cz0, r.kz);
+  const float Ax = Akx - r.Sx * Akz, Ay = Aky - r.Sy * Akz;
+  const float Bx = Bkx - r.Sx * Bkz, By = Bky - r.Sy * Bkz;
+  const float Cx = Ckx - r.Sx * Ckz, Cy = Cky - r.Sy * Ckz;
+  float U = Cx * By - Cy * Bx;
+  float V = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
+    double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
+    double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
+    U = (float)(CxBy - CyBx);
+    V = (float)(AxCy - AyCx);
+    W = (float)(BxAy - ByAx);
+  }
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+  const float det = (U + V) + W;
+  if (det == 0.0f) return false;
+  const float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
+  const float T = (U * Az + V * Bz) + W * Cz;
+  const float t = T / det;
+  if (!(t > 0.0f)) return false;
+  // Entry distance into the triangle's own bounding box.
+  const float ex = fminf(Ax0 * r.ix, fminf(Bx0 * r.ix, Cx0 * r.ix));
+  const float ey = fminf(Ay0 * r.iy, fminf(By0 * r.iy, Cy0 * r.iy));
+  const float ez = fminf(Az0 * r.iz, fminf(Bz0 * r.iz, Cz0 * r.iz));
+  const float e = fmaxf(ex, fmaxf(ey, ez));
+  *tout = fmaxf(t, e);
+  return true;
+}
+
+// Closest hit over the threaded BVH; returns the original facet index or -1.
+__device__ __forceinline__ int trace_closest(const float4 *__restrict__ nodes,
+                                             const float4 *__restrict__ tris, const DRay &r,
+                                             float *tbest_out) {
+  float tbest = __builtin_huge_valf();
+  int best = -1;
+  int node = 0;
+  while (node >= 0) {
+    const float4 lo = nodes[2 * node], hi = nodes[2 * node + 1];
+    const int miss = __float_as_int(lo.w), info = __float_as_int(hi.w);
+    if (!box_visit(r, lo, hi, tbest)) { node = miss; continue; }
+    if (info < 0) { node = node + 1; continue; }
+    const int first = info & 0xFFFFFF, cnt = info >> 24;
+    for (int k = 0; k < cnt; ++k) {
+      const float4 a = tris[3 * (first + k)], b = tris[3 * (first + k) + 1],
+                   c = tris[3 * (first + k) + 2];
+      const int idx = __float_as_int(a.w);
+      float t;
+      if (tri_hit(r, a, b, c, &t) && (t < tbest || (t == tbest && idx < best))) {
+        tbest = t;
+        best = idx;
+      }
+    }
+    node = miss;
+  }
+  *tbest_out = tbest;
+  return best;
+}
+
+// Any hit (t > 0) over the threaded BVH.
+__device__ __forceinline__ bool trace_any(const float4 *__restrict__ nodes,
+                                          const float4 *__restrict__ tris, const DRay &r) {
+  const float inf = __builtin_huge_valf();
+  int node = 0;
+  while (node >= 0) {
+    const float4 lo = nodes[2 * node], hi = nodes[2 * node + 1];
+    const int miss = __float_as_int(lo.w), info = __float_as_int(hi.w);
+    if (!box_visit(r, lo, hi, inf)) { node = miss; continue; }
+    if (info < 0) { node = node + 1; continue; }
+    const int first = info & 0xFFFFFF, cnt = info >> 24;
+    for (int k = 0; k < cnt; ++k) {
+      float t;
+      if (tri_hit(r, tris[3 * (first + k)], tris[3 * (first + k) + 1],
+                  tris[3 * (first + k) + 2], &t))
+        return true;
+    }
+    node = miss;
+  }
+  return false;
+}
+
+}  // namespace illum

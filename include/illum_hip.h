@@ -1,0 +1,208 @@
+/*
+ * illum_hip.h -- C ABI of the MI355X (gfx950) form-factor / horizon / radiosity /
+ * Crank-Nicolson library.
+ *
+ * Every entry point returns an int status (ILLUM_OK == 0).  On failure the
+ * message is available from illum_last_error() (thread-local).  No entry point
+ * exits the process or throws.  Buffers are caller-owned unless a handle says
+ * otherwise; a pointer argument documented as "host or device" is read/written
+ * on the device when the corresponding ILLUM_*_ON_DEVICE flag is set.
+ *
+ * Reference interfaces replaced (paths relative to the illum source tree):
+ *   illum_mesh_*          obj_util::get_objects           src/cpp/obj_util.cpp:8-96
+ *                         illum_context ctor               src/cpp/illum.cpp:22-27
+ *   illum_bvh_build       Fast-BVH BVH{&objects} ctor      src/cpp/illum.cpp:25
+ *   illum_form_factors    illum_context::compute_F         src/cpp/illum.cpp:31-237, illum.hpp:18-20
+ *   illum_horizons        illum_context::make_horizons     src/cpp/illum.cpp:326-363, illum.hpp:22-23
+ *   illum_direct_radiosity illum_context::get_direct_radiosity src/cpp/illum.cpp:382-466, illum.hpp:29-30
+ *   illum_fib_spiral      fib_spiral                       src/cpp/illum.cpp:468-483, illum.hpp:50
+ *   illum_radiosity_solve Gauss-Seidel loop of do_radiosity_task
+ *                                                          src/cpp/illum_cli.cpp:425-440,
+ *                                                          src/cpp/arma_util.cpp:3-19
+ *   illum_thermal_*       thermal_model                    src/cpp/thermal.hpp:7-105
+ *   illum_conductionQ     conductionQ (C++ wrapper)        src/cpp/conduction.hpp:16-28,
+ *                                                          src/cpp/conduction.cpp:17-30
+ */
+#ifndef ILLUM_HIP_H
+#define ILLUM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define ILLUM_OK                 0
+#define ILLUM_ERR_ARG            1  /* bad argument (null, size, range)            */
+#define ILLUM_ERR_NAN            2  /* a NaN form factor (illum.cpp:121-123)        */
+#define ILLUM_ERR_ZERO_PIVOT     3  /* tridiagonal b(1) == 0 (tridag.for:10-12)     */
+#define ILLUM_ERR_OOM            4  /* device or host allocation failed             */
+#define ILLUM_ERR_HIP            5  /* HIP runtime error                            */
+#define ILLUM_ERR_IO             6  /* file could not be read/parsed                */
+#define ILLUM_ERR_CAPACITY       7  /* problem exceeds a documented size limit      */
+#define ILLUM_ERR_NOT_CONVERGED  8  /* iterative solve hit max_iter above tolerance */
+#define ILLUM_ERR_SHAPE          9  /* array of incompatible size (thermal.hpp:53-57) */
+
+/* ---- flags ------------------------------------------------------------- */
+#define ILLUM_IN_ON_DEVICE   1   /* input arrays are device pointers  */
+#define ILLUM_OUT_ON_DEVICE  2   /* output arrays are device pointers */
+
+/* radiosity solve methods */
+#define ILLUM_RAD_JACOBI_FP64   1   /* FP64 Jacobi sweeps B <- E + F B                  */
+#define ILLUM_RAD_MIXED         2   /* FP32 Jacobi sweeps + FP64 residual refinement    */
+
+/* thermal precision modes */
+#define ILLUM_THERM_FP64        1   /* FP64 Crank-Nicolson, FP64 Thomas                  */
+#define ILLUM_THERM_MIXED       2   /* FP32 Thomas sweeps + FP64 refinement / surface    */
+
+typedef struct illum_ctx     illum_ctx;
+typedef struct illum_mesh    illum_mesh;
+typedef struct illum_bvh     illum_bvh;
+typedef struct illum_sparse  illum_sparse;
+typedef struct illum_thermal illum_thermal;
+
+const char *illum_last_error(void);
+int illum_version(void);
+
+/* ---- context: one HIP device + one stream; calls on a context are serialised */
+int illum_ctx_create(int device, illum_ctx **out);
+int illum_ctx_destroy(illum_ctx *ctx);
+int illum_ctx_synchronize(illum_ctx *ctx);
+/* Per-stage kernel timing with hipEvents recorded on the context's stream.
+ * When enabled, every launch of a named stage accumulates its duration. */
+int illum_ctx_set_profiling(illum_ctx *ctx, int enable);
+int illum_ctx_reset_profile(illum_ctx *ctx);
+/* Fills up to cap entries; names are written as NUL-separated list into
+ * names (names_cap bytes).  *n receives the number of stages. */
+int illum_ctx_get_profile(illum_ctx *ctx, int cap, double *total_ms, int64_t *launches,
+                          char *names, int names_cap, int *n);
+
+/* ---- synthetic shape models (host, deterministic, seeded) --------------
+ * Icosahedron subdivided `subdiv` times, projected to the unit sphere,
+ * scaled to a triaxial ellipsoid (a, b, c), radially displaced by
+ * `ncraters` Gaussian depressions with depth U(dmin,dmax), width U(wmin,wmax).
+ * V: nv x 3 row-major doubles, F: nf x 3 row-major int32 (outward winding). */
+int illum_synth_sizes(int subdiv, int64_t *nv, int64_t *nf);
+int illum_synth_ellipsoid(int subdiv, double a, double b, double c, int ncraters,
+                          double dmin, double dmax, double wmin, double wmax,
+                          uint64_t seed, double *V, int32_t *F);
+
+/* ---- meshes -------------------------------------------------------------
+ * Vertices are rounded to FP32 and divided (in FP32) by vertex_divisor (100.0
+ * reproduces the reference OBJ loader, obj_util.cpp:90-92; 1.0 = none).  When
+ * VN (nv x 3 vertex normals) is given, each face normal is flipped to agree
+ * with the sum of its vertex normals (obj_util.cpp:64-88). */
+int illum_mesh_create(illum_ctx *ctx, const double *V, int64_t nv, const int32_t *F,
+                      int64_t nf, const double *VN, double vertex_divisor,
+                      illum_mesh **out);
+int illum_mesh_from_obj(illum_ctx *ctx, const char *path, double vertex_divisor,
+                        illum_mesh **out);
+int illum_mesh_destroy(illum_mesh *mesh);
+int illum_mesh_num_faces(const illum_mesh *mesh, int64_t *nf);
+/* Derived per-facet geometry (host copies; any pointer may be null):
+ * centroid (nf x 3 float), normal (nf x 3 float), bounding radius (nf float),
+ * area (nf double), frame (nf x 9 double: columns b, t, n as in
+ * get_frenet_frame, illum.cpp:244-262). */
+int illum_mesh_geometry(const illum_mesh *mesh, float *centroid, float *normal,
+                        float *radius, double *area, double *frame);
+/* FP32 vertices of every facet after scaling, nf x 9 floats (v0, v1, v2). */
+int illum_mesh_triangles(const illum_mesh *mesh, float *tri9);
+
+/* ---- BVH (host SAH build, flattened threaded layout uploaded to HBM) ---- */
+int illum_bvh_build(illum_mesh *mesh, int leaf_size, illum_bvh **out);
+int illum_bvh_destroy(illum_bvh *bvh);
+int illum_bvh_info(const illum_bvh *bvh, int64_t *nodes, int *depth, int *leaf_size);
+/* Closest-hit query for arbitrary rays (testing/diagnostics): origins and
+ * directions are nray x 3 floats; hit receives the original facet index or
+ * -1, t the parametric distance (in units of |d|). */
+int illum_bvh_closest_hit(illum_bvh *bvh, const float *orig, const float *dir,
+                          int64_t nray, int32_t *hit, float *t);
+
+/* ---- form factors (A2 + A3) ---------------------------------------------
+ * Builds the N x N sparse matrix F of compute_F: column j = source facet,
+ * F(i,j) = mu_ij mu_ji A_j / (pi r_ij^2) over the union of closest hits of
+ * rays from q_j = p_j + offset n_j towards every un-culled centroid,
+ * exact zeros dropped, rows scaled by albedo (albedo_vec[i] if non-null,
+ * else the scalar albedo).  Stored both as CSC (reference layout) and CSR. */
+int illum_form_factors(illum_bvh *bvh, double albedo, const double *albedo_vec,
+                       double offset, illum_sparse **out);
+int illum_sparse_create_csr(illum_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rowptr,
+                            const int32_t *colidx, const double *val, illum_sparse **out);
+int illum_sparse_destroy(illum_sparse *sp);
+int illum_sparse_info(const illum_sparse *sp, int64_t *n, int64_t *nnz);
+int illum_sparse_copy_csc(const illum_sparse *sp, int64_t *colptr, int32_t *rowidx,
+                          double *val);
+int illum_sparse_copy_csr(const illum_sparse *sp, int64_t *rowptr, int32_t *colidx,
+                          double *val);
+/* Visibility set of the last illum_form_factors call before the zero drop:
+ * the union of closest-hit facets per source, as nf rows of ceil(nf/32)
+ * uint32 words (bit i of row j set <=> i in hits_j).  Testing aid. */
+int illum_sparse_visibility_words(const illum_sparse *sp, int64_t *words_per_row);
+int illum_sparse_copy_visibility(const illum_sparse *sp, uint32_t *bits);
+
+/* ---- horizons (A4) -------------------------------------------------------
+ * out: nphi x nf column-major doubles (illum.hpp:47), host or device.
+ * occluder may be null (use bvh); otherwise rays are cast against the
+ * occluder mesh's BVH (illum.cpp:333-349). */
+int illum_horizons(illum_bvh *bvh, illum_bvh *occluder, int nphi, double theta_eps,
+                   double offset, double *out, int flags);
+
+/* ---- direct radiosity (A5) -----------------------------------------------
+ * horizons: nphi x nf column-major; sun_km: 3 x S column-major; disk_xy:
+ * ndisk x 2 column-major (fib_spiral layout); E: nf x S column-major.
+ * The reference picks a random disk tangent per facet (illum.cpp:406); this
+ * library uses the deterministic tangent normalise(N x e_k), e_k the
+ * coordinate axis least aligned with N. */
+int illum_fib_spiral(double *xy, int n);
+int illum_direct_radiosity(illum_mesh *mesh, const double *horizons, int nphi,
+                           const double *sun_km, int S, const double *disk_xy,
+                           int ndisk, double *E, int flags);
+
+/* ---- radiosity solve (A6) ------------------------------------------------
+ * Solves (I - F) B = E for S right-hand sides; E, B are nf x S column-major
+ * (host or device per flags).  Stops when max_s ||E - (I-F)B||_2/||E||_2
+ * <= rel_tol.  resid (nullable, max_iter+1 doubles) receives the relative
+ * residual after each FP64 check; *iters the number of F-products. */
+int illum_radiosity_solve(illum_sparse *F, const double *E, int S, int method,
+                          double rel_tol, int max_iter, double *B, double *resid,
+                          int *iters, int flags);
+
+/* ---- thermal model (A7 + A8 + A9) --------------------------------------
+ * ngrid layer grids: grid g has depths z[grid_off[g] .. grid_off[g+1]) (at
+ * least 2 layers, strictly increasing, z > 0) with per-layer thermal inertia
+ * ti and volumetric heat capacity rhoc at the same offsets.  Facet f uses
+ * grid grid_of_facet[f] (null => all use grid 0) and scales its thermal
+ * inertia by ti_scale[f] (null => 1).  T0: scalar initial temperature if
+ * T0_full is null, else T0_full is (nzmax+1) x nf column-major with row 0 the
+ * surface (thermal.hpp:47-58).  emissivity: thermal.hpp:77 uses 0.999. */
+int illum_thermal_create(illum_ctx *ctx, int64_t nf, int ngrid, const int64_t *grid_off,
+                         const double *z, const double *ti, const double *rhoc,
+                         const int32_t *grid_of_facet, const double *ti_scale,
+                         double emissivity, double T0, const double *T0_full,
+                         int precision, illum_thermal **out);
+int illum_thermal_destroy(illum_thermal *th);
+int illum_thermal_info(const illum_thermal *th, int64_t *nf, int *nzmax);
+/* One step of thermal_model::step(dt, Q) (thermal.hpp:65-105): Q is nf
+ * doubles (host or device), Qprev is the previous step's Q (zero initially). */
+int illum_thermal_step(illum_thermal *th, double dt, const double *Q, int flags);
+/* nsteps consecutive steps, Q series nsteps x nf (row k = Q at step k+1),
+ * run as one persistent launch sequence. */
+int illum_thermal_run(illum_thermal *th, double dt, const double *Qseries, int nsteps,
+                      int flags);
+/* T: (nzmax+1) x nf column-major (ragged facets padded with NaN), Tsurf and
+ * Fsurf: nf; host or device per flags; any may be null. */
+int illum_thermal_get(illum_thermal *th, double *T, double *Tsurf, double *Fsurf,
+                      int flags);
+
+/* Single-facet drop-in for conductionQ (conduction.hpp:16-28), run on the
+ * GPU.  T (nz) and Tsurf, Fsurf are updated in place. */
+int illum_conductionQ(int nz, const double *z, double dt, double Qn, double Qnp1,
+                      double *T, const double *ti, const double *rhoc, double emiss,
+                      double *Tsurf, double Fgeotherm, double *Fsurf);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ILLUM_HIP_H */

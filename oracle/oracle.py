@@ -1,0 +1,229 @@
+"""TEST INFRASTRUCTURE ONLY: Python binding of the CPU oracle (liboracle.so)
+and of the reference's own Fortran conductionQ compiled into _ref/.
+
+Imported only by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+leg, always as the checker.  See oracle.cpp for the reference file:line each
+function restates.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libref_conduction.so")
+
+_P = C.c_void_p
+_I = C.c_int
+_I64 = C.c_int64
+_D = C.c_double
+
+_SIG = {
+    "oracle_geometry": ([_P, _I64, _P, _I64, _P, _D, _P, _P, _P, _P, _P, _P], _I),
+    "oracle_closest_hit": ([_P, _I64, _P, _P, _I64, _I, _I, _P, _P], _I),
+    "oracle_visibility": ([_P, _P, _P, _P, _I64, _D, _I64, _I64, _I, _I, _P], _I64),
+    "oracle_form_factors": ([_P, _P, _P, _I64, _P, _I64, _I64, _D, _P, _I64, _P, _P, _P], _I64),
+    "oracle_horizons": ([_P, _P, _P, _I64, _I64, _P, _I64, _I, _D, _D, _I, _I, _P], _I64),
+    "oracle_fib_spiral": ([_P, _I], _I),
+    "oracle_direct": ([_P, _P, _I64, _P, _I, _P, _I, _P, _I, _I, _P], _I),
+    "oracle_gauss_seidel": ([_I64, _P, _P, _P, _P, _I, _D, _P, _P], _I),
+    "oracle_conductionQ": ([_I, _P, _D, _D, _D, _P, _P, _P, _D, _P, _D, _P], _I),
+    "oracle_thermal_run": ([_I64, _P, _P, _P, _P, _P, _P, _D, _I, _D, _I, _P, _P, _P, _P, _P, _I],
+                           _I),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise ImportError(f"{LIB} not built (make -C oracle)")
+        L = C.CDLL(LIB)
+        for k, (a, r) in _SIG.items():
+            getattr(L, k).argtypes = a
+            getattr(L, k).restype = r
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def nthreads_default():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def geometry(V, F, VN=None, divisor=1.0):
+    V = np.ascontiguousarray(V, np.float64)
+    F = np.ascontiguousarray(F, np.int32)
+    nf = F.shape[0]
+    out = dict(tri=np.zeros((nf, 9), np.float32), centroid=np.zeros((nf, 3), np.float32),
+               normal=np.zeros((nf, 3), np.float32), radius=np.zeros(nf, np.float32),
+               area=np.zeros(nf), frame=np.zeros((nf, 9)))
+    VN = None if VN is None else np.ascontiguousarray(VN, np.float64)
+    st = lib().oracle_geometry(_p(V), V.shape[0], _p(F), nf, _p(VN), divisor, _p(out["tri"]),
+                               _p(out["centroid"]), _p(out["normal"]), _p(out["radius"]),
+                               _p(out["area"]), _p(out["frame"]))
+    if st:
+        raise ValueError("oracle_geometry failed")
+    return out
+
+
+def closest_hit(tri, orig, dirs, use_bvh=True, nthreads=None):
+    tri = np.ascontiguousarray(tri, np.float32)
+    orig = np.ascontiguousarray(orig, np.float32)
+    dirs = np.ascontiguousarray(dirs, np.float32)
+    n = orig.shape[0]
+    hit = np.zeros(n, np.int32)
+    t = np.zeros(n, np.float32)
+    lib().oracle_closest_hit(_p(tri), tri.shape[0], _p(orig), _p(dirs), n, int(use_bvh),
+                             nthreads or nthreads_default(), _p(hit), _p(t))
+    return hit, t
+
+
+def visibility(g, offset=1e-5, j0=0, j1=None, use_bvh=True, nthreads=None):
+    """Raw visibility rows (union of closest hits) for sources [j0, j1)."""
+    nf = g["tri"].shape[0]
+    j1 = nf if j1 is None else j1
+    words = (nf + 31) // 32
+    bits = np.zeros((j1 - j0, words), np.uint32)
+    nr = lib().oracle_visibility(_p(g["tri"]), _p(g["centroid"]), _p(g["normal"]),
+                                 _p(g["radius"]), nf, offset, j0, j1, int(use_bvh),
+                                 nthreads or nthreads_default(), _p(bits))
+    return bits, nr
+
+
+def form_factors(g, bits, j0=0, j1=None, albedo=1.0, albedo_vec=None):
+    """CSC (colptr, rowidx, val) of columns [j0, j1) from visibility rows."""
+    nf = g["tri"].shape[0]
+    j1 = nf if j1 is None else j1
+    cap = int(bits.astype(np.uint64).sum() and np.unpackbits(bits.view(np.uint8)).sum()) + 1
+    colptr = np.zeros(j1 - j0 + 1, np.int64)
+    rowidx = np.zeros(cap, np.int32)
+    val = np.zeros(cap)
+    av = None if albedo_vec is None else np.ascontiguousarray(albedo_vec, np.float64)
+    nnz = lib().oracle_form_factors(_p(g["centroid"]), _p(g["normal"]), _p(g["area"]), nf,
+                                    _p(np.ascontiguousarray(bits)), j0, j1, albedo, _p(av), cap,
+                                    _p(colptr), _p(rowidx), _p(val))
+    if nnz == -2:
+        raise FloatingPointError("computed a NaN form factor")
+    if nnz < 0:
+        raise RuntimeError("oracle_form_factors capacity")
+    return colptr, rowidx[:nnz].copy(), val[:nnz].copy()
+
+
+def horizons(g, nphi=361, theta_eps=1e-3, offset=1e-5, j0=0, j1=None, occ_tri=None,
+             use_bvh=True, nthreads=None):
+    """nphi x (j1 - j0) horizon angles."""
+    nf = g["tri"].shape[0]
+    j1 = nf if j1 is None else j1
+    occ = g["tri"] if occ_tri is None else np.ascontiguousarray(occ_tri, np.float32)
+    out = np.zeros((j1 - j0, nphi))
+    nr = lib().oracle_horizons(_p(g["centroid"]), _p(g["normal"]), _p(g["frame"]), j0, j1,
+                               _p(occ), occ.shape[0], nphi, theta_eps, offset, int(use_bvh),
+                               nthreads or nthreads_default(), _p(out))
+    return out.T, nr
+
+
+def fib_spiral(n):
+    xy = np.zeros((2, n))
+    lib().oracle_fib_spiral(_p(xy), n)
+    return xy.T.copy()
+
+
+def direct(g, hz, sun, disk_xy, nthreads=None):
+    """E (N,) or (N, S) for sun (3,) or (3, S) km; hz nphi x N."""
+    sun = np.asarray(sun, np.float64)
+    single = sun.ndim == 1
+    sunr = np.ascontiguousarray(sun.reshape(3, -1).T)
+    S = sunr.shape[0]
+    nf = g["tri"].shape[0]
+    hzc = np.ascontiguousarray(np.asarray(hz).T)  # N x nphi row-major == nphi x N col-major
+    disk = np.ascontiguousarray(np.asarray(disk_xy, np.float64).T)  # 2 x n == n x 2 col-major
+    E = np.zeros((S, nf))
+    lib().oracle_direct(_p(g["centroid"]), _p(g["frame"]), nf, _p(hzc), hzc.shape[1], _p(sunr),
+                        S, _p(disk), disk.shape[1], nthreads or nthreads_default(), _p(E))
+    return E[0] if single else E.T
+
+
+def gauss_seidel(n, colptr, rowidx, val, E, max_sweeps=50, rel_tol=1e-10):
+    """Reference GS; returns (B, sweeps, absolute residual per sweep)."""
+    colptr = np.ascontiguousarray(colptr, np.int64)
+    rowidx = np.ascontiguousarray(rowidx, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    E = np.ascontiguousarray(E, np.float64)
+    B = np.zeros(n)
+    res = np.zeros(max(1, max_sweeps))
+    k = lib().oracle_gauss_seidel(n, _p(colptr), _p(rowidx), _p(val), _p(E), max_sweeps, rel_tol,
+                                  _p(B), _p(res))
+    return B, k, res[:k]
+
+
+def conductionQ(nz, z, dt, Qn, Qnp1, T, ti, rhoc, emiss, Tsurf, Fgeotherm):
+    T = np.array(T, np.float64)
+    ts = C.c_double(Tsurf)
+    fs = C.c_double(0.0)
+    z = np.ascontiguousarray(z, np.float64)
+    ti = np.ascontiguousarray(np.broadcast_to(ti, (nz,)), np.float64)
+    rhoc = np.ascontiguousarray(np.broadcast_to(rhoc, (nz,)), np.float64)
+    st = lib().oracle_conductionQ(nz, _p(z), dt, Qn, Qnp1, _p(T), _p(ti), _p(rhoc), emiss,
+                                  C.byref(ts), Fgeotherm, C.byref(fs))
+    if st:
+        raise ZeroDivisionError("tridag: rewrite equations")
+    return T, ts.value, fs.value
+
+
+def thermal_run(goff, z, ti, rhoc, T, Qprev, Q, dt, emiss, grid=None, ti_scale=None,
+                record=False, nthreads=None):
+    """Runs Q.shape[0] steps; T is (nzmax+1) x N and is updated (a copy is returned)."""
+    goff = np.ascontiguousarray(goff, np.int64)
+    z = np.ascontiguousarray(z, np.float64)
+    ti = np.ascontiguousarray(ti, np.float64)
+    rhoc = np.ascontiguousarray(rhoc, np.float64)
+    T = np.asfortranarray(np.array(T, np.float64))
+    Qprev = np.array(Qprev, np.float64)
+    Q = np.ascontiguousarray(Q, np.float64)
+    nsteps, nf = Q.shape
+    nzmax = T.shape[0] - 1
+    Fs = np.zeros(nf)
+    series = np.zeros((nsteps, nf)) if record else None
+    g = None if grid is None else np.ascontiguousarray(grid, np.int32)
+    s = None if ti_scale is None else np.ascontiguousarray(ti_scale, np.float64)
+    st = lib().oracle_thermal_run(nf, _p(goff), _p(z), _p(ti), _p(rhoc), _p(g), _p(s), emiss,
+                                  nzmax, dt, nsteps, _p(Q), _p(Qprev), T.ctypes.data_as(C.c_void_p),
+                                  _p(Fs), _p(series), nthreads or nthreads_default())
+    if st:
+        raise ZeroDivisionError("tridag: rewrite equations")
+    return T, Qprev, Fs, series
+
+
+# --- the reference's own Fortran (compiled from its sources into _ref/) ------
+_ref = None
+
+
+def ref_available():
+    return os.path.exists(REF_LIB)
+
+
+def ref_conductionQ(nz, z, dt, Qn, Qnp1, T, ti, rhoc, emiss, Tsurf, Fgeotherm):
+    """conductionq_ from conductionQ.f + tridag.for, called through the
+    gfortran-convention C ABI that conduction.cpp:17-21 declares."""
+    global _ref
+    if _ref is None:
+        _ref = C.CDLL(REF_LIB)
+        _ref.conductionq_.argtypes = [_P] * 12
+        _ref.conductionq_.restype = None
+    T = np.array(T, np.float64)
+    z = np.ascontiguousarray(z, np.float64)
+    ti = np.ascontiguousarray(np.broadcast_to(ti, (nz,)), np.float64)
+    rhoc = np.ascontiguousarray(np.broadcast_to(rhoc, (nz,)), np.float64)
+    a = [C.c_int(nz), None, C.c_double(dt), C.c_double(Qn), C.c_double(Qnp1), None, None, None,
+         C.c_double(emiss), C.c_double(Tsurf), C.c_double(Fgeotherm), C.c_double(0.0)]
+    _ref.conductionq_(C.byref(a[0]), _p(z), C.byref(a[2]), C.byref(a[3]), C.byref(a[4]), _p(T),
+                      _p(ti), _p(rhoc), C.byref(a[8]), C.byref(a[9]), C.byref(a[10]),
+                      C.byref(a[11]))
+    return T, a[9].value, a[11].value
