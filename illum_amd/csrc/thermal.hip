@@ -63,9 +63,9 @@ __global__ __launch_bounds__(256) void cn_kernel(
     const double Qnp1 = Q[(int64_t)step * nf + f];
     // Radiative boundary linearised at the old surface temperature.
     const double Tr = Tsurf;
-    const double Tr2 = Tr * Tr;
-    const double arad = ((-3.0 * emiss) * SIGMA_SB) * (Tr2 * Tr2);
-    const double brad = ((2.0 * emiss) * SIGMA_SB) * (Tr2 * Tr);
+    const double Tr3 = (Tr * Tr) * Tr, Tr4 = Tr3 * Tr;  // Tr**3, Tr**4 left to right
+    const double arad = ((-3.0 * emiss) * SIGMA_SB) * Tr4;
+    const double brad = ((2.0 * emiss) * SIGMA_SB) * Tr3;
     const double den = k1 + brad;
     const double ann = (Qn - arad) / den;
     const double annp1 = (Qnp1 - arad) / den;

@@ -672,9 +672,11 @@ int oracle_conductionQ(int nz, const double *z, double dt, double Qn, double Qnp
   }
   b[nz - 1] = 1. + gamma[nz - 1];
   const double Tr = *Tsurf;
-  const double Tr2 = Tr * Tr;
-  const double arad = -(((3. * emiss) * sigSB) * (Tr2 * Tr2));
-  const double brad = ((2. * emiss) * sigSB) * (Tr2 * Tr);
+  // Tr**3 and Tr**4 as repeated multiplication, left to right (the order the
+  // compiled reference Fortran evaluates them in).
+  const double Tr3 = (Tr * Tr) * Tr, Tr4 = Tr3 * Tr;
+  const double arad = -(((3. * emiss) * sigSB) * Tr4);
+  const double brad = ((2. * emiss) * sigSB) * Tr3;
   const double ann = (Qn - arad) / (k1 + brad);
   const double annp1 = (Qnp1 - arad) / (k1 + brad);
   const double bn = (k1 - brad) / (k1 + brad);

@@ -1,0 +1,223 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bars: visibility (the union-of-closest-hits sets of compute_F, illum.cpp:42-95),
+horizons and ray hits bit-exact; form factors within 1e-12 relative of the
+oracle's FP64 values (the FP32-stored copy within 1e-7); radiosity at the
+stated relative residual; Crank-Nicolson FP64 within 1e-9 K of the oracle
+and the mixed mode within 1e-3 K.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def api():
+    from illum_amd import api as A
+    A.default_context()
+    return A
+
+
+def _mesh(api, subdiv, ncraters, seed=0):
+    V, F = api.synth_ellipsoid(subdiv, ncraters=ncraters, seed=seed)
+    return V, F
+
+
+def _bits_to_sets(bits, nf):
+    return np.unpackbits(bits.view(np.uint8), bitorder="little", axis=1)[:, :nf]
+
+
+def test_geometry_bitwise(api):
+    V, F = _mesh(api, 3, 20)
+    m = api.Mesh(api.default_context(), V=V, F=F)
+    g = m.geometry()
+    o = O.geometry(V, F)
+    for k in ("centroid", "normal", "radius", "area", "frame"):
+        assert np.array_equal(g[k], o[k]), k
+    assert np.array_equal(m.triangles(), o["tri"])
+
+
+def test_closest_hit_random_rays_bitwise(api):
+    V, F = _mesh(api, 3, 30, seed=3)
+    ctx = api.IllumContext(V=V, F=F, leaf_size=4)
+    o = O.geometry(V, F)
+    rng = np.random.default_rng(7)
+    n = 50000
+    orig = rng.uniform(-14, 14, (n, 3)).astype(np.float32)
+    dirs = rng.normal(size=(n, 3)).astype(np.float32)
+    # also rays aimed exactly at vertices and edge midpoints (watertightness)
+    tri = o["tri"].reshape(-1, 3, 3)
+    targets = np.concatenate([tri[:2000, 0], 0.5 * (tri[:2000, 0] + tri[:2000, 1])]).astype(np.float32)
+    o2 = np.zeros_like(targets)
+    orig = np.concatenate([orig, o2])
+    dirs = np.concatenate([dirs, targets])
+    hg, tg = ctx.bvh.closest_hit(orig, dirs)
+    hb, tb = O.closest_hit(o["tri"], orig, dirs, use_bvh=False)
+    assert np.array_equal(hg, hb)
+    hit = hb >= 0
+    assert np.array_equal(tg[hit], tb[hit])
+    # rays from the centre towards vertices/edges of a closed mesh never miss
+    assert np.all(hb[n:] >= 0)
+
+
+@pytest.mark.parametrize("subdiv,ncraters,albedo", [(3, 40, 0.1), (4, 40, 0.1)])
+def test_form_factors_vs_oracle(api, subdiv, ncraters, albedo):
+    V, F = _mesh(api, subdiv, ncraters, seed=subdiv)
+    ctx = api.IllumContext(V=V, F=F, leaf_size=4)
+    Fm = ctx.compute_F(albedo)
+    o = O.geometry(V, F)
+    bits, nrays = O.visibility(o, offset=1e-5)
+    gvis = Fm.visibility()[:, : bits.shape[1]]
+    assert np.array_equal(gvis, bits), "visibility sets differ"
+    cp, ri, v = O.form_factors(o, bits, albedo=albedo)
+    gcp, gri, gv = Fm.csc()
+    assert np.array_equal(gcp, cp)
+    assert np.array_equal(gri, ri)
+    np.testing.assert_allclose(gv, v, rtol=1e-12, atol=0)
+    # CSR is the transpose of CSC, rows sorted
+    import scipy.sparse as sps
+    A = sps.csc_matrix((v, ri, cp), shape=(len(F), len(F))).tocsr()
+    A.sort_indices()
+    rp, ci, rv = Fm.csr()
+    assert np.array_equal(rp, A.indptr)
+    assert np.array_equal(ci, A.indices)
+    np.testing.assert_allclose(rv, A.data, rtol=1e-12, atol=0)
+    assert Fm.nnz > 0
+
+
+def test_form_factors_albedo_vector(api):
+    V, F = _mesh(api, 3, 40, seed=5)
+    ctx = api.IllumContext(V=V, F=F)
+    rho = np.random.default_rng(1).uniform(0.05, 0.15, len(F))
+    Fm = ctx.compute_F(rho)
+    o = O.geometry(V, F)
+    bits, _ = O.visibility(o)
+    cp, ri, v = O.form_factors(o, bits, albedo_vec=rho)
+    gcp, gri, gv = Fm.csc()
+    assert np.array_equal(gri, ri)
+    np.testing.assert_allclose(gv, v, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("nphi,theta_eps", [(65, 1e-3), (17, 1e-2)])
+def test_horizons_bitwise(api, nphi, theta_eps):
+    V, F = _mesh(api, 3, 40, seed=2)
+    ctx = api.IllumContext(V=V, F=F)
+    hz = ctx.make_horizons(nphi=nphi, theta_eps=theta_eps, offset=1e-5)
+    o = O.geometry(V, F)
+    ho, _ = O.horizons(o, nphi=nphi, theta_eps=theta_eps, offset=1e-5)
+    assert hz.shape == (nphi, len(F))
+    assert np.array_equal(hz, ho)
+
+
+def test_direct_radiosity(api):
+    V, F = _mesh(api, 3, 40, seed=4)
+    ctx = api.IllumContext(V=V, F=F)
+    hz = ctx.make_horizons(nphi=65)
+    disk = api.fib_spiral(100)
+    assert np.array_equal(disk, O.fib_spiral(100))
+    rng = np.random.default_rng(3)
+    S = 6
+    sun = rng.normal(size=(3, S))
+    sun = sun / np.linalg.norm(sun, axis=0) * 227390024.0
+    E = ctx.get_direct_radiosity(sun, disk)
+    Eo = O.direct(O.geometry(V, F), hz, sun, disk)
+    np.testing.assert_allclose(E, Eo, rtol=1e-12, atol=1e-12)
+    assert E.max() > 0
+
+
+@pytest.mark.parametrize("S", [1, 5, 64])
+@pytest.mark.parametrize("method", ["fp64", "mixed"])
+def test_radiosity_vs_gauss_seidel(api, S, method):
+    V, F = _mesh(api, 4, 40, seed=6)
+    ctx = api.IllumContext(V=V, F=F)
+    Fm = ctx.compute_F(0.5)  # large albedo: a harder solve than the configs
+    rng = np.random.default_rng(S)
+    E = rng.uniform(0, 500, (len(F), S))
+    B, res, iters = api.radiosity_solve(Fm, E if S > 1 else E[:, 0], method=method, rel_tol=1e-10,
+                                        max_iter=400)
+    B = B.reshape(len(F), -1)
+    cp, ri, v = Fm.csc()
+    for s in range(min(S, 3)):
+        Bo, sweeps, _ = O.gauss_seidel(len(F), cp, ri, v, E[:, s], max_sweeps=200, rel_tol=1e-13)
+        np.testing.assert_allclose(B[:, s], Bo, rtol=1e-8, atol=1e-8 * np.abs(Bo).max())
+    # FP64 residual of the returned B against the FP64 operator
+    import scipy.sparse as sps
+    A = sps.csc_matrix((v, ri, cp), shape=(len(F), len(F)))
+    r = E - (B - A @ B)
+    rel = np.linalg.norm(r, axis=0) / np.linalg.norm(E, axis=0)
+    assert rel.max() <= 1.2e-10
+    assert res[-1] <= 1e-10
+
+
+def _cn_case(nf, nz, seed):
+    rng = np.random.default_rng(seed)
+    z = 0.5 * (np.exp(np.linspace(0, 3, nz)) - 1 + 0.01) / (np.exp(3) - 1 + 0.01)
+    Q = rng.uniform(0, 600, (64, nf))
+    return z, Q
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("mixed", 1e-3)])
+def test_thermal_vs_oracle(api, precision, tol):
+    nf, nz = 700, 40
+    z, Q = _cn_case(nf, nz, 11)
+    dt = 5.342 * 3600 / 256
+    th = api.ThermalModel(nf, z, 200.0, 1.19177e6, 233.0, emissivity=0.95, precision=precision)
+    th.run(dt, Q[:32])
+    for k in range(32, 64):
+        th.step(dt, Q[k])
+    T0 = np.full((nz + 1, nf), 233.0)
+    To, _, Fo, _ = O.thermal_run([0, nz], z, np.full(nz, 200.0), np.full(nz, 1.19177e6), T0,
+                                 np.zeros(nf), Q, dt, 0.95)
+    T = th.T
+    assert np.max(np.abs(T - To)) <= tol
+    Ts, Fs = th.surface()
+    assert np.max(np.abs(Ts - To[0])) <= tol
+    assert np.allclose(th.get_radiosity(), -Fo, rtol=1e-6 if precision == "mixed" else 1e-9,
+                       atol=1e-6)
+
+
+def test_thermal_ragged_layers_and_ti_scale(api):
+    rng = np.random.default_rng(5)
+    nf = 500
+    grids = []
+    for nz in (32, 57, 96):
+        grids.append(0.5 * (np.exp(np.linspace(0, 3, nz)) - 1 + 0.01) / (np.exp(3) - 1 + 0.01))
+    gid = rng.integers(0, 3, nf).astype(np.int32)
+    s = np.exp(rng.normal(0, 0.5, nf))
+    dt = 60.0
+    Q = rng.uniform(0, 600, (40, nf))
+    th = api.ThermalModel(nf, grids, 200.0, 1.19177e6, 250.0, emissivity=0.95,
+                          grid_of_facet=gid, ti_scale=s)
+    th.run(dt, Q)
+    goff = np.cumsum([0] + [len(g) for g in grids])
+    zc = np.concatenate(grids)
+    nzmax = 96
+    T0 = np.full((nzmax + 1, nf), 250.0)
+    To, _, _, _ = O.thermal_run(goff, zc, np.full(len(zc), 200.0), np.full(len(zc), 1.19177e6), T0,
+                                np.zeros(nf), Q, dt, 0.95, grid=gid, ti_scale=s)
+    T = th.T
+    mask = ~np.isnan(T)
+    nz = np.array([len(grids[g]) for g in gid])
+    assert np.array_equal(mask, np.arange(nzmax + 1)[:, None] <= nz[None, :])
+    assert np.max(np.abs(T[mask] - To[mask])) <= 1e-9
+
+
+def test_conductionQ_dropin_vs_reference_fortran_golden(api):
+    import os
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "cn_golden.npz"))
+    for n in gold["names"]:
+        z, ti, rhoc, Q = (gold[f"{n}.{k}"] for k in ("z", "ti", "rhoc", "Q"))
+        dt, emiss, T0, Fgeo = gold[f"{n}.scalars"]
+        nz = len(z)
+        T = np.full(nz, T0)
+        Ts = T0
+        Qp = 0.0
+        for k, q in enumerate(Q[:8]):
+            T, Ts, Fs = api.conductionQ(nz, z, dt, Qp, q, T, ti, rhoc, emiss, Ts, Fgeo)
+            Qp = q
+            np.testing.assert_allclose(T, gold[f"{n}.T"][k], rtol=0, atol=1e-10)
+            assert abs(Ts - gold[f"{n}.Tsurf"][k]) <= 1e-10
+            assert abs(Fs - gold[f"{n}.Fsurf"][k]) <= 1e-8 * max(1.0, abs(Fs))
