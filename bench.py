@@ -68,68 +68,97 @@ def hemisphere_suns(S, seed):
 
 
 def bench_horizons(api, illum, ctx, nphi=361, theta_eps=1e-3, offset=1e-5):
-    import torch
     nf = illum.num_faces
-    hz = torch.empty((nf, nphi), dtype=torch.float64, device="cuda")
+    hz = api.DeviceArray(ctx, (nf, nphi))  # nphi x nf column-major
     illum.make_horizons(nphi, theta_eps, offset, out=hz)  # warm-up
-    ctx.synchronize()
+    ctx.reset_profile()
+    ctx.set_profiling(True)
     t = time.perf_counter()
     illum.make_horizons(nphi, theta_eps, offset, out=hz)
     ctx.synchronize()
     dt = time.perf_counter() - t
+    ctx.set_profiling(False)
+    kms = ctx.profile().get("horizons", (float("nan"), 1))[0]
     rays = nf * nphi * 11
-    return hz, {"seconds": dt, "rays_per_s": rays / dt, "rays": rays, "nphi": nphi}
+    return hz, {"seconds": dt, "kernel_ms": kms, "rays_per_s": rays / (kms * 1e-3), "rays": rays,
+                "nphi": nphi, "theta_eps": theta_eps}
 
 
-def bench_radiosity(api, illum, ctx, Fm, hz_dev, S=512, tol=1e-8):
-    import torch
+def bench_radiosity(api, illum, ctx, Fm, hz, S=512, tol=1e-8):
     from illum_amd import _lib
     nf = illum.num_faces
-    nphi = hz_dev.shape[1]
-    sun = np.ascontiguousarray(hemisphere_suns(S, 1).T)  # S x 3 == 3 x S col-major
-    disk = np.asfortranarray(api.fib_spiral(100))
-    sun_d = torch.from_numpy(sun).cuda()
-    disk_d = torch.from_numpy(np.ascontiguousarray(disk.T)).cuda()
-    E = torch.empty((S, nf), dtype=torch.float64, device="cuda")  # nf x S col-major
-    _lib.call("illum_direct_radiosity", illum.mesh.h, api._ptr(hz_dev), nphi, api._ptr(sun_d), S,
-              api._ptr(disk_d), 100, api._ptr(E), _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_OUT_ON_DEVICE)
-    ctx.synchronize()
-    out = {"S": S, "rel_tol": tol, "E_mean": float(E.mean())}
-    Bs = {}
+    nphi = hz.shape[1]
+    sun = api.DeviceArray.from_host(ctx, np.ascontiguousarray(hemisphere_suns(S, 1).T))
+    disk = api.DeviceArray.from_host(ctx, np.ascontiguousarray(api.fib_spiral(100).T))
+    E = api.DeviceArray(ctx, (S, nf))  # nf x S column-major
+    dev = _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_OUT_ON_DEVICE
+    _lib.call("illum_direct_radiosity", illum.mesh.h, api._ptr(hz), nphi, api._ptr(sun), S,
+              api._ptr(disk), 100, api._ptr(E), dev)
+    out = {"S": S, "rel_tol": tol, "facets": nf, "nnz": Fm.nnz}
+    Bh = {}
     for method in ("fp64", "mixed"):
-        B = torch.empty_like(E)
+        B = api.DeviceArray(ctx, (S, nf))
         resid = np.zeros(201)
         iters = api.C.c_int()
         m = {"fp64": _lib.ILLUM_RAD_JACOBI_FP64, "mixed": _lib.ILLUM_RAD_MIXED}[method]
         best = None
         for rep in range(2):
             ctx.synchronize()
+            ctx.reset_profile()
+            ctx.set_profiling(True)
             t = time.perf_counter()
             _lib.call("illum_radiosity_solve", Fm.h, api._ptr(E), S, m, tol, 200, api._ptr(B),
-                      api._ptr(resid), api.C.byref(iters),
-                      _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_OUT_ON_DEVICE)
+                      api._ptr(resid), api.C.byref(iters), dev)
             ctx.synchronize()
             dt = time.perf_counter() - t
+            ctx.set_profiling(False)
             best = dt if best is None else min(best, dt)
         nres = int(np.count_nonzero(resid))
         out[method] = {"seconds": best, "products": iters.value,
-                       "final_rel_resid": float(resid[max(0, nres - 1)])}
-        Bs[method] = B
+                       "final_rel_resid": float(resid[max(0, nres - 1)]),
+                       "kernels_ms": {k: v[0] for k, v in ctx.profile().items()}}
+        Bh[method] = B.to_host()
+        B.close()
         log(f"radiosity {method}: {best:.3f}s, {iters.value} products, "
             f"resid {resid[max(0, nres - 1)]:.2e}")
     out["mixed_speedup"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
-    d = (Bs["mixed"] - Bs["fp64"]).abs().max().item()
-    out["max_abs_diff_mixed_vs_fp64_W_m2"] = d
-    out["B_max"] = Bs["fp64"].abs().max().item()
+    out["max_rel_diff_mixed_vs_fp64"] = float(np.max(np.abs(Bh["mixed"] - Bh["fp64"])) /
+                                              np.max(np.abs(Bh["fp64"])))
+    # FP64 relative residual of both answers, recomputed on the host from CSC
+    import scipy.sparse as sps
+    cp, ri, v = Fm.csc()
+    A = sps.csc_matrix((v, ri, cp), shape=(nf, nf))
+    Eh = E.to_host().T
+    for method in ("fp64", "mixed"):
+        Bm = Bh[method].T
+        r = Eh - (Bm - A @ Bm)
+        out[method]["host_check_rel_resid"] = float(np.max(np.linalg.norm(r, axis=0) /
+                                                           np.linalg.norm(Eh, axis=0)))
+    E.close()
     return out
 
 
-def bench_thermal(api, nsteps=4096, subdiv=7, seed=0):
-    import torch
+def insolation_chunks(normals, steps_per_rot, chunk):
+    """Q (W/m^2) for a sun on a 20-degree-inclined orbit about the body's z axis,
+    no shadowing: one rotation in steps_per_rot steps, returned in chunks."""
+    AU = 149.5978707e6
+    S0 = 1372 * (AU / SUN_KM) ** 2 * (1 - 0.1)
+    inc = np.deg2rad(20.0)
+    chunks = []
+    for c0 in range(0, steps_per_rot, chunk):
+        ang = 2 * np.pi * np.arange(c0 + 1, c0 + chunk + 1) / steps_per_rot
+        s = np.stack([np.cos(ang) * np.cos(inc), np.sin(ang) * np.cos(inc),
+                      np.full(chunk, np.sin(inc))])
+        chunks.append(np.ascontiguousarray(S0 * np.clip(s.T @ normals.T, 0, None)))
+    return chunks
+
+
+def bench_thermal(api, nsteps=4096, subdiv=7, seed=0, chunk=256):
+    ctx = api.default_context()
     V, F = api.synth_ellipsoid(subdiv, ncraters=150, seed=seed)
     nf = len(F)
-    m = api.Mesh(api.default_context(), V=V, F=F)
-    nrm = torch.from_numpy(m.geometry()["normal"].astype(np.float64)).cuda()
+    m = api.Mesh(ctx, V=V, F=F)
+    nrm = m.geometry()["normal"].astype(np.float64)
     m.close()
     rng = np.random.default_rng(seed)
     nzs = np.arange(32, 97)
@@ -137,97 +166,101 @@ def bench_thermal(api, nsteps=4096, subdiv=7, seed=0):
     gid = rng.integers(0, len(nzs), nf).astype(np.int32)
     tis = np.exp(rng.normal(0, 0.5, nf))
     rotations = 4
-    dt = rotations * ROT_PERIOD_S / nsteps
-    ang = 2 * np.pi * rotations * np.arange(1, nsteps + 1) / nsteps
-    inc = np.deg2rad(20.0)
-    sdir = np.stack([np.cos(ang) * np.cos(inc), np.sin(ang) * np.cos(inc), np.full(nsteps, np.sin(inc))])
-    sd = torch.from_numpy(sdir).cuda()
-    AU = 149.5978707e6
-    S0 = 1372 * (AU / SUN_KM) ** 2 * (1 - 0.1)
-    Q = (S0 * torch.clamp(sd.T @ nrm.T, min=0)).contiguous()  # nsteps x nf, device
+    spr = nsteps // rotations
+    dt = ROT_PERIOD_S / spr
+    qh = insolation_chunks(nrm, spr, chunk)
+    qd = [api.DeviceArray.from_host(ctx, q) for q in qh]
     out = {"facets": nf, "steps": nsteps, "dt_s": dt, "layers": "U{32..96}",
-           "cell_steps": int(np.sum(nzs[gid])) * nsteps}
+           "cell_steps": int(np.sum(nzs[gid])) * nsteps, "launches": nsteps // chunk}
     res = {}
     for prec in ("fp64", "mixed"):
         th = api.ThermalModel(nf, grids, 200.0, 1.19177e6, 200.0, emissivity=0.95,
                               precision=prec, grid_of_facet=gid, ti_scale=tis)
-        th.run(dt, Q[:8])  # warm-up (also lays down Qprev)
-        th2 = api.ThermalModel(nf, grids, 200.0, 1.19177e6, 200.0, emissivity=0.95,
-                               precision=prec, grid_of_facet=gid, ti_scale=tis)
+        th.run(dt, qd[0])  # warm-up (code load)
         th.close()
-        api.default_context().synchronize()
+        th = api.ThermalModel(nf, grids, 200.0, 1.19177e6, 200.0, emissivity=0.95,
+                              precision=prec, grid_of_facet=gid, ti_scale=tis)
+        ctx.synchronize()
+        ctx.reset_profile()
+        ctx.set_profiling(True)
         t = time.perf_counter()
-        th2.run(dt, Q)
-        api.default_context().synchronize()
+        for k in range(nsteps // chunk):
+            th.run(dt, qd[k % len(qd)])
+        ctx.synchronize()
         sec = time.perf_counter() - t
-        Ts, _ = th2.surface()
-        res[prec] = (th2.T, Ts)
-        out[prec] = {"seconds": sec, "steps_per_s": nsteps / sec,
-                     "cell_steps_per_s": out["cell_steps"] / sec}
-        th2.close()
-        log(f"thermal {prec}: {sec:.3f}s ({nsteps / sec:.0f} steps/s)")
-    out["mixed_speedup"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
+        ctx.set_profiling(False)
+        kname = "cn_mixed" if prec == "mixed" else "cn_fp64"
+        kms = ctx.profile().get(kname, (float("nan"), 1))[0]
+        res[prec] = (th.T, th.surface()[0])
+        th.close()
+        bytes_per_cell = 40 if prec == "mixed" else 48
+        gbs = out["cell_steps"] * bytes_per_cell / (kms * 1e-3) / 1e9
+        out[prec] = {"seconds": sec, "kernel_ms": kms, "steps_per_s": nsteps / sec,
+                     "cell_steps_per_s": out["cell_steps"] / (kms * 1e-3),
+                     "algorithmic_GBs": gbs, "hbm_frac": gbs / PEAK_HBM_GBS}
+        log(f"thermal {prec}: {sec:.3f}s wall, kernel {kms:.1f} ms ({nsteps / sec:.0f} steps/s)")
+    out["mixed_speedup"] = out["fp64"]["kernel_ms"] / out["mixed"]["kernel_ms"]
     out["max_abs_Tsurf_diff_mixed_vs_fp64_K"] = float(np.max(np.abs(res["mixed"][1] - res["fp64"][1])))
     Tm, Tf = res["mixed"][0], res["fp64"][0]
     ok = ~np.isnan(Tf)
     out["max_abs_T_diff_mixed_vs_fp64_K"] = float(np.max(np.abs(Tm[ok] - Tf[ok])))
-    # FP64 GPU vs the CPU oracle on a sample of facets
+    # FP64 GPU and mixed GPU vs the CPU oracle on a sample of facets
     from oracle import oracle as O
     samp = np.arange(0, nf, max(1, nf // 256))
     goff = np.cumsum([0] + [len(g) for g in grids])
     zc = np.concatenate(grids)
-    Qh = Q[:, torch.from_numpy(samp).cuda()].cpu().numpy()
+    Qs = np.concatenate([qh[k % len(qh)][:, samp] for k in range(nsteps // chunk)])
     T0 = np.full((97, len(samp)), 200.0)
     To, _, _, _ = O.thermal_run(goff, zc, np.full(len(zc), 200.0), np.full(len(zc), 1.19177e6),
-                                T0, np.zeros(len(samp)), Qh, dt, 0.95, grid=gid[samp],
+                                T0, np.zeros(len(samp)), Qs, dt, 0.95, grid=gid[samp],
                                 ti_scale=tis[samp])
-    Tg = res["fp64"][0][:, samp]
+    Tg = Tf[:, samp]
     okk = ~np.isnan(Tg)
     out["max_abs_T_diff_fp64_vs_oracle_K"] = float(np.max(np.abs(Tg[okk] - To[okk])))
-    Tgm = res["mixed"][0][:, samp]
-    out["max_abs_Tsurf_diff_mixed_vs_oracle_K"] = float(np.max(np.abs(Tgm[0] - To[0])))
+    out["max_abs_Tsurf_diff_mixed_vs_oracle_K"] = float(np.max(np.abs(Tm[0, samp] - To[0])))
+    for d in qd:
+        d.close()
     return out
 
 
-def bench_coupled(api, illum, ctx, Fm, hz_dev, nsteps=32):
-    """radiosity + thermal time loop (illum_cli.cpp:403-456) on device buffers."""
-    import torch
+def bench_coupled(api, illum, ctx, Fm, hz, nsteps=32):
+    """radiosity + thermal time loop (illum_cli.cpp:403-456): direct -> E -= F_surf ->
+    radiosity solve -> CN step, all on device buffers."""
     from illum_amd import _lib
     nf = illum.num_faces
-    nphi = hz_dev.shape[1]
+    nphi = hz.shape[1]
     z = 0.5 * (np.exp(np.linspace(0, 3, 40)) - 1 + 0.01) / (np.exp(3) - 1 + 0.01)
     dt = ROT_PERIOD_S / 256
-    disk_d = torch.from_numpy(np.ascontiguousarray(api.fib_spiral(100).T)).cuda()
+    disk = api.DeviceArray.from_host(ctx, np.ascontiguousarray(api.fib_spiral(100).T))
     ang = 2 * np.pi * np.arange(nsteps) / 256
-    suns = [torch.from_numpy(SUN_KM * np.array([np.cos(a) * 0.94, np.sin(a) * 0.94, 0.34])).cuda()
+    suns = [api.DeviceArray.from_host(ctx, SUN_KM * np.array([np.cos(a) * 0.94, np.sin(a) * 0.94, 0.34]))
             for a in ang]
     out = {"steps": nsteps, "facets": nf, "layers": 40}
     finals = {}
+    dev = _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_OUT_ON_DEVICE
     for prec in ("fp64", "mixed"):
-        th = api.ThermalModel(nf, z, 200.0, 1.19177e6, 200.0, emissivity=0.95, precision=prec)
-        E = torch.empty(nf, dtype=torch.float64, device="cuda")
-        B = torch.empty_like(E)
-        Fs = torch.empty_like(E)
+        th = api.ThermalModel(nf, z, 200.0, 1.19177e6, 200.0, emissivity=0.95, precision=prec, ctx=ctx)
+        E = api.DeviceArray(ctx, (nf,))
+        B = api.DeviceArray(ctx, (nf,))
         resid = np.zeros(101)
         it = api.C.c_int()
         method = _lib.ILLUM_RAD_JACOBI_FP64 if prec == "fp64" else _lib.ILLUM_RAD_MIXED
-        flags = _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_OUT_ON_DEVICE
         ctx.synchronize()
         t = time.perf_counter()
         for k in range(nsteps):
-            _lib.call("illum_direct_radiosity", illum.mesh.h, api._ptr(hz_dev), nphi, api._ptr(suns[k]),
-                      1, api._ptr(disk_d), 100, api._ptr(E), flags)
-            _lib.call("illum_thermal_get", th.h, None, None, api._ptr(Fs), _lib.ILLUM_OUT_ON_DEVICE)
-            E.sub_(Fs)  # rad += -F_surf (illum_cli.cpp:416-418)
-            _lib.call("illum_radiosity_solve", Fm.h, api._ptr(E), 1, method, 1e-8, 100, api._ptr(B),
-                      api._ptr(resid), api.C.byref(it), flags)
-            _lib.call("illum_thermal_step", th.h, dt, api._ptr(B), _lib.ILLUM_IN_ON_DEVICE)
+            _lib.call("illum_direct_radiosity", illum.mesh.h, api._ptr(hz), nphi, api._ptr(suns[k]),
+                      1, api._ptr(disk), 100, api._ptr(E), dev)
+            # rad = -F_surf + direct (illum_cli.cpp:411-423), formed on the host
+            Eh = E.to_host() + th.get_radiosity()
+            B_h, _, _ = api.radiosity_solve(Fm, Eh, method=prec, rel_tol=1e-8, max_iter=100)
+            th.step(dt, B_h)
         ctx.synchronize()
-        api.default_context().synchronize()
         sec = time.perf_counter() - t
         finals[prec] = th.surface()[0]
         out[prec] = {"seconds": sec, "steps_per_s": nsteps / sec}
         th.close()
+        E.close()
+        B.close()
         log(f"coupled {prec}: {nsteps / sec:.1f} steps/s")
     out["mixed_speedup"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
     out["max_abs_Tsurf_diff_mixed_vs_fp64_K"] = float(np.max(np.abs(finals["mixed"] - finals["fp64"])))
@@ -308,7 +341,7 @@ def main():
         log("radiosity:", secondary["radiosity"])
         secondary["coupled"] = bench_coupled(api, illum, ctx, Fm, hz)
         log("coupled:", secondary["coupled"])
-        del hz
+        hz.close()
         secondary["thermal"] = bench_thermal(api, nsteps=args.thermal_steps)
         log("thermal:", secondary["thermal"])
 

@@ -43,6 +43,9 @@ SIGNATURES = {
     "illum_ctx_set_profiling": [_P, _I],
     "illum_ctx_reset_profile": [_P],
     "illum_ctx_get_profile": [_P, _I, _P, _P, _P, _I, _P],
+    "illum_dev_alloc": [_P, _I64, _PP],
+    "illum_dev_free": [_P, _P],
+    "illum_dev_copy": [_P, _P, _P, _I64, _I],
     "illum_synth_sizes": [_I, _P, _P],
     "illum_synth_ellipsoid": [_I, _D, _D, _D, _I, _D, _D, _D, _D, C.c_uint64, _P, _P],
     "illum_mesh_create": [_P, _P, _I64, _P, _I64, _P, _D, _PP],

@@ -80,6 +80,51 @@ class Context:
         return {k.decode(): (float(ms[i]), int(cnt[i])) for i, k in enumerate(keys)}
 
 
+class DeviceArray:
+    """A buffer in the context GPU's HBM (illum_dev_alloc), usable wherever the
+    API takes a device pointer.  Row-major numpy-style shape and dtype."""
+
+    is_cuda = True
+
+    def __init__(self, ctx, shape, dtype=np.float64):
+        self.ctx = ctx
+        self.shape = tuple(shape) if np.ndim(shape) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        call("illum_dev_alloc", ctx.h, self.nbytes, C.byref(p))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, ctx, arr):
+        a = np.ascontiguousarray(arr)
+        d = cls(ctx, a.shape, a.dtype)
+        call("illum_dev_copy", ctx.h, d.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes, 0)
+        return d
+
+    def to_host(self):
+        a = np.empty(self.shape, self.dtype)
+        call("illum_dev_copy", self.ctx.h, a.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes, 1)
+        return a
+
+    def data_ptr(self):
+        return self.ptr.value
+
+    def dim(self):
+        return len(self.shape)
+
+    def close(self):
+        if self.ptr:
+            call("illum_dev_free", self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 _default_ctx = None
 
 

@@ -199,6 +199,43 @@ HostBVH build_bvh(const HostGeometry &g, int leaf_size) {
     o[4] = bn.box.hi[0] + pad; o[5] = bn.box.hi[1] + pad; o[6] = bn.box.hi[2] + pad;
     o[7] = as_float(info);
   }
+  // Two-wide layout: inner nodes in pre-order, each holding both child boxes.
+  {
+    std::vector<int> inner_of(nn, -1);
+    int ninner = 0;
+    for (int k = 0; k < nn; ++k)
+      if (B.nodes[order[k]].left >= 0) inner_of[order[k]] = ninner++;
+    auto leaf_info = [&](const BuildNode &bn) {
+      if (bn.count >= 128) throw Error(ILLUM_ERR_CAPACITY, "leaf too large");
+      return (int32_t)((bn.count << 24) | bn.first);
+    };
+    auto put_box = [&](float *o, const Box &b, int32_t ref) {
+      o[0] = b.lo[0] - pad; o[1] = b.lo[1] - pad; o[2] = b.lo[2] - pad; o[3] = as_float(ref);
+      o[4] = b.hi[0] + pad; o[5] = b.hi[1] + pad; o[6] = b.hi[2] + pad; o[7] = 0.0f;
+    };
+    if (ninner == 0) {  // root is a leaf: one inner root, empty right box
+      h.ninner = 1;
+      h.wide.assign(16, 0.0f);
+      put_box(&h.wide[0], B.nodes[0].box, ~leaf_info(B.nodes[0]));
+      Box empty;  // lo = +max, hi = -max: never entered
+      for (int c = 0; c < 3; ++c) { h.wide[8 + c] = empty.lo[c]; h.wide[12 + c] = empty.hi[c]; }
+      h.wide[11] = as_float(~leaf_info(B.nodes[0]));
+    } else {
+      h.ninner = ninner;
+      h.wide.assign(16 * (size_t)ninner, 0.0f);
+      for (int k = 0; k < nn; ++k) {
+        const BuildNode &bn = B.nodes[order[k]];
+        if (bn.left < 0) continue;
+        float *o = &h.wide[16 * (size_t)inner_of[order[k]]];
+        const int ch[2] = {bn.left, bn.right};
+        for (int s = 0; s < 2; ++s) {
+          const BuildNode &cn = B.nodes[ch[s]];
+          const int32_t ref = cn.left >= 0 ? inner_of[ch[s]] : ~leaf_info(cn);
+          put_box(o + 8 * s, cn.box, ref);
+        }
+      }
+    }
+  }
   for (int64_t k = 0; k < nf; ++k) {
     int t = B.perm[k];
     float *o = &h.tris[12 * (size_t)k];

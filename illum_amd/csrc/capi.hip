@@ -107,7 +107,7 @@ static void destroy_mesh(illum_mesh *m) {
   delete m;
 }
 static void destroy_bvh(illum_bvh *b) {
-  dfree(b->d_nodes); dfree(b->d_tris);
+  dfree(b->d_nodes); dfree(b->d_tris); dfree(b->d_wide);
   delete b;
 }
 void destroy_sparse(illum_sparse *s) {
@@ -245,6 +245,34 @@ int illum_ctx_get_profile(illum_ctx *ctx, int cap, double *total_ms, int64_t *la
   });
 }
 
+int illum_dev_alloc(illum_ctx *ctx, int64_t bytes, void **out) {
+  return guarded([&] {
+    CHECK_ARG(ctx && out && bytes >= 0, "bad argument");
+    set_device(ctx);
+    *out = dalloc<char>((size_t)bytes);
+  });
+}
+
+int illum_dev_free(illum_ctx *ctx, void *p) {
+  return guarded([&] {
+    CHECK_ARG(ctx, "null context");
+    set_device(ctx);
+    sync(ctx);
+    if (p) ILLUM_HIP(hipFree(p));
+  });
+}
+
+int illum_dev_copy(illum_ctx *ctx, void *dst, const void *src, int64_t bytes, int kind) {
+  return guarded([&] {
+    CHECK_ARG(ctx && dst && src && bytes >= 0 && kind >= 0 && kind <= 2, "bad argument");
+    set_device(ctx);
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                                     : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+    ILLUM_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, k, stream_of(ctx)));
+    sync(ctx);
+  });
+}
+
 int illum_synth_sizes(int subdiv, int64_t *nv, int64_t *nf) {
   return guarded([&] { synth_sizes(subdiv, nv, nf); });
 }
@@ -321,6 +349,9 @@ int illum_bvh_build(illum_mesh *mesh, int leaf_size, illum_bvh **out) {
     hipStream_t s = stream_of(ctx);
     b->d_nodes = dalloc<float>(b->h.nodes.size());
     b->d_tris = dalloc<float>(b->h.tris.size());
+    b->d_wide = dalloc<float>(b->h.wide.size());
+    ILLUM_HIP(hipMemcpyAsync(b->d_wide, b->h.wide.data(), 4 * b->h.wide.size(),
+                             hipMemcpyHostToDevice, s));
     ILLUM_HIP(hipMemcpyAsync(b->d_nodes, b->h.nodes.data(), 4 * b->h.nodes.size(),
                              hipMemcpyHostToDevice, s));
     ILLUM_HIP(hipMemcpyAsync(b->d_tris, b->h.tris.data(), 4 * b->h.tris.size(),

@@ -20,12 +20,20 @@
 
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
+#include <string>
 
 namespace illum {
 
 constexpr int FF_BLOCK = 256;
 constexpr int FF_PER = 8;                          // candidates per thread per chunk
 constexpr int FF_QCAP = FF_BLOCK * (FF_PER + 1);   // queue capacity (ints)
+constexpr int FF_STACK = 32;                       // traversal stack entries per lane
+// First pass of each visibility ray prunes at this distance: the ray aims at a
+// centroid lying on its target triangle, so the closest hit is almost always
+// at t <= 1; a ray with no hit below the bound is traced again unbounded,
+// which keeps the result identical to an unbounded closest-hit query.
+constexpr float FF_TINIT = 1.0009765625f;
 
 // Form factor F(i, j) for receiver i and source j (illum.cpp:100-119), in FP64
 // from the FP32 centroids/normals.  Returns 0 for r < DBL_EPSILON.
@@ -51,15 +59,18 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
   return x * per + (x < rem ? x : rem) + k;
 }
 
+template <bool WIDE>
 __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
     int nf, int nwords, const float4 *__restrict__ pr4, const float4 *__restrict__ n4,
-    const float4 *__restrict__ nodes, const float4 *__restrict__ tris, float offset,
+    const float4 *__restrict__ nodes, const float4 *__restrict__ wide,
+    const float4 *__restrict__ tris, float offset,
     const double *__restrict__ area, const double *__restrict__ albedo_vec, double albedo,
     uint32_t *__restrict__ vis, uint32_t *__restrict__ mask, int32_t *__restrict__ counts,
     int *__restrict__ status) {
   extern __shared__ uint32_t smem[];
   uint32_t *bits = smem;
   int *queue = (int *)(smem + nwords);
+  int *stack = queue + FF_QCAP;  // FF_STACK x FF_BLOCK, used when WIDE
   __shared__ int s_qn;
   __shared__ int s_red[FF_BLOCK / 64];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -101,7 +112,14 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
         const float4 pi = pr4[i];
         const DRay r = make_ray(qx, qy, qz, pi.x - qx, pi.y - qy, pi.z - qz);
         float t;
-        const int h = trace_closest(nodes, tris, r, &t);
+        int h;
+        if (WIDE) {
+          h = trace_closest_wide(wide, tris, r, FF_TINIT, stack + tid, FF_BLOCK, &t);
+          if (h < 0)
+            h = trace_closest_wide(wide, tris, r, __builtin_huge_valf(), stack + tid, FF_BLOCK, &t);
+        } else {
+          h = trace_closest(nodes, tris, r, &t);
+        }
         if (h >= 0 && h != j) atomicOr(&bits[h >> 5], 1u << (h & 31));
       }
       qrem -= take;
@@ -248,18 +266,28 @@ __global__ void popcount_rows_kernel(int nrows, int nwords, const uint32_t *__re
   if (lane == 0) counts[row] = c;
 }
 
-__global__ void closest_hit_kernel(int64_t nray, const float *__restrict__ orig,
-                                   const float *__restrict__ dir,
-                                   const float4 *__restrict__ nodes,
-                                   const float4 *__restrict__ tris, int32_t *__restrict__ hit,
-                                   float *__restrict__ tout) {
+__global__ __launch_bounds__(256) void closest_hit_kernel(
+    int64_t nray, const float *__restrict__ orig, const float *__restrict__ dir,
+    const float4 *__restrict__ nodes, const float4 *__restrict__ wide,
+    const float4 *__restrict__ tris, int32_t *__restrict__ hit, float *__restrict__ tout) {
+  __shared__ int stack[FF_STACK * 256];
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nray) return;
   const DRay r = make_ray(orig[3 * k], orig[3 * k + 1], orig[3 * k + 2], dir[3 * k],
                           dir[3 * k + 1], dir[3 * k + 2]);
   float t;
-  hit[k] = trace_closest(nodes, tris, r, &t);
+  hit[k] = wide ? trace_closest_wide(wide, tris, r, __builtin_huge_valf(), stack + threadIdx.x,
+                                     256, &t)
+                : trace_closest(nodes, tris, r, &t);
   tout[k] = t;
+}
+
+// The near-first layout is used when the tree fits the traversal stack and
+// ILLUM_FF_TRAVERSAL is not "threaded".
+static bool use_wide(const illum_bvh *bvh) {
+  const char *e = getenv("ILLUM_FF_TRAVERSAL");
+  if (e && std::string(e) == "threaded") return false;
+  return bvh->d_wide && bvh->h.depth <= FF_STACK;
 }
 
 // ---------------------------------------------------------------------------
@@ -273,8 +301,10 @@ void bvh_closest_hit(illum_bvh *bvh, const float *orig, const float *dir, int64_
   ILLUM_HIP(hipMemcpyAsync(o.p, orig, 12 * nray, hipMemcpyHostToDevice, s));
   ILLUM_HIP(hipMemcpyAsync(d.p, dir, 12 * nray, hipMemcpyHostToDevice, s));
   const int blocks = (int)((nray + 255) / 256);
-  closest_hit_kernel<<<blocks, 256, 0, s>>>(nray, o.p, d.p, (const float4 *)bvh->d_nodes,
-                                            (const float4 *)bvh->d_tris, h.p, tt.p);
+  closest_hit_kernel<<<blocks, 256, 0, s>>>(
+      nray, o.p, d.p, (const float4 *)bvh->d_nodes,
+      use_wide(bvh) ? (const float4 *)bvh->d_wide : nullptr, (const float4 *)bvh->d_tris, h.p,
+      tt.p);
   ILLUM_LAUNCH_CHECK();
   ILLUM_HIP(hipMemcpyAsync(hit, h.p, 4 * nray, hipMemcpyDeviceToHost, s));
   ILLUM_HIP(hipMemcpyAsync(t, tt.p, 4 * nray, hipMemcpyDeviceToHost, s));
@@ -289,12 +319,14 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   const int64_t nf = mesh->g.nf;
   const int nwords = (int)(((nf + 511) / 512) * 16);
   const int64_t rows_p = (int64_t)nwords * 32;
-  const size_t lds = (size_t)(nwords + FF_QCAP) * 4;
+  const bool wide = use_wide(bvh);
+  const size_t lds = (size_t)(nwords + FF_QCAP + (wide ? FF_STACK * FF_BLOCK : 0)) * 4;
   if (lds > 160 * 1024)
     throw Error(ILLUM_ERR_CAPACITY, "too many facets for the LDS visibility bitset (max ~1.2M)");
+  const void *kfn = wide ? (const void *)ff_visibility_kernel<true>
+                         : (const void *)ff_visibility_kernel<false>;
   if (lds > 64 * 1024)
-    ILLUM_HIP(hipFuncSetAttribute((const void *)ff_visibility_kernel,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    ILLUM_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 
   std::unique_ptr<illum_sparse, void (*)(illum_sparse *)> sp(new illum_sparse(), destroy_sparse);
   sp->ctx = ctx;
@@ -315,10 +347,11 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
 
   {
     StageTimer st(ctx, "ff_visibility");
-    ff_visibility_kernel<<<(int)nf, FF_BLOCK, lds, s>>>(
+    auto launch = wide ? ff_visibility_kernel<true> : ff_visibility_kernel<false>;
+    launch<<<(int)nf, FF_BLOCK, lds, s>>>(
         (int)nf, nwords, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4,
-        (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_tris, (float)offset,
-        mesh->d_area, d_alb.p, albedo, sp->d_vis, M.p, counts.p, status.p);
+        (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_wide, (const float4 *)bvh->d_tris,
+        (float)offset, mesh->d_area, d_alb.p, albedo, sp->d_vis, M.p, counts.p, status.p);
     ILLUM_LAUNCH_CHECK();
   }
   // CSC

@@ -56,10 +56,19 @@ void read_obj(const char *path, std::vector<double> &V, std::vector<int32_t> &F,
 //                                                   (count << 24) | first
 // Triangles are stored in leaf order as three float4 each:
 //   (v0.xyz, as_float(orig_index)), (v1.xyz, 0), (v2.xyz, 0).
+//
+// The same tree is also stored in a two-wide layout for near-first traversal:
+// inner node k (pre-order over inner nodes only) holds both children's boxes,
+//   f4[0] = (left.bmin, as_float(left)),  f4[1] = (left.bmax, 0),
+//   f4[2] = (right.bmin, as_float(right)), f4[3] = (right.bmax, 0),
+// where a child reference >= 0 is an inner node index and < 0 is ~leaf_info.
+// A tree whose root is a leaf gets one inner root with an empty right box.
 struct HostBVH {
   std::vector<float> nodes;    // 8 floats per node
+  std::vector<float> wide;     // 16 floats per inner node
   std::vector<float> tris;     // 12 floats per triangle
   int64_t nnodes = 0;
+  int64_t ninner = 0;
   int depth = 0;
   int leaf_size = 0;
 };
@@ -92,6 +101,7 @@ struct illum_bvh {
   illum::HostBVH h;
   float *d_nodes = nullptr;
   float *d_tris = nullptr;
+  float *d_wide = nullptr;
 };
 
 struct illum_sparse {

@@ -143,6 +143,72 @@ __device__ __forceinline__ int trace_closest(const float4 *__restrict__ nodes,
   return best;
 }
 
+// Slab test returning the entry distance.
+__device__ __forceinline__ bool box_enter(const DRay &r, float4 lo, float4 hi, float tbest,
+                                          float *tn_out) {
+  float t0x = (lo.x - r.ox) * r.ix, t1x = (hi.x - r.ox) * r.ix;
+  float t0y = (lo.y - r.oy) * r.iy, t1y = (hi.y - r.oy) * r.iy;
+  float t0z = (lo.z - r.oz) * r.iz, t1z = (hi.z - r.oz) * r.iz;
+  float tn = fmaxf(fminf(t0x, t1x), fmaxf(fminf(t0y, t1y), fminf(t0z, t1z)));
+  float tf = fminf(fmaxf(t0x, t1x), fminf(fmaxf(t0y, t1y), fmaxf(t0z, t1z)));
+  *tn_out = tn;
+  return tn <= tf && tf > 0.0f && tn <= tbest;
+}
+
+// Near-first closest hit over the two-wide layout.  `stack` points at this
+// lane's first slot of an LDS stack laid out slot-major (entry k of the lane
+// at stack[k * stride]) so lanes at equal depth touch consecutive words.
+// The tree depth must not exceed the stack capacity (checked on the host).
+// Boxes are pruned against the best distance so far, starting from tinit;
+// because every ancestor box enters no later than any hit inside it, the
+// result equals the argmin over all triangles with distance <= tinit.
+__device__ __forceinline__ int trace_closest_wide(const float4 *__restrict__ wide,
+                                                  const float4 *__restrict__ tris,
+                                                  const DRay &r, float tinit, int *stack,
+                                                  int stride, float *tbest_out) {
+  float tbest = tinit;
+  int best = -1;
+  int sp = 0;
+  int cur = 0;
+  for (;;) {
+    if (cur >= 0) {
+      const float4 a0 = wide[4 * cur], a1 = wide[4 * cur + 1];
+      const float4 b0 = wide[4 * cur + 2], b1 = wide[4 * cur + 3];
+      float tl, tr;
+      const bool hl = box_enter(r, a0, a1, tbest, &tl);
+      const bool hr = box_enter(r, b0, b1, tbest, &tr);
+      const int cl = __float_as_int(a0.w), cr = __float_as_int(b0.w);
+      if (hl && hr) {
+        const bool lnear = tl <= tr;
+        stack[sp * stride] = lnear ? cr : cl;
+        ++sp;
+        cur = lnear ? cl : cr;
+        continue;
+      }
+      if (hl) { cur = cl; continue; }
+      if (hr) { cur = cr; continue; }
+    } else {
+      const int info = ~cur;
+      const int first = info & 0xFFFFFF, cnt = info >> 24;
+      for (int k = 0; k < cnt; ++k) {
+        const float4 a = tris[3 * (first + k)], b = tris[3 * (first + k) + 1],
+                     c = tris[3 * (first + k) + 2];
+        const int idx = __float_as_int(a.w);
+        float t;
+        if (tri_hit(r, a, b, c, &t) && (t < tbest || (t == tbest && idx < best))) {
+          tbest = t;
+          best = idx;
+        }
+      }
+    }
+    if (sp == 0) break;
+    --sp;
+    cur = stack[sp * stride];
+  }
+  *tbest_out = tbest;
+  return best;
+}
+
 // Any hit (t > 0) over the threaded BVH.
 __device__ __forceinline__ bool trace_any(const float4 *__restrict__ nodes,
                                           const float4 *__restrict__ tris, const DRay &r) {

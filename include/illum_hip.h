@@ -78,6 +78,15 @@ int illum_ctx_reset_profile(illum_ctx *ctx);
 int illum_ctx_get_profile(illum_ctx *ctx, int cap, double *total_ms, int64_t *launches,
                           char *names, int names_cap, int *n);
 
+/* ---- device memory on the context's GPU (for the *_ON_DEVICE paths) -----
+ * No counterpart in the reference (it has no device); lets C/C++ callers keep
+ * E, B, horizons and Q series resident in HBM between calls.  kind: 0 host to
+ * device, 1 device to host, 2 device to device; copies are ordered on the
+ * context stream and complete before the call returns. */
+int illum_dev_alloc(illum_ctx *ctx, int64_t bytes, void **out);
+int illum_dev_free(illum_ctx *ctx, void *p);
+int illum_dev_copy(illum_ctx *ctx, void *dst, const void *src, int64_t bytes, int kind);
+
 /* ---- synthetic shape models (host, deterministic, seeded) --------------
  * Icosahedron subdivided `subdiv` times, projected to the unit sphere,
  * scaled to a triaxial ellipsoid (a, b, c), radially displaced by
