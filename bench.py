@@ -274,11 +274,25 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--subdiv", type=int, default=6)
     ap.add_argument("--craters", type=int, default=150)
-    ap.add_argument("--cpu-sample", type=int, default=512, help="source facets in the CPU sample")
+    ap.add_argument("--cpu-sample", type=int, default=16384,
+                    help="source facets in the CPU sample (about 10 s on 16 threads)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--leaf-size", type=int, default=0, help="BVH leaf size (0 = autotune)")
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--thermal-steps", type=int, default=4096)
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run one device-to-device copy of this many bytes (a known-size "
+                         "kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.copy_bytes:
+        from illum_amd import api as A
+        c = A.default_context()
+        src = A.DeviceArray(c, (args.copy_bytes,), np.uint8)
+        dst = A.DeviceArray(c, (args.copy_bytes,), np.uint8)
+        A._lib.call("illum_dev_copy", c.h, dst.ptr, src.ptr, args.copy_bytes, 2)
+        c.synchronize()
+        log(f"copied {args.copy_bytes} bytes device to device")
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the form-factor path "
               "shards by source facet with no communication, see DESIGN.md", file=sys.stderr)
@@ -291,8 +305,9 @@ def main():
     nf = len(F)
     albedo = np.random.default_rng(0).uniform(0.05, 0.15, nf)
     ctx = api.default_context()
-    illum = api.IllumContext(V=V, F=F, leaf_size=4, ctx=ctx)
-    log(f"mesh {nf} facets, bvh {illum.bvh.info()}, setup {time.perf_counter() - t0:.1f}s")
+    illum = api.IllumContext(V=V, F=F, leaf_size=args.leaf_size, ctx=ctx)
+    log(f"mesh {nf} facets, bvh {illum.bvh.info()}, leaf autotune {illum.bvh.tuning()}, "
+        f"setup {time.perf_counter() - t0:.1f}s")
 
     for w in range(args.warmup):
         t = time.perf_counter()

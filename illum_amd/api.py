@@ -217,6 +217,14 @@ class BVH:
         call("illum_bvh_info", self.h, C.byref(n), C.byref(d), C.byref(l))
         return dict(nodes=n.value, depth=d.value, leaf_size=l.value)
 
+    def tuning(self):
+        """[(leaf_size, ms)] of the autotuning trials (empty unless leaf_size was 0)."""
+        leaves = np.zeros(8, np.int32)
+        ms = np.zeros(8)
+        n = C.c_int()
+        call("illum_bvh_tuning", self.h, 8, _ptr(leaves), _ptr(ms), C.byref(n))
+        return [(int(leaves[k]), float(ms[k])) for k in range(n.value)]
+
     def closest_hit(self, orig, dirs):
         orig = np.ascontiguousarray(orig, np.float32)
         dirs = np.ascontiguousarray(dirs, np.float32)

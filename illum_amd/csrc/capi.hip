@@ -11,6 +11,7 @@
 namespace illum {
 
 // Implemented in the kernel translation units.
+double time_visibility_sample(illum_bvh *bvh, int max_sources);
 illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_vec,
                            double offset);
 void bvh_closest_hit(illum_bvh *bvh, const float *orig, const float *dir, int64_t nray,
@@ -338,26 +339,70 @@ int illum_mesh_triangles(const illum_mesh *mesh, float *tri9) {
   });
 }
 
+static illum_bvh *make_bvh(illum_mesh *mesh, int leaf_size) {
+  illum_ctx *ctx = mesh->ctx;
+  auto b = std::unique_ptr<illum_bvh, void (*)(illum_bvh *)>(new illum_bvh(), destroy_bvh);
+  b->mesh = mesh;
+  b->h = build_bvh(mesh->g, leaf_size);
+  hipStream_t s = stream_of(ctx);
+  b->d_nodes = dalloc<float>(b->h.nodes.size());
+  b->d_tris = dalloc<float>(b->h.tris.size());
+  b->d_wide = dalloc<float>(b->h.wide.size());
+  ILLUM_HIP(hipMemcpyAsync(b->d_wide, b->h.wide.data(), 4 * b->h.wide.size(),
+                           hipMemcpyHostToDevice, s));
+  ILLUM_HIP(hipMemcpyAsync(b->d_nodes, b->h.nodes.data(), 4 * b->h.nodes.size(),
+                           hipMemcpyHostToDevice, s));
+  ILLUM_HIP(hipMemcpyAsync(b->d_tris, b->h.tris.data(), 4 * b->h.tris.size(),
+                           hipMemcpyHostToDevice, s));
+  sync(ctx);
+  return b.release();
+}
+
 int illum_bvh_build(illum_mesh *mesh, int leaf_size, illum_bvh **out) {
   return guarded([&] {
     CHECK_ARG(mesh && out, "null argument");
-    illum_ctx *ctx = mesh->ctx;
-    set_device(ctx);
-    auto b = std::unique_ptr<illum_bvh, void (*)(illum_bvh *)>(new illum_bvh(), destroy_bvh);
-    b->mesh = mesh;
-    b->h = build_bvh(mesh->g, leaf_size);
-    hipStream_t s = stream_of(ctx);
-    b->d_nodes = dalloc<float>(b->h.nodes.size());
-    b->d_tris = dalloc<float>(b->h.tris.size());
-    b->d_wide = dalloc<float>(b->h.wide.size());
-    ILLUM_HIP(hipMemcpyAsync(b->d_wide, b->h.wide.data(), 4 * b->h.wide.size(),
-                             hipMemcpyHostToDevice, s));
-    ILLUM_HIP(hipMemcpyAsync(b->d_nodes, b->h.nodes.data(), 4 * b->h.nodes.size(),
-                             hipMemcpyHostToDevice, s));
-    ILLUM_HIP(hipMemcpyAsync(b->d_tris, b->h.tris.data(), 4 * b->h.tris.size(),
-                             hipMemcpyHostToDevice, s));
-    sync(ctx);
-    *out = b.release();
+    CHECK_ARG(leaf_size >= 0, "leaf_size must be >= 0 (0 = autotune)");
+    set_device(mesh->ctx);
+    if (leaf_size > 0) {
+      *out = make_bvh(mesh, leaf_size);
+      return;
+    }
+    // Autotune: time the visibility kernel on a strided sample of up to 8192
+    // sources for each candidate leaf size and keep the fastest tree.
+    static const int cand[] = {2, 3, 4, 6, 8};
+    std::unique_ptr<illum_bvh, void (*)(illum_bvh *)> best(nullptr, destroy_bvh);
+    double best_ms = 0;
+    int leaves[8];
+    double times[8];
+    int n = 0;
+    for (int c : cand) {
+      std::unique_ptr<illum_bvh, void (*)(illum_bvh *)> b(make_bvh(mesh, c), destroy_bvh);
+      const double ms = time_visibility_sample(b.get(), 8192);
+      leaves[n] = c;
+      times[n] = ms;
+      ++n;
+      if (!best || ms < best_ms) {
+        best = std::move(b);
+        best_ms = ms;
+      }
+    }
+    for (int k = 0; k < n; ++k) {
+      best->tune_leaf[k] = leaves[k];
+      best->tune_ms[k] = times[k];
+    }
+    best->ntuned = n;
+    *out = best.release();
+  });
+}
+
+int illum_bvh_tuning(const illum_bvh *bvh, int cap, int *leaf_sizes, double *ms, int *n) {
+  return guarded([&] {
+    CHECK_ARG(bvh && n, "null argument");
+    for (int k = 0; k < bvh->ntuned && k < cap; ++k) {
+      if (leaf_sizes) leaf_sizes[k] = bvh->tune_leaf[k];
+      if (ms) ms[k] = bvh->tune_ms[k];
+    }
+    *n = bvh->ntuned;
   });
 }
 

@@ -20,13 +20,14 @@
 
 #include <cfloat>
 #include <cmath>
+#include <algorithm>
 #include <cstdlib>
 #include <string>
 
 namespace illum {
 
 constexpr int FF_BLOCK = 256;
-constexpr int FF_PER = 8;                          // candidates per thread per chunk
+constexpr int FF_PER = 4;                          // candidates per thread per chunk
 constexpr int FF_QCAP = FF_BLOCK * (FF_PER + 1);   // queue capacity (ints)
 constexpr int FF_STACK = 32;                       // traversal stack entries per lane
 // First pass of each visibility ray prunes at this distance: the ray aims at a
@@ -66,15 +67,17 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
     const float4 *__restrict__ tris, float offset,
     const double *__restrict__ area, const double *__restrict__ albedo_vec, double albedo,
     uint32_t *__restrict__ vis, uint32_t *__restrict__ mask, int32_t *__restrict__ counts,
-    int *__restrict__ status) {
+    int *__restrict__ status, int nrows, int jstride) {
   extern __shared__ uint32_t smem[];
   uint32_t *bits = smem;
   int *queue = (int *)(smem + nwords);
-  int *stack = queue + FF_QCAP;  // FF_STACK x FF_BLOCK, used when WIDE
+  int *stack = queue + FF_QCAP;  // (tree depth + 1) x FF_BLOCK slots, used when WIDE
   __shared__ int s_qn;
   __shared__ int s_red[FF_BLOCK / 64];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int j = xcd_remap(blockIdx.x, nf);
+  // row = output row; source j = row * jstride (jstride 1 except when sampling)
+  const int row = xcd_remap(blockIdx.x, nrows);
+  const int j = row * jstride;
 
   for (int w = tid; w < nwords; w += FF_BLOCK) bits[w] = 0u;
   if (tid == 0) s_qn = 0;
@@ -146,8 +149,8 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
       if (v != v) nan = true;
       if (v != 0.0) keep |= 1u << b;
     }
-    vis[(size_t)j * nwords + w] = word;
-    mask[(size_t)j * nwords + w] = keep;
+    vis[(size_t)row * nwords + w] = word;
+    mask[(size_t)row * nwords + w] = keep;
     cnt += __popc(keep);
   }
   if (nan) atomicOr(status, 1);
@@ -157,7 +160,7 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
   if (tid == 0) {
     int s = 0;
     for (int w = 0; w < FF_BLOCK / 64; ++w) s += s_red[w];
-    counts[j] = s;
+    counts[row] = s;
   }
 }
 
@@ -225,33 +228,50 @@ __global__ __launch_bounds__(256) void ff_fill_kernel(
 }
 
 // Transposes a (rows_p x 32*nwords)-bit matrix stored as rows of nwords words
-// (rows_p == 32*nwords, both multiples of 512) in 512x512-bit tiles.
+// (rows_p == 32*nwords, both multiples of 512) in 512x512-bit tiles.  The
+// tile is staged through LDS for coalesced 64-byte row segments; each 32x32
+// sub-block (one word per lane, two sub-blocks per wavefront) is transposed in
+// registers by the 5-stage block-swap network (Warren, Hacker's Delight,
+// section 7-3) with cross-lane shuffles.
 __global__ __launch_bounds__(256) void bit_transpose_kernel(const uint32_t *__restrict__ in,
                                                             uint32_t *__restrict__ out,
                                                             int nwords) {
-  __shared__ uint32_t tin[512][17];
-  __shared__ uint32_t tout[512][17];
+  __shared__ uint32_t tile[512][17];
   const int tc = blockIdx.x, tr = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   for (int k = 0; k < 32; ++k) {
     const int idx = tid + 256 * k, r = idx >> 4, w = idx & 15;
-    tin[r][w] = in[(size_t)(512 * tr + r) * nwords + 16 * tc + w];
+    tile[r][w] = in[(size_t)(512 * tr + r) * nwords + 16 * tc + w];
   }
   __syncthreads();
-  for (int combo = wv; combo < 128; combo += 4) {
+  uint32_t res[32];
+#pragma unroll
+  for (int c = 0; c < 32; ++c) {
+    const int combo = wv * 32 + c;  // (input word column w, pair of row blocks qp)
     const int w = combo >> 3, qp = combo & 7;
     const int q = 2 * qp + (lane >> 5);
-    const uint32_t v = tin[32 * q + (lane & 31)][w];
-    for (int b = 0; b < 32; ++b) {
-      const uint64_t mb = __ballot((v >> b) & 1u);
-      if (lane == b) tout[32 * w + b][2 * qp] = (uint32_t)mb;
-      if (lane == 32 + b) tout[32 * w + b][2 * qp + 1] = (uint32_t)(mb >> 32);
+    uint32_t x = tile[32 * q + (lane & 31)][w];
+    uint32_t m = 0x0000FFFFu;
+#pragma unroll
+    for (int j = 16; j > 0; j >>= 1) {
+      const uint32_t p = __shfl_xor(x, j);
+      x = (lane & j) ? ((x & ~m) | ((p >> j) & m)) : ((x & m) | ((p & m) << j));
+      m = m ^ (m << (j >> 1));
     }
+    res[c] = x;  // lane l of the half holds column l of the sub-block
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 32; ++c) {
+    const int combo = wv * 32 + c;
+    const int w = combo >> 3, qp = combo & 7;
+    const int q = 2 * qp + (lane >> 5);
+    tile[32 * w + (lane & 31)][q] = res[c];
   }
   __syncthreads();
   for (int k = 0; k < 32; ++k) {
     const int idx = tid + 256 * k, o = idx >> 4, q = idx & 15;
-    out[(size_t)(512 * tc + o) * nwords + 16 * tr + q] = tout[o][q];
+    out[(size_t)(512 * tc + o) * nwords + 16 * tr + q] = tile[o][q];
   }
 }
 
@@ -320,7 +340,9 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   const int nwords = (int)(((nf + 511) / 512) * 16);
   const int64_t rows_p = (int64_t)nwords * 32;
   const bool wide = use_wide(bvh);
-  const size_t lds = (size_t)(nwords + FF_QCAP + (wide ? FF_STACK * FF_BLOCK : 0)) * 4;
+  // traversal stack: one slot per tree level (a push happens at most once per level)
+  const int stack_slots = wide ? bvh->h.depth + 1 : 0;
+  const size_t lds = (size_t)(nwords + FF_QCAP + stack_slots * FF_BLOCK) * 4;
   if (lds > 160 * 1024)
     throw Error(ILLUM_ERR_CAPACITY, "too many facets for the LDS visibility bitset (max ~1.2M)");
   const void *kfn = wide ? (const void *)ff_visibility_kernel<true>
@@ -351,7 +373,8 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
     launch<<<(int)nf, FF_BLOCK, lds, s>>>(
         (int)nf, nwords, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4,
         (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_wide, (const float4 *)bvh->d_tris,
-        (float)offset, mesh->d_area, d_alb.p, albedo, sp->d_vis, M.p, counts.p, status.p);
+        (float)offset, mesh->d_area, d_alb.p, albedo, sp->d_vis, M.p, counts.p, status.p,
+        (int)nf, 1);
     ILLUM_LAUNCH_CHECK();
   }
   // CSC
@@ -400,6 +423,50 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   }
   sync(ctx);
   return sp.release();
+}
+
+// Times ff_visibility on a strided sample of sources for one BVH (ms, from
+// hipEvents on the context stream).  Used by the leaf-size autotuner.
+double time_visibility_sample(illum_bvh *bvh, int max_sources) {
+  illum_mesh *mesh = bvh->mesh;
+  illum_ctx *ctx = mesh->ctx;
+  hipStream_t s = stream_of(ctx);
+  const int64_t nf = mesh->g.nf;
+  const int jstride = (int)std::max<int64_t>(1, nf / std::max(1, max_sources));
+  const int nrows = (int)((nf + jstride - 1) / jstride);
+  const int nwords = (int)(((nf + 511) / 512) * 16);
+  const bool wide = use_wide(bvh);
+  const int stack_slots = wide ? bvh->h.depth + 1 : 0;
+  const size_t lds = (size_t)(nwords + FF_QCAP + stack_slots * FF_BLOCK) * 4;
+  if (lds > 160 * 1024) throw Error(ILLUM_ERR_CAPACITY, "too many facets for the LDS bitset");
+  const void *kfn = wide ? (const void *)ff_visibility_kernel<true>
+                         : (const void *)ff_visibility_kernel<false>;
+  if (lds > 64 * 1024)
+    ILLUM_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  DBuf<uint32_t> vis((size_t)nrows * nwords), M((size_t)nrows * nwords);
+  DBuf<int32_t> counts(nrows);
+  DBuf<int> status(1);
+  hipEvent_t e0, e1;
+  ILLUM_HIP(hipEventCreate(&e0));
+  ILLUM_HIP(hipEventCreate(&e1));
+  auto launch = wide ? ff_visibility_kernel<true> : ff_visibility_kernel<false>;
+  float best = 1e30f;
+  for (int rep = 0; rep < 2; ++rep) {  // first launch also loads the code object
+    ILLUM_HIP(hipEventRecord(e0, s));
+    launch<<<nrows, FF_BLOCK, lds, s>>>(
+        (int)nf, nwords, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4,
+        (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_wide, (const float4 *)bvh->d_tris,
+        1e-5f, mesh->d_area, nullptr, 1.0, vis.p, M.p, counts.p, status.p, nrows, jstride);
+    ILLUM_LAUNCH_CHECK();
+    ILLUM_HIP(hipEventRecord(e1, s));
+    ILLUM_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    ILLUM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    best = std::min(best, ms);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return best;
 }
 
 }  // namespace illum

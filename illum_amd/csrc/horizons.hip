@@ -14,16 +14,22 @@
 #include "ray.hpp"
 
 #include <cmath>
+#include <cstdlib>
+#include <string>
 #include <vector>
 
 namespace illum {
 
+// WIDE: near-first any-hit over the two-wide layout with a per-lane stack in
+// dynamic LDS ((depth + 1) x 256 ints); otherwise the stackless threaded walk.
+template <bool WIDE>
 __global__ __launch_bounds__(256) void horizons_kernel(
     int64_t nf, int nphi, const float4 *__restrict__ pr4, const float4 *__restrict__ n4,
     const double *__restrict__ frame, const float4 *__restrict__ nodes,
-    const float4 *__restrict__ tris, float offset, const double *__restrict__ cphi,
-    const double *__restrict__ sphi, const double *__restrict__ th, int tree_nodes,
-    double *__restrict__ out) {
+    const float4 *__restrict__ wide, const float4 *__restrict__ tris, float offset,
+    const double *__restrict__ cphi, const double *__restrict__ sphi,
+    const double *__restrict__ th, int tree_nodes, double *__restrict__ out) {
+  extern __shared__ int hstack[];
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nf * nphi) return;
   const int64_t j = idx / nphi;
@@ -43,7 +49,8 @@ __global__ __launch_bounds__(256) void horizons_kernel(
     const float dy = (float)((F10 * vx + F11 * vy) + F12 * vz);
     const float dz = (float)((F20 * vx + F21 * vy) + F22 * vz);
     const DRay r = make_ray(ox, oy, oz, dx, dy, dz);
-    const bool hit = trace_any(nodes, tris, r);
+    const bool hit = WIDE ? trace_any_wide(wide, tris, r, hstack + threadIdx.x, 256)
+                          : trace_any(nodes, tris, r);
     m = 2 * m + (hit ? 0 : 1);
     if (m >= tree_nodes) { out[idx] = __builtin_nan(""); return; }
     if (th[4 * m + 3] != 0.0) break;
@@ -123,10 +130,15 @@ void horizons(illum_bvh *bvh, illum_bvh *occl, int nphi, double theta_eps, doubl
   if (!(flags & ILLUM_OUT_ON_DEVICE)) { d_out.reset(total); dst = d_out.p; }
   {
     StageTimer st(ctx, "horizons");
-    horizons_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(
+    const char *e = getenv("ILLUM_HZ_TRAVERSAL");
+    const bool wide = !(e && std::string(e) == "threaded") && target->d_wide &&
+                      target->h.depth <= 32;
+    const size_t lds = wide ? (size_t)(target->h.depth + 1) * 256 * 4 : 0;
+    auto kern = wide ? horizons_kernel<true> : horizons_kernel<false>;
+    kern<<<(unsigned)((total + 255) / 256), 256, lds, s>>>(
         nf, nphi, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4, mesh->d_frame,
-        (const float4 *)target->d_nodes, (const float4 *)target->d_tris, (float)offset, d_c.p,
-        d_s.p, d_tab.p, tree_nodes, dst);
+        (const float4 *)target->d_nodes, (const float4 *)target->d_wide,
+        (const float4 *)target->d_tris, (float)offset, d_c.p, d_s.p, d_tab.p, tree_nodes, dst);
     ILLUM_LAUNCH_CHECK();
   }
   if (!(flags & ILLUM_OUT_ON_DEVICE))

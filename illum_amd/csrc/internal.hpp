@@ -102,6 +102,10 @@ struct illum_bvh {
   float *d_nodes = nullptr;
   float *d_tris = nullptr;
   float *d_wide = nullptr;
+  // leaf-size autotuning record (illum_bvh_build with leaf_size 0)
+  int ntuned = 0;
+  int tune_leaf[8] = {0};
+  double tune_ms[8] = {0};
 };
 
 struct illum_sparse {

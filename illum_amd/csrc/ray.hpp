@@ -209,6 +209,48 @@ __device__ __forceinline__ int trace_closest_wide(const float4 *__restrict__ wid
   return best;
 }
 
+// Any hit (t > 0) over the two-wide layout, nearer child first (stack as in
+// trace_closest_wide).
+__device__ __forceinline__ bool trace_any_wide(const float4 *__restrict__ wide,
+                                               const float4 *__restrict__ tris, const DRay &r,
+                                               int *stack, int stride) {
+  const float inf = __builtin_huge_valf();
+  int sp = 0;
+  int cur = 0;
+  for (;;) {
+    if (cur >= 0) {
+      const float4 a0 = wide[4 * cur], a1 = wide[4 * cur + 1];
+      const float4 b0 = wide[4 * cur + 2], b1 = wide[4 * cur + 3];
+      float tl, tr;
+      const bool hl = box_enter(r, a0, a1, inf, &tl);
+      const bool hr = box_enter(r, b0, b1, inf, &tr);
+      const int cl = __float_as_int(a0.w), cr = __float_as_int(b0.w);
+      if (hl && hr) {
+        const bool lnear = tl <= tr;
+        stack[sp * stride] = lnear ? cr : cl;
+        ++sp;
+        cur = lnear ? cl : cr;
+        continue;
+      }
+      if (hl) { cur = cl; continue; }
+      if (hr) { cur = cr; continue; }
+    } else {
+      const int info = ~cur;
+      const int first = info & 0xFFFFFF, cnt = info >> 24;
+      for (int k = 0; k < cnt; ++k) {
+        float t;
+        if (tri_hit(r, tris[3 * (first + k)], tris[3 * (first + k) + 1],
+                    tris[3 * (first + k) + 2], &t))
+          return true;
+      }
+    }
+    if (sp == 0) break;
+    --sp;
+    cur = stack[sp * stride];
+  }
+  return false;
+}
+
 // Any hit (t > 0) over the threaded BVH.
 __device__ __forceinline__ bool trace_any(const float4 *__restrict__ nodes,
                                           const float4 *__restrict__ tris, const DRay &r) {

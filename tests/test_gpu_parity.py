@@ -88,6 +88,34 @@ def test_form_factors_vs_oracle(api, subdiv, ncraters, albedo):
     assert Fm.nnz > 0
 
 
+def test_autotuned_bvh_and_traversal_choice_do_not_change_results(api, monkeypatch):
+    # Visibility is a property of the triangles, not of the tree: the autotuned
+    # leaf size, leaf size 1, and the stackless threaded walk must all agree.
+    V, F = _mesh(api, 4, 40, seed=8)
+    ref = api.IllumContext(V=V, F=F, leaf_size=4).compute_F(0.1)
+    tuned = api.IllumContext(V=V, F=F, leaf_size=0)
+    trials = tuned.bvh.tuning()
+    assert [l for l, _ in trials] == [2, 3, 4, 6, 8] and all(ms > 0 for _, ms in trials)
+    assert tuned.bvh.info()["leaf_size"] == min(trials, key=lambda x: x[1])[0]
+    out = [tuned.compute_F(0.1), api.IllumContext(V=V, F=F, leaf_size=1).compute_F(0.1)]
+    monkeypatch.setenv("ILLUM_FF_TRAVERSAL", "threaded")
+    out.append(api.IllumContext(V=V, F=F, leaf_size=4).compute_F(0.1))
+    r = ref.csc()
+    for m in out:
+        assert np.array_equal(m.visibility(), ref.visibility())
+        c = m.csc()
+        assert all(np.array_equal(a, b) for a, b in zip(c, r))
+
+
+def test_horizon_traversal_choice_does_not_change_results(api, monkeypatch):
+    V, F = _mesh(api, 3, 40, seed=9)
+    ctx = api.IllumContext(V=V, F=F)
+    a = ctx.make_horizons(nphi=33).copy()
+    monkeypatch.setenv("ILLUM_HZ_TRAVERSAL", "threaded")
+    b = ctx.make_horizons(nphi=33)
+    assert np.array_equal(a, b)
+
+
 def test_form_factors_albedo_vector(api):
     V, F = _mesh(api, 3, 40, seed=5)
     ctx = api.IllumContext(V=V, F=F)
