@@ -280,6 +280,8 @@ def main():
     ap.add_argument("--leaf-size", type=int, default=0, help="BVH leaf size (0 = autotune)")
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--thermal-steps", type=int, default=4096)
+    ap.add_argument("--thermal-only", action="store_true",
+                    help="only run the conduction leg and print its record")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run one device-to-device copy of this many bytes (a known-size "
                          "kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE) and exit")
@@ -287,7 +289,8 @@ def main():
     if args.copy_bytes:
         from illum_amd import api as A
         c = A.default_context()
-        src = A.DeviceArray(c, (args.copy_bytes,), np.uint8)
+        # source written first (from the host) so every page holds data
+        src = A.DeviceArray.from_host(c, np.ones(args.copy_bytes, np.uint8))
         dst = A.DeviceArray(c, (args.copy_bytes,), np.uint8)
         A._lib.call("illum_dev_copy", c.h, dst.ptr, src.ptr, args.copy_bytes, 2)
         c.synchronize()
@@ -299,6 +302,9 @@ def main():
         sys.exit(2)
 
     from illum_amd import api
+    if args.thermal_only:
+        print(json.dumps({"thermal": bench_thermal(api, nsteps=args.thermal_steps)}), flush=True)
+        return
     offset = 1e-5
     t0 = time.perf_counter()
     V, F = api.synth_ellipsoid(args.subdiv, ncraters=args.craters, seed=0)
@@ -338,15 +344,26 @@ def main():
 
     vis_ms, vis_n = prof.get("ff_visibility", (float("nan"), 1))
     vis_ms /= max(1, vis_n)
-    # Algorithmic bytes of ff_visibility per launch: every candidate pair
-    # streams one 16-byte (centroid, radius) record; every source writes its
-    # visibility row and its nonzero row (2 x N/8 bytes).
+    rays = Fm.rays()
+    # Algorithmic HBM bytes of ff_visibility per launch: read the facet records
+    # (centroid+radius, normal: 32 B) and areas (8 B) once, write each
+    # source's visibility row and nonzero row (2 x W words of 4 B).  The
+    # 16 B-per-pair receiver stream of the facing cull (pairs x 16 B) is the
+    # same data re-read from L2/MALL for every source, not HBM traffic.
     words = ((nf + 511) // 512) * 16
-    vis_bytes = pairs * 16 + nf * words * 4 * 2
+    vis_bytes = nf * 40 + nf * words * 4 * 2
     achieved = vis_bytes / (vis_ms * 1e-3) / 1e9
+    traffic = None
+    pmc_path = os.path.join(ROOT, "profiles", f"ff_visibility_{nf}_pmc.json")
+    if os.path.exists(pmc_path):
+        pmc = json.load(open(pmc_path))
+        traffic = (pmc["fetch_kib"] * pmc["fetch_scale"] + pmc["write_kib"] * pmc["write_scale"]) * 1024
     roofline = {"bound": "hbm", "kernel": "ff_visibility", "achieved": achieved,
                 "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                "traffic": None, "avg_ms": vis_ms}
+                "traffic": traffic, "avg_ms": vis_ms,
+                "note": "latency/VALU-bound ray traversal: HBM is not the limit",
+                "rays": rays, "rays_per_s": rays / (vis_ms * 1e-3),
+                "l2_stream_GBs": pairs * 16 / (vis_ms * 1e-3) / 1e9}
 
     secondary = {}
     if not args.quick:

@@ -67,6 +67,7 @@ SIGNATURES = {
     "illum_sparse_copy_csr": [_P, _P, _P, _P],
     "illum_sparse_visibility_words": [_P, _P],
     "illum_sparse_copy_visibility": [_P, _P],
+    "illum_sparse_rays": [_P, _P],
     "illum_horizons": [_P, _P, _I, _D, _D, _P, _I],
     "illum_fib_spiral": [_P, _I],
     "illum_direct_radiosity": [_P, _P, _I, _P, _I, _P, _I, _P, _I],

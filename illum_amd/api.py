@@ -290,6 +290,12 @@ class SparseMatrix:
         call("illum_sparse_copy_visibility", self.h, _ptr(bits))
         return bits
 
+    def rays(self):
+        """Visibility rays traced when the matrix was assembled."""
+        n = C.c_int64()
+        call("illum_sparse_rays", self.h, C.byref(n))
+        return n.value
+
     def to_scipy(self):
         import scipy.sparse as sps
         cp, ri, v = self.csc()

@@ -114,13 +114,13 @@ static void destroy_bvh(illum_bvh *b) {
 void destroy_sparse(illum_sparse *s) {
   dfree(s->d_colptr); dfree(s->d_rowidx); dfree(s->d_cval);
   dfree(s->d_rowptr); dfree(s->d_colidx); dfree(s->d_rval); dfree(s->d_rval32);
-  dfree(s->d_vis);
+  dfree(s->d_vis); dfree(s->d_nrays);
   delete s;
 }
 void destroy_thermal(illum_thermal *t) {
   dfree(t->d_nz); dfree(t->d_grid); dfree(t->d_s2); dfree(t->d_goff); dfree(t->d_T);
   dfree(t->d_Qprev); dfree(t->d_F); dfree(t->d_work); dfree(t->d_work32); dfree(t->d_Qdev);
-  dfree(t->d_status);
+  dfree(t->d_status); dfree(t->d_fidx); dfree(t->d_pos); dfree(t->d_wgrid);
   thermal_free_impl(t);
   delete t;
 }
@@ -514,6 +514,16 @@ int illum_sparse_visibility_words(const illum_sparse *sp, int64_t *words_per_row
     CHECK_ARG(sp && words_per_row, "null argument");
     if (!sp->d_vis) throw Error(ILLUM_ERR_ARG, "matrix carries no visibility rows");
     *words_per_row = sp->vis_words;
+  });
+}
+
+int illum_sparse_rays(const illum_sparse *sp, int64_t *rays) {
+  return guarded([&] {
+    CHECK_ARG(sp && rays, "null argument");
+    if (!sp->d_nrays) throw Error(ILLUM_ERR_ARG, "matrix was not assembled by illum_form_factors");
+    unsigned long long n = 0;
+    ILLUM_HIP(hipMemcpy(&n, sp->d_nrays, 8, hipMemcpyDeviceToHost));
+    *rays = (int64_t)n;
   });
 }
 

@@ -67,12 +67,12 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
     const float4 *__restrict__ tris, float offset,
     const double *__restrict__ area, const double *__restrict__ albedo_vec, double albedo,
     uint32_t *__restrict__ vis, uint32_t *__restrict__ mask, int32_t *__restrict__ counts,
-    int *__restrict__ status, int nrows, int jstride) {
+    int *__restrict__ status, int nrows, int jstride, unsigned long long *__restrict__ nrays) {
   extern __shared__ uint32_t smem[];
   uint32_t *bits = smem;
   int *queue = (int *)(smem + nwords);
   int *stack = queue + FF_QCAP;  // (tree depth + 1) x FF_BLOCK slots, used when WIDE
-  __shared__ int s_qn;
+  __shared__ int s_qn, s_rays;
   __shared__ int s_red[FF_BLOCK / 64];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   // row = output row; source j = row * jstride (jstride 1 except when sampling)
@@ -80,7 +80,7 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
   const int j = row * jstride;
 
   for (int w = tid; w < nwords; w += FF_BLOCK) bits[w] = 0u;
-  if (tid == 0) s_qn = 0;
+  if (tid == 0) { s_qn = 0; s_rays = 0; }
   const float4 pj = pr4[j], nj = n4[j];
   const float qx = pj.x + offset * nj.x, qy = pj.y + offset * nj.y, qz = pj.z + offset * nj.z;
   __syncthreads();
@@ -100,7 +100,10 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
       const uint64_t m = __ballot(s);
       if (m) {
         int slot = 0;
-        if (lane == 0) slot = atomicAdd(&s_qn, __popcll(m));
+        if (lane == 0) {
+          slot = atomicAdd(&s_qn, __popcll(m));
+          atomicAdd(&s_rays, __popcll(m));
+        }
         slot = __shfl(slot, 0);
         if (s) queue[slot + __popcll(m & lanemask_lt())] = i;
       }
@@ -161,6 +164,7 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
     int s = 0;
     for (int w = 0; w < FF_BLOCK / 64; ++w) s += s_red[w];
     counts[row] = s;
+    if (nrays) atomicAdd(nrays, (unsigned long long)s_rays);
   }
 }
 
@@ -366,6 +370,8 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   DBuf<int> status(1);
   ILLUM_HIP(hipMemsetAsync(M.p, 0, (size_t)rows_p * nwords * 4, s));
   ILLUM_HIP(hipMemsetAsync(status.p, 0, 4, s));
+  sp->d_nrays = dalloc<unsigned long long>(1);
+  ILLUM_HIP(hipMemsetAsync(sp->d_nrays, 0, 8, s));
 
   {
     StageTimer st(ctx, "ff_visibility");
@@ -374,7 +380,7 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
         (int)nf, nwords, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4,
         (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_wide, (const float4 *)bvh->d_tris,
         (float)offset, mesh->d_area, d_alb.p, albedo, sp->d_vis, M.p, counts.p, status.p,
-        (int)nf, 1);
+        (int)nf, 1, sp->d_nrays);
     ILLUM_LAUNCH_CHECK();
   }
   // CSC
@@ -456,7 +462,7 @@ double time_visibility_sample(illum_bvh *bvh, int max_sources) {
     launch<<<nrows, FF_BLOCK, lds, s>>>(
         (int)nf, nwords, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4,
         (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_wide, (const float4 *)bvh->d_tris,
-        1e-5f, mesh->d_area, nullptr, 1.0, vis.p, M.p, counts.p, status.p, nrows, jstride);
+        1e-5f, mesh->d_area, nullptr, 1.0, vis.p, M.p, counts.p, status.p, nrows, jstride, nullptr);
     ILLUM_LAUNCH_CHECK();
     ILLUM_HIP(hipEventRecord(e1, s));
     ILLUM_HIP(hipEventSynchronize(e1));

@@ -121,12 +121,21 @@ struct illum_sparse {
   float *d_rval32 = nullptr;
   int64_t vis_words = 0;
   uint32_t *d_vis = nullptr;   // raw visibility rows (union of first hits)
+  unsigned long long *d_nrays = nullptr;  // rays cast by the visibility kernel
 };
 
 struct illum_thermal {
   illum_ctx *ctx = nullptr;
   int64_t nf = 0;
-  int64_t nfp = 0;             // padded facet stride of the layer-major arrays
+  // Facets are permuted into "slots": each 64-slot wavefront holds facets of
+  // one layer grid (padding slots hold -1); per-layer state is stored
+  // slot-major, T[l * nslot + slot].
+  int64_t nslot = 0;
+  int32_t *d_fidx = nullptr;   // slot -> facet (or -1)
+  int32_t *d_pos = nullptr;    // facet -> slot
+  int32_t *d_wgrid = nullptr;  // wavefront -> grid
+  struct Launch { int nz_bucket; int64_t block0, nblocks; };
+  std::vector<Launch> launches;  // per layer-count bucket (0 = streaming kernel)
   int nzmax = 0;
   int precision = ILLUM_THERM_FP64;
   int ngrid = 0;
@@ -136,11 +145,11 @@ struct illum_thermal {
   double *d_s2 = nullptr;      // per facet ti_scale^2
   void *impl = nullptr;        // host tables + per-dt device tables (thermal.hip)
   int64_t *d_goff = nullptr;
-  double *d_T = nullptr;       // (nzmax+1) x nfp layer-major, row 0 = surface
+  double *d_T = nullptr;       // (nzmax+1) x nslot, row 0 = surface
   double *d_Qprev = nullptr;   // nf
   double *d_F = nullptr;       // nf surface flux
-  double *d_work = nullptr;    // scratch (nzmax x nfp)
-  float *d_work32 = nullptr;   // scratch (2 nzmax x nfp)
+  double *d_work = nullptr;    // streaming-kernel scratch ((nzmax+1) x nslot), if needed
+  float *d_work32 = nullptr;   // streaming-kernel scratch (2 (nzmax+1) x nslot), if needed
   double *d_Qdev = nullptr;    // staging for host Q
   int64_t qdev_cap = 0;
   int *d_status = nullptr;

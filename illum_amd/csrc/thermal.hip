@@ -3,119 +3,283 @@
 // tridiagonal solve per tridag.for:17-27, i.e. the Thomas algorithm; see
 // Golub & Van Loan, Matrix Computations, section 4.3).
 //
-// Layout: facets are independent; one lane owns one facet for all steps of a
-// launch.  Temperatures live in HBM layer-major, T[l * nfp + f] (l = 0 is the
-// surface, l = 1..nz the layers), so a wavefront touches 64 consecutive
-// facets of one layer per access (fully coalesced).  Ragged layer counts use
-// the same stride; a facet with fewer layers simply stops early.
+// Layout.  Facets are independent.  They are permuted into slots so that each
+// 64-slot wavefront holds facets of a single layer grid (each grid's list is
+// padded to a multiple of 64 with empty slots); per-layer state is slot-major,
+// T[l * nslot + slot] (l = 0 surface, 1..nz layers), so state loads and
+// stores are coalesced and a wavefront's grid coefficients are uniform.
 //
-// Coefficients: the time-invariant tridiagonal entries of each layer grid are
-// tabulated on the host in conductionQ's own operation order for the current
-// dt; a facet's thermal-inertia scale s multiplies every conductivity by s^2.
+// Engines.
+//   * cn_reg_kernel<NZ>: grids with at most NZ layers (NZ in 32/64/96/128).
+//     A lane keeps its whole temperature column in registers for every step
+//     of the launch; the wavefront's alpha/gamma table sits in LDS; the Thomas
+//     multipliers go through a wavefront-private scratch block (written in the
+//     forward sweep, read back in the backward sweep).  ILLUM_CN_ENGINE=stream
+//     forces the streaming kernel for comparison.
+//   * cn_stream_kernel: any depth (<= 1000 layers, conductionQ.f:38 NMAX);
+//     the column streams through HBM each step (coalesced across the
+//     wavefront).
 //
-// FP64 mode: the reference step verbatim -- right-hand side, Thomas sweep
-// with one division per row, surface temperature and flux, all in FP64.
-// Mixed mode: iterative refinement from the previous state.  The FP64
-// residual rho = r - A T^n is formed in the forward pass, the correction
-// A d = rho is solved by an FP32 Thomas sweep (hardware reciprocal), and
-// T^{n+1} = T^n + d in FP64.  Since |d| is the per-step temperature change,
-// the FP32 error is relative to that change rather than to T.  The radiative
-// surface linearisation and surface flux stay FP64.
+// Coefficients: the time-invariant tridiagonal entries of each grid are
+// tabulated on the host in conductionQ's operation order for the current dt; a
+// facet's thermal-inertia scale s multiplies every conductivity by s^2.
+//
+// FP64 mode: the reference step verbatim -- right-hand side, Thomas sweep with
+// one division per row, surface temperature and flux, all in FP64.
+// Mixed mode: iterative refinement from the previous state.  The FP64 residual
+// rho = r - A T^n is formed in the forward pass, A d = rho is solved by an
+// FP32 Thomas sweep (hardware reciprocal), and T^{n+1} = T^n + d in FP64.
+// |d| is the per-step temperature change, so the FP32 error scales with the
+// change, not with T.  The radiative surface linearisation and the surface
+// flux stay FP64.
 #include "hip_util.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <string>
+#include <type_traits>
 #include <vector>
 
 namespace illum {
 
 constexpr double SIGMA_SB = 5.6704e-8;  // conductionQ.f:39
 
-// Per-grid table entry for layer i (0-based): alpha, gamma.
 // Per-grid header (6 doubles): k1 = k(1)/dz, k(1), z(1), dt/rhoc(nz),
-// z(nz) - z(nz-1), nz.
+// z(nz) - z(nz-1), nz.  Per-layer table: interleaved alpha, gamma.
 struct GridHdr {
   double k1, kfirst, z1, geo, dzlast, nz;
 };
 
+// Linearised radiative boundary at the old surface temperature
+// (conductionQ.f:76-82): returns ann, annp1, bn.
+__device__ __forceinline__ void surface_bc(double Tsurf, double Qn, double Qnp1, double k1,
+                                           double emiss, double *ann, double *annp1,
+                                           double *bn) {
+  const double Tr = Tsurf;
+  const double Tr3 = (Tr * Tr) * Tr, Tr4 = Tr3 * Tr;  // Tr**3, Tr**4 left to right
+  const double arad = ((-3.0 * emiss) * SIGMA_SB) * Tr4;
+  const double brad = ((2.0 * emiss) * SIGMA_SB) * Tr3;
+  const double den = k1 + brad;
+  *ann = (Qn - arad) / den;
+  *annp1 = (Qnp1 - arad) / den;
+  *bn = (k1 - brad) / den;
+}
+
+// ---------------------------------------------------------------------------
+// Register-resident engine: the temperature column of a lane stays in
+// registers for all steps of the launch; the Thomas multipliers (and in mixed
+// mode the FP32 forward values) go to slot-major scratch rows, written in the
+// forward sweep and read back in the backward sweep (fire-and-forget stores,
+// independent loads the compiler can issue early).
+template <int NZ, bool MIXED>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void cn_reg_kernel(
+    const int32_t *__restrict__ fidx, const int32_t *__restrict__ wgrid, int64_t block0,
+    int64_t nf, int64_t nslot, const double *__restrict__ s2f, const double *__restrict__ hdr,
+    const double *__restrict__ coef, const int64_t *__restrict__ goff, double emiss, double Fgeo,
+    int nsteps, const double *__restrict__ Q, const double *__restrict__ Qprev,
+    double *__restrict__ T, double *__restrict__ Fs, double *__restrict__ G,
+    float *__restrict__ G32, double *__restrict__ Tsurf_series, int *__restrict__ status,
+    int rows) {
+  __shared__ double s_al[NZ], s_ga[NZ];
+  const int64_t blk = block0 + blockIdx.x;
+  const int64_t slot = blk * 64 + threadIdx.x;
+  const int f = fidx[slot];
+  const bool act = f >= 0;
+  const int g = wgrid[blk];
+  const GridHdr H = *(const GridHdr *)(hdr + 6 * g);
+  const int nz = (int)H.nz;  // uniform across the wavefront, <= NZ
+  const double *al = coef + 2 * goff[g];
+  for (int j = threadIdx.x; j < nz; j += 64) {
+    s_al[j] = al[2 * j];
+    s_ga[j] = al[2 * j + 1];
+  }
+  __syncthreads();
+  const double s2 = (act && s2f) ? s2f[f] : 1.0;
+  const double k1 = s2 * H.k1, kfirst = s2 * H.kfirst;
+  const double geo_term = (H.geo * Fgeo) / H.dzlast;
+  // Wavefront-private scratch block, element [row][lane]: the per-layer offset
+  // is a compile-time immediate on one base address.
+  double *Gw = G + (MIXED ? 0 : blk * (int64_t)rows * 64 + threadIdx.x);
+  float *Hw = G32 + (MIXED ? blk * (int64_t)(2 * rows) * 64 + threadIdx.x : 0);
+
+  double Tl[NZ + 2];  // Tl[0] surface, Tl[1..nz] layers, one spare slot
+#pragma unroll
+  for (int l = 0; l < NZ + 2; ++l) Tl[l] = (act && l <= nz) ? T[(int64_t)l * nslot + slot] : 0.0;
+  double Qn = act ? Qprev[f] : 0.0;
+  double Fsurf = act ? Fs[f] : 0.0;
+  bool zero_pivot = false;
+
+  for (int step = 0; step < nsteps; ++step) {
+    const double Qnp1 = act ? Q[(int64_t)step * nf + f] : 0.0;
+    double ann, annp1, bn;
+    surface_bc(Tl[0], Qn, Qnp1, k1, emiss, &ann, &annp1, &bn);
+    const double a1 = s2 * s_al[0], g1 = s2 * s_ga[0];
+    const double b1 = ((1.0 + a1) + g1) - g1 * bn;
+    if (b1 == 0.0) zero_pivot = true;
+    double bet = 0.0, uprev = 0.0, cprev = 0.0, Told = 0.0;
+    float bet32 = 0.0f, up32 = 0.0f, cprev32 = 0.0f;
+#pragma unroll
+    for (int j = 1; j <= NZ; ++j) {
+      if (j <= nz) {
+        const double Tc = Tl[j];
+        const double Tn = j < nz ? Tl[j + 1] : 0.0;
+        const double aj = j == 1 ? a1 : s2 * s_al[j - 1];
+        const double gj = j == 1 ? g1 : s2 * s_ga[j - 1];
+        double r, bj;
+        if (j == 1) {
+          r = (g1 * (annp1 + ann) + (((1.0 - a1) - g1) + g1 * bn) * Tc) + a1 * Tn;
+          bj = b1;
+        } else if (j < nz) {
+          r = (gj * Told + ((1.0 - aj) - gj) * Tc) + aj * Tn;
+          bj = (1.0 + aj) + gj;
+        } else {
+          r = (gj * Told + (1.0 - gj) * Tc) + geo_term;
+          bj = 1.0 + gj;
+        }
+        if (!MIXED) {
+          double u;
+          if (j == 1) {
+            bet = bj;
+            u = r / bet;
+          } else {
+            const double a_j = -gj;
+            const double gm = cprev / bet;
+            Gw[j * 64] = gm;
+            bet = bj - a_j * gm;
+            u = (r - a_j * uprev) / bet;
+          }
+          Tl[j] = u;  // old T_j is kept in Told for the next row
+          uprev = u;
+        } else {
+          double Arow;
+          if (j == 1) Arow = bj * Tc + (-a1) * Tn;
+          else if (j < nz) Arow = ((-gj) * Told + bj * Tc) + (-aj) * Tn;
+          else Arow = (-gj) * Told + bj * Tc;
+          const float rho = (float)(r - Arow);
+          float u;
+          if (j == 1) {
+            bet32 = (float)bj;
+            u = rho * __builtin_amdgcn_rcpf(bet32);
+          } else {
+            const float a_j = (float)(-gj);
+            const float gm = cprev32 * __builtin_amdgcn_rcpf(bet32);
+            Hw[(2 * j) * 64] = gm;
+            bet32 = (float)bj - a_j * gm;
+            u = (rho - a_j * up32) * __builtin_amdgcn_rcpf(bet32);
+          }
+          Hw[(2 * j + 1) * 64] = u;
+          up32 = u;
+          cprev32 = (float)(-aj);
+        }
+        cprev = -aj;
+        Told = Tc;
+      }
+      // Keep the scheduler from hoisting loads across groups of 8 layers,
+      // which bounds register pressure (the column itself needs 2 VGPRs per layer).
+      if ((j & 7) == 0) __builtin_amdgcn_sched_barrier(0);
+    }
+    if (!MIXED) {
+#pragma unroll
+      for (int j = NZ - 1; j >= 1; --j) {
+        if (j < nz) Tl[j] = Tl[j] - Gw[(j + 1) * 64] * Tl[j + 1];
+        if ((j & 7) == 0) __builtin_amdgcn_sched_barrier(0);
+      }
+    } else {
+      float dnext = 0.0f;
+#pragma unroll
+      for (int j = NZ; j >= 1; --j) {
+        if (j <= nz) {
+          const float u = Hw[(2 * j + 1) * 64];
+          const float d = j == nz ? u : u - Hw[(2 * (j + 1)) * 64] * dnext;
+          Tl[j] = Tl[j] + (double)d;
+          dnext = d;
+        }
+        if ((j & 7) == 0) __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    const double T1 = Tl[1];
+    const double Tsurf = 0.5 * ((annp1 + bn * T1) + T1);
+    Fsurf = ((-kfirst) * (T1 - Tsurf)) / H.z1;
+    Tl[0] = Tsurf;
+    if (Tsurf_series && act) Tsurf_series[(int64_t)step * nf + f] = Tsurf;
+    Qn = Qnp1;
+  }
+  if (act) {
+#pragma unroll
+    for (int l = 0; l < NZ + 2; ++l)
+      if (l <= nz) T[(int64_t)l * nslot + slot] = Tl[l];
+    Fs[f] = Fsurf;
+    if (zero_pivot) atomicOr(status, 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Streaming engine (any depth): the column lives in HBM; scratch rows hold
+// the Thomas multipliers (FP64) or the FP32 multipliers and forward values.
 template <bool MIXED>
-__global__ __launch_bounds__(256) void cn_kernel(
-    int64_t nf, int64_t nfp, const int32_t *__restrict__ nzf, const int32_t *__restrict__ gridf,
-    const double *__restrict__ s2f, const double *__restrict__ hdr,
-    const double *__restrict__ coef, const int64_t *__restrict__ goff, double emiss,
-    double Fgeo, int nsteps, const double *__restrict__ Q, const double *__restrict__ Qprev,
+__global__ __launch_bounds__(64) void cn_stream_kernel(
+    const int32_t *__restrict__ fidx, const int32_t *__restrict__ wgrid, int64_t block0,
+    int64_t nf, int64_t nslot, const double *__restrict__ s2f, const double *__restrict__ hdr,
+    const double *__restrict__ coef, const int64_t *__restrict__ goff, double emiss, double Fgeo,
+    int nsteps, const double *__restrict__ Q, const double *__restrict__ Qprev,
     double *__restrict__ T, double *__restrict__ Fs, double *__restrict__ G,
     float *__restrict__ G32, double *__restrict__ Tsurf_series, int *__restrict__ status) {
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= nf) return;
-  const int nz = nzf[f];
-  const int g = gridf ? gridf[f] : 0;
-  const double s2 = s2f ? s2f[f] : 1.0;
+  const int64_t blk = block0 + blockIdx.x;
+  const int64_t slot = blk * 64 + threadIdx.x;
+  const int f = fidx[slot];
+  if (f < 0) return;
+  const int g = wgrid[blk];
   const GridHdr H = *(const GridHdr *)(hdr + 6 * g);
-  const double *al = coef + 2 * goff[g];  // interleaved alpha, gamma
+  const int nz = (int)H.nz;
+  const double *al = coef + 2 * goff[g];
+  const double s2 = s2f ? s2f[f] : 1.0;
   const double k1 = s2 * H.k1, kfirst = s2 * H.kfirst;
-  const double geo_term = ((H.geo) * Fgeo) / H.dzlast;
-  float *gam32 = G32;  // row 2j: FP32 multiplier gam_j, row 2j+1: FP32 forward value u_j
-  double Tsurf = T[f];
+  const double geo_term = (H.geo * Fgeo) / H.dzlast;
+  auto Tat = [&](int l) -> double & { return T[(int64_t)l * nslot + slot]; };
   double Qn = Qprev[f];
   double Fsurf = Fs[f];
   for (int step = 0; step < nsteps; ++step) {
     const double Qnp1 = Q[(int64_t)step * nf + f];
-    // Radiative boundary linearised at the old surface temperature.
-    const double Tr = Tsurf;
-    const double Tr3 = (Tr * Tr) * Tr, Tr4 = Tr3 * Tr;  // Tr**3, Tr**4 left to right
-    const double arad = ((-3.0 * emiss) * SIGMA_SB) * Tr4;
-    const double brad = ((2.0 * emiss) * SIGMA_SB) * Tr3;
-    const double den = k1 + brad;
-    const double ann = (Qn - arad) / den;
-    const double annp1 = (Qnp1 - arad) / den;
-    const double bn = (k1 - brad) / den;
+    double ann, annp1, bn;
+    surface_bc(Tat(0), Qn, Qnp1, k1, emiss, &ann, &annp1, &bn);
     const double a1 = s2 * al[0], g1 = s2 * al[1];
     const double b1 = ((1.0 + a1) + g1) - g1 * bn;
     if (b1 == 0.0) { atomicOr(status, 1); return; }
-
-    // Forward sweep over layers 1..nz, right-hand side formed on the fly
-    // from the old temperatures.
-    double Tm = 0.0;                 // old T_{j-1}
-    double Tc = T[1 * nfp + f];      // old T_j
-    double aj = a1, gj = g1;         // alpha_j, gamma_j
-    double bj = b1;
-    double cprev = 0.0;              // c_{j-1} = -alpha_{j-1}
-    double bet = 0.0;
-    double uprev = 0.0;
+    double Told = 0.0, Tc = Tat(1);
+    double cprev = 0.0, bet = 0.0, uprev = 0.0;
     float bet32 = 0.0f, up32 = 0.0f;
     for (int j = 1; j <= nz; ++j) {
-      const double Tn = j < nz ? T[(int64_t)(j + 1) * nfp + f] : 0.0;
-      double r, Arow;
+      const double Tn = j < nz ? Tat(j + 1) : 0.0;
+      const double aj = j == 1 ? a1 : s2 * al[2 * (j - 1)];
+      const double gj = j == 1 ? g1 : s2 * al[2 * (j - 1) + 1];
+      double r, Arow, bj;
       if (j == 1) {
         r = (g1 * (annp1 + ann) + (((1.0 - a1) - g1) + g1 * bn) * Tc) + a1 * Tn;
-        Arow = b1 * Tc + (-a1) * Tn;
+        bj = b1;
+        Arow = bj * Tc + (-a1) * Tn;
+      } else if (j < nz) {
+        r = (gj * Told + ((1.0 - aj) - gj) * Tc) + aj * Tn;
+        bj = (1.0 + aj) + gj;
+        Arow = ((-gj) * Told + bj * Tc) + (-aj) * Tn;
       } else {
-        aj = s2 * al[2 * (j - 1)];
-        gj = s2 * al[2 * (j - 1) + 1];
-        if (j < nz) {
-          r = (gj * Tm + ((1.0 - aj) - gj) * Tc) + aj * Tn;
-          bj = (1.0 + aj) + gj;
-          Arow = ((-gj) * Tm + bj * Tc) + (-aj) * Tn;
-        } else {
-          r = (gj * Tm + (1.0 - gj) * Tc) + geo_term;
-          bj = 1.0 + gj;
-          Arow = (-gj) * Tm + bj * Tc;
-        }
+        r = (gj * Told + (1.0 - gj) * Tc) + geo_term;
+        bj = 1.0 + gj;
+        Arow = (-gj) * Told + bj * Tc;
       }
-      const double a_j = -gj;  // sub-diagonal (unused for j == 1)
+      const double a_j = -gj;
       if (!MIXED) {
         double u;
         if (j == 1) {
           bet = bj;
           u = r / bet;
         } else {
-          const double gam = cprev / bet;
-          G[(int64_t)j * nfp + f] = gam;
-          bet = bj - a_j * gam;
+          const double gm = cprev / bet;
+          G[(int64_t)j * nslot + slot] = gm;
+          bet = bj - a_j * gm;
           u = (r - a_j * uprev) / bet;
         }
-        T[(int64_t)j * nfp + f] = u;  // old T_j is held in Tc
+        Tat(j) = u;  // old T_j is held in Tc
         uprev = u;
       } else {
         const float rho = (float)(r - Arow);
@@ -124,42 +288,41 @@ __global__ __launch_bounds__(256) void cn_kernel(
           bet32 = (float)bj;
           u = rho * __builtin_amdgcn_rcpf(bet32);
         } else {
-          const float gam = (float)cprev * __builtin_amdgcn_rcpf(bet32);
-          gam32[(int64_t)(2 * j) * nfp + f] = gam;
-          bet32 = (float)bj - (float)a_j * gam;
+          const float gm = (float)cprev * __builtin_amdgcn_rcpf(bet32);
+          G32[(int64_t)(2 * j) * nslot + slot] = gm;
+          bet32 = (float)bj - (float)a_j * gm;
           u = (rho - (float)a_j * up32) * __builtin_amdgcn_rcpf(bet32);
         }
-        gam32[(int64_t)(2 * j + 1) * nfp + f] = u;
+        G32[(int64_t)(2 * j + 1) * nslot + slot] = u;
         up32 = u;
       }
       cprev = -aj;
-      Tm = Tc;
+      Told = Tc;
       Tc = Tn;
     }
-    // Back substitution.
     double T1;
     if (!MIXED) {
-      double unext = uprev;  // u_nz, already stored
+      double unext = uprev;
       for (int j = nz - 1; j >= 1; --j) {
-        const double u = T[(int64_t)j * nfp + f] - G[(int64_t)(j + 1) * nfp + f] * unext;
-        T[(int64_t)j * nfp + f] = u;
+        const double u = Tat(j) - G[(int64_t)(j + 1) * nslot + slot] * unext;
+        Tat(j) = u;
         unext = u;
       }
       T1 = unext;
     } else {
       float dnext = up32;
-      T[(int64_t)nz * nfp + f] = T[(int64_t)nz * nfp + f] + (double)dnext;
+      Tat(nz) = Tat(nz) + (double)dnext;
       for (int j = nz - 1; j >= 1; --j) {
-        const float d = gam32[(int64_t)(2 * j + 1) * nfp + f] -
-                        gam32[(int64_t)(2 * (j + 1)) * nfp + f] * dnext;
-        T[(int64_t)j * nfp + f] = T[(int64_t)j * nfp + f] + (double)d;
+        const float d = G32[(int64_t)(2 * j + 1) * nslot + slot] -
+                        G32[(int64_t)(2 * (j + 1)) * nslot + slot] * dnext;
+        Tat(j) = Tat(j) + (double)d;
         dnext = d;
       }
-      T1 = T[1 * nfp + f];
+      T1 = Tat(1);
     }
-    Tsurf = 0.5 * ((annp1 + bn * T1) + T1);
+    const double Tsurf = 0.5 * ((annp1 + bn * T1) + T1);
     Fsurf = ((-kfirst) * (T1 - Tsurf)) / H.z1;
-    T[f] = Tsurf;
+    Tat(0) = Tsurf;
     if (Tsurf_series) Tsurf_series[(int64_t)step * nf + f] = Tsurf;
     Qn = Qnp1;
   }
@@ -172,22 +335,31 @@ __global__ void copy_qprev_kernel(int64_t nf, const double *__restrict__ Qlast,
   if (f < nf) Qprev[f] = Qlast[f];
 }
 
-__global__ void gather_T_kernel(int64_t nf, int64_t nfp, int nzmax, const int32_t *__restrict__ nzf,
+// out[f * (nzmax + 1) + l] = T of facet f, layer l (NaN below its depth).
+__global__ void gather_T_kernel(int64_t nf, int64_t nslot, int nzmax,
+                                const int32_t *__restrict__ nzf, const int32_t *__restrict__ pos,
                                 const double *__restrict__ T, double *__restrict__ out) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nf * (nzmax + 1)) return;
   const int64_t f = idx / (nzmax + 1);
   const int l = (int)(idx - f * (nzmax + 1));
-  out[idx] = l <= nzf[f] ? T[(int64_t)l * nfp + f] : __builtin_nan("");
+  out[idx] = l <= nzf[f] ? T[(int64_t)l * nslot + pos[f]] : __builtin_nan("");
 }
 
-__global__ void scatter_T_kernel(int64_t nf, int64_t nfp, int nzmax, const double *__restrict__ in,
+__global__ void gather_surface_kernel(int64_t nf, const int32_t *__restrict__ pos,
+                                      const double *__restrict__ T, double *__restrict__ out) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f < nf) out[f] = T[pos[f]];
+}
+
+__global__ void scatter_T_kernel(int64_t nf, int64_t nslot, int nzmax,
+                                 const int32_t *__restrict__ pos, const double *__restrict__ in,
                                  double *__restrict__ T) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nf * (nzmax + 1)) return;
   const int64_t f = idx / (nzmax + 1);
   const int l = (int)(idx - f * (nzmax + 1));
-  T[(int64_t)l * nfp + f] = in[idx];
+  T[(int64_t)l * nslot + pos[f]] = in[idx];
 }
 
 __global__ void fill_kernel(int64_t n, double v, double *__restrict__ p) {
@@ -237,6 +409,16 @@ void make_tables(const std::vector<int64_t> &goff, const double *z, const double
   }
 }
 
+// Register buckets (layer capacity of cn_reg_kernel instantiations); 0 means
+// the streaming kernel.
+constexpr int BUCKETS[] = {32, 64, 96, 128};
+
+int bucket_of(int nz) {
+  for (int b : BUCKETS)
+    if (nz <= b) return b;
+  return 0;
+}
+
 }  // namespace
 
 struct ThermalImpl {
@@ -246,7 +428,8 @@ struct ThermalImpl {
   ~ThermalImpl() { dfree(d_hdr); dfree(d_coef); }
 };
 
-static ThermalImpl *impl_of(illum_thermal *th);
+static ThermalImpl *impl_of(illum_thermal *th) { return (ThermalImpl *)th->impl; }
+
 illum_thermal *thermal_create(illum_ctx *ctx, int64_t nf, int ngrid, const int64_t *grid_off,
                               const double *z, const double *ti, const double *rhoc,
                               const int32_t *grid_of_facet, const double *ti_scale,
@@ -254,6 +437,7 @@ illum_thermal *thermal_create(illum_ctx *ctx, int64_t nf, int ngrid, const int64
                               int precision) {
   if (nf <= 0 || ngrid <= 0 || !grid_off || !z || !ti || !rhoc)
     throw Error(ILLUM_ERR_ARG, "thermal model needs facets and at least one layer grid");
+  if (nf >= ((int64_t)1 << 31)) throw Error(ILLUM_ERR_CAPACITY, "too many facets");
   if (precision != ILLUM_THERM_FP64 && precision != ILLUM_THERM_MIXED)
     throw Error(ILLUM_ERR_ARG, "unknown thermal precision");
   if (grid_off[0] != 0) throw Error(ILLUM_ERR_ARG, "grid_off[0] must be 0");
@@ -270,11 +454,14 @@ illum_thermal *thermal_create(illum_ctx *ctx, int64_t nf, int ngrid, const int64
     }
     nzmax = std::max<int>(nzmax, (int)nz);
   }
-  std::vector<int32_t> nzh(nf);
+  std::vector<int32_t> nzh(nf), gof(nf);
   std::vector<double> s2(nf, 1.0);
+  std::vector<std::vector<int32_t>> members(ngrid);
   for (int64_t f = 0; f < nf; ++f) {
     const int g = grid_of_facet ? grid_of_facet[f] : 0;
     if (g < 0 || g >= ngrid) throw Error(ILLUM_ERR_ARG, "grid_of_facet out of range");
+    gof[f] = g;
+    members[g].push_back((int32_t)f);
     nzh[f] = (int32_t)(goff[g + 1] - goff[g]);
     if (ti_scale) {
       if (!(ti_scale[f] > 0)) throw Error(ILLUM_ERR_ARG, "ti_scale must be positive");
@@ -284,47 +471,79 @@ illum_thermal *thermal_create(illum_ctx *ctx, int64_t nf, int ngrid, const int64
   std::unique_ptr<illum_thermal, void (*)(illum_thermal *)> th(new illum_thermal(), destroy_thermal);
   th->ctx = ctx;
   th->nf = nf;
-  th->nfp = (nf + 63) / 64 * 64;
   th->nzmax = nzmax;
   th->precision = precision;
   th->ngrid = ngrid;
   th->emiss = emissivity;
   th->goff = goff;
   th->nz_host = nzh;
+
+  // Slot permutation: grids grouped by layer bucket, each grid's facets
+  // padded to whole wavefronts.
+  std::vector<int32_t> fidx, wgrid, pos(nf, -1);
+  bool need_stream = false;
+  for (int bi = 0; bi <= 4; ++bi) {
+    const int bucket = bi < 4 ? BUCKETS[bi] : 0;
+    const int64_t block0 = (int64_t)wgrid.size();
+    for (int g = 0; g < ngrid; ++g) {
+      if (members[g].empty() || bucket_of((int)(goff[g + 1] - goff[g])) != bucket) continue;
+      for (size_t k = 0; k < members[g].size(); k += 64) {
+        for (size_t q = k; q < k + 64; ++q) {
+          if (q < members[g].size()) {
+            pos[members[g][q]] = (int32_t)fidx.size();
+            fidx.push_back(members[g][q]);
+          } else {
+            fidx.push_back(-1);
+          }
+        }
+        wgrid.push_back(g);
+      }
+    }
+    const int64_t nb = (int64_t)wgrid.size() - block0;
+    if (nb > 0) {
+      th->launches.push_back({bucket, block0, nb});
+      if (bucket == 0) need_stream = true;
+    }
+  }
+  const int64_t nslot = (int64_t)fidx.size();
+  th->nslot = nslot;
   hipStream_t s = stream_of(ctx);
-  const int64_t nfp = th->nfp;
-  th->d_nz = dalloc<int32_t>(nf);
-  ILLUM_HIP(hipMemcpyAsync(th->d_nz, nzh.data(), 4 * nf, hipMemcpyHostToDevice, s));
-  if (grid_of_facet) {
-    th->d_grid = dalloc<int32_t>(nf);
-    ILLUM_HIP(hipMemcpyAsync(th->d_grid, grid_of_facet, 4 * nf, hipMemcpyHostToDevice, s));
-  }
-  if (ti_scale) {
-    th->d_s2 = dalloc<double>(nf);
-    ILLUM_HIP(hipMemcpyAsync(th->d_s2, s2.data(), 8 * nf, hipMemcpyHostToDevice, s));
-  }
-  th->d_goff = dalloc<int64_t>(ngrid + 1);
-  ILLUM_HIP(hipMemcpyAsync(th->d_goff, goff.data(), 8 * (ngrid + 1), hipMemcpyHostToDevice, s));
-  th->d_T = dalloc<double>((size_t)(nzmax + 1) * nfp);
+  auto up = [&](auto *&dst, const auto &v) {
+    using T = typename std::remove_reference<decltype(*dst)>::type;
+    dst = dalloc<T>(v.size());
+    ILLUM_HIP(hipMemcpyAsync(dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s));
+  };
+  up(th->d_fidx, fidx);
+  up(th->d_pos, pos);
+  up(th->d_wgrid, wgrid);
+  up(th->d_nz, nzh);
+  up(th->d_grid, gof);
+  if (ti_scale) up(th->d_s2, s2);
+  up(th->d_goff, goff);
+  th->d_T = dalloc<double>((size_t)(nzmax + 1) * nslot);
   th->d_Qprev = dalloc<double>(nf);
   th->d_F = dalloc<double>(nf);
-  if (precision == ILLUM_THERM_FP64) th->d_work = dalloc<double>((size_t)(nzmax + 1) * nfp);
-  else th->d_work32 = dalloc<float>((size_t)(2 * nzmax + 2) * nfp);
+  // Thomas scratch rows (every engine streams its multipliers through them)
+  if (precision == ILLUM_THERM_FP64) th->d_work = dalloc<double>((size_t)(nzmax + 1) * nslot);
+  else th->d_work32 = dalloc<float>((size_t)(2 * nzmax + 2) * nslot);
+  (void)need_stream;
   th->d_status = dalloc<int>(1);
   ILLUM_HIP(hipMemsetAsync(th->d_Qprev, 0, 8 * nf, s));
   ILLUM_HIP(hipMemsetAsync(th->d_F, 0, 8 * nf, s));
   ILLUM_HIP(hipMemsetAsync(th->d_status, 0, 4, s));
+  {
+    const int64_t tot = (int64_t)(nzmax + 1) * nslot;
+    fill_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(tot, T0, th->d_T);
+    ILLUM_LAUNCH_CHECK();
+  }
   if (T0_full) {
     DBuf<double> tmp((size_t)nf * (nzmax + 1));
     ILLUM_HIP(hipMemcpyAsync(tmp.p, T0_full, 8 * nf * (nzmax + 1), hipMemcpyHostToDevice, s));
     const int64_t tot = nf * (nzmax + 1);
-    scatter_T_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(nf, nfp, nzmax, tmp.p, th->d_T);
+    scatter_T_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(nf, nslot, nzmax, th->d_pos,
+                                                                 tmp.p, th->d_T);
     ILLUM_LAUNCH_CHECK();
     sync(ctx);
-  } else {
-    const int64_t tot = (int64_t)(nzmax + 1) * nfp;
-    fill_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(tot, T0, th->d_T);
-    ILLUM_LAUNCH_CHECK();
   }
   auto *im = new ThermalImpl();
   im->z.assign(z, z + goff[ngrid]);
@@ -334,8 +553,6 @@ illum_thermal *thermal_create(illum_ctx *ctx, int64_t nf, int ngrid, const int64
   sync(ctx);
   return th.release();
 }
-
-static ThermalImpl *impl_of(illum_thermal *th) { return (ThermalImpl *)th->impl; }
 
 void thermal_free_impl(illum_thermal *th) {
   delete impl_of(th);
@@ -356,6 +573,36 @@ static void ensure_tables(illum_thermal *th, double dt) {
   ILLUM_HIP(hipMemcpyAsync(im->d_coef, coef.data(), 8 * coef.size(), hipMemcpyHostToDevice, s));
   sync(th->ctx);
   im->dt = dt;
+}
+
+template <bool MIXED>
+static void launch_all(illum_thermal *th, ThermalImpl *im, hipStream_t s, double Fgeo, int nsteps,
+                       const double *dQ, double *Tss) {
+  const char *eng = getenv("ILLUM_CN_ENGINE");
+  const bool force_stream = eng && std::string(eng) == "stream";
+  for (const auto &L : th->launches) {
+#define ILLUM_CN_ARGS                                                                        \
+  th->d_fidx, th->d_wgrid, L.block0, th->nf, th->nslot, th->d_s2, im->d_hdr, im->d_coef,     \
+      th->d_goff, th->emiss, Fgeo, nsteps, dQ, th->d_Qprev, th->d_T, th->d_F
+    const unsigned nb = (unsigned)L.nblocks;
+    switch (force_stream ? 0 : L.nz_bucket) {
+#define ILLUM_CN_REG(NZ)                                                                     \
+  case NZ:                                                                                   \
+    cn_reg_kernel<NZ, MIXED><<<nb, 64, 0, s>>>(ILLUM_CN_ARGS, th->d_work, th->d_work32, Tss, \
+                                               th->d_status, th->nzmax + 1);                 \
+    break;
+      ILLUM_CN_REG(32)
+      ILLUM_CN_REG(64)
+      ILLUM_CN_REG(96)
+      ILLUM_CN_REG(128)
+#undef ILLUM_CN_REG
+      default:
+        cn_stream_kernel<MIXED><<<nb, 64, 0, s>>>(ILLUM_CN_ARGS, th->d_work, th->d_work32, Tss,
+                                                  th->d_status);
+    }
+#undef ILLUM_CN_ARGS
+    ILLUM_LAUNCH_CHECK();
+  }
 }
 
 void thermal_run(illum_thermal *th, double dt, const double *Q, int nsteps, int flags,
@@ -379,23 +626,12 @@ void thermal_run(illum_thermal *th, double dt, const double *Q, int nsteps, int 
     ILLUM_HIP(hipMemcpyAsync(th->d_Qdev, Q, 8 * need, hipMemcpyHostToDevice, s));
     dQ = th->d_Qdev;
   }
-  const unsigned blocks = (unsigned)((nf + 255) / 256);
   {
     StageTimer st(ctx, th->precision == ILLUM_THERM_MIXED ? "cn_mixed" : "cn_fp64");
-    if (th->precision == ILLUM_THERM_MIXED)
-      cn_kernel<true><<<blocks, 256, 0, s>>>(nf, th->nfp, th->d_nz, th->d_grid, th->d_s2,
-                                             im->d_hdr, im->d_coef, th->d_goff, th->emiss, Fgeo,
-                                             nsteps, dQ, th->d_Qprev, th->d_T, th->d_F,
-                                             th->d_work, th->d_work32, Tsurf_series,
-                                             th->d_status);
-    else
-      cn_kernel<false><<<blocks, 256, 0, s>>>(nf, th->nfp, th->d_nz, th->d_grid, th->d_s2,
-                                              im->d_hdr, im->d_coef, th->d_goff, th->emiss, Fgeo,
-                                              nsteps, dQ, th->d_Qprev, th->d_T, th->d_F,
-                                              th->d_work, th->d_work32, Tsurf_series,
-                                              th->d_status);
-    ILLUM_LAUNCH_CHECK();
+    if (th->precision == ILLUM_THERM_MIXED) launch_all<true>(th, im, s, Fgeo, nsteps, dQ, Tsurf_series);
+    else launch_all<false>(th, im, s, Fgeo, nsteps, dQ, Tsurf_series);
   }
+  const unsigned blocks = (unsigned)((nf + 255) / 256);
   copy_qprev_kernel<<<blocks, 256, 0, s>>>(nf, dQ + (int64_t)(nsteps - 1) * nf, th->d_Qprev);
   ILLUM_LAUNCH_CHECK();
   int st = 0;
@@ -409,20 +645,28 @@ void thermal_get(illum_thermal *th, double *T, double *Tsurf, double *Fsurf, int
   hipStream_t s = stream_of(ctx);
   const int64_t nf = th->nf;
   const bool dev = flags & ILLUM_OUT_ON_DEVICE;
-  const auto kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
   if (T) {
     const int64_t tot = nf * (th->nzmax + 1);
     DBuf<double> tmp;
     double *dst = T;
     if (!dev) { tmp.reset(tot); dst = tmp.p; }
-    gather_T_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(nf, th->nfp, th->nzmax,
-                                                                th->d_nz, th->d_T, dst);
+    gather_T_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(
+        nf, th->nslot, th->nzmax, th->d_nz, th->d_pos, th->d_T, dst);
     ILLUM_LAUNCH_CHECK();
     if (!dev) ILLUM_HIP(hipMemcpyAsync(T, dst, 8 * tot, hipMemcpyDeviceToHost, s));
     sync(ctx);
   }
-  if (Tsurf) ILLUM_HIP(hipMemcpyAsync(Tsurf, th->d_T, 8 * nf, kind, s));
-  if (Fsurf) ILLUM_HIP(hipMemcpyAsync(Fsurf, th->d_F, 8 * nf, kind, s));
+  if (Tsurf) {
+    DBuf<double> tmp;
+    double *dst = Tsurf;
+    if (!dev) { tmp.reset(nf); dst = tmp.p; }
+    gather_surface_kernel<<<(unsigned)((nf + 255) / 256), 256, 0, s>>>(nf, th->d_pos, th->d_T, dst);
+    ILLUM_LAUNCH_CHECK();
+    if (!dev) ILLUM_HIP(hipMemcpyAsync(Tsurf, dst, 8 * nf, hipMemcpyDeviceToHost, s));
+    sync(ctx);
+  }
+  if (Fsurf)
+    ILLUM_HIP(hipMemcpyAsync(Fsurf, th->d_F, 8 * nf, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
   sync(ctx);
 }
 

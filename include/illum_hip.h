@@ -153,6 +153,8 @@ int illum_sparse_copy_csr(const illum_sparse *sp, int64_t *rowptr, int32_t *coli
  * uint32 words (bit i of row j set <=> i in hits_j).  Testing aid. */
 int illum_sparse_visibility_words(const illum_sparse *sp, int64_t *words_per_row);
 int illum_sparse_copy_visibility(const illum_sparse *sp, uint32_t *bits);
+/* Number of visibility rays (facing-cull survivors) the assembly traced. */
+int illum_sparse_rays(const illum_sparse *sp, int64_t *rays);
 
 /* ---- horizons (A4) -------------------------------------------------------
  * out: nphi x nf column-major doubles (illum.hpp:47), host or device.

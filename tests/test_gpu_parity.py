@@ -72,6 +72,7 @@ def test_form_factors_vs_oracle(api, subdiv, ncraters, albedo):
     bits, nrays = O.visibility(o, offset=1e-5)
     gvis = Fm.visibility()[:, : bits.shape[1]]
     assert np.array_equal(gvis, bits), "visibility sets differ"
+    assert Fm.rays() == nrays  # same facing-cull survivors
     cp, ri, v = O.form_factors(o, bits, albedo=albedo)
     gcp, gri, gv = Fm.csc()
     assert np.array_equal(gcp, cp)
