@@ -31,6 +31,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+# Crank-Nicolson work per cell-step and the vector peak it is priced against:
+# FP64 Thomas step (right-hand side 9, elimination 6 + 2 divisions, back
+# substitution 2) on FP64 vector 78.6 TFLOP/s; mixed step (FP64 difference and
+# update 3, FP32 coefficients, residual, factorisation, reciprocal and sweeps
+# 16) on FP32 vector 157.3 TFLOP/s.
+CN_FLOPS = {"fp64": (19, 78.6), "mixed": (19, 157.3)}
 SUN_KM = 227390024.0          # illum_cli.cpp:337 (m -> km)
 ROT_PERIOD_S = 5.342 * 3600   # Vesta, the reference's body (runjob.sh:3-4)
 
@@ -193,11 +199,14 @@ def bench_thermal(api, nsteps=4096, subdiv=7, seed=0, chunk=256):
         kms = ctx.profile().get(kname, (float("nan"), 1))[0]
         res[prec] = (th.T, th.surface()[0])
         th.close()
-        bytes_per_cell = 40 if prec == "mixed" else 48
-        gbs = out["cell_steps"] * bytes_per_cell / (kms * 1e-3) / 1e9
+        # The column is register-resident (no per-step HBM stream), so the bound
+        # is VALU: flops per cell-step counted from the kernels' arithmetic.
+        fl, peak = CN_FLOPS[prec]
+        tfs = out["cell_steps"] * fl / (kms * 1e-3) / 1e12
         out[prec] = {"seconds": sec, "kernel_ms": kms, "steps_per_s": nsteps / sec,
                      "cell_steps_per_s": out["cell_steps"] / (kms * 1e-3),
-                     "algorithmic_GBs": gbs, "hbm_frac": gbs / PEAK_HBM_GBS}
+                     "flop_per_cell_step": fl, "TFLOPs": tfs, "valu_peak_TFLOPs": peak,
+                     "valu_frac": tfs / peak}
         log(f"thermal {prec}: {sec:.3f}s wall, kernel {kms:.1f} ms ({nsteps / sec:.0f} steps/s)")
     out["mixed_speedup"] = out["fp64"]["kernel_ms"] / out["mixed"]["kernel_ms"]
     out["max_abs_Tsurf_diff_mixed_vs_fp64_K"] = float(np.max(np.abs(res["mixed"][1] - res["fp64"][1])))
@@ -242,18 +251,15 @@ def bench_coupled(api, illum, ctx, Fm, hz, nsteps=32):
         th = api.ThermalModel(nf, z, 200.0, 1.19177e6, 200.0, emissivity=0.95, precision=prec, ctx=ctx)
         E = api.DeviceArray(ctx, (nf,))
         B = api.DeviceArray(ctx, (nf,))
-        resid = np.zeros(101)
-        it = api.C.c_int()
-        method = _lib.ILLUM_RAD_JACOBI_FP64 if prec == "fp64" else _lib.ILLUM_RAD_MIXED
         ctx.synchronize()
         t = time.perf_counter()
         for k in range(nsteps):
             _lib.call("illum_direct_radiosity", illum.mesh.h, api._ptr(hz), nphi, api._ptr(suns[k]),
                       1, api._ptr(disk), 100, api._ptr(E), dev)
-            # rad = -F_surf + direct (illum_cli.cpp:411-423), formed on the host
-            Eh = E.to_host() + th.get_radiosity()
-            B_h, _, _ = api.radiosity_solve(Fm, Eh, method=prec, rel_tol=1e-8, max_iter=100)
-            th.step(dt, B_h)
+            # rad = direct - F_surf (illum_cli.cpp:411-423), summed on the device
+            th.get_radiosity(out=E, accumulate=True)
+            api.radiosity_solve(Fm, E, method=prec, rel_tol=1e-8, max_iter=100, B=B)
+            th.step(dt, B)
         ctx.synchronize()
         sec = time.perf_counter() - t
         finals[prec] = th.surface()[0]

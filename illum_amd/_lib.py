@@ -22,6 +22,7 @@ ILLUM_ERR_SHAPE = 9
 
 ILLUM_IN_ON_DEVICE = 1
 ILLUM_OUT_ON_DEVICE = 2
+ILLUM_ACCUMULATE = 4
 ILLUM_RAD_JACOBI_FP64 = 1
 ILLUM_RAD_MIXED = 2
 ILLUM_THERM_FP64 = 1
@@ -78,6 +79,7 @@ SIGNATURES = {
     "illum_thermal_step": [_P, _D, _P, _I],
     "illum_thermal_run": [_P, _D, _P, _I, _I],
     "illum_thermal_get": [_P, _P, _P, _P, _I],
+    "illum_thermal_get_radiosity": [_P, _P, _I],
     "illum_conductionQ": [_I, _P, _D, _D, _D, _P, _P, _P, _D, _P, _D, _P],
 }
 _RET = {"illum_last_error": C.c_char_p}

@@ -498,8 +498,21 @@ class ThermalModel:
         call("illum_thermal_get", self.h, None, _ptr(Ts), _ptr(Fs), 0)
         return Ts, Fs
 
-    def get_radiosity(self):
-        return -self.surface()[1]
+    def get_radiosity(self, out=None, accumulate=False):
+        """-Fsurf (thermal.hpp:61-63).  With `out` (host array or DeviceArray of
+        N doubles) the result is written, or added when `accumulate`, in place."""
+        flags = _lib.ILLUM_ACCUMULATE if accumulate else 0
+        if out is None:
+            if accumulate:
+                raise ValueError("accumulate needs an output array")
+            out = np.zeros(self.nf)
+        elif _is_dev(out):
+            flags |= _lib.ILLUM_OUT_ON_DEVICE
+        elif not (isinstance(out, np.ndarray) and out.dtype == np.float64 and
+                  out.flags.c_contiguous and out.size == self.nf):
+            raise ValueError("out must be a contiguous float64 array of N values")
+        call("illum_thermal_get_radiosity", self.h, _ptr(out), flags)
+        return out
 
 
 def conductionQ(nz, z, dt, Qn, Qnp1, T, ti, rhoc, emiss, Tsurf, Fgeotherm):

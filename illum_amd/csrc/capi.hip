@@ -30,6 +30,7 @@ illum_thermal *thermal_create(illum_ctx *ctx, int64_t nf, int ngrid, const int64
 void thermal_run(illum_thermal *th, double dt, const double *Q, int nsteps, int flags,
                  double Fgeo, double *Tsurf_series);
 void thermal_get(illum_thermal *th, double *T, double *Tsurf, double *Fsurf, int flags);
+void thermal_get_radiosity(illum_thermal *th, double *out, int flags);
 void thermal_free_impl(illum_thermal *th);
 
 static thread_local std::string g_last_error;
@@ -104,7 +105,7 @@ void scan_counts(illum_ctx *ctx, const int32_t *counts, int64_t n, int64_t *offs
 static void set_device(illum_ctx *ctx) { ILLUM_HIP(hipSetDevice(ctx->device)); }
 
 static void destroy_mesh(illum_mesh *m) {
-  dfree(m->d_pr4); dfree(m->d_n4); dfree(m->d_area); dfree(m->d_frame);
+  dfree(m->d_pr4); dfree(m->d_n4); dfree(m->d_area); dfree(m->d_frame); dfree(m->d_clus);
   delete m;
 }
 static void destroy_bvh(illum_bvh *b) {
@@ -148,6 +149,10 @@ static illum_mesh *upload_mesh(illum_ctx *ctx, HostGeometry &&g) {
   ILLUM_HIP(hipMemcpyAsync(m->d_n4, n4.data(), 16 * nf, hipMemcpyHostToDevice, s));
   ILLUM_HIP(hipMemcpyAsync(m->d_area, m->g.area.data(), 8 * nf, hipMemcpyHostToDevice, s));
   ILLUM_HIP(hipMemcpyAsync(m->d_frame, m->g.frame.data(), 72 * nf, hipMemcpyHostToDevice, s));
+  const std::vector<float> clus = cull_clusters(m->g);
+  m->nclus = (int64_t)(clus.size() / 4);
+  m->d_clus = dalloc<float>(clus.size());
+  ILLUM_HIP(hipMemcpyAsync(m->d_clus, clus.data(), 4 * clus.size(), hipMemcpyHostToDevice, s));
   sync(ctx);
   return m.release();
 }
@@ -627,6 +632,14 @@ int illum_thermal_get(illum_thermal *th, double *T, double *Tsurf, double *Fsurf
     CHECK_ARG(th, "null thermal model");
     set_device(th->ctx);
     thermal_get(th, T, Tsurf, Fsurf, flags);
+  });
+}
+
+int illum_thermal_get_radiosity(illum_thermal *th, double *out, int flags) {
+  return guarded([&] {
+    CHECK_ARG(th && out, "null argument");
+    set_device(th->ctx);
+    thermal_get_radiosity(th, out, flags);
   });
 }
 

@@ -51,15 +51,6 @@ __device__ __forceinline__ double ff_value(float4 pi, float4 ni, float4 pj, floa
   return ((mij * mji) * Aj) / ((M_PI * r) * r);
 }
 
-// Maps consecutive workgroup ids (dispatched round-robin over the 8 XCDs) to
-// contiguous source ranges per XCD, so each XCD's L2 serves spatially
-// coherent sources.  Bijective on [0, n).
-__device__ __forceinline__ int xcd_remap(int b, int n) {
-  const int x = b & 7, k = b >> 3;
-  const int per = n >> 3, rem = n & 7;
-  return x * per + (x < rem ? x : rem) + k;
-}
-
 template <bool WIDE>
 __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
     int nf, int nwords, const float4 *__restrict__ pr4, const float4 *__restrict__ n4,
@@ -67,12 +58,14 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
     const float4 *__restrict__ tris, float offset,
     const double *__restrict__ area, const double *__restrict__ albedo_vec, double albedo,
     uint32_t *__restrict__ vis, uint32_t *__restrict__ mask, int32_t *__restrict__ counts,
-    int *__restrict__ status, int nrows, int jstride, unsigned long long *__restrict__ nrays) {
+    int *__restrict__ status, int nrows, int jstride, unsigned long long *__restrict__ nrays,
+    const float4 *__restrict__ clus, int nclus) {
   extern __shared__ uint32_t smem[];
   uint32_t *bits = smem;
   int *queue = (int *)(smem + nwords);
-  int *stack = queue + FF_QCAP;  // (tree depth + 1) x FF_BLOCK slots, used when WIDE
-  __shared__ int s_qn, s_rays;
+  int *clist = queue + FF_QCAP;  // surviving receiver clusters (nclus slots)
+  int *stack = clist + nclus;    // (tree depth + 1) x FF_BLOCK slots, used when WIDE
+  __shared__ int s_qn, s_rays, s_nc;
   __shared__ int s_red[FF_BLOCK / 64];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   // row = output row; source j = row * jstride (jstride 1 except when sampling)
@@ -80,16 +73,45 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
   const int j = row * jstride;
 
   for (int w = tid; w < nwords; w += FF_BLOCK) bits[w] = 0u;
-  if (tid == 0) { s_qn = 0; s_rays = 0; }
+  if (tid == 0) { s_qn = 0; s_rays = 0; s_nc = 0; }
   const float4 pj = pr4[j], nj = n4[j];
   const float qx = pj.x + offset * nj.x, qy = pj.y + offset * nj.y, qz = pj.z + offset * nj.z;
   __syncthreads();
 
-  const int chunk = FF_BLOCK * FF_PER;
-  for (int base = 0; base < nf; base += chunk) {
+  // Cluster pass: a cluster of 64 consecutive receivers whose bound proves
+  // every member fails the facing test is dropped whole (the bound and its
+  // rounding margin are documented with cull_clusters in mesh.cpp); the
+  // others are listed for the per-pair test.  clus == nullptr keeps all.
+  for (int c0 = 0; c0 < nclus; c0 += FF_BLOCK) {
+    const int c = c0 + tid;
+    bool keep = false;
+    if (c < nclus) {
+      keep = true;
+      if (clus) {
+        const float4 cc = clus[c];
+        const float t = ((cc.x - pj.x) * nj.x + (cc.y - pj.y) * nj.y) + (cc.z - pj.z) * nj.z;
+        keep = !(t <= -cc.w);
+      }
+    }
+    const uint64_t m = __ballot(keep);
+    if (m) {
+      int slot = 0;
+      if (lane == 0) slot = atomicAdd(&s_nc, __popcll(m));
+      slot = __shfl(slot, 0);
+      if (keep) clist[slot + __popcll(m & lanemask_lt())] = c;
+    }
+  }
+  __syncthreads();
+  const int ncl = s_nc;
+
+  // Per-pair facing test over the listed clusters, FF_PER * 4 clusters (one
+  // per wave and k) per round; survivors go to the LDS ray queue.
+  constexpr int CPR = FF_PER * (FF_BLOCK / 64);
+  for (int g0 = 0; g0 < ncl; g0 += CPR) {
 #pragma unroll
     for (int k = 0; k < FF_PER; ++k) {
-      const int i = base + k * FF_BLOCK + tid;
+      const int ci = g0 + k * (FF_BLOCK / 64) + wid;
+      const int i = ci < ncl ? clist[ci] * 64 + lane : nf;
       bool s = false;
       if (i < nf && i != j) {
         const float4 pi = pr4[i];
@@ -109,7 +131,7 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
       }
     }
     __syncthreads();
-    const bool last = base + chunk >= nf;
+    const bool last = g0 + CPR >= ncl;
     int qrem = s_qn;
     while (qrem >= FF_BLOCK || (last && qrem > 0)) {
       const int take = qrem < FF_BLOCK ? qrem : FF_BLOCK;
@@ -314,6 +336,13 @@ static bool use_wide(const illum_bvh *bvh) {
   return bvh->d_wide && bvh->h.depth <= FF_STACK;
 }
 
+// The cluster pass of the facing cull is on unless ILLUM_FF_CLUSTER_CULL=0
+// (results are identical either way; the switch exists for comparison).
+static bool cluster_cull() {
+  const char *e = getenv("ILLUM_FF_CLUSTER_CULL");
+  return !(e && std::string(e) == "0");
+}
+
 // ---------------------------------------------------------------------------
 void bvh_closest_hit(illum_bvh *bvh, const float *orig, const float *dir, int64_t nray,
                      int32_t *hit, float *t) {
@@ -346,7 +375,7 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   const bool wide = use_wide(bvh);
   // traversal stack: one slot per tree level (a push happens at most once per level)
   const int stack_slots = wide ? bvh->h.depth + 1 : 0;
-  const size_t lds = (size_t)(nwords + FF_QCAP + stack_slots * FF_BLOCK) * 4;
+  const size_t lds = (size_t)(nwords + FF_QCAP + mesh->nclus + stack_slots * FF_BLOCK) * 4;
   if (lds > 160 * 1024)
     throw Error(ILLUM_ERR_CAPACITY, "too many facets for the LDS visibility bitset (max ~1.2M)");
   const void *kfn = wide ? (const void *)ff_visibility_kernel<true>
@@ -380,7 +409,8 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
         (int)nf, nwords, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4,
         (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_wide, (const float4 *)bvh->d_tris,
         (float)offset, mesh->d_area, d_alb.p, albedo, sp->d_vis, M.p, counts.p, status.p,
-        (int)nf, 1, sp->d_nrays);
+        (int)nf, 1, sp->d_nrays, cluster_cull() ? (const float4 *)mesh->d_clus : nullptr,
+        (int)mesh->nclus);
     ILLUM_LAUNCH_CHECK();
   }
   // CSC
@@ -443,7 +473,7 @@ double time_visibility_sample(illum_bvh *bvh, int max_sources) {
   const int nwords = (int)(((nf + 511) / 512) * 16);
   const bool wide = use_wide(bvh);
   const int stack_slots = wide ? bvh->h.depth + 1 : 0;
-  const size_t lds = (size_t)(nwords + FF_QCAP + stack_slots * FF_BLOCK) * 4;
+  const size_t lds = (size_t)(nwords + FF_QCAP + mesh->nclus + stack_slots * FF_BLOCK) * 4;
   if (lds > 160 * 1024) throw Error(ILLUM_ERR_CAPACITY, "too many facets for the LDS bitset");
   const void *kfn = wide ? (const void *)ff_visibility_kernel<true>
                          : (const void *)ff_visibility_kernel<false>;
@@ -462,7 +492,8 @@ double time_visibility_sample(illum_bvh *bvh, int max_sources) {
     launch<<<nrows, FF_BLOCK, lds, s>>>(
         (int)nf, nwords, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4,
         (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_wide, (const float4 *)bvh->d_tris,
-        1e-5f, mesh->d_area, nullptr, 1.0, vis.p, M.p, counts.p, status.p, nrows, jstride, nullptr);
+        1e-5f, mesh->d_area, nullptr, 1.0, vis.p, M.p, counts.p, status.p, nrows, jstride, nullptr,
+        cluster_cull() ? (const float4 *)mesh->d_clus : nullptr, (int)mesh->nclus);
     ILLUM_LAUNCH_CHECK();
     ILLUM_HIP(hipEventRecord(e1, s));
     ILLUM_HIP(hipEventSynchronize(e1));

@@ -80,6 +80,15 @@ void sync(illum_ctx *ctx);
 // Exclusive scan of n int32 counts into n+1 int64 offsets (device), on stream.
 void scan_counts(illum_ctx *ctx, const int32_t *counts, int64_t n, int64_t *offsets);
 
+// Maps consecutive workgroup ids (dispatched round-robin over the 8 XCDs) to
+// contiguous ranges per XCD, so each XCD's L2 serves spatially coherent
+// work.  Bijective on [0, n).
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+  const int x = b & 7, k = b >> 3;
+  const int per = n >> 3, rem = n & 7;
+  return x * per + (x < rem ? x : rem) + k;
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
   const unsigned lane = __lane_id();
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));

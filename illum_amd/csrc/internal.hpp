@@ -36,6 +36,10 @@ struct HostGeometry {
 HostGeometry make_geometry(const double *V, int64_t nv, const int32_t *F, int64_t nf,
                            const double *VN, double vertex_divisor);
 
+// Receiver clusters of 64 consecutive facets for the form-factor facing cull:
+// (centre, threshold) float4 per cluster; the bound is documented in mesh.cpp.
+std::vector<float> cull_clusters(const HostGeometry &g);
+
 // Synthetic cratered ellipsoid.
 void synth_sizes(int subdiv, int64_t *nv, int64_t *nf);
 void synth_ellipsoid(int subdiv, double a, double b, double c, int ncraters, double dmin,
@@ -94,6 +98,8 @@ struct illum_mesh {
   float *d_n4 = nullptr;       // nf x float4 (normal, 0)
   double *d_area = nullptr;    // nf
   double *d_frame = nullptr;   // nf x 9
+  float *d_clus = nullptr;     // ceil(nf / 64) x float4 (cull_clusters)
+  int64_t nclus = 0;
 };
 
 struct illum_bvh {

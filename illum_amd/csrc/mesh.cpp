@@ -12,6 +12,7 @@
 // evaluated exactly as written (IEEE round-to-nearest, no fused ops).
 #include "internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -274,6 +275,54 @@ HostGeometry make_geometry(const double *V, int64_t nv, const int32_t *F, int64_
     fr[6] = nu[0]; fr[7] = nu[1]; fr[8] = nu[2];
   }
   return g;
+}
+
+// Receiver clusters for the form-factor facing cull: facets [64c, 64c + 64)
+// form cluster c, stored as (centre, threshold).  For a source j with FP32
+// centroid p_j and normal n_j, when
+//   t = ((cx - pjx) * njx + (cy - pjy) * njy) + (cz - pjz) * njz   (FP32)
+// satisfies t <= -threshold, the FP32 facing test (p_i - p_j).n_j <= -r_i
+// holds for every member i, so the whole cluster is culled without testing
+// its members.  Exact bound: (p_i - p_j).n_j + r_i <= (c - p_j).n_j +
+// |n_j| (|p_i - c| + r_i), with |n_j| <= 1 + 2^-20 for an FP32 unit normal.
+// Rounding: each FP32 dot product of differences of coordinates bounded by E
+// (the largest centroid coordinate) errs by at most 4u|d||n| <= 14uE
+// (u = 2^-24, |d| <= 2 sqrt(3) E), so the pair test and t together err by
+// less than 28uE; the threshold adds 64uE.  Facet order decides how tight
+// the clusters are (index-local meshes such as subdivided icosahedra give
+// compact clusters); a loose cluster is simply never culled.
+std::vector<float> cull_clusters(const HostGeometry &g) {
+  const int64_t nf = g.nf;
+  const int64_t nc = (nf + 63) / 64;
+  std::vector<float> out(4 * nc);
+  double E = 0.0;
+  for (int64_t k = 0; k < 3 * nf; ++k) E = std::max(E, std::fabs((double)g.cen[k]));
+  const double margin = 64.0 * std::ldexp(1.0, -24) * E;
+  for (int64_t c = 0; c < nc; ++c) {
+    const int64_t i0 = 64 * c, i1 = std::min<int64_t>(nf, i0 + 64);
+    double m[3] = {0, 0, 0};
+    for (int64_t i = i0; i < i1; ++i)
+      for (int a = 0; a < 3; ++a) m[a] += g.cen[3 * i + a];
+    float cf[3];
+    for (int a = 0; a < 3; ++a) cf[a] = (float)(m[a] / (double)(i1 - i0));
+    double R = 0.0;
+    for (int64_t i = i0; i < i1; ++i) {
+      double d2 = 0.0;
+      for (int a = 0; a < 3; ++a) {
+        const double d = (double)g.cen[3 * i + a] - (double)cf[a];
+        d2 += d * d;
+      }
+      R = std::max(R, std::sqrt(d2) + (double)g.rad[i]);
+    }
+    const double thr = R * (1.0 + std::ldexp(1.0, -20)) + margin;
+    float tf = (float)thr;
+    if ((double)tf < thr) tf = std::nextafter(tf, INFINITY);
+    out[4 * c] = cf[0];
+    out[4 * c + 1] = cf[1];
+    out[4 * c + 2] = cf[2];
+    out[4 * c + 3] = tf;
+  }
+  return out;
 }
 
 }  // namespace illum

@@ -9,29 +9,26 @@
 // T[l * nslot + slot] (l = 0 surface, 1..nz layers), so state loads and
 // stores are coalesced and a wavefront's grid coefficients are uniform.
 //
-// Engines.
-//   * cn_reg_kernel<NZ>: grids with at most NZ layers (NZ in 32/64/96/128).
-//     A lane keeps its whole temperature column in registers for every step
-//     of the launch; the wavefront's alpha/gamma table sits in LDS; the Thomas
-//     multipliers go through a wavefront-private scratch block (written in the
-//     forward sweep, read back in the backward sweep).  ILLUM_CN_ENGINE=stream
-//     forces the streaming kernel for comparison.
-//   * cn_stream_kernel: any depth (<= 1000 layers, conductionQ.f:38 NMAX);
-//     the column streams through HBM each step (coalesced across the
-//     wavefront).
+// Engines.  Grids with at most NZ layers (NZ in 32/64/96/128) run on
+// register kernels: a lane keeps its whole temperature column in registers
+// for every step of the launch.  Deeper grids (<= 1000 layers,
+// conductionQ.f:38 NMAX) run on cn_stream_kernel, whose column streams
+// through HBM each step (coalesced across the wavefront).
+// ILLUM_CN_ENGINE=stream forces the streaming kernel for comparison.
 //
 // Coefficients: the time-invariant tridiagonal entries of each grid are
 // tabulated on the host in conductionQ's operation order for the current dt; a
 // facet's thermal-inertia scale s multiplies every conductivity by s^2.
 //
-// FP64 mode: the reference step verbatim -- right-hand side, Thomas sweep with
-// one division per row, surface temperature and flux, all in FP64.
-// Mixed mode: iterative refinement from the previous state.  The FP64 residual
-// rho = r - A T^n is formed in the forward pass, A d = rho is solved by an
-// FP32 Thomas sweep (hardware reciprocal), and T^{n+1} = T^n + d in FP64.
+// FP64 mode (cn_fp64_kernel): the reference step verbatim -- right-hand side,
+// Thomas sweep with one division per row, surface temperature and flux, all
+// in FP64.
+// Mixed mode (cn_mixed_kernel): the increment d = T^{n+1} - T^n is solved in
+// FP32 against an FP64 residual (see the kernel), T^{n+1} = T^n + d in FP64.
 // |d| is the per-step temperature change, so the FP32 error scales with the
 // change, not with T.  The radiative surface linearisation and the surface
-// flux stay FP64.
+// flux stay FP64.  The streaming kernel's mixed mode forms rho = r - A T^n
+// directly and runs a top-down FP32 Thomas sweep.
 #include "hip_util.hpp"
 
 #include <algorithm>
@@ -66,21 +63,24 @@ __device__ __forceinline__ void surface_bc(double Tsurf, double Qn, double Qnp1,
   *bn = (k1 - brad) / den;
 }
 
+// Waves per SIMD of a register kernel: columns of up to 32 layers fit two
+// waves' register files; deeper columns take the whole file (512 VGPR+AGPR).
+constexpr int cn_waves(int nz) { return nz <= 32 ? 2 : 1; }
+
 // ---------------------------------------------------------------------------
-// Register-resident engine: the temperature column of a lane stays in
-// registers for all steps of the launch; the Thomas multipliers (and in mixed
-// mode the FP32 forward values) go to slot-major scratch rows, written in the
-// forward sweep and read back in the backward sweep (fire-and-forget stores,
-// independent loads the compiler can issue early).
-template <int NZ, bool MIXED>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void cn_reg_kernel(
+// FP64 register engine (the reference step verbatim): the temperature column
+// of a lane stays in registers for all steps of the launch.  The Thomas
+// multipliers stay in registers too when GMREG (columns of up to 96 layers);
+// otherwise they go to a wavefront-private scratch block, written in the
+// forward sweep and read back in the backward sweep.
+template <int NZ, bool GMREG>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ), cn_waves(NZ)))) void cn_fp64_kernel(
     const int32_t *__restrict__ fidx, const int32_t *__restrict__ wgrid, int64_t block0,
     int64_t nf, int64_t nslot, const double *__restrict__ s2f, const double *__restrict__ hdr,
     const double *__restrict__ coef, const int64_t *__restrict__ goff, double emiss, double Fgeo,
     int nsteps, const double *__restrict__ Q, const double *__restrict__ Qprev,
     double *__restrict__ T, double *__restrict__ Fs, double *__restrict__ G,
-    float *__restrict__ G32, double *__restrict__ Tsurf_series, int *__restrict__ status,
-    int rows) {
+    double *__restrict__ Tsurf_series, int *__restrict__ status, int rows) {
   __shared__ double s_al[NZ], s_ga[NZ];
   const int64_t blk = block0 + blockIdx.x;
   const int64_t slot = blk * 64 + threadIdx.x;
@@ -100,8 +100,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
   const double geo_term = (H.geo * Fgeo) / H.dzlast;
   // Wavefront-private scratch block, element [row][lane]: the per-layer offset
   // is a compile-time immediate on one base address.
-  double *Gw = G + (MIXED ? 0 : blk * (int64_t)rows * 64 + threadIdx.x);
-  float *Hw = G32 + (MIXED ? blk * (int64_t)(2 * rows) * 64 + threadIdx.x : 0);
+  double *Gw = GMREG ? nullptr : G + blk * (int64_t)rows * 64 + threadIdx.x;
+  double Gm[GMREG ? NZ + 2 : 1];
 
   double Tl[NZ + 2];  // Tl[0] surface, Tl[1..nz] layers, one spare slot
 #pragma unroll
@@ -118,7 +118,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     const double b1 = ((1.0 + a1) + g1) - g1 * bn;
     if (b1 == 0.0) zero_pivot = true;
     double bet = 0.0, uprev = 0.0, cprev = 0.0, Told = 0.0;
-    float bet32 = 0.0f, up32 = 0.0f, cprev32 = 0.0f;
 #pragma unroll
     for (int j = 1; j <= NZ; ++j) {
       if (j <= nz) {
@@ -137,41 +136,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
           r = (gj * Told + (1.0 - gj) * Tc) + geo_term;
           bj = 1.0 + gj;
         }
-        if (!MIXED) {
-          double u;
-          if (j == 1) {
-            bet = bj;
-            u = r / bet;
-          } else {
-            const double a_j = -gj;
-            const double gm = cprev / bet;
-            Gw[j * 64] = gm;
-            bet = bj - a_j * gm;
-            u = (r - a_j * uprev) / bet;
-          }
-          Tl[j] = u;  // old T_j is kept in Told for the next row
-          uprev = u;
+        double u;
+        if (j == 1) {
+          bet = bj;
+          u = r / bet;
         } else {
-          double Arow;
-          if (j == 1) Arow = bj * Tc + (-a1) * Tn;
-          else if (j < nz) Arow = ((-gj) * Told + bj * Tc) + (-aj) * Tn;
-          else Arow = (-gj) * Told + bj * Tc;
-          const float rho = (float)(r - Arow);
-          float u;
-          if (j == 1) {
-            bet32 = (float)bj;
-            u = rho * __builtin_amdgcn_rcpf(bet32);
-          } else {
-            const float a_j = (float)(-gj);
-            const float gm = cprev32 * __builtin_amdgcn_rcpf(bet32);
-            Hw[(2 * j) * 64] = gm;
-            bet32 = (float)bj - a_j * gm;
-            u = (rho - a_j * up32) * __builtin_amdgcn_rcpf(bet32);
-          }
-          Hw[(2 * j + 1) * 64] = u;
-          up32 = u;
-          cprev32 = (float)(-aj);
+          const double a_j = -gj;
+          const double gm = cprev / bet;
+          if (GMREG) Gm[j] = gm;
+          else Gw[j * 64] = gm;
+          bet = bj - a_j * gm;
+          u = (r - a_j * uprev) / bet;
         }
+        Tl[j] = u;  // old T_j is kept in Told for the next row
+        uprev = u;
         cprev = -aj;
         Told = Tc;
       }
@@ -179,24 +157,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
       // which bounds register pressure (the column itself needs 2 VGPRs per layer).
       if ((j & 7) == 0) __builtin_amdgcn_sched_barrier(0);
     }
-    if (!MIXED) {
 #pragma unroll
-      for (int j = NZ - 1; j >= 1; --j) {
-        if (j < nz) Tl[j] = Tl[j] - Gw[(j + 1) * 64] * Tl[j + 1];
-        if ((j & 7) == 0) __builtin_amdgcn_sched_barrier(0);
-      }
-    } else {
-      float dnext = 0.0f;
-#pragma unroll
-      for (int j = NZ; j >= 1; --j) {
-        if (j <= nz) {
-          const float u = Hw[(2 * j + 1) * 64];
-          const float d = j == nz ? u : u - Hw[(2 * (j + 1)) * 64] * dnext;
-          Tl[j] = Tl[j] + (double)d;
-          dnext = d;
-        }
-        if ((j & 7) == 0) __builtin_amdgcn_sched_barrier(0);
-      }
+    for (int j = NZ - 1; j >= 1; --j) {
+      if (j < nz) Tl[j] = Tl[j] - (GMREG ? Gm[j + 1] : Gw[(j + 1) * 64]) * Tl[j + 1];
+      if ((j & 7) == 0) __builtin_amdgcn_sched_barrier(0);
     }
     const double T1 = Tl[1];
     const double Tsurf = 0.5 * ((annp1 + bn * T1) + T1);
@@ -211,6 +175,106 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
       if (l <= nz) T[(int64_t)l * nslot + slot] = Tl[l];
     Fs[f] = Fsurf;
     if (zero_pivot) atomicOr(status, 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision register engine.  A step A T^{n+1} = r(T^n) is solved in
+// increment form, A d = rho with rho = r - A T^n and T^{n+1} = T^n + d.
+// For rows j >= 2 rho reduces to the explicit diffusion term
+//   rho_j = 2 (alpha_j (T_{j+1} - T_j) - gamma_j (T_j - T_{j-1}))  (+ geothermal
+// term on the bottom row), formed from FP64 differences of T so it carries the
+// per-step change at full relative precision; row 1 keeps the linearised
+// radiative terms in FP64.  The FP32 solve factors A from the bottom row up:
+//   beta_nz = b_nz,  beta_j = b_j - alpha_j gamma_{j+1} / beta_{j+1},
+//   z_j = (rho_j + alpha_j z_{j+1}) / beta_j,
+//   d_1 = z_1,  d_j = z_j + (gamma_j / beta_j) d_{j-1},
+// so only row 1 (whose diagonal follows the surface temperature) differs
+// between steps and the recurrence needs one reciprocal per row.  The
+// column T (FP64) and z, gamma/beta (FP32) all stay in registers.  rho is
+// carried halved (the factor 2 is applied in the FP64 update).
+template <int NZ>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ), cn_waves(NZ)))) void cn_mixed_kernel(
+    const int32_t *__restrict__ fidx, const int32_t *__restrict__ wgrid, int64_t block0,
+    int64_t nf, int64_t nslot, const double *__restrict__ s2f, const double *__restrict__ hdr,
+    const double *__restrict__ coef, const float *__restrict__ coef32,
+    const int64_t *__restrict__ goff, double emiss, double Fgeo, int nsteps,
+    const double *__restrict__ Q, const double *__restrict__ Qprev, double *__restrict__ T,
+    double *__restrict__ Fs, double *__restrict__ Tsurf_series) {
+  const int64_t blk = block0 + blockIdx.x;
+  const int64_t slot = blk * 64 + threadIdx.x;
+  const int f = fidx[slot];
+  const bool act = f >= 0;
+  const int g = wgrid[blk];
+  const GridHdr H = *(const GridHdr *)(hdr + 6 * g);
+  const int nz = (int)H.nz;  // uniform across the wavefront, <= NZ
+  const double *al = coef + 2 * goff[g];
+  const float *c32 = coef32 + 3 * goff[g];  // per layer: alpha, gamma, alpha + gamma
+  const double s2 = (act && s2f) ? s2f[f] : 1.0;
+  const float s2s = (float)s2;
+  const double k1 = s2 * H.k1, kfirst = s2 * H.kfirst;
+  const float geo_half = (float)(0.5 * ((H.geo * Fgeo) / H.dzlast));
+  const double a1 = s2 * al[0], g1 = s2 * al[1];
+  const float a1s = (float)a1;
+
+  double Tl[NZ + 1];  // Tl[0] surface, Tl[1..nz] layers
+#pragma unroll
+  for (int l = 0; l <= NZ; ++l) Tl[l] = (act && l <= nz) ? T[(int64_t)l * nslot + slot] : 0.0;
+  double Qn = act ? Qprev[f] : 0.0;
+  double Fsurf = act ? Fs[f] : 0.0;
+
+  for (int step = 0; step < nsteps; ++step) {
+    const double Qnp1 = act ? Q[(int64_t)step * nf + f] : 0.0;
+    double ann, annp1, bn;
+    surface_bc(Tl[0], Qn, Qnp1, k1, emiss, &ann, &annp1, &bn);
+    float z[NZ + 1], M[NZ + 1];
+    float znext = 0.0f, Mnext = 0.0f, enext = 0.0f;  // z, gamma/beta, T diff of row j+1
+#pragma unroll
+    for (int j = NZ; j >= 2; --j) {
+      if (j <= nz) {
+        const float aj = s2s * c32[3 * (j - 1)];
+        const float gj = s2s * c32[3 * (j - 1) + 1];
+        const float bj = 1.0f + s2s * c32[3 * (j - 1) + 2];
+        const float ej = (float)(Tl[j] - Tl[j - 1]);
+        float rho = aj * enext - gj * ej;
+        if (j == nz) rho += geo_half;
+        const float P = __builtin_amdgcn_rcpf(bj - aj * Mnext);
+        z[j] = (rho + aj * znext) * P;
+        M[j] = gj * P;
+        znext = z[j];
+        Mnext = M[j];
+        enext = ej;
+      }
+      if ((j & 7) == 0) __builtin_amdgcn_sched_barrier(0);
+    }
+    {
+      const double b1 = ((1.0 + a1) + g1) - g1 * bn;
+      const double surf = (0.5 * g1) * ((ann + annp1) + (2.0 * (bn - 1.0)) * Tl[1]);
+      const float rho1 = (float)surf + a1s * enext;
+      z[1] = (rho1 + a1s * znext) * __builtin_amdgcn_rcpf((float)b1 - a1s * Mnext);
+    }
+    float d = z[1];
+    Tl[1] = Tl[1] + 2.0 * (double)d;
+#pragma unroll
+    for (int j = 2; j <= NZ; ++j) {
+      if (j <= nz) {
+        d = z[j] + M[j] * d;
+        Tl[j] = Tl[j] + 2.0 * (double)d;
+      }
+      if ((j & 7) == 0) __builtin_amdgcn_sched_barrier(0);
+    }
+    const double T1 = Tl[1];
+    const double Tsurf = 0.5 * ((annp1 + bn * T1) + T1);
+    Fsurf = ((-kfirst) * (T1 - Tsurf)) / H.z1;
+    Tl[0] = Tsurf;
+    if (Tsurf_series && act) Tsurf_series[(int64_t)step * nf + f] = Tsurf;
+    Qn = Qnp1;
+  }
+  if (act) {
+#pragma unroll
+    for (int l = 0; l <= NZ; ++l)
+      if (l <= nz) T[(int64_t)l * nslot + slot] = Tl[l];
+    Fs[f] = Fsurf;
   }
 }
 
@@ -362,6 +426,14 @@ __global__ void scatter_T_kernel(int64_t nf, int64_t nslot, int nzmax,
   T[(int64_t)l * nslot + pos[f]] = in[idx];
 }
 
+// out = -F (or out += -F): thermal_model::get_radiosity, thermal.hpp:61-63.
+__global__ void neg_flux_kernel(int64_t nf, const double *__restrict__ F, double *__restrict__ out,
+                                int accumulate) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  out[f] = accumulate ? out[f] - F[f] : -F[f];
+}
+
 __global__ void fill_kernel(int64_t n, double v, double *__restrict__ p) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) p[k] = v;
@@ -373,10 +445,11 @@ namespace {
 // Host tables for a given dt, conductionQ.f:50-65 operation by operation.
 void make_tables(const std::vector<int64_t> &goff, const double *z, const double *ti,
                  const double *rhoc, double dt, std::vector<double> &hdr,
-                 std::vector<double> &coef) {
+                 std::vector<double> &coef, std::vector<float> &coef32) {
   const int ng = (int)goff.size() - 1;
   hdr.assign(6 * ng, 0.0);
   coef.assign(2 * goff[ng], 0.0);
+  coef32.assign(3 * goff[ng], 0.0f);
   for (int g = 0; g < ng; ++g) {
     const int64_t o = goff[g];
     const int nz = (int)(goff[g + 1] - o);
@@ -398,6 +471,9 @@ void make_tables(const std::vector<int64_t> &goff, const double *z, const double
     for (int i = 0; i < nz; ++i) {
       coef[2 * (o + i)] = alpha[i];
       coef[2 * (o + i) + 1] = gamma[i];
+      coef32[3 * (o + i)] = (float)alpha[i];
+      coef32[3 * (o + i) + 1] = (float)gamma[i];
+      coef32[3 * (o + i) + 2] = (float)(alpha[i] + gamma[i]);
     }
     double *h = &hdr[6 * g];
     h[0] = k[0] / dz;
@@ -425,7 +501,8 @@ struct ThermalImpl {
   std::vector<double> z, ti, rhoc;
   double dt = -1;
   double *d_hdr = nullptr, *d_coef = nullptr;
-  ~ThermalImpl() { dfree(d_hdr); dfree(d_coef); }
+  float *d_coef32 = nullptr;
+  ~ThermalImpl() { dfree(d_hdr); dfree(d_coef); dfree(d_coef32); }
 };
 
 static ThermalImpl *impl_of(illum_thermal *th) { return (ThermalImpl *)th->impl; }
@@ -524,6 +601,9 @@ illum_thermal *thermal_create(illum_ctx *ctx, int64_t nf, int ngrid, const int64
   th->d_Qprev = dalloc<double>(nf);
   th->d_F = dalloc<double>(nf);
   // Thomas scratch rows (every engine streams its multipliers through them)
+  // Scratch rows for the engines that stream Thomas multipliers: the
+  // streaming kernel (either precision, or forced for comparison) and the
+  // FP64 register kernel above 96 layers.
   if (precision == ILLUM_THERM_FP64) th->d_work = dalloc<double>((size_t)(nzmax + 1) * nslot);
   else th->d_work32 = dalloc<float>((size_t)(2 * nzmax + 2) * nslot);
   (void)need_stream;
@@ -563,14 +643,18 @@ static void ensure_tables(illum_thermal *th, double dt) {
   ThermalImpl *im = impl_of(th);
   if (im->dt == dt && im->d_hdr) return;
   std::vector<double> hdr, coef;
-  make_tables(th->goff, im->z.data(), im->ti.data(), im->rhoc.data(), dt, hdr, coef);
+  std::vector<float> coef32;
+  make_tables(th->goff, im->z.data(), im->ti.data(), im->rhoc.data(), dt, hdr, coef, coef32);
   dfree(im->d_hdr);
   dfree(im->d_coef);
+  dfree(im->d_coef32);
   im->d_hdr = dalloc<double>(hdr.size());
   im->d_coef = dalloc<double>(coef.size());
+  im->d_coef32 = dalloc<float>(coef32.size());
   hipStream_t s = stream_of(th->ctx);
   ILLUM_HIP(hipMemcpyAsync(im->d_hdr, hdr.data(), 8 * hdr.size(), hipMemcpyHostToDevice, s));
   ILLUM_HIP(hipMemcpyAsync(im->d_coef, coef.data(), 8 * coef.size(), hipMemcpyHostToDevice, s));
+  ILLUM_HIP(hipMemcpyAsync(im->d_coef32, coef32.data(), 4 * coef32.size(), hipMemcpyHostToDevice, s));
   sync(th->ctx);
   im->dt = dt;
 }
@@ -586,15 +670,21 @@ static void launch_all(illum_thermal *th, ThermalImpl *im, hipStream_t s, double
       th->d_goff, th->emiss, Fgeo, nsteps, dQ, th->d_Qprev, th->d_T, th->d_F
     const unsigned nb = (unsigned)L.nblocks;
     switch (force_stream ? 0 : L.nz_bucket) {
-#define ILLUM_CN_REG(NZ)                                                                     \
+#define ILLUM_CN_REG(NZ, GMREG)                                                              \
   case NZ:                                                                                   \
-    cn_reg_kernel<NZ, MIXED><<<nb, 64, 0, s>>>(ILLUM_CN_ARGS, th->d_work, th->d_work32, Tss, \
-                                               th->d_status, th->nzmax + 1);                 \
+    if (MIXED)                                                                               \
+      cn_mixed_kernel<NZ><<<nb, 64, 0, s>>>(                                                 \
+          th->d_fidx, th->d_wgrid, L.block0, th->nf, th->nslot, th->d_s2, im->d_hdr,         \
+          im->d_coef, im->d_coef32, th->d_goff, th->emiss, Fgeo, nsteps, dQ, th->d_Qprev,    \
+          th->d_T, th->d_F, Tss);                                                            \
+    else                                                                                     \
+      cn_fp64_kernel<NZ, GMREG><<<nb, 64, 0, s>>>(ILLUM_CN_ARGS, th->d_work, Tss,            \
+                                                  th->d_status, th->nzmax + 1);              \
     break;
-      ILLUM_CN_REG(32)
-      ILLUM_CN_REG(64)
-      ILLUM_CN_REG(96)
-      ILLUM_CN_REG(128)
+      ILLUM_CN_REG(32, true)
+      ILLUM_CN_REG(64, true)
+      ILLUM_CN_REG(96, true)
+      ILLUM_CN_REG(128, false)
 #undef ILLUM_CN_REG
       default:
         cn_stream_kernel<MIXED><<<nb, 64, 0, s>>>(ILLUM_CN_ARGS, th->d_work, th->d_work32, Tss,
@@ -668,6 +758,27 @@ void thermal_get(illum_thermal *th, double *T, double *Tsurf, double *Fsurf, int
   if (Fsurf)
     ILLUM_HIP(hipMemcpyAsync(Fsurf, th->d_F, 8 * nf, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
   sync(ctx);
+}
+
+void thermal_get_radiosity(illum_thermal *th, double *out, int flags) {
+  illum_ctx *ctx = th->ctx;
+  hipStream_t s = stream_of(ctx);
+  const int64_t nf = th->nf;
+  const bool dev = flags & ILLUM_OUT_ON_DEVICE;
+  const bool acc = flags & ILLUM_ACCUMULATE;
+  DBuf<double> tmp;
+  double *dst = out;
+  if (!dev) {
+    tmp.reset(nf);
+    dst = tmp.p;
+    if (acc) ILLUM_HIP(hipMemcpyAsync(dst, out, 8 * nf, hipMemcpyHostToDevice, s));
+  }
+  neg_flux_kernel<<<(unsigned)((nf + 255) / 256), 256, 0, s>>>(nf, th->d_F, dst, acc ? 1 : 0);
+  ILLUM_LAUNCH_CHECK();
+  if (!dev) {
+    ILLUM_HIP(hipMemcpyAsync(out, dst, 8 * nf, hipMemcpyDeviceToHost, s));
+    sync(ctx);
+  }
 }
 
 }  // namespace illum

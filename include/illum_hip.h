@@ -47,10 +47,11 @@ extern "C" {
 /* ---- flags ------------------------------------------------------------- */
 #define ILLUM_IN_ON_DEVICE   1   /* input arrays are device pointers  */
 #define ILLUM_OUT_ON_DEVICE  2   /* output arrays are device pointers */
+#define ILLUM_ACCUMULATE     4   /* add into the output instead of overwriting it */
 
 /* radiosity solve methods */
 #define ILLUM_RAD_JACOBI_FP64   1   /* FP64 Jacobi sweeps B <- E + F B                  */
-#define ILLUM_RAD_MIXED         2   /* FP32 Jacobi sweeps + FP64 residual refinement    */
+#define ILLUM_RAD_MIXED         2   /* FP32 correction sweeps, FP64 residual refinement */
 
 /* thermal precision modes */
 #define ILLUM_THERM_FP64        1   /* FP64 Crank-Nicolson, FP64 Thomas                  */
@@ -209,6 +210,10 @@ int illum_thermal_run(illum_thermal *th, double dt, const double *Qseries, int n
  * Fsurf: nf; host or device per flags; any may be null. */
 int illum_thermal_get(illum_thermal *th, double *T, double *Tsurf, double *Fsurf,
                       int flags);
+/* thermal_model::get_radiosity (thermal.hpp:61-63): out = -Fsurf (nf
+ * doubles), or out += -Fsurf with ILLUM_ACCUMULATE (the rad = direct -
+ * Fsurf sum of illum_cli.cpp:411-423 without leaving the device). */
+int illum_thermal_get_radiosity(illum_thermal *th, double *out, int flags);
 
 /* Single-facet drop-in for conductionQ (conduction.hpp:16-28), run on the
  * GPU.  T (nz) and Tsurf, Fsurf are updated in place. */
