@@ -173,6 +173,20 @@ static illum_ctx *default_ctx() {
   return ctx;
 }
 
+void *ctx_workspace(illum_ctx *ctx, int slot, size_t bytes) {
+  if ((int)ctx->workspace.size() <= slot) ctx->workspace.resize(slot + 1, {nullptr, 0});
+  auto &w = ctx->workspace[slot];
+  if (w.second < bytes) {
+    ILLUM_HIP(hipStreamSynchronize(stream_of(ctx)));
+    if (w.first) ILLUM_HIP(hipFree(w.first));
+    w.first = nullptr;
+    w.second = 0;
+    ILLUM_HIP(hipMalloc(&w.first, bytes));
+    w.second = bytes;
+  }
+  return w.first;
+}
+
 }  // namespace illum
 
 using namespace illum;
@@ -207,6 +221,7 @@ int illum_ctx_destroy(illum_ctx *ctx) {
     (void)hipStreamSynchronize(stream_of(ctx));
     collect_profile(ctx);
     (void)hipStreamDestroy(stream_of(ctx));
+    for (auto &w : ctx->workspace) (void)hipFree(w.first);
     delete ctx;
   });
 }

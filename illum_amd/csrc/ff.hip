@@ -29,7 +29,7 @@ namespace illum {
 constexpr int FF_BLOCK = 256;
 constexpr int FF_PER = 4;                          // candidates per thread per chunk
 constexpr int FF_QCAP = FF_BLOCK * (FF_PER + 1);   // queue capacity (ints)
-constexpr int FF_STACK = 32;                       // traversal stack entries per lane
+constexpr int FF_STACK = 8;  // LDS traversal stack entries per lane (deeper: threaded walk)
 // First pass of each visibility ray prunes at this distance: the ray aims at a
 // centroid lying on its target triangle, so the closest hit is almost always
 // at t <= 1; a ray with no hit below the bound is traced again unbounded,
@@ -59,12 +59,14 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
     const double *__restrict__ area, const double *__restrict__ albedo_vec, double albedo,
     uint32_t *__restrict__ vis, uint32_t *__restrict__ mask, int32_t *__restrict__ counts,
     int *__restrict__ status, int nrows, int jstride, unsigned long long *__restrict__ nrays,
-    const float4 *__restrict__ clus, int nclus) {
+    const float4 *__restrict__ clus, int nclus, int scap) {
   extern __shared__ uint32_t smem[];
   uint32_t *bits = smem;
   int *queue = (int *)(smem + nwords);
-  int *clist = queue + FF_QCAP;  // surviving receiver clusters (nclus slots)
-  int *stack = clist + nclus;    // (tree depth + 1) x FF_BLOCK slots, used when WIDE
+  // surviving receiver clusters (16-bit ids: the LDS bitset bounds nf far
+  // below 64 x 65536), then scap x FF_BLOCK stack slots (WIDE)
+  uint16_t *clist = (uint16_t *)(queue + FF_QCAP);
+  int *stack = (int *)(clist + ((nclus + 1) & ~1));
   __shared__ int s_qn, s_rays, s_nc;
   __shared__ int s_red[FF_BLOCK / 64];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -98,7 +100,7 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
       int slot = 0;
       if (lane == 0) slot = atomicAdd(&s_nc, __popcll(m));
       slot = __shfl(slot, 0);
-      if (keep) clist[slot + __popcll(m & lanemask_lt())] = c;
+      if (keep) clist[slot + __popcll(m & lanemask_lt())] = (uint16_t)c;
     }
   }
   __syncthreads();
@@ -142,9 +144,13 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
         float t;
         int h;
         if (WIDE) {
-          h = trace_closest_wide(wide, tris, r, FF_TINIT, stack + tid, FF_BLOCK, &t);
-          if (h < 0)
-            h = trace_closest_wide(wide, tris, r, __builtin_huge_valf(), stack + tid, FF_BLOCK, &t);
+          h = trace_closest_wide(wide, tris, r, FF_TINIT, stack + tid, FF_BLOCK, scap, &t);
+          if (h == -1)
+            h = trace_closest_wide(wide, tris, r, __builtin_huge_valf(), stack + tid, FF_BLOCK,
+                                   scap, &t);
+          // a walk deeper than the LDS stack: the stackless walk answers the
+          // same unbounded query (identical result)
+          if (h == -2) h = trace_closest(nodes, tris, r, &t);
         } else {
           h = trace_closest(nodes, tris, r, &t);
         }
@@ -322,18 +328,29 @@ __global__ __launch_bounds__(256) void closest_hit_kernel(
   const DRay r = make_ray(orig[3 * k], orig[3 * k + 1], orig[3 * k + 2], dir[3 * k],
                           dir[3 * k + 1], dir[3 * k + 2]);
   float t;
-  hit[k] = wide ? trace_closest_wide(wide, tris, r, __builtin_huge_valf(), stack + threadIdx.x,
-                                     256, &t)
-                : trace_closest(nodes, tris, r, &t);
+  int h = -2;
+  if (wide)
+    h = trace_closest_wide(wide, tris, r, __builtin_huge_valf(), stack + threadIdx.x, 256,
+                           FF_STACK, &t);
+  if (h == -2) h = trace_closest(nodes, tris, r, &t);  // walk deeper than the LDS stack
+  hit[k] = h;
   tout[k] = t;
 }
 
-// The near-first layout is used when the tree fits the traversal stack and
-// ILLUM_FF_TRAVERSAL is not "threaded".
+// The near-first layout is used unless ILLUM_FF_TRAVERSAL is "threaded".  Its
+// LDS stack holds min(depth + 1, FF_STACK) entries per lane; a walk that needs
+// more is answered by the stackless threaded walk (same result).
 static bool use_wide(const illum_bvh *bvh) {
   const char *e = getenv("ILLUM_FF_TRAVERSAL");
   if (e && std::string(e) == "threaded") return false;
-  return bvh->d_wide && bvh->h.depth <= FF_STACK;
+  return bvh->d_wide != nullptr;
+}
+
+// Dynamic LDS of ff_visibility_kernel: bitset, ray queue, 16-bit cluster list
+// (rounded to whole words), stack.
+static size_t ff_lds_bytes(int nwords, int64_t nclus, int stack_slots) {
+  return (size_t)(nwords + FF_QCAP) * 4 + (size_t)((nclus + 1) & ~1) * 2 +
+         (size_t)stack_slots * FF_BLOCK * 4;
 }
 
 // The cluster pass of the facing cull is on unless ILLUM_FF_CLUSTER_CULL=0
@@ -374,8 +391,8 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   const int64_t rows_p = (int64_t)nwords * 32;
   const bool wide = use_wide(bvh);
   // traversal stack: one slot per tree level (a push happens at most once per level)
-  const int stack_slots = wide ? bvh->h.depth + 1 : 0;
-  const size_t lds = (size_t)(nwords + FF_QCAP + mesh->nclus + stack_slots * FF_BLOCK) * 4;
+  const int stack_slots = wide ? std::min(bvh->h.depth + 1, FF_STACK) : 0;
+  const size_t lds = ff_lds_bytes(nwords, mesh->nclus, stack_slots);
   if (lds > 160 * 1024)
     throw Error(ILLUM_ERR_CAPACITY, "too many facets for the LDS visibility bitset (max ~1.2M)");
   const void *kfn = wide ? (const void *)ff_visibility_kernel<true>
@@ -394,10 +411,15 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
     d_alb.reset(nf);
     ILLUM_HIP(hipMemcpyAsync(d_alb.p, albedo_vec, 8 * nf, hipMemcpyHostToDevice, s));
   }
-  DBuf<uint32_t> M((size_t)rows_p * nwords), MT((size_t)rows_p * nwords);
-  DBuf<int32_t> counts(rows_p);
+  // M and its transpose live in the context's workspace.  Rows >= nf of M
+  // (padding to whole 512-row tiles) must read as zero for the transpose.
+  const size_t mbytes = (size_t)rows_p * nwords * 4;
+  uint32_t *M = (uint32_t *)ctx_workspace(ctx, WS_FF_MASK, mbytes);
+  uint32_t *MT = (uint32_t *)ctx_workspace(ctx, WS_FF_MASK_T, mbytes);
+  int32_t *counts = (int32_t *)ctx_workspace(ctx, WS_FF_COUNTS, 4 * rows_p);
   DBuf<int> status(1);
-  ILLUM_HIP(hipMemsetAsync(M.p, 0, (size_t)rows_p * nwords * 4, s));
+  if (rows_p > nf)
+    ILLUM_HIP(hipMemsetAsync(M + (size_t)nf * nwords, 0, (size_t)(rows_p - nf) * nwords * 4, s));
   ILLUM_HIP(hipMemsetAsync(status.p, 0, 4, s));
   sp->d_nrays = dalloc<unsigned long long>(1);
   ILLUM_HIP(hipMemsetAsync(sp->d_nrays, 0, 8, s));
@@ -408,16 +430,16 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
     launch<<<(int)nf, FF_BLOCK, lds, s>>>(
         (int)nf, nwords, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4,
         (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_wide, (const float4 *)bvh->d_tris,
-        (float)offset, mesh->d_area, d_alb.p, albedo, sp->d_vis, M.p, counts.p, status.p,
+        (float)offset, mesh->d_area, d_alb.p, albedo, sp->d_vis, M, counts, status.p,
         (int)nf, 1, sp->d_nrays, cluster_cull() ? (const float4 *)mesh->d_clus : nullptr,
-        (int)mesh->nclus);
+        (int)mesh->nclus, stack_slots);
     ILLUM_LAUNCH_CHECK();
   }
   // CSC
   sp->d_colptr = dalloc<int64_t>(nf + 1);
   {
     StageTimer st(ctx, "ff_csc");
-    scan_counts(ctx, counts.p, nf, sp->d_colptr);
+    scan_counts(ctx, counts, nf, sp->d_colptr);
   }
   int64_t nnz = 0;
   int hstatus = 0;
@@ -432,7 +454,7 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   sp->d_cval = dalloc<double>(nnz);
   {
     StageTimer st(ctx, "ff_csc");
-    ff_fill_kernel<<<(int)nf, 256, 0, s>>>((int)nf, nwords, M.p, sp->d_colptr, 0,
+    ff_fill_kernel<<<(int)nf, 256, 0, s>>>((int)nf, nwords, M, sp->d_colptr, 0,
                                            (const float4 *)mesh->d_pr4,
                                            (const float4 *)mesh->d_n4, mesh->d_area, d_alb.p,
                                            albedo, sp->d_rowidx, sp->d_cval, nullptr);
@@ -446,12 +468,12 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   {
     StageTimer st(ctx, "ff_csr");
     dim3 g(nwords / 16, (unsigned)(rows_p / 512));
-    bit_transpose_kernel<<<g, 256, 0, s>>>(M.p, MT.p, nwords);
+    bit_transpose_kernel<<<g, 256, 0, s>>>(M, MT, nwords);
     ILLUM_LAUNCH_CHECK();
-    popcount_rows_kernel<<<(int)((nf + 3) / 4), 256, 0, s>>>((int)nf, nwords, MT.p, counts.p);
+    popcount_rows_kernel<<<(int)((nf + 3) / 4), 256, 0, s>>>((int)nf, nwords, MT, counts);
     ILLUM_LAUNCH_CHECK();
-    scan_counts(ctx, counts.p, nf, sp->d_rowptr);
-    ff_fill_kernel<<<(int)nf, 256, 0, s>>>((int)nf, nwords, MT.p, sp->d_rowptr, 1,
+    scan_counts(ctx, counts, nf, sp->d_rowptr);
+    ff_fill_kernel<<<(int)nf, 256, 0, s>>>((int)nf, nwords, MT, sp->d_rowptr, 1,
                                            (const float4 *)mesh->d_pr4,
                                            (const float4 *)mesh->d_n4, mesh->d_area, d_alb.p,
                                            albedo, sp->d_colidx, sp->d_rval, sp->d_rval32);
@@ -472,8 +494,8 @@ double time_visibility_sample(illum_bvh *bvh, int max_sources) {
   const int nrows = (int)((nf + jstride - 1) / jstride);
   const int nwords = (int)(((nf + 511) / 512) * 16);
   const bool wide = use_wide(bvh);
-  const int stack_slots = wide ? bvh->h.depth + 1 : 0;
-  const size_t lds = (size_t)(nwords + FF_QCAP + mesh->nclus + stack_slots * FF_BLOCK) * 4;
+  const int stack_slots = wide ? std::min(bvh->h.depth + 1, FF_STACK) : 0;
+  const size_t lds = ff_lds_bytes(nwords, mesh->nclus, stack_slots);
   if (lds > 160 * 1024) throw Error(ILLUM_ERR_CAPACITY, "too many facets for the LDS bitset");
   const void *kfn = wide ? (const void *)ff_visibility_kernel<true>
                          : (const void *)ff_visibility_kernel<false>;
@@ -493,7 +515,7 @@ double time_visibility_sample(illum_bvh *bvh, int max_sources) {
         (int)nf, nwords, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4,
         (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_wide, (const float4 *)bvh->d_tris,
         1e-5f, mesh->d_area, nullptr, 1.0, vis.p, M.p, counts.p, status.p, nrows, jstride, nullptr,
-        cluster_cull() ? (const float4 *)mesh->d_clus : nullptr, (int)mesh->nclus);
+        cluster_cull() ? (const float4 *)mesh->d_clus : nullptr, (int)mesh->nclus, stack_slots);
     ILLUM_LAUNCH_CHECK();
     ILLUM_HIP(hipEventRecord(e1, s));
     ILLUM_HIP(hipEventSynchronize(e1));

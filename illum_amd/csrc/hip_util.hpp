@@ -73,6 +73,11 @@ struct StageTimer {
 void destroy_sparse(illum_sparse *s);
 void destroy_thermal(illum_thermal *t);
 
+// Device scratch slot `slot` of the context, at least `bytes` long (grown by
+// reallocation after the stream drains; contents are not preserved).
+void *ctx_workspace(illum_ctx *ctx, int slot, size_t bytes);
+enum WorkspaceSlot { WS_FF_MASK = 0, WS_FF_MASK_T = 1, WS_FF_COUNTS = 2 };
+
 // Drains pending stage events into the profile (after the stream is synced).
 void collect_profile(illum_ctx *ctx);
 void sync(illum_ctx *ctx);

@@ -88,6 +88,10 @@ struct illum_ctx {
   struct Stage { double ms = 0; int64_t launches = 0; };
   std::map<std::string, Stage> profile;
   std::vector<std::pair<std::string, std::pair<void *, void *>>> pending;  // events
+  // Grow-only device scratch kept between calls (slot -> pointer, bytes), so
+  // repeated calls skip large allocations.  Calls on one context are issued
+  // from one host thread at a time, in stream order.
+  std::vector<std::pair<void *, size_t>> workspace;
 };
 
 struct illum_mesh {

@@ -155,58 +155,74 @@ __device__ __forceinline__ bool box_enter(const DRay &r, float4 lo, float4 hi, f
   return tn <= tf && tf > 0.0f && tn <= tbest;
 }
 
-// Near-first closest hit over the two-wide layout.  `stack` points at this
-// lane's first slot of an LDS stack laid out slot-major (entry k of the lane
-// at stack[k * stride]) so lanes at equal depth touch consecutive words.
-// The tree depth must not exceed the stack capacity (checked on the host).
-// Boxes are pruned against the best distance so far, starting from tinit;
-// because every ancestor box enters no later than any hit inside it, the
-// result equals the argmin over all triangles with distance <= tinit.
+// One step of the near-first closest-hit walk over the two-wide layout: one
+// inner node (descend into the nearer hit child, pushing the farther), or one
+// leaf followed by a pop.  Returns true when the walk has finished.  `stack`
+// points at this lane's first slot of an LDS stack laid out slot-major (entry
+// k of the lane at stack[k * stride]) so lanes at equal depth touch
+// consecutive words.  A push beyond `cap` entries sets sp = -1 and ends the
+// walk (the caller then answers the query another way).  Boxes are pruned
+// against the best distance so far; because every ancestor box enters no
+// later than any hit inside it, a walk started with tbest = tinit ends with
+// the argmin over all triangles with distance <= tinit.
+__device__ __forceinline__ bool closest_wide_step(const float4 *__restrict__ wide,
+                                                  const float4 *__restrict__ tris,
+                                                  const DRay &r, int &cur, int &sp,
+                                                  float &tbest, int &best, int *stack,
+                                                  int stride, int cap) {
+  if (cur >= 0) {
+    const float4 a0 = wide[4 * cur], a1 = wide[4 * cur + 1];
+    const float4 b0 = wide[4 * cur + 2], b1 = wide[4 * cur + 3];
+    float tl, tr;
+    const bool hl = box_enter(r, a0, a1, tbest, &tl);
+    const bool hr = box_enter(r, b0, b1, tbest, &tr);
+    const int cl = __float_as_int(a0.w), cr = __float_as_int(b0.w);
+    if (hl && hr) {
+      if (sp == cap) {
+        sp = -1;
+        return true;
+      }
+      const bool lnear = tl <= tr;
+      stack[sp * stride] = lnear ? cr : cl;
+      ++sp;
+      cur = lnear ? cl : cr;
+      return false;
+    }
+    if (hl) { cur = cl; return false; }
+    if (hr) { cur = cr; return false; }
+  } else {
+    const int info = ~cur;
+    const int first = info & 0xFFFFFF, cnt = info >> 24;
+    for (int k = 0; k < cnt; ++k) {
+      const float4 a = tris[3 * (first + k)], b = tris[3 * (first + k) + 1],
+                   c = tris[3 * (first + k) + 2];
+      const int idx = __float_as_int(a.w);
+      float t;
+      if (tri_hit(r, a, b, c, &t) && (t < tbest || (t == tbest && idx < best))) {
+        tbest = t;
+        best = idx;
+      }
+    }
+  }
+  if (sp == 0) return true;
+  --sp;
+  cur = stack[sp * stride];
+  return false;
+}
+
+// Near-first closest hit over the two-wide layout (closest_wide_step to the
+// end of the walk); returns the original facet index, -1 for no hit, or -2
+// when the walk needed more than `cap` stack entries.
 __device__ __forceinline__ int trace_closest_wide(const float4 *__restrict__ wide,
                                                   const float4 *__restrict__ tris,
                                                   const DRay &r, float tinit, int *stack,
-                                                  int stride, float *tbest_out) {
+                                                  int stride, int cap, float *tbest_out) {
   float tbest = tinit;
-  int best = -1;
-  int sp = 0;
-  int cur = 0;
-  for (;;) {
-    if (cur >= 0) {
-      const float4 a0 = wide[4 * cur], a1 = wide[4 * cur + 1];
-      const float4 b0 = wide[4 * cur + 2], b1 = wide[4 * cur + 3];
-      float tl, tr;
-      const bool hl = box_enter(r, a0, a1, tbest, &tl);
-      const bool hr = box_enter(r, b0, b1, tbest, &tr);
-      const int cl = __float_as_int(a0.w), cr = __float_as_int(b0.w);
-      if (hl && hr) {
-        const bool lnear = tl <= tr;
-        stack[sp * stride] = lnear ? cr : cl;
-        ++sp;
-        cur = lnear ? cl : cr;
-        continue;
-      }
-      if (hl) { cur = cl; continue; }
-      if (hr) { cur = cr; continue; }
-    } else {
-      const int info = ~cur;
-      const int first = info & 0xFFFFFF, cnt = info >> 24;
-      for (int k = 0; k < cnt; ++k) {
-        const float4 a = tris[3 * (first + k)], b = tris[3 * (first + k) + 1],
-                     c = tris[3 * (first + k) + 2];
-        const int idx = __float_as_int(a.w);
-        float t;
-        if (tri_hit(r, a, b, c, &t) && (t < tbest || (t == tbest && idx < best))) {
-          tbest = t;
-          best = idx;
-        }
-      }
-    }
-    if (sp == 0) break;
-    --sp;
-    cur = stack[sp * stride];
+  int best = -1, sp = 0, cur = 0;
+  while (!closest_wide_step(wide, tris, r, cur, sp, tbest, best, stack, stride, cap)) {
   }
   *tbest_out = tbest;
-  return best;
+  return sp < 0 ? -2 : best;
 }
 
 // Any hit (t > 0) over the two-wide layout, nearer child first (stack as in
