@@ -10,7 +10,7 @@ Secondary measurements on the same JSON line (skip with --quick):
   horizons   360 azimuths (nphi 361), theta_eps 1e-3 on the same mesh: rays/s
   radiosity  512 sun directions on the +z hemisphere, E from the direct
              insolation, (I - F) B = E to 1e-8 relative residual, FP64 Jacobi
-             vs mixed (FP32 sweeps + FP64 refinement)
+             vs mixed (FP32-gathered corrections, FP64 residual refinement)
   thermal    327,680 facets, 4,096 CN steps over 4 rotations, layer counts
              U{32..96}, thermal inertia lognormal (median 200, sigma 0.5),
              FP64 vs mixed
@@ -31,6 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFS = 157.3
 # Crank-Nicolson work per cell-step and the vector peak it is priced against:
 # FP64 Thomas step (right-hand side 9, elimination 6 + 2 divisions, back
 # substitution 2) on FP64 vector 78.6 TFLOP/s; mixed step (FP64 difference and
@@ -364,12 +365,18 @@ def main():
     if os.path.exists(pmc_path):
         pmc = json.load(open(pmc_path))
         traffic = (pmc["fetch_kib"] * pmc["fetch_scale"] + pmc["write_kib"] * pmc["write_scale"]) * 1024
+    # VALU work per launch from the kernel's own counters: 8 FLOP per facing
+    # test, 12 per ray/box slab test, 40 per ray/triangle test.
+    st = illum.form_factor_work(offset)  # separate instrumented launch
+    flops = 8 * st["pair_tests"] + 12 * st["box_tests"] + 40 * st["tri_tests"]
+    tflops = flops / (vis_ms * 1e-3) / 1e12
     roofline = {"bound": "hbm", "kernel": "ff_visibility", "achieved": achieved,
                 "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                 "traffic": traffic, "avg_ms": vis_ms,
                 "note": "latency/VALU-bound ray traversal: HBM is not the limit",
                 "rays": rays, "rays_per_s": rays / (vis_ms * 1e-3),
-                "l2_stream_GBs": pairs * 16 / (vis_ms * 1e-3) / 1e9}
+                "valu": {"work": st, "flop": flops, "achieved": tflops,
+                         "peak": PEAK_FP32_TFS, "unit": "TFLOP/s", "frac": tflops / PEAK_FP32_TFS}}
 
     secondary = {}
     if not args.quick:

@@ -14,9 +14,9 @@ Reference names kept:
   conductionQ                       conduction.hpp:16-28
   gauss_seidel-style solve          radiosity_solve (illum_cli.cpp:425-440)
 
-Arrays are numpy (host) by default; torch CUDA tensors are passed as device
-pointers (no host staging).  Every compute call runs on the GPU through
-libillum_hip; nothing here computes results on the CPU.
+Arrays are numpy (host) by default; DeviceArray (HBM allocated by the library)
+is passed as a device pointer with no host staging.  Every compute call runs
+on the GPU through libillum_hip; nothing here computes results on the CPU.
 """
 import ctypes as C
 
@@ -296,6 +296,7 @@ class SparseMatrix:
         call("illum_sparse_rays", self.h, C.byref(n))
         return n.value
 
+
     def to_scipy(self):
         import scipy.sparse as sps
         cp, ri, v = self.csc()
@@ -348,6 +349,13 @@ class IllumContext:
                 raise ValueError("albedo vector must have one entry per facet")
             call("illum_form_factors", self.bvh.h, 0.0, _ptr(a), offset, C.byref(h))
         return SparseMatrix(self.ctx, h)
+
+    def form_factor_work(self, offset=1e-5):
+        """Work of one visibility pass (separate instrumented launch): rays, pair
+        (facing) tests after the cluster pass, box tests, triangle tests."""
+        out = np.zeros(4, np.int64)
+        call("illum_form_factors_work", self.bvh.h, offset, _ptr(out))
+        return dict(zip(("rays", "pair_tests", "box_tests", "tri_tests"), map(int, out)))
 
     def make_horizons(self, nphi=361, theta_eps=1e-3, offset=1e-5, out=None):
         occ = self._horizon_ctx[1].h if self._horizon_ctx else None

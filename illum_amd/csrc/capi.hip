@@ -14,6 +14,7 @@ namespace illum {
 double time_visibility_sample(illum_bvh *bvh, int max_sources);
 illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_vec,
                            double offset);
+void form_factor_work(illum_bvh *bvh, double offset, int64_t *out4);
 void bvh_closest_hit(illum_bvh *bvh, const float *orig, const float *dir, int64_t nray,
                      int32_t *hit, float *t);
 void horizons(illum_bvh *bvh, illum_bvh *occl, int nphi, double theta_eps, double offset,
@@ -455,6 +456,15 @@ int illum_form_factors(illum_bvh *bvh, double albedo, const double *albedo_vec, 
     CHECK_ARG(std::isfinite(offset), "offset must be finite");
     set_device(bvh->mesh->ctx);
     *out = form_factors(bvh, albedo, albedo_vec, offset);
+  });
+}
+
+int illum_form_factors_work(illum_bvh *bvh, double offset, int64_t *out4) {
+  return guarded([&] {
+    CHECK_ARG(bvh && out4, "null argument");
+    CHECK_ARG(std::isfinite(offset), "offset must be finite");
+    set_device(bvh->mesh->ctx);
+    form_factor_work(bvh, offset, out4);
   });
 }
 

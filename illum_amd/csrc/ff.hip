@@ -51,14 +51,16 @@ __device__ __forceinline__ double ff_value(float4 pi, float4 ni, float4 pj, floa
   return ((mij * mji) * Aj) / ((M_PI * r) * r);
 }
 
-template <bool WIDE>
+// COUNT adds the work counters (pair, box and triangle tests) to stats[1..3];
+// stats[0] always receives the number of rays.
+template <bool WIDE, bool COUNT>
 __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
     int nf, int nwords, const float4 *__restrict__ pr4, const float4 *__restrict__ n4,
     const float4 *__restrict__ nodes, const float4 *__restrict__ wide,
     const float4 *__restrict__ tris, float offset,
     const double *__restrict__ area, const double *__restrict__ albedo_vec, double albedo,
     uint32_t *__restrict__ vis, uint32_t *__restrict__ mask, int32_t *__restrict__ counts,
-    int *__restrict__ status, int nrows, int jstride, unsigned long long *__restrict__ nrays,
+    int *__restrict__ status, int nrows, int jstride, unsigned long long *__restrict__ stats,
     const float4 *__restrict__ clus, int nclus, int scap) {
   extern __shared__ uint32_t smem[];
   uint32_t *bits = smem;
@@ -76,6 +78,7 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
 
   for (int w = tid; w < nwords; w += FF_BLOCK) bits[w] = 0u;
   if (tid == 0) { s_qn = 0; s_rays = 0; s_nc = 0; }
+  unsigned npair = 0, nbox = 0, ntri = 0;  // work counters of this thread
   const float4 pj = pr4[j], nj = n4[j];
   const float qx = pj.x + offset * nj.x, qy = pj.y + offset * nj.y, qz = pj.z + offset * nj.z;
   __syncthreads();
@@ -116,6 +119,7 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
       const int i = ci < ncl ? clist[ci] * 64 + lane : nf;
       bool s = false;
       if (i < nf && i != j) {
+        ++npair;
         const float4 pi = pr4[i];
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float dot = (dx * nj.x + dy * nj.y) + dz * nj.z;
@@ -144,15 +148,16 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
         float t;
         int h;
         if (WIDE) {
-          h = trace_closest_wide(wide, tris, r, FF_TINIT, stack + tid, FF_BLOCK, scap, &t);
+          h = trace_closest_wide(wide, tris, r, FF_TINIT, stack + tid, FF_BLOCK, scap, &t,
+                                 nbox, ntri);
           if (h == -1)
             h = trace_closest_wide(wide, tris, r, __builtin_huge_valf(), stack + tid, FF_BLOCK,
-                                   scap, &t);
+                                   scap, &t, nbox, ntri);
           // a walk deeper than the LDS stack: the stackless walk answers the
           // same unbounded query (identical result)
-          if (h == -2) h = trace_closest(nodes, tris, r, &t);
+          if (h == -2) h = trace_closest(nodes, tris, r, &t, nbox, ntri);
         } else {
-          h = trace_closest(nodes, tris, r, &t);
+          h = trace_closest(nodes, tris, r, &t, nbox, ntri);
         }
         if (h >= 0 && h != j) atomicOr(&bits[h >> 5], 1u << (h & 31));
       }
@@ -192,7 +197,17 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
     int s = 0;
     for (int w = 0; w < FF_BLOCK / 64; ++w) s += s_red[w];
     counts[row] = s;
-    if (nrays) atomicAdd(nrays, (unsigned long long)s_rays);
+  }
+  if (stats && tid == 0) atomicAdd(stats, (unsigned long long)s_rays);
+  if (COUNT && stats) {
+    // pair tests, box tests, triangle tests: wave sums, one atomic each.
+    // Without COUNT the counters are dead and compile away.
+    unsigned long long w3[3] = {npair, nbox, ntri};
+    for (int q = 0; q < 3; ++q) {
+      unsigned long long v = w3[q];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+      if (lane == 0 && v) atomicAdd(stats + 1 + q, v);
+    }
   }
 }
 
@@ -329,10 +344,11 @@ __global__ __launch_bounds__(256) void closest_hit_kernel(
                           dir[3 * k + 1], dir[3 * k + 2]);
   float t;
   int h = -2;
+  unsigned nbox = 0, ntri = 0;
   if (wide)
     h = trace_closest_wide(wide, tris, r, __builtin_huge_valf(), stack + threadIdx.x, 256,
-                           FF_STACK, &t);
-  if (h == -2) h = trace_closest(nodes, tris, r, &t);  // walk deeper than the LDS stack
+                           FF_STACK, &t, nbox, ntri);
+  if (h == -2) h = trace_closest(nodes, tris, r, &t, nbox, ntri);  // deeper than the LDS stack
   hit[k] = h;
   tout[k] = t;
 }
@@ -395,8 +411,8 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   const size_t lds = ff_lds_bytes(nwords, mesh->nclus, stack_slots);
   if (lds > 160 * 1024)
     throw Error(ILLUM_ERR_CAPACITY, "too many facets for the LDS visibility bitset (max ~1.2M)");
-  const void *kfn = wide ? (const void *)ff_visibility_kernel<true>
-                         : (const void *)ff_visibility_kernel<false>;
+  const void *kfn = wide ? (const void *)ff_visibility_kernel<true, false>
+                         : (const void *)ff_visibility_kernel<false, false>;
   if (lds > 64 * 1024)
     ILLUM_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 
@@ -426,7 +442,7 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
 
   {
     StageTimer st(ctx, "ff_visibility");
-    auto launch = wide ? ff_visibility_kernel<true> : ff_visibility_kernel<false>;
+    auto launch = wide ? ff_visibility_kernel<true, false> : ff_visibility_kernel<false, false>;
     launch<<<(int)nf, FF_BLOCK, lds, s>>>(
         (int)nf, nwords, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4,
         (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_wide, (const float4 *)bvh->d_tris,
@@ -497,8 +513,8 @@ double time_visibility_sample(illum_bvh *bvh, int max_sources) {
   const int stack_slots = wide ? std::min(bvh->h.depth + 1, FF_STACK) : 0;
   const size_t lds = ff_lds_bytes(nwords, mesh->nclus, stack_slots);
   if (lds > 160 * 1024) throw Error(ILLUM_ERR_CAPACITY, "too many facets for the LDS bitset");
-  const void *kfn = wide ? (const void *)ff_visibility_kernel<true>
-                         : (const void *)ff_visibility_kernel<false>;
+  const void *kfn = wide ? (const void *)ff_visibility_kernel<true, false>
+                         : (const void *)ff_visibility_kernel<false, false>;
   if (lds > 64 * 1024)
     ILLUM_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   DBuf<uint32_t> vis((size_t)nrows * nwords), M((size_t)nrows * nwords);
@@ -507,7 +523,7 @@ double time_visibility_sample(illum_bvh *bvh, int max_sources) {
   hipEvent_t e0, e1;
   ILLUM_HIP(hipEventCreate(&e0));
   ILLUM_HIP(hipEventCreate(&e1));
-  auto launch = wide ? ff_visibility_kernel<true> : ff_visibility_kernel<false>;
+  auto launch = wide ? ff_visibility_kernel<true, false> : ff_visibility_kernel<false, false>;
   float best = 1e30f;
   for (int rep = 0; rep < 2; ++rep) {  // first launch also loads the code object
     ILLUM_HIP(hipEventRecord(e0, s));
@@ -526,6 +542,43 @@ double time_visibility_sample(illum_bvh *bvh, int max_sources) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return best;
+}
+
+// Work of one full visibility pass (all sources), counted by the COUNT
+// instantiation: out4 = {rays, pair tests, box tests, triangle tests}.  The
+// assembly kernel itself carries no counters; this is a separate launch.
+void form_factor_work(illum_bvh *bvh, double offset, int64_t *out4) {
+  illum_mesh *mesh = bvh->mesh;
+  illum_ctx *ctx = mesh->ctx;
+  hipStream_t s = stream_of(ctx);
+  const int64_t nf = mesh->g.nf;
+  const int nwords = (int)(((nf + 511) / 512) * 16);
+  const bool wide = use_wide(bvh);
+  const int stack_slots = wide ? std::min(bvh->h.depth + 1, FF_STACK) : 0;
+  const size_t lds = ff_lds_bytes(nwords, mesh->nclus, stack_slots);
+  if (lds > 160 * 1024) throw Error(ILLUM_ERR_CAPACITY, "too many facets for the LDS bitset");
+  const void *kfn = wide ? (const void *)ff_visibility_kernel<true, true>
+                         : (const void *)ff_visibility_kernel<false, true>;
+  if (lds > 64 * 1024)
+    ILLUM_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const size_t mbytes = (size_t)nf * nwords * 4;
+  uint32_t *vis = (uint32_t *)ctx_workspace(ctx, WS_FF_MASK_T, mbytes);
+  uint32_t *M = (uint32_t *)ctx_workspace(ctx, WS_FF_MASK, mbytes);
+  int32_t *counts = (int32_t *)ctx_workspace(ctx, WS_FF_COUNTS, 4 * nf);
+  DBuf<int> status(1);
+  DBuf<unsigned long long> stats(4);
+  ILLUM_HIP(hipMemsetAsync(stats.p, 0, 32, s));
+  auto launch = wide ? ff_visibility_kernel<true, true> : ff_visibility_kernel<false, true>;
+  launch<<<(int)nf, FF_BLOCK, lds, s>>>(
+      (int)nf, nwords, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4,
+      (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_wide, (const float4 *)bvh->d_tris,
+      (float)offset, mesh->d_area, nullptr, 1.0, vis, M, counts, status.p, (int)nf, 1, stats.p,
+      cluster_cull() ? (const float4 *)mesh->d_clus : nullptr, (int)mesh->nclus, stack_slots);
+  ILLUM_LAUNCH_CHECK();
+  unsigned long long h[4];
+  ILLUM_HIP(hipMemcpyAsync(h, stats.p, 32, hipMemcpyDeviceToHost, s));
+  sync(ctx);
+  for (int k = 0; k < 4; ++k) out4[k] = (int64_t)h[k];
 }
 
 }  // namespace illum

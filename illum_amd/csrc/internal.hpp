@@ -131,7 +131,8 @@ struct illum_sparse {
   float *d_rval32 = nullptr;
   int64_t vis_words = 0;
   uint32_t *d_vis = nullptr;   // raw visibility rows (union of first hits)
-  unsigned long long *d_nrays = nullptr;  // rays cast by the visibility kernel
+  // visibility-kernel work: rays, pair tests, box tests, triangle tests
+  unsigned long long *d_nrays = nullptr;
 };
 
 struct illum_thermal {

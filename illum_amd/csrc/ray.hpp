@@ -115,18 +115,22 @@ __device__ __forceinline__ bool tri_hit(const DRay &r, float4 a, float4 b, float
 }
 
 // Closest hit over the threaded BVH; returns the original facet index or -1.
+// nbox / ntri count the box and triangle tests made.
 __device__ __forceinline__ int trace_closest(const float4 *__restrict__ nodes,
                                              const float4 *__restrict__ tris, const DRay &r,
-                                             float *tbest_out) {
+                                             float *tbest_out, unsigned &nbox,
+                                             unsigned &ntri) {
   float tbest = __builtin_huge_valf();
   int best = -1;
   int node = 0;
   while (node >= 0) {
     const float4 lo = nodes[2 * node], hi = nodes[2 * node + 1];
     const int miss = __float_as_int(lo.w), info = __float_as_int(hi.w);
+    ++nbox;
     if (!box_visit(r, lo, hi, tbest)) { node = miss; continue; }
     if (info < 0) { node = node + 1; continue; }
     const int first = info & 0xFFFFFF, cnt = info >> 24;
+    ntri += cnt;
     for (int k = 0; k < cnt; ++k) {
       const float4 a = tris[3 * (first + k)], b = tris[3 * (first + k) + 1],
                    c = tris[3 * (first + k) + 2];
@@ -169,8 +173,10 @@ __device__ __forceinline__ bool closest_wide_step(const float4 *__restrict__ wid
                                                   const float4 *__restrict__ tris,
                                                   const DRay &r, int &cur, int &sp,
                                                   float &tbest, int &best, int *stack,
-                                                  int stride, int cap) {
+                                                  int stride, int cap, unsigned &nbox,
+                                                  unsigned &ntri) {
   if (cur >= 0) {
+    nbox += 2;
     const float4 a0 = wide[4 * cur], a1 = wide[4 * cur + 1];
     const float4 b0 = wide[4 * cur + 2], b1 = wide[4 * cur + 3];
     float tl, tr;
@@ -193,6 +199,7 @@ __device__ __forceinline__ bool closest_wide_step(const float4 *__restrict__ wid
   } else {
     const int info = ~cur;
     const int first = info & 0xFFFFFF, cnt = info >> 24;
+    ntri += cnt;
     for (int k = 0; k < cnt; ++k) {
       const float4 a = tris[3 * (first + k)], b = tris[3 * (first + k) + 1],
                    c = tris[3 * (first + k) + 2];
@@ -216,10 +223,12 @@ __device__ __forceinline__ bool closest_wide_step(const float4 *__restrict__ wid
 __device__ __forceinline__ int trace_closest_wide(const float4 *__restrict__ wide,
                                                   const float4 *__restrict__ tris,
                                                   const DRay &r, float tinit, int *stack,
-                                                  int stride, int cap, float *tbest_out) {
+                                                  int stride, int cap, float *tbest_out,
+                                                  unsigned &nbox, unsigned &ntri) {
   float tbest = tinit;
   int best = -1, sp = 0, cur = 0;
-  while (!closest_wide_step(wide, tris, r, cur, sp, tbest, best, stack, stride, cap)) {
+  while (!closest_wide_step(wide, tris, r, cur, sp, tbest, best, stack, stride, cap, nbox,
+                            ntri)) {
   }
   *tbest_out = tbest;
   return sp < 0 ? -2 : best;

@@ -141,6 +141,12 @@ int illum_bvh_closest_hit(illum_bvh *bvh, const float *orig, const float *dir,
  * else the scalar albedo).  Stored both as CSC (reference layout) and CSR. */
 int illum_form_factors(illum_bvh *bvh, double albedo, const double *albedo_vec,
                        double offset, illum_sparse **out);
+/* Work of one visibility pass over all sources, counted by a separate
+ * instrumented launch (the assembly kernel carries no counters): out4 =
+ * {rays, facing tests of candidate pairs after the cluster pass, ray/box
+ * slab tests, ray/triangle tests}.  Measurement aid, no reference
+ * counterpart. */
+int illum_form_factors_work(illum_bvh *bvh, double offset, int64_t *out4);
 int illum_sparse_create_csr(illum_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rowptr,
                             const int32_t *colidx, const double *val, illum_sparse **out);
 int illum_sparse_destroy(illum_sparse *sp);

@@ -73,6 +73,9 @@ def test_form_factors_vs_oracle(api, subdiv, ncraters, albedo):
     gvis = Fm.visibility()[:, : bits.shape[1]]
     assert np.array_equal(gvis, bits), "visibility sets differ"
     assert Fm.rays() == nrays  # same facing-cull survivors
+    st = ctx.form_factor_work(1e-5)
+    assert st["rays"] == nrays and nrays <= st["pair_tests"] <= len(F) * (len(F) - 1)
+    assert st["box_tests"] >= 2 * nrays and st["tri_tests"] >= nrays
     cp, ri, v = O.form_factors(o, bits, albedo=albedo)
     gcp, gri, gv = Fm.csc()
     assert np.array_equal(gcp, cp)
@@ -115,9 +118,15 @@ def test_cluster_cull_does_not_change_results(api, monkeypatch):
     on = api.IllumContext(V=V, F=F).compute_F(0.1)
     monkeypatch.setenv("ILLUM_FF_CLUSTER_CULL", "0")
     off = api.IllumContext(V=V, F=F).compute_F(0.1)
+    work_off = api.IllumContext(V=V, F=F).form_factor_work()
     monkeypatch.delenv("ILLUM_FF_CLUSTER_CULL")
+    work_on = api.IllumContext(V=V, F=F).form_factor_work()
     assert on.rays() == off.rays() > 0
     assert np.array_equal(on.visibility(), off.visibility())
+    nf = len(F)
+    assert work_off["pair_tests"] == nf * (nf - 1) and work_off["rays"] == off.rays()
+    assert work_on["pair_tests"] < nf * (nf - 1) // 4  # clusters behind the plane
+    assert work_on["rays"] == on.rays()
     # Shuffled facets make loose clusters; still bit-exact against the oracle
     # on the shuffled mesh (closest-hit ties go to the smaller index, so the
     # oracle must see the same order).
