@@ -53,6 +53,7 @@ SIGNATURES = {
     "illum_mesh_from_obj": [_P, C.c_char_p, _D, _PP],
     "illum_mesh_destroy": [_P],
     "illum_mesh_num_faces": [_P, _P],
+    "illum_mesh_set_bodies": [_P, _P, _I],
     "illum_mesh_geometry": [_P, _P, _P, _P, _P, _P],
     "illum_mesh_triangles": [_P, _P],
     "illum_bvh_build": [_P, _I, _PP],

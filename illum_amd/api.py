@@ -148,8 +148,24 @@ def synth_ellipsoid(subdiv, axes=(13.0, 11.0, 9.0), ncraters=40, depth=(0.05, 0.
     return V, F
 
 
+def concat_bodies(bodies):
+    """[(V, F), ...] -> (V, F, offsets): one mesh of several bodies, facet
+    offsets for Mesh(bodies=...)."""
+    Vs, Fs, off, vbase = [], [], [0], 0
+    for V, F in bodies:
+        Vs.append(np.asarray(V, np.float64))
+        Fs.append(np.asarray(F, np.int32) + vbase)
+        vbase += len(V)
+        off.append(off[-1] + len(F))
+    return np.concatenate(Vs), np.ascontiguousarray(np.concatenate(Fs)), np.asarray(off, np.int64)
+
+
 class Mesh:
-    def __init__(self, ctx, V=None, F=None, VN=None, vertex_divisor=1.0, obj_path=None):
+    """bodies: optional facet offsets (nbody + 1, from 0 to N): facets
+    [bodies[b], bodies[b+1]) form an independent body (illum_mesh_set_bodies)."""
+
+    def __init__(self, ctx, V=None, F=None, VN=None, vertex_divisor=1.0, obj_path=None,
+                 bodies=None):
         self.ctx = ctx
         h = C.c_void_p()
         if obj_path is not None:
@@ -164,6 +180,9 @@ class Mesh:
         n = C.c_int64()
         call("illum_mesh_num_faces", h, C.byref(n))
         self.nf = n.value
+        if bodies is not None:
+            off = np.ascontiguousarray(bodies, np.int64)
+            call("illum_mesh_set_bodies", h, _ptr(off), len(off) - 1)
 
     def close(self):
         if self.h:
@@ -314,14 +333,14 @@ class IllumContext:
     """Mirror of illum_context (illum.hpp:6-48) over the GPU library."""
 
     def __init__(self, obj_path=None, V=None, F=None, VN=None, vertex_divisor=None,
-                 leaf_size=4, ctx=None):
+                 leaf_size=4, ctx=None, bodies=None):
         self.ctx = ctx or default_context()
         if obj_path is not None:
             div = 100.0 if vertex_divisor is None else vertex_divisor
-            self.mesh = Mesh(self.ctx, obj_path=obj_path, vertex_divisor=div)
+            self.mesh = Mesh(self.ctx, obj_path=obj_path, vertex_divisor=div, bodies=bodies)
         else:
             div = 1.0 if vertex_divisor is None else vertex_divisor
-            self.mesh = Mesh(self.ctx, V=V, F=F, VN=VN, vertex_divisor=div)
+            self.mesh = Mesh(self.ctx, V=V, F=F, VN=VN, vertex_divisor=div, bodies=bodies)
         self.bvh = BVH(self.mesh, leaf_size)
         self.num_faces = self.mesh.nf
         self.horizons = None

@@ -9,6 +9,12 @@
 // Boxes are padded by 4e-6 of the scene's largest coordinate magnitude so a
 // ray that the FP32 watertight predicate reports as hitting a triangle always
 // passes through every ancestor box with a wide margin against rounding.
+//
+// A mesh of several bodies (contiguous facet ranges, HostGeometry::body_off)
+// gets one tree per body, concatenated: body b's trees start at root_thr[b]
+// (threaded) and root_wide[b] (two-wide), its threaded walk ends at miss link
+// -1, and its leaves index its own facets.  A walk from a body's root sees
+// only that body.
 #include "internal.hpp"
 
 #include <algorithm>
@@ -132,27 +138,25 @@ inline float as_float(int32_t i) { float f; std::memcpy(&f, &i, 4); return f; }
 
 }  // namespace
 
-HostBVH build_bvh(const HostGeometry &g, int leaf_size) {
-  if (leaf_size < 1 || leaf_size > 64) throw Error(ILLUM_ERR_ARG, "leaf_size must be in [1, 64]");
-  const int64_t nf = g.nf;
+// One tree over facets [f0, f1); leaf ranges index the reordered facets of
+// that range (0-based within it).
+static HostBVH build_one(const HostGeometry &g, int64_t f0, int64_t f1, int leaf_size,
+                         float pad) {
+  const int64_t nf = f1 - f0;
   Builder B(g, leaf_size);
   B.tbox.resize(nf);
   B.tcen.resize(3 * nf);
   B.perm.resize(nf);
-  float scene_max = 0.0f;
-  for (int64_t t = 0; t < nf; ++t) {
+  for (int64_t k = 0; k < nf; ++k) {
+    const int64_t t = f0 + k;
     Box b;
-    for (int k = 0; k < 3; ++k) b.grow(&g.tri[9 * t + 3 * k]);
-    B.tbox[t] = b;
-    for (int c = 0; c < 3; ++c) {
-      B.tcen[3 * t + c] = 0.5f * (b.lo[c] + b.hi[c]);
-      scene_max = std::max(scene_max, std::max(std::fabs(b.lo[c]), std::fabs(b.hi[c])));
-    }
-    B.perm[t] = (int)t;
+    for (int v = 0; v < 3; ++v) b.grow(&g.tri[9 * t + 3 * v]);
+    B.tbox[k] = b;
+    for (int c = 0; c < 3; ++c) B.tcen[3 * k + c] = 0.5f * (b.lo[c] + b.hi[c]);
+    B.perm[k] = (int)k;
   }
   B.nodes.reserve(2 * nf / std::max(1, leaf_size) + 16);
   B.build(0, (int)nf, 0);
-  const float pad = scene_max * 4e-6f + FLT_MIN;
 
   // Flatten in DFS pre-order, computing miss links.
   HostBVH h;
@@ -237,14 +241,63 @@ HostBVH build_bvh(const HostGeometry &g, int leaf_size) {
     }
   }
   for (int64_t k = 0; k < nf; ++k) {
-    int t = B.perm[k];
+    const int64_t t = f0 + B.perm[k];
     float *o = &h.tris[12 * (size_t)k];
     for (int v = 0; v < 3; ++v) {
       o[4 * v + 0] = g.tri[9 * t + 3 * v + 0];
       o[4 * v + 1] = g.tri[9 * t + 3 * v + 1];
       o[4 * v + 2] = g.tri[9 * t + 3 * v + 2];
-      o[4 * v + 3] = v == 0 ? as_float(t) : 0.0f;
+      o[4 * v + 3] = v == 0 ? as_float((int32_t)t) : 0.0f;
     }
+  }
+  return h;
+}
+
+static inline int32_t as_int(float f) { int32_t i; std::memcpy(&i, &f, 4); return i; }
+
+HostBVH build_bvh(const HostGeometry &g, int leaf_size) {
+  if (leaf_size < 1 || leaf_size > 64) throw Error(ILLUM_ERR_ARG, "leaf_size must be in [1, 64]");
+  const int64_t nf = g.nf;
+  float scene_max = 0.0f;
+  for (int64_t k = 0; k < 9 * nf; ++k) scene_max = std::max(scene_max, std::fabs(g.tri[k]));
+  const float pad = scene_max * 4e-6f + FLT_MIN;
+  std::vector<int64_t> off = g.body_off;
+  if (off.size() < 2) off = {0, nf};
+  const int nb = (int)off.size() - 1;
+  if (nb == 1) {
+    HostBVH h = build_one(g, 0, nf, leaf_size, pad);
+    h.root_thr = {0};
+    h.root_wide = {0};
+    return h;
+  }
+  // Forest: concatenate per-body trees, shifting node links and leaf ranges.
+  HostBVH h;
+  h.leaf_size = leaf_size;
+  h.tris.resize(12 * (size_t)nf);
+  for (int b = 0; b < nb; ++b) {
+    HostBVH t = build_one(g, off[b], off[b + 1], leaf_size, pad);
+    const int32_t nbase = (int32_t)h.nnodes, ibase = (int32_t)h.ninner;
+    const int32_t tbase = (int32_t)off[b];
+    if (off[b + 1] >= (1 << 24)) throw Error(ILLUM_ERR_CAPACITY, "too many facets for leaf ranges");
+    h.root_thr.push_back(nbase);
+    h.root_wide.push_back(ibase);
+    h.depth = std::max(h.depth, t.depth);
+    for (int64_t k = 0; k < t.nnodes; ++k) {
+      float *o = &t.nodes[8 * (size_t)k];
+      const int32_t miss = as_int(o[3]), info = as_int(o[7]);
+      o[3] = as_float(miss < 0 ? -1 : miss + nbase);
+      if (info >= 0) o[7] = as_float(info + tbase);  // count << 24 | first
+    }
+    for (int64_t k = 0; k < 2 * t.ninner; ++k) {
+      float *o = &t.wide[8 * (size_t)k];
+      const int32_t ref = as_int(o[3]);
+      o[3] = as_float(ref >= 0 ? ref + ibase : ~(~ref + tbase));
+    }
+    h.nodes.insert(h.nodes.end(), t.nodes.begin(), t.nodes.end());
+    h.wide.insert(h.wide.end(), t.wide.begin(), t.wide.end());
+    std::copy(t.tris.begin(), t.tris.end(), h.tris.begin() + 12 * (size_t)off[b]);
+    h.nnodes += t.nnodes;
+    h.ninner += t.ninner;
   }
   return h;
 }

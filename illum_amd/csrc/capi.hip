@@ -107,10 +107,11 @@ static void set_device(illum_ctx *ctx) { ILLUM_HIP(hipSetDevice(ctx->device)); }
 
 static void destroy_mesh(illum_mesh *m) {
   dfree(m->d_pr4); dfree(m->d_n4); dfree(m->d_area); dfree(m->d_frame); dfree(m->d_clus);
+  dfree(m->d_boff); dfree(m->d_body);
   delete m;
 }
 static void destroy_bvh(illum_bvh *b) {
-  dfree(b->d_nodes); dfree(b->d_tris); dfree(b->d_wide);
+  dfree(b->d_nodes); dfree(b->d_tris); dfree(b->d_wide); dfree(b->d_roots);
   delete b;
 }
 void destroy_sparse(illum_sparse *s) {
@@ -125,6 +126,30 @@ void destroy_thermal(illum_thermal *t) {
   dfree(t->d_status); dfree(t->d_fidx); dfree(t->d_pos); dfree(t->d_wgrid);
   thermal_free_impl(t);
   delete t;
+}
+
+// Device copy of the body partition (HostGeometry::body_off).
+static void upload_bodies(illum_mesh *m) {
+  hipStream_t s = stream_of(m->ctx);
+  const int64_t nf = m->g.nf;
+  if (m->g.body_off.size() < 2) m->g.body_off = {0, nf};
+  const int nb = (int)m->g.body_off.size() - 1;
+  std::vector<int32_t> off(m->g.body_off.begin(), m->g.body_off.end());
+  dfree(m->d_boff);
+  dfree(m->d_body);
+  m->d_boff = nullptr;
+  m->d_body = nullptr;
+  m->nbody = nb;
+  m->d_boff = dalloc<int32_t>(nb + 1);
+  ILLUM_HIP(hipMemcpyAsync(m->d_boff, off.data(), 4 * (nb + 1), hipMemcpyHostToDevice, s));
+  if (nb > 1) {
+    std::vector<int32_t> body(nf);
+    for (int b = 0; b < nb; ++b)
+      for (int64_t f = off[b]; f < off[b + 1]; ++f) body[f] = b;
+    m->d_body = dalloc<int32_t>(nf);
+    ILLUM_HIP(hipMemcpyAsync(m->d_body, body.data(), 4 * nf, hipMemcpyHostToDevice, s));
+  }
+  sync(m->ctx);
 }
 
 static illum_mesh *upload_mesh(illum_ctx *ctx, HostGeometry &&g) {
@@ -154,6 +179,7 @@ static illum_mesh *upload_mesh(illum_ctx *ctx, HostGeometry &&g) {
   m->nclus = (int64_t)(clus.size() / 4);
   m->d_clus = dalloc<float>(clus.size());
   ILLUM_HIP(hipMemcpyAsync(m->d_clus, clus.data(), 4 * clus.size(), hipMemcpyHostToDevice, s));
+  upload_bodies(m.get());
   sync(ctx);
   return m.release();
 }
@@ -340,6 +366,20 @@ int illum_mesh_num_faces(const illum_mesh *mesh, int64_t *nf) {
   return guarded([&] { CHECK_ARG(mesh && nf, "null argument"); *nf = mesh->g.nf; });
 }
 
+int illum_mesh_set_bodies(illum_mesh *mesh, const int64_t *body_face_off, int nbody) {
+  return guarded([&] {
+    CHECK_ARG(mesh && body_face_off && nbody >= 1, "bad argument");
+    const int64_t nf = mesh->g.nf;
+    CHECK_ARG(body_face_off[0] == 0 && body_face_off[nbody] == nf,
+              "body offsets must start at 0 and end at the facet count");
+    for (int b = 0; b < nbody; ++b)
+      CHECK_ARG(body_face_off[b + 1] > body_face_off[b], "every body needs at least one facet");
+    set_device(mesh->ctx);
+    mesh->g.body_off.assign(body_face_off, body_face_off + nbody + 1);
+    upload_bodies(mesh);
+  });
+}
+
 int illum_mesh_geometry(const illum_mesh *mesh, float *centroid, float *normal, float *radius,
                         double *area, double *frame) {
   return guarded([&] {
@@ -375,6 +415,15 @@ static illum_bvh *make_bvh(illum_mesh *mesh, int leaf_size) {
                            hipMemcpyHostToDevice, s));
   ILLUM_HIP(hipMemcpyAsync(b->d_tris, b->h.tris.data(), 4 * b->h.tris.size(),
                            hipMemcpyHostToDevice, s));
+  b->nbody = (int)b->h.root_thr.size();
+  b->body_off = mesh->g.body_off;
+  std::vector<int32_t> roots(2 * b->nbody);
+  for (int k = 0; k < b->nbody; ++k) {
+    roots[2 * k] = b->h.root_thr[k];
+    roots[2 * k + 1] = b->h.root_wide[k];
+  }
+  b->d_roots = dalloc<int32_t>(roots.size());
+  ILLUM_HIP(hipMemcpyAsync(b->d_roots, roots.data(), 4 * roots.size(), hipMemcpyHostToDevice, s));
   sync(ctx);
   return b.release();
 }

@@ -61,7 +61,8 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
     const double *__restrict__ area, const double *__restrict__ albedo_vec, double albedo,
     uint32_t *__restrict__ vis, uint32_t *__restrict__ mask, int32_t *__restrict__ counts,
     int *__restrict__ status, int nrows, int jstride, unsigned long long *__restrict__ stats,
-    const float4 *__restrict__ clus, int nclus, int scap) {
+    const float4 *__restrict__ clus, int nclus, int scap, const int32_t *__restrict__ body_of,
+    const int32_t *__restrict__ body_off, const int32_t *__restrict__ roots) {
   extern __shared__ uint32_t smem[];
   uint32_t *bits = smem;
   int *queue = (int *)(smem + nwords);
@@ -79,6 +80,11 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
   for (int w = tid; w < nwords; w += FF_BLOCK) bits[w] = 0u;
   if (tid == 0) { s_qn = 0; s_rays = 0; s_nc = 0; }
   unsigned npair = 0, nbox = 0, ntri = 0;  // work counters of this thread
+  // The source's body: receivers [lo, hi), its trees' roots.
+  const int body = body_of ? body_of[j] : 0;
+  const int lo = body_off[body], hi = body_off[body + 1];
+  const int root_thr = roots[2 * body], root_wide = roots[2 * body + 1];
+  const int clo = lo >> 6, chi = (hi + 63) >> 6;  // clusters overlapping [lo, hi)
   const float4 pj = pr4[j], nj = n4[j];
   const float qx = pj.x + offset * nj.x, qy = pj.y + offset * nj.y, qz = pj.z + offset * nj.z;
   __syncthreads();
@@ -87,10 +93,10 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
   // every member fails the facing test is dropped whole (the bound and its
   // rounding margin are documented with cull_clusters in mesh.cpp); the
   // others are listed for the per-pair test.  clus == nullptr keeps all.
-  for (int c0 = 0; c0 < nclus; c0 += FF_BLOCK) {
+  for (int c0 = clo; c0 < chi; c0 += FF_BLOCK) {
     const int c = c0 + tid;
     bool keep = false;
-    if (c < nclus) {
+    if (c < chi) {
       keep = true;
       if (clus) {
         const float4 cc = clus[c];
@@ -116,9 +122,9 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
 #pragma unroll
     for (int k = 0; k < FF_PER; ++k) {
       const int ci = g0 + k * (FF_BLOCK / 64) + wid;
-      const int i = ci < ncl ? clist[ci] * 64 + lane : nf;
+      const int i = ci < ncl ? clist[ci] * 64 + lane : hi;
       bool s = false;
-      if (i < nf && i != j) {
+      if (i >= lo && i < hi && i != j) {
         ++npair;
         const float4 pi = pr4[i];
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -148,16 +154,16 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
         float t;
         int h;
         if (WIDE) {
-          h = trace_closest_wide(wide, tris, r, FF_TINIT, stack + tid, FF_BLOCK, scap, &t,
-                                 nbox, ntri);
+          h = trace_closest_wide(wide, tris, r, root_wide, FF_TINIT, stack + tid, FF_BLOCK, scap,
+                                 &t, nbox, ntri);
           if (h == -1)
-            h = trace_closest_wide(wide, tris, r, __builtin_huge_valf(), stack + tid, FF_BLOCK,
-                                   scap, &t, nbox, ntri);
+            h = trace_closest_wide(wide, tris, r, root_wide, __builtin_huge_valf(), stack + tid,
+                                   FF_BLOCK, scap, &t, nbox, ntri);
           // a walk deeper than the LDS stack: the stackless walk answers the
           // same unbounded query (identical result)
-          if (h == -2) h = trace_closest(nodes, tris, r, &t, nbox, ntri);
+          if (h == -2) h = trace_closest(nodes, tris, r, root_thr, &t, nbox, ntri);
         } else {
-          h = trace_closest(nodes, tris, r, &t, nbox, ntri);
+          h = trace_closest(nodes, tris, r, root_thr, &t, nbox, ntri);
         }
         if (h >= 0 && h != j) atomicOr(&bits[h >> 5], 1u << (h & 31));
       }
@@ -333,24 +339,35 @@ __global__ void popcount_rows_kernel(int nrows, int nwords, const uint32_t *__re
   if (lane == 0) counts[row] = c;
 }
 
+// Closest hit of arbitrary rays over every body's tree: each tree is walked
+// unbounded and the (distance, index) minima are combined exactly.
 __global__ __launch_bounds__(256) void closest_hit_kernel(
     int64_t nray, const float *__restrict__ orig, const float *__restrict__ dir,
     const float4 *__restrict__ nodes, const float4 *__restrict__ wide,
-    const float4 *__restrict__ tris, int32_t *__restrict__ hit, float *__restrict__ tout) {
+    const float4 *__restrict__ tris, const int32_t *__restrict__ roots, int nbody,
+    int32_t *__restrict__ hit, float *__restrict__ tout) {
   __shared__ int stack[FF_STACK * 256];
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nray) return;
   const DRay r = make_ray(orig[3 * k], orig[3 * k + 1], orig[3 * k + 2], dir[3 * k],
                           dir[3 * k + 1], dir[3 * k + 2]);
-  float t;
-  int h = -2;
   unsigned nbox = 0, ntri = 0;
-  if (wide)
-    h = trace_closest_wide(wide, tris, r, __builtin_huge_valf(), stack + threadIdx.x, 256,
-                           FF_STACK, &t, nbox, ntri);
-  if (h == -2) h = trace_closest(nodes, tris, r, &t, nbox, ntri);  // deeper than the LDS stack
-  hit[k] = h;
-  tout[k] = t;
+  int best = -1;
+  float tbest = __builtin_huge_valf();
+  for (int b = 0; b < nbody; ++b) {
+    float t;
+    int h = -2;
+    if (wide)
+      h = trace_closest_wide(wide, tris, r, roots[2 * b + 1], __builtin_huge_valf(),
+                             stack + threadIdx.x, 256, FF_STACK, &t, nbox, ntri);
+    if (h == -2) h = trace_closest(nodes, tris, r, roots[2 * b], &t, nbox, ntri);  // deep walk
+    if (h >= 0 && (t < tbest || (t == tbest && h < best))) {
+      tbest = t;
+      best = h;
+    }
+  }
+  hit[k] = best;
+  tout[k] = tbest;
 }
 
 // The near-first layout is used unless ILLUM_FF_TRAVERSAL is "threaded".  Its
@@ -389,8 +406,8 @@ void bvh_closest_hit(illum_bvh *bvh, const float *orig, const float *dir, int64_
   const int blocks = (int)((nray + 255) / 256);
   closest_hit_kernel<<<blocks, 256, 0, s>>>(
       nray, o.p, d.p, (const float4 *)bvh->d_nodes,
-      use_wide(bvh) ? (const float4 *)bvh->d_wide : nullptr, (const float4 *)bvh->d_tris, h.p,
-      tt.p);
+      use_wide(bvh) ? (const float4 *)bvh->d_wide : nullptr, (const float4 *)bvh->d_tris,
+      bvh->d_roots, bvh->nbody, h.p, tt.p);
   ILLUM_LAUNCH_CHECK();
   ILLUM_HIP(hipMemcpyAsync(hit, h.p, 4 * nray, hipMemcpyDeviceToHost, s));
   ILLUM_HIP(hipMemcpyAsync(t, tt.p, 4 * nray, hipMemcpyDeviceToHost, s));
@@ -399,6 +416,7 @@ void bvh_closest_hit(illum_bvh *bvh, const float *orig, const float *dir, int64_
 
 illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_vec,
                            double offset) {
+  check_bvh_bodies(bvh);
   illum_mesh *mesh = bvh->mesh;
   illum_ctx *ctx = mesh->ctx;
   hipStream_t s = stream_of(ctx);
@@ -448,7 +466,8 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
         (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_wide, (const float4 *)bvh->d_tris,
         (float)offset, mesh->d_area, d_alb.p, albedo, sp->d_vis, M, counts, status.p,
         (int)nf, 1, sp->d_nrays, cluster_cull() ? (const float4 *)mesh->d_clus : nullptr,
-        (int)mesh->nclus, stack_slots);
+        (int)mesh->nclus, stack_slots,
+        mesh->d_body, mesh->d_boff, bvh->d_roots);
     ILLUM_LAUNCH_CHECK();
   }
   // CSC
@@ -502,6 +521,7 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
 // Times ff_visibility on a strided sample of sources for one BVH (ms, from
 // hipEvents on the context stream).  Used by the leaf-size autotuner.
 double time_visibility_sample(illum_bvh *bvh, int max_sources) {
+  check_bvh_bodies(bvh);
   illum_mesh *mesh = bvh->mesh;
   illum_ctx *ctx = mesh->ctx;
   hipStream_t s = stream_of(ctx);
@@ -531,7 +551,8 @@ double time_visibility_sample(illum_bvh *bvh, int max_sources) {
         (int)nf, nwords, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4,
         (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_wide, (const float4 *)bvh->d_tris,
         1e-5f, mesh->d_area, nullptr, 1.0, vis.p, M.p, counts.p, status.p, nrows, jstride, nullptr,
-        cluster_cull() ? (const float4 *)mesh->d_clus : nullptr, (int)mesh->nclus, stack_slots);
+        cluster_cull() ? (const float4 *)mesh->d_clus : nullptr, (int)mesh->nclus, stack_slots,
+        mesh->d_body, mesh->d_boff, bvh->d_roots);
     ILLUM_LAUNCH_CHECK();
     ILLUM_HIP(hipEventRecord(e1, s));
     ILLUM_HIP(hipEventSynchronize(e1));
@@ -548,6 +569,7 @@ double time_visibility_sample(illum_bvh *bvh, int max_sources) {
 // instantiation: out4 = {rays, pair tests, box tests, triangle tests}.  The
 // assembly kernel itself carries no counters; this is a separate launch.
 void form_factor_work(illum_bvh *bvh, double offset, int64_t *out4) {
+  check_bvh_bodies(bvh);
   illum_mesh *mesh = bvh->mesh;
   illum_ctx *ctx = mesh->ctx;
   hipStream_t s = stream_of(ctx);
@@ -573,7 +595,8 @@ void form_factor_work(illum_bvh *bvh, double offset, int64_t *out4) {
       (int)nf, nwords, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4,
       (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_wide, (const float4 *)bvh->d_tris,
       (float)offset, mesh->d_area, nullptr, 1.0, vis, M, counts, status.p, (int)nf, 1, stats.p,
-      cluster_cull() ? (const float4 *)mesh->d_clus : nullptr, (int)mesh->nclus, stack_slots);
+      cluster_cull() ? (const float4 *)mesh->d_clus : nullptr, (int)mesh->nclus, stack_slots,
+        mesh->d_body, mesh->d_boff, bvh->d_roots);
   ILLUM_LAUNCH_CHECK();
   unsigned long long h[4];
   ILLUM_HIP(hipMemcpyAsync(h, stats.p, 32, hipMemcpyDeviceToHost, s));

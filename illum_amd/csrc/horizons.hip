@@ -28,7 +28,8 @@ __global__ __launch_bounds__(256) void horizons_kernel(
     const double *__restrict__ frame, const float4 *__restrict__ nodes,
     const float4 *__restrict__ wide, const float4 *__restrict__ tris, float offset,
     const double *__restrict__ cphi, const double *__restrict__ sphi,
-    const double *__restrict__ th, int tree_nodes, double *__restrict__ out) {
+    const double *__restrict__ th, int tree_nodes, double *__restrict__ out,
+    const int32_t *__restrict__ roots, int nroots, const int32_t *__restrict__ body_of) {
   extern __shared__ int hstack[];
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nf * nphi) return;
@@ -49,8 +50,13 @@ __global__ __launch_bounds__(256) void horizons_kernel(
     const float dy = (float)((F10 * vx + F11 * vy) + F12 * vz);
     const float dz = (float)((F20 * vx + F21 * vy) + F22 * vz);
     const DRay r = make_ray(ox, oy, oz, dx, dy, dz);
-    const bool hit = WIDE ? trace_any_wide(wide, tris, r, hstack + threadIdx.x, 256)
-                          : trace_any(nodes, tris, r);
+    // Rays see the facet's own body (body_of set), else every tree in turn
+    // (one body, or an occluder mesh).
+    const int b0 = body_of ? body_of[j] : 0, b1 = body_of ? b0 + 1 : nroots;
+    bool hit = false;
+    for (int b = b0; b < b1 && !hit; ++b)
+      hit = WIDE ? trace_any_wide(wide, tris, r, roots[2 * b + 1], hstack + threadIdx.x, 256)
+                 : trace_any(nodes, tris, r, roots[2 * b]);
     m = 2 * m + (hit ? 0 : 1);
     if (m >= tree_nodes) { out[idx] = __builtin_nan(""); return; }
     if (th[4 * m + 3] != 0.0) break;
@@ -109,6 +115,8 @@ void horizons(illum_bvh *bvh, illum_bvh *occl, int nphi, double theta_eps, doubl
   if (!(theta_eps > 0)) throw Error(ILLUM_ERR_ARG, "theta_eps must be positive");
   if (!out) throw Error(ILLUM_ERR_ARG, "null output");
   illum_bvh *target = occl ? occl : bvh;
+  check_bvh_bodies(bvh);
+  check_bvh_bodies(target);
   const int64_t nf = mesh->g.nf;
   int tree_nodes = 0;
   std::vector<double> tab = bisection_table(theta_eps, &tree_nodes);
@@ -138,7 +146,8 @@ void horizons(illum_bvh *bvh, illum_bvh *occl, int nphi, double theta_eps, doubl
     kern<<<(unsigned)((total + 255) / 256), 256, lds, s>>>(
         nf, nphi, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4, mesh->d_frame,
         (const float4 *)target->d_nodes, (const float4 *)target->d_wide,
-        (const float4 *)target->d_tris, (float)offset, d_c.p, d_s.p, d_tab.p, tree_nodes, dst);
+        (const float4 *)target->d_tris, (float)offset, d_c.p, d_s.p, d_tab.p, tree_nodes, dst,
+        target->d_roots, target->nbody, occl ? nullptr : mesh->d_body);
     ILLUM_LAUNCH_CHECK();
   }
   if (!(flags & ILLUM_OUT_ON_DEVICE))

@@ -30,6 +30,9 @@ struct HostGeometry {
   std::vector<float> rad;      // nf     : bounding radius
   std::vector<double> area;    // nf     : |(v1-v0)x(v2-v0)|/2 in FP64
   std::vector<double> frame;   // nf x 9 : b, t, n columns (FP64)
+  // Bodies: facets [body_off[b], body_off[b+1]) form body b, a scene of its
+  // own (empty or {0, nf}: one body).
+  std::vector<int64_t> body_off;
 };
 
 // Builds HostGeometry from double vertices (rounded to float) and faces.
@@ -75,6 +78,7 @@ struct HostBVH {
   int64_t ninner = 0;
   int depth = 0;
   int leaf_size = 0;
+  std::vector<int32_t> root_thr, root_wide;  // per body (one entry for one body)
 };
 HostBVH build_bvh(const HostGeometry &g, int leaf_size);
 
@@ -104,6 +108,11 @@ struct illum_mesh {
   double *d_frame = nullptr;   // nf x 9
   float *d_clus = nullptr;     // ceil(nf / 64) x float4 (cull_clusters)
   int64_t nclus = 0;
+  // bodies (see HostGeometry::body_off): facet ranges and, with several
+  // bodies, the body of each facet
+  int nbody = 1;
+  int32_t *d_boff = nullptr;   // nbody + 1
+  int32_t *d_body = nullptr;   // nf, null for one body
 };
 
 struct illum_bvh {
@@ -112,6 +121,9 @@ struct illum_bvh {
   float *d_nodes = nullptr;
   float *d_tris = nullptr;
   float *d_wide = nullptr;
+  int32_t *d_roots = nullptr;  // per body: threaded root, two-wide root
+  int nbody = 1;               // bodies of the mesh when the tree was built
+  std::vector<int64_t> body_off;  // the partition the forest was built for
   // leaf-size autotuning record (illum_bvh_build with leaf_size 0)
   int ntuned = 0;
   int tune_leaf[8] = {0};
@@ -168,3 +180,11 @@ struct illum_thermal {
   std::vector<int64_t> goff;
   std::vector<int32_t> nz_host;
 };
+
+namespace illum {
+// A forest is only valid for the body partition it was built for.
+inline void check_bvh_bodies(const illum_bvh *bvh) {
+  if (bvh->body_off != bvh->mesh->g.body_off)
+    throw Error(ILLUM_ERR_ARG, "the BVH was built for another body partition; rebuild it");
+}
+}  // namespace illum

@@ -114,15 +114,15 @@ __device__ __forceinline__ bool tri_hit(const DRay &r, float4 a, float4 b, float
   return true;
 }
 
-// Closest hit over the threaded BVH; returns the original facet index or -1.
-// nbox / ntri count the box and triangle tests made.
+// Closest hit over the threaded tree starting at node `root`; returns the
+// original facet index or -1.  nbox / ntri count the box and triangle tests.
 __device__ __forceinline__ int trace_closest(const float4 *__restrict__ nodes,
                                              const float4 *__restrict__ tris, const DRay &r,
-                                             float *tbest_out, unsigned &nbox,
+                                             int root, float *tbest_out, unsigned &nbox,
                                              unsigned &ntri) {
   float tbest = __builtin_huge_valf();
   int best = -1;
-  int node = 0;
+  int node = root;
   while (node >= 0) {
     const float4 lo = nodes[2 * node], hi = nodes[2 * node + 1];
     const int miss = __float_as_int(lo.w), info = __float_as_int(hi.w);
@@ -217,16 +217,18 @@ __device__ __forceinline__ bool closest_wide_step(const float4 *__restrict__ wid
   return false;
 }
 
-// Near-first closest hit over the two-wide layout (closest_wide_step to the
-// end of the walk); returns the original facet index, -1 for no hit, or -2
-// when the walk needed more than `cap` stack entries.
+// Near-first closest hit over the two-wide tree rooted at inner node `root`
+// (closest_wide_step to the end of the walk); returns the original facet
+// index, -1 for no hit, or -2 when the walk needed more than `cap` stack
+// entries.
 __device__ __forceinline__ int trace_closest_wide(const float4 *__restrict__ wide,
                                                   const float4 *__restrict__ tris,
-                                                  const DRay &r, float tinit, int *stack,
-                                                  int stride, int cap, float *tbest_out,
-                                                  unsigned &nbox, unsigned &ntri) {
+                                                  const DRay &r, int root, float tinit,
+                                                  int *stack, int stride, int cap,
+                                                  float *tbest_out, unsigned &nbox,
+                                                  unsigned &ntri) {
   float tbest = tinit;
-  int best = -1, sp = 0, cur = 0;
+  int best = -1, sp = 0, cur = root;
   while (!closest_wide_step(wide, tris, r, cur, sp, tbest, best, stack, stride, cap, nbox,
                             ntri)) {
   }
@@ -234,14 +236,14 @@ __device__ __forceinline__ int trace_closest_wide(const float4 *__restrict__ wid
   return sp < 0 ? -2 : best;
 }
 
-// Any hit (t > 0) over the two-wide layout, nearer child first (stack as in
-// trace_closest_wide).
+// Any hit (t > 0) over the two-wide tree rooted at `root`, nearer child
+// first (stack as in trace_closest_wide; the tree depth must fit it).
 __device__ __forceinline__ bool trace_any_wide(const float4 *__restrict__ wide,
                                                const float4 *__restrict__ tris, const DRay &r,
-                                               int *stack, int stride) {
+                                               int root, int *stack, int stride) {
   const float inf = __builtin_huge_valf();
   int sp = 0;
-  int cur = 0;
+  int cur = root;
   for (;;) {
     if (cur >= 0) {
       const float4 a0 = wide[4 * cur], a1 = wide[4 * cur + 1];
@@ -276,11 +278,12 @@ __device__ __forceinline__ bool trace_any_wide(const float4 *__restrict__ wide,
   return false;
 }
 
-// Any hit (t > 0) over the threaded BVH.
+// Any hit (t > 0) over the threaded tree starting at node `root`.
 __device__ __forceinline__ bool trace_any(const float4 *__restrict__ nodes,
-                                          const float4 *__restrict__ tris, const DRay &r) {
+                                          const float4 *__restrict__ tris, const DRay &r,
+                                          int root) {
   const float inf = __builtin_huge_valf();
-  int node = 0;
+  int node = root;
   while (node >= 0) {
     const float4 lo = nodes[2 * node], hi = nodes[2 * node + 1];
     const int miss = __float_as_int(lo.w), info = __float_as_int(hi.w);

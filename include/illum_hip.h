@@ -110,6 +110,14 @@ int illum_mesh_from_obj(illum_ctx *ctx, const char *path, double vertex_divisor,
                         illum_mesh **out);
 int illum_mesh_destroy(illum_mesh *mesh);
 int illum_mesh_num_faces(const illum_mesh *mesh, int64_t *nf);
+/* Partition the mesh into nbody independent bodies, facets
+ * [body_face_off[b], body_face_off[b+1]) forming body b (offsets start at 0,
+ * increase strictly and end at the facet count).  Visibility, form factors
+ * and horizons of a facet then involve only its own body, so many bodies of
+ * uneven size run in one launch; F comes out block-diagonal.  Set before
+ * building BVHs (a tree built for another partition is rejected).  No
+ * reference counterpart (the reference handles one body per process). */
+int illum_mesh_set_bodies(illum_mesh *mesh, const int64_t *body_face_off, int nbody);
 /* Derived per-facet geometry (host copies; any pointer may be null):
  * centroid (nf x 3 float), normal (nf x 3 float), bounding radius (nf float),
  * area (nf double), frame (nf x 9 double: columns b, t, n as in

@@ -311,3 +311,57 @@ def test_conductionQ_dropin_vs_reference_fortran_golden(api):
             np.testing.assert_allclose(T, gold[f"{n}.T"][k], rtol=0, atol=1e-10)
             assert abs(Ts - gold[f"{n}.Tsurf"][k]) <= 1e-10
             assert abs(Fs - gold[f"{n}.Fsurf"][k]) <= 1e-8 * max(1.0, abs(Fs))
+
+
+def test_multibody_batch_vs_oracle_per_body(api):
+    """Bodies of uneven size in one launch sequence (the multi-body batch
+    configuration, scaled down): each body's visibility, form factors,
+    horizons and scattered radiosity equal the oracle run on that body alone,
+    and nothing couples bodies (F is block-diagonal)."""
+    import scipy.sparse as sps
+    rng = np.random.default_rng(21)
+    bodies = []
+    for k, sub in enumerate((2, 3, 4, 3, 2)):
+        ax = 13.0 * rng.uniform(0.6, 1.0, 3)
+        bodies.append(api.synth_ellipsoid(sub, axes=tuple(ax), ncraters=20, seed=100 + k))
+    V, F, off = api.concat_bodies(bodies)
+    nf = len(F)
+    ctx = api.IllumContext(V=V, F=F, bodies=off, leaf_size=0)  # autotuned forest
+    Fm = ctx.compute_F(0.1)
+    vis = _bits_to_sets(Fm.visibility(), nf)
+    cp, ri, v = Fm.csc()
+    A = sps.csc_matrix((v, ri, cp), shape=(nf, nf))
+    hz = ctx.make_horizons(nphi=17)
+    E = rng.uniform(0, 500, (nf, 3))
+    B, res, _ = api.radiosity_solve(Fm, E, method="mixed", rel_tol=1e-10, max_iter=400)
+    rays = 0
+    for b, (Vb, Fb) in enumerate(bodies):
+        lo, hi = int(off[b]), int(off[b + 1])
+        g = O.geometry(Vb, Fb)
+        bits, nr = O.visibility(g, offset=1e-5)
+        rays += nr
+        assert np.array_equal(vis[lo:hi, lo:hi], _bits_to_sets(bits, hi - lo)), f"body {b}"
+        assert not vis[lo:hi, :lo].any() and not vis[lo:hi, hi:].any()
+        ocp, ori, ov = O.form_factors(g, bits, albedo=0.1)
+        blk = A[:, lo:hi].tocsc()
+        blk.sort_indices()
+        assert np.array_equal(blk.indptr, ocp) and np.array_equal(blk.indices - lo, ori)
+        np.testing.assert_allclose(blk.data, ov, rtol=1e-12, atol=0)
+        ho, _ = O.horizons(g, nphi=17, theta_eps=1e-3, offset=1e-5)
+        assert np.array_equal(hz[:, lo:hi], ho), f"body {b} horizons"
+        Bo, _, _ = O.gauss_seidel(hi - lo, ocp, ori, ov, E[lo:hi, 0], max_sweeps=200,
+                                  rel_tol=1e-13)
+        np.testing.assert_allclose(B[lo:hi, 0], Bo, rtol=1e-8, atol=1e-8 * np.abs(Bo).max())
+    assert Fm.rays() == rays
+    assert res[-1] <= 1e-10
+
+
+def test_bvh_built_for_another_partition_is_rejected(api):
+    V, F = _mesh(api, 2, 5, seed=1)
+    V2, F2, off = api.concat_bodies([(V, F), (V, F)])
+    m = api.Mesh(api.default_context(), V=V2, F=F2)
+    t = api.BVH(m, 4)
+    api._lib.call("illum_mesh_set_bodies", m.h, api._ptr(off), len(off) - 1)
+    h = api.C.c_void_p()
+    with pytest.raises(api._lib.IllumError):
+        api._lib.call("illum_form_factors", t.h, 0.1, None, 1e-5, api.C.byref(h))
