@@ -72,6 +72,7 @@ SIGNATURES = {
     "illum_sparse_copy_visibility": [_P, _P],
     "illum_sparse_rays": [_P, _P],
     "illum_horizons": [_P, _P, _I, _D, _D, _P, _I],
+    "illum_render": [_P, _I, _I, _I, _D, _D, _P, _D, _D, _P, _P, _I],
     "illum_fib_spiral": [_P, _I],
     "illum_direct_radiosity": [_P, _P, _I, _P, _I, _P, _I, _P, _I],
     "illum_radiosity_solve": [_P, _P, _I, _I, _D, _I, _P, _P, _P, _I],

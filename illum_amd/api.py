@@ -369,6 +369,26 @@ class IllumContext:
             call("illum_form_factors", self.bvh.h, 0.0, _ptr(a), offset, C.byref(h))
         return SparseMatrix(self.ctx, h)
 
+    def render(self, data=None, mode="orthographic", width=512, height=512, az=0.0, el=0.0,
+               minval=None, maxval=None):
+        """The renderer's ray-cast (illum_renderer_cli.cpp:174-261): returns
+        (facet image, value image), height x width, row 0 at the top; facet -1
+        and value NaN where no facet is hit; value is None without data."""
+        m = {"orthographic": 0, "equal_area_cylindrical": 1}[mode]
+        facet = np.zeros((height, width), np.int32)
+        value = None
+        d = None
+        if data is not None:
+            d = np.ascontiguousarray(data, np.float64).ravel()
+            if d.shape != (self.num_faces,):
+                raise ValueError("input data and mesh have incompatible sizes")
+            value = np.zeros((height, width))
+        lo = -np.inf if minval is None else float(minval)
+        hi = np.inf if maxval is None else float(maxval)
+        call("illum_render", self.bvh.h, m, width, height, float(az), float(el), _ptr(d), lo, hi,
+             _ptr(facet), _ptr(value), 0)
+        return facet, value
+
     def form_factor_work(self, offset=1e-5):
         """Work of one visibility pass (separate instrumented launch): rays, pair
         (facing) tests after the cluster pass, box tests, triangle tests."""

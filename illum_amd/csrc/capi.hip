@@ -14,6 +14,9 @@ namespace illum {
 double time_visibility_sample(illum_bvh *bvh, int max_sources);
 illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_vec,
                            double offset);
+void render(illum_bvh *bvh, int mode, int width, int height, double az, double el,
+            const double *data, double lo, double hi, int32_t *facet, double *value,
+            int flags);
 void form_factor_work(illum_bvh *bvh, double offset, int64_t *out4);
 void bvh_closest_hit(illum_bvh *bvh, const float *orig, const float *dir, int64_t nray,
                      int32_t *hit, float *t);
@@ -505,6 +508,17 @@ int illum_form_factors(illum_bvh *bvh, double albedo, const double *albedo_vec, 
     CHECK_ARG(std::isfinite(offset), "offset must be finite");
     set_device(bvh->mesh->ctx);
     *out = form_factors(bvh, albedo, albedo_vec, offset);
+  });
+}
+
+int illum_render(illum_bvh *bvh, int mode, int width, int height, double az_deg,
+                 double el_deg, const double *data, double minval, double maxval,
+                 int32_t *facet, double *value, int flags) {
+  return guarded([&] {
+    CHECK_ARG(bvh && (facet || value), "null argument");
+    CHECK_ARG(!(minval > maxval), "minval must not exceed maxval");
+    set_device(bvh->mesh->ctx);
+    render(bvh, mode, width, height, az_deg, el_deg, data, minval, maxval, facet, value, flags);
   });
 }
 

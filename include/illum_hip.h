@@ -20,6 +20,7 @@
  *                                                          src/cpp/illum_cli.cpp:425-440,
  *                                                          src/cpp/arma_util.cpp:3-19
  *   illum_thermal_*       thermal_model                    src/cpp/thermal.hpp:7-105
+ *   illum_render          illum_renderer_cli's ray-cast    src/cpp/illum_renderer_cli.cpp:174-261
  *   illum_conductionQ     conductionQ (C++ wrapper)        src/cpp/conduction.hpp:16-28,
  *                                                          src/cpp/conduction.cpp:17-30
  */
@@ -188,6 +189,19 @@ int illum_fib_spiral(double *xy, int n);
 int illum_direct_radiosity(illum_mesh *mesh, const double *horizons, int nphi,
                            const double *sun_km, int S, const double *disk_xy,
                            int ndisk, double *E, int flags);
+
+/* ---- renderer ray-cast (illum_renderer_cli.cpp:174-261) ----------------
+ * Per-pixel closest-hit facet of the renderer's camera, height x width
+ * row-major (row 0 = top image row): mode 0 orthographic at azimuth az_deg,
+ * elevation el_deg; mode 1 equal-area cylindrical (az, el used as given, as
+ * the reference does).  facet[p] = hit facet or -1; value[p] = data[facet]
+ * clamped to [minval, maxval] (pass -inf / +inf for no clamp), NaN where
+ * nothing is hit.  Either output may be null; data (nf doubles) is needed
+ * for value.  Host or device per flags.  Colour mapping and image files are
+ * left to the caller. */
+int illum_render(illum_bvh *bvh, int mode, int width, int height, double az_deg,
+                 double el_deg, const double *data, double minval, double maxval,
+                 int32_t *facet, double *value, int flags);
 
 /* ---- radiosity solve (A6) ------------------------------------------------
  * Solves (I - F) B = E for S right-hand sides; E, B are nf x S column-major

@@ -17,6 +17,7 @@
 // arma::sp_mat layout) or CSR.  Errors throw illum::error carrying the status.
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -176,6 +177,23 @@ class illum_context {
     if ((int64_t)h.size() != (int64_t)nphi * num_faces) throw error(ILLUM_ERR_SHAPE, "horizon map size");
     horizons = std::move(h);
     nphi_ = nphi;
+  }
+
+  // The renderer's ray-cast (illum_renderer_cli.cpp:174-261): hit facet per
+  // pixel (-1 = none), height x width row-major; mode 0 orthographic, 1
+  // equal-area cylindrical.  With data (N values), value gets data[facet]
+  // clamped to [minval, maxval] (NaN where nothing is hit).
+  std::vector<int32_t> render(int mode, int width, int height, double az, double el,
+                              const std::vector<double> *data = nullptr,
+                              std::vector<double> *value = nullptr,
+                              double minval = -HUGE_VAL, double maxval = HUGE_VAL) const {
+    std::vector<int32_t> facet((size_t)width * height);
+    if (value) value->assign((size_t)width * height, 0.0);
+    if (data && (int64_t)data->size() != num_faces) throw error(ILLUM_ERR_SHAPE, "data size");
+    check(illum_render(bvh_.get(), mode, width, height, az, el, data ? data->data() : nullptr,
+                       minval, maxval, facet.data(), value && data ? value->data() : nullptr,
+                       0));
+    return facet;
   }
 
   int nphi() const { return nphi_; }

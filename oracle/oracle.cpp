@@ -24,6 +24,7 @@
 //   oracle_conductionQ       conductionQ.f:35-98 with the Thomas algorithm of
 //                            tridag.for:17-27 (textbook form, Golub & Van Loan 4.3)
 //   oracle_thermal_run       thermal_model::step, thermal.hpp:65-84
+//   oracle_render            the renderer's per-pixel ray-cast, illum_renderer_cli.cpp:174-261
 //
 // Compiled with -ffp-contract=off: every expression is evaluated as written.
 #include <algorithm>
@@ -734,6 +735,80 @@ int oracle_thermal_run(int64_t nf, const int64_t *goff, const double *z, const d
     Qprev[f] = qn;
   });
   return status.load();
+}
+
+// --------------------------------------------------------------------------
+// Renderer ray-cast (illum_renderer_cli.cpp:174-261): facet hit by each pixel's
+// ray, -1 for none, row-major height x width (row 0 at the top).
+// mode 0: orthographic camera at azimuth az, elevation el (degrees) looking
+// at the mesh centroid from 1.1 x the largest centroid distance; mode 1:
+// equal-area cylindrical projection, az and el used as given (the reference
+// converts degrees only in the orthographic branch).  Vector arithmetic is
+// FP32 component-wise, scalars are FP64 rounded to FP32 where they meet a
+// vector; |v| = sqrtf(x*x + y*y + z*z) summed left to right.
+namespace {
+struct V3 { float x, y, z; };
+inline V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 mul(float s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline V3 cross(V3 a, V3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline float len(V3 a) { return (float)std::sqrt((double)((a.x * a.x + a.y * a.y) + a.z * a.z)); }
+inline V3 unit(V3 a) { const float l = len(a); return {a.x / l, a.y / l, a.z / l}; }
+inline V3 sph2xyz(double phi, double theta, double rad) {
+  return {(float)((rad * std::cos(phi)) * std::sin(theta)),
+          (float)((rad * std::sin(phi)) * std::sin(theta)), (float)(rad * std::cos(theta))};
+}
+}  // namespace
+
+int oracle_render(const float *tri9, const float *cen, int64_t nf, int mode, int width,
+                  int height, double az, double el, int nthreads, int32_t *facet) {
+  const double PI = 3.141592653589793;
+  V3 c{0.0f, 0.0f, 0.0f};
+  for (int64_t i = 0; i < nf; ++i) c = add(c, V3{cen[3 * i], cen[3 * i + 1], cen[3 * i + 2]});
+  c = V3{c.x / (float)nf, c.y / (float)nf, c.z / (float)nf};
+  double r_model = 0.0;
+  for (int64_t i = 0; i < nf; ++i)
+    r_model = std::fmax(r_model, (double)len(sub(V3{cen[3 * i], cen[3 * i + 1], cen[3 * i + 2]}, c)));
+  const double rad = 1.1 * r_model;
+  BVH bvh;
+  bvh.build(tri9, nf);
+  V3 p_cam{}, n_up{}, n_right{}, n_eye{};
+  double aspect = (double)width / height;
+  if (mode == 0) {
+    az *= PI / 180;
+    el *= PI / 180;
+    p_cam = sph2xyz(az, el, rad);
+    const V3 u = unit(p_cam);
+    n_eye = {-u.x, -u.y, -u.z};
+    n_up = sph2xyz(az, el - PI / 2, 1.0);
+    n_right = cross(n_eye, n_up);
+    p_cam = add(p_cam, c);
+  }
+  parallel_for((int64_t)width * height, nthreads, [&](int64_t k) {
+    const int i = (int)(k / width), j = (int)(k % width);
+    V3 o, d;
+    if (mode == 0) {
+      double y_eye = 2 * (1. - (double)i / (height - 1)) - 1;
+      y_eye *= rad;
+      double x_eye = 2 * (double)j / (width - 1) - 1;
+      x_eye *= aspect * rad;
+      o = add(add(p_cam, mul((float)y_eye, n_up)), mul((float)x_eye, n_right));
+      d = n_eye;
+    } else {
+      const double theta = std::asin(2 * (double)(i + 1) / (height + 1) - 1);
+      const double phi = 2 * PI * (double)j / width;
+      const V3 pc = sph2xyz(phi - az, theta - el, rad);
+      const V3 u = unit(pc);
+      d = {-u.x, -u.y, -u.z};
+      o = add(pc, c);
+    }
+    const float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
+    float tb;
+    facet[k] = bvh.closest(setup(of, df), &tb, false);
+  });
+  return 0;
 }
 
 }  // extern "C"

@@ -31,6 +31,7 @@ _SIG = {
     "oracle_conductionQ": ([_I, _P, _D, _D, _D, _P, _P, _P, _D, _P, _D, _P], _I),
     "oracle_thermal_run": ([_I64, _P, _P, _P, _P, _P, _P, _D, _I, _D, _I, _P, _P, _P, _P, _P, _I],
                            _I),
+    "oracle_render": ([_P, _P, _I64, _I, _I, _I, _D, _D, _I, _P], _I),
 }
 
 _lib = None
@@ -83,6 +84,14 @@ def closest_hit(tri, orig, dirs, use_bvh=True, nthreads=None):
     lib().oracle_closest_hit(_p(tri), tri.shape[0], _p(orig), _p(dirs), n, int(use_bvh),
                              nthreads or nthreads_default(), _p(hit), _p(t))
     return hit, t
+
+
+def render(g, mode=0, width=64, height=64, az=0.0, el=0.0, nthreads=None):
+    """Per-pixel hit facet of the renderer's ray-cast (height x width, -1 = none)."""
+    out = np.zeros((height, width), np.int32)
+    lib().oracle_render(_p(g["tri"]), _p(g["centroid"]), g["tri"].shape[0], mode, width, height,
+                        az, el, nthreads or nthreads_default(), _p(out))
+    return out
 
 
 def visibility(g, offset=1e-5, j0=0, j1=None, use_bvh=True, nthreads=None):

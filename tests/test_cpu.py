@@ -282,3 +282,22 @@ def test_armaio_roundtrip(tmp_path):
     head = open(p, "rb").read(30)
     assert head.startswith(b"ARMA_MAT_BIN_FN008\n7 3\n")
     assert np.array_equal(load_mat(p), X)
+
+
+def test_oracle_render_sphere_closed_form():
+    # Orthographic view of a fine unit icosphere: the camera half-width is 1.1
+    # times the centroid radius, so the disk covers pi / (4 * 1.1^2) of a square
+    # image up to the polygonal rim; every hit facet faces the camera, and the
+    # image is mirror-symmetric about the vertical axis for a view down z.
+    from illum_amd import api
+    V, F = api.synth_ellipsoid(5, axes=(1.0, 1.0, 1.0), ncraters=0, seed=0)
+    g = O.geometry(V, F)
+    img = O.render(g, 0, 160, 160, 0.0, 0.0)
+    r_cen = np.linalg.norm(g["centroid"].astype(np.float64), axis=1).max()
+    frac = np.mean(img >= 0)
+    assert abs(frac - np.pi / (4 * 1.1 ** 2)) < 0.01
+    assert np.all(g["normal"][img[img >= 0]] @ np.array([0.0, 0.0, 1.0]) > 0)
+    assert np.array_equal(img >= 0, (img >= 0)[:, ::-1])
+    assert 0.99 < r_cen < 1.0
+    # every equal-area pixel looks at the sphere from outside: all hit
+    assert np.all(O.render(g, 1, 48, 24, 0.0, 0.0) >= 0)

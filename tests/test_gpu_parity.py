@@ -231,11 +231,15 @@ def test_thermal_vs_oracle(api, precision, tol):
     To, _, Fo, _ = O.thermal_run([0, nz], z, np.full(nz, 200.0), np.full(nz, 1.19177e6), T0,
                                  np.zeros(nf), Q, dt, 0.95)
     T = th.T
-    assert np.max(np.abs(T - To)) <= tol
     Ts, Fs = th.surface()
+    dT, dF = np.max(np.abs(T - To)), np.max(np.abs(th.get_radiosity() + Fo))
+    print(f"{precision}: max |T - oracle| {dT:.3e} K, max |Fsurf - oracle| {dF:.3e} W/m^2")
+    assert dT <= tol
     assert np.max(np.abs(Ts - To[0])) <= tol
-    assert np.allclose(th.get_radiosity(), -Fo, rtol=1e-6 if precision == "mixed" else 1e-9,
-                       atol=1e-6)
+    # Fsurf = -k (T1 - Tsurf) / z1 amplifies temperature errors by k / z1
+    # (about 130 W m^-2 K^-1 here), so its bound follows the temperature's
+    assert dF <= 200 * tol if precision == "mixed" else np.allclose(
+        th.get_radiosity(), -Fo, rtol=1e-9, atol=1e-6)
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("mixed", 1e-3)])
@@ -365,3 +369,36 @@ def test_bvh_built_for_another_partition_is_rejected(api):
     h = api.C.c_void_p()
     with pytest.raises(api._lib.IllumError):
         api._lib.call("illum_form_factors", t.h, 0.1, None, 1e-5, api.C.byref(h))
+
+
+@pytest.mark.parametrize("mode,az,el", [(0, 30.0, 20.0), (0, 0.0, 90.0), (1, 0.3, 0.1)])
+def test_render_vs_oracle(api, mode, az, el):
+    V, F = _mesh(api, 4, 40, seed=13)
+    ctx = api.IllumContext(V=V, F=F)
+    data = np.random.default_rng(2).uniform(100, 400, len(F))
+    name = "orthographic" if mode == 0 else "equal_area_cylindrical"
+    fac, val = ctx.render(data, mode=name, width=96, height=64, az=az, el=el, minval=150,
+                          maxval=350)
+    fo = O.render(O.geometry(V, F), mode, 96, 64, az, el)
+    assert np.array_equal(fac, fo)
+    hit = fo >= 0
+    np.testing.assert_array_equal(val[hit], np.clip(data[fo[hit]], 150, 350))
+    assert np.all(np.isnan(val[~hit]))
+
+
+def test_render_sphere_known_answers(api):
+    # A fine icosphere viewed orthographically fills pi / (4 * 1.1^2) of a
+    # square image (camera half-width = 1.1 x the centroid radius), and each
+    # pixel sees a facet facing the camera; the equal-area projection sees the
+    # sphere from all around, every pixel hitting the facet in its direction.
+    V, F = api.synth_ellipsoid(5, axes=(1.0, 1.0, 1.0), ncraters=0, seed=0)
+    ctx = api.IllumContext(V=V, F=F)
+    n = ctx.mesh.geometry()["normal"].astype(np.float64)
+    fac, _ = ctx.render(None, width=200, height=200, az=40.0, el=70.0)
+    frac = np.mean(fac >= 0)
+    assert abs(frac - np.pi / (4 * 1.1 ** 2)) < 0.01
+    az, el = np.deg2rad(40.0), np.deg2rad(70.0)
+    toward_cam = np.array([np.cos(az) * np.sin(el), np.sin(az) * np.sin(el), np.cos(el)])
+    assert np.all(n[fac[fac >= 0]] @ toward_cam > 0)
+    fac2, _ = ctx.render(None, mode="equal_area_cylindrical", width=64, height=32)
+    assert np.all(fac2 >= 0)
