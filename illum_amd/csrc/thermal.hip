@@ -190,9 +190,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ)
 //   z_j = (rho_j + alpha_j z_{j+1}) / beta_j,
 //   d_1 = z_1,  d_j = z_j + (gamma_j / beta_j) d_{j-1},
 // so only row 1 (whose diagonal follows the surface temperature) differs
-// between steps and the recurrence needs one reciprocal per row.  The
-// column T (FP64) and z, gamma/beta (FP32) all stay in registers.  rho is
-// carried halved (the factor 2 is applied in the FP64 update).
+// between steps.  The pivots of rows >= 2 are therefore computed once per
+// launch, pre-scaled by the facet's conductivity factor s2 (P'_j = s2 /
+// beta_j), which folds s2 out of every step: with the grid's unscaled
+// alpha, gamma (wavefront-uniform scalars)
+//   z_j = (alpha_j P'_j) z_{j+1} + (alpha_j e_{j+1} - gamma_j e_j) P'_j,
+//   e_j = T_j - T_{j-1},
+//   d_j = z_j + gamma_j P'_j d_{j-1}.
+// The column T (FP64), z and P' (FP32) stay in registers.  rho is carried
+// halved (the factor 2 is applied in the FP64 update); the FP32 sweeps use
+// fused multiply-adds.
 template <int NZ>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ), cn_waves(NZ)))) void cn_mixed_kernel(
     const int32_t *__restrict__ fidx, const int32_t *__restrict__ wgrid, int64_t block0,
@@ -223,26 +230,41 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ)
   double Qn = act ? Qprev[f] : 0.0;
   double Fsurf = act ? Fs[f] : 0.0;
 
+  // Time-invariant pivots of rows nz..2, pre-scaled: P'_j = s2 / beta_j.
+  float Pp[NZ + 1];
+  float M2 = 0.0f;  // gamma_2 P'_2: enters row 1's pivot every step
+  {
+    float Mn = 0.0f;  // gamma_{j+1} P'_{j+1}
+#pragma unroll
+    for (int j = NZ; j >= 2; --j) {
+      if (j <= nz) {
+        const float al_j = c32[3 * (j - 1)], ga_j = c32[3 * (j - 1) + 1];
+        const float bj = __builtin_fmaf(s2s, c32[3 * (j - 1) + 2], 1.0f);
+        const float beta = __builtin_fmaf(-s2s * al_j, Mn, bj);
+        Pp[j] = s2s * __builtin_amdgcn_rcpf(beta);
+        Mn = ga_j * Pp[j];
+      }
+    }
+    M2 = Mn;
+  }
+  const float geo_z = geo_half / s2s;  // bottom-row geothermal term in z's scaling
+
   for (int step = 0; step < nsteps; ++step) {
     const double Qnp1 = act ? Q[(int64_t)step * nf + f] : 0.0;
     double ann, annp1, bn;
     surface_bc(Tl[0], Qn, Qnp1, k1, emiss, &ann, &annp1, &bn);
-    float z[NZ + 1], M[NZ + 1];
-    float znext = 0.0f, Mnext = 0.0f, enext = 0.0f;  // z, gamma/beta, T diff of row j+1
+    float z[NZ + 1];
+    float znext = 0.0f, enext = 0.0f;  // z and T difference of row j+1
 #pragma unroll
     for (int j = NZ; j >= 2; --j) {
       if (j <= nz) {
-        const float aj = s2s * c32[3 * (j - 1)];
-        const float gj = s2s * c32[3 * (j - 1) + 1];
-        const float bj = 1.0f + s2s * c32[3 * (j - 1) + 2];
+        const float al_j = c32[3 * (j - 1)], ga_j = c32[3 * (j - 1) + 1];
         const float ej = (float)(Tl[j] - Tl[j - 1]);
-        float rho = aj * enext - gj * ej;
-        if (j == nz) rho += geo_half;
-        const float P = __builtin_amdgcn_rcpf(bj - aj * Mnext);
-        z[j] = (rho + aj * znext) * P;
-        M[j] = gj * P;
+        float w = __builtin_fmaf(al_j, enext, -ga_j * ej);
+        if (j == nz) w += geo_z;
+        // one fused multiply-add on the z recurrence; the rest overlaps
+        z[j] = __builtin_fmaf(al_j * Pp[j], znext, w * Pp[j]);
         znext = z[j];
-        Mnext = M[j];
         enext = ej;
       }
       if ((j & 7) == 0) __builtin_amdgcn_sched_barrier(0);
@@ -250,16 +272,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ)
     {
       const double b1 = ((1.0 + a1) + g1) - g1 * bn;
       const double surf = (0.5 * g1) * ((ann + annp1) + (2.0 * (bn - 1.0)) * Tl[1]);
-      const float rho1 = (float)surf + a1s * enext;
-      z[1] = (rho1 + a1s * znext) * __builtin_amdgcn_rcpf((float)b1 - a1s * Mnext);
+      const float rho1 = __builtin_fmaf(a1s, enext, (float)surf);
+      z[1] = __builtin_fmaf(a1s, znext, rho1) *
+             __builtin_amdgcn_rcpf(__builtin_fmaf(-a1s, M2, (float)b1));
     }
     float d = z[1];
-    Tl[1] = Tl[1] + 2.0 * (double)d;
+    Tl[1] = __builtin_fma(2.0, (double)d, Tl[1]);
 #pragma unroll
     for (int j = 2; j <= NZ; ++j) {
       if (j <= nz) {
-        d = z[j] + M[j] * d;
-        Tl[j] = Tl[j] + 2.0 * (double)d;
+        d = __builtin_fmaf(c32[3 * (j - 1) + 1] * Pp[j], d, z[j]);
+        Tl[j] = __builtin_fma(2.0, (double)d, Tl[j]);
       }
       if ((j & 7) == 0) __builtin_amdgcn_sched_barrier(0);
     }
