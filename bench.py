@@ -322,6 +322,8 @@ def main():
     log(f"mesh {nf} facets, bvh {illum.bvh.info()}, leaf autotune {illum.bvh.tuning()}, "
         f"setup {time.perf_counter() - t0:.1f}s")
 
+    tune = illum.bvh.tuning()
+    walk = {2: "two-wide", 4: "four-wide"}[min(tune, key=lambda x: x[2])[1]] if tune else "two-wide"
     for w in range(args.warmup):
         t = time.perf_counter()
         Fm = illum.compute_F(albedo, offset)
@@ -417,7 +419,8 @@ def main():
         "config": {"workload": f"compute_F on seeded cratered ellipsoid, {nf} facets "
                                f"(icosphere subdiv {args.subdiv}, {args.craters} craters), "
                                "per-facet albedo U(0.05,0.15), CSC+CSR assembly",
-                   "facets": nf, "nnz": nnz, "parallelism": "single GPU"},
+                   "facets": nf, "nnz": nnz, "parallelism": "single GPU",
+                   "bvh": dict(illum.bvh.info(), walk=walk)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "profile_ms": {k: v[0] / max(1, v[1]) for k, v in prof.items()},

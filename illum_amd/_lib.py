@@ -58,7 +58,7 @@ SIGNATURES = {
     "illum_mesh_triangles": [_P, _P],
     "illum_bvh_build": [_P, _I, _PP],
     "illum_bvh_destroy": [_P],
-    "illum_bvh_tuning": [_P, _I, _P, _P, _P],
+    "illum_bvh_tuning": [_P, _I, _P, _P, _P, _P],
     "illum_bvh_info": [_P, _P, _P, _P],
     "illum_bvh_closest_hit": [_P, _P, _P, _I64, _P, _P],
     "illum_form_factors": [_P, _D, _P, _D, _PP],

@@ -237,12 +237,14 @@ class BVH:
         return dict(nodes=n.value, depth=d.value, leaf_size=l.value)
 
     def tuning(self):
-        """[(leaf_size, ms)] of the autotuning trials (empty unless leaf_size was 0)."""
-        leaves = np.zeros(8, np.int32)
-        ms = np.zeros(8)
+        """[(leaf_size, layout, ms)] of the autotuning trials (empty unless
+        leaf_size was 0); layout 2 = two-wide, 4 = four-wide walk."""
+        leaves = np.zeros(16, np.int32)
+        layouts = np.zeros(16, np.int32)
+        ms = np.zeros(16)
         n = C.c_int()
-        call("illum_bvh_tuning", self.h, 8, _ptr(leaves), _ptr(ms), C.byref(n))
-        return [(int(leaves[k]), float(ms[k])) for k in range(n.value)]
+        call("illum_bvh_tuning", self.h, 16, _ptr(leaves), _ptr(layouts), _ptr(ms), C.byref(n))
+        return [(int(leaves[k]), int(layouts[k]), float(ms[k])) for k in range(n.value)]
 
     def closest_hit(self, orig, dirs):
         orig = np.ascontiguousarray(orig, np.float32)

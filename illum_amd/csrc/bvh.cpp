@@ -240,6 +240,72 @@ static HostBVH build_one(const HostGeometry &g, int64_t f0, int64_t f1, int leaf
       }
     }
   }
+  // Four-wide layout: the binary tree collapsed so each node holds up to 4
+  // children -- a binary node's two children, the inner child with the
+  // largest box area replaced by its own two children until there are four.
+  // Node = 8 float4: lo.x, lo.y, lo.z, hi.x, hi.y, hi.z of children 0..3
+  // (SoA), child refs (>= 0 node, < 0 ~leaf info), (child count, 0, 0, 0).
+  {
+    std::vector<std::vector<int>> kids;
+    std::vector<int> n4_of(nn, -1), d4;
+    auto expand = [&](int id) {
+      std::vector<int> ch = {B.nodes[id].left, B.nodes[id].right};
+      while (ch.size() < 4) {
+        int best = -1;
+        double ba = -1.0;
+        for (size_t q = 0; q < ch.size(); ++q)
+          if (B.nodes[ch[q]].left >= 0 && B.nodes[ch[q]].box.area() > ba) {
+            ba = B.nodes[ch[q]].box.area();
+            best = (int)q;
+          }
+        if (best < 0) break;
+        const int m = ch[best];
+        ch.erase(ch.begin() + best);
+        ch.push_back(B.nodes[m].left);
+        ch.push_back(B.nodes[m].right);
+      }
+      return ch;
+    };
+    int depth4 = 0;
+    if (B.nodes[0].left < 0) {
+      kids.push_back({0});  // root is a leaf
+    } else {
+      std::vector<std::pair<int, int>> st = {{0, 0}};
+      while (!st.empty()) {
+        const int id = st.back().first, dep = st.back().second;
+        st.pop_back();
+        depth4 = std::max(depth4, dep);
+        n4_of[id] = (int)kids.size();
+        kids.push_back(expand(id));
+        const auto &ch = kids.back();
+        for (int q = (int)ch.size() - 1; q >= 0; --q)
+          if (B.nodes[ch[q]].left >= 0) st.push_back({ch[q], dep + 1});
+      }
+    }
+    h.ninner4 = (int64_t)kids.size();
+    h.depth4 = depth4;
+    h.wide4.assign(32 * kids.size(), 0.0f);
+    for (size_t k = 0; k < kids.size(); ++k) {
+      float *o = &h.wide4[32 * k];
+      const auto &ch = kids[k];
+      for (size_t q = 0; q < ch.size(); ++q) {
+        const BuildNode &cn = B.nodes[ch[q]];
+        for (int a = 0; a < 3; ++a) {
+          o[4 * a + q] = cn.box.lo[a] - pad;
+          o[4 * (3 + a) + q] = cn.box.hi[a] + pad;
+        }
+        int32_t ref;
+        if (cn.left >= 0) {
+          ref = n4_of[ch[q]];
+        } else {
+          if (cn.count >= 128) throw Error(ILLUM_ERR_CAPACITY, "leaf too large");
+          ref = ~((cn.count << 24) | cn.first);
+        }
+        o[24 + q] = as_float(ref);
+      }
+      o[28] = as_float((int32_t)ch.size());
+    }
+  }
   for (int64_t k = 0; k < nf; ++k) {
     const int64_t t = f0 + B.perm[k];
     float *o = &h.tris[12 * (size_t)k];
@@ -268,6 +334,7 @@ HostBVH build_bvh(const HostGeometry &g, int leaf_size) {
     HostBVH h = build_one(g, 0, nf, leaf_size, pad);
     h.root_thr = {0};
     h.root_wide = {0};
+    h.root_wide4 = {0};
     return h;
   }
   // Forest: concatenate per-body trees, shifting node links and leaf ranges.
@@ -277,11 +344,24 @@ HostBVH build_bvh(const HostGeometry &g, int leaf_size) {
   for (int b = 0; b < nb; ++b) {
     HostBVH t = build_one(g, off[b], off[b + 1], leaf_size, pad);
     const int32_t nbase = (int32_t)h.nnodes, ibase = (int32_t)h.ninner;
+    const int32_t qbase = (int32_t)h.ninner4;
     const int32_t tbase = (int32_t)off[b];
     if (off[b + 1] >= (1 << 24)) throw Error(ILLUM_ERR_CAPACITY, "too many facets for leaf ranges");
     h.root_thr.push_back(nbase);
     h.root_wide.push_back(ibase);
+    h.root_wide4.push_back(qbase);
     h.depth = std::max(h.depth, t.depth);
+    h.depth4 = std::max(h.depth4, t.depth4);
+    for (int64_t k = 0; k < t.ninner4; ++k) {
+      float *o = &t.wide4[32 * (size_t)k];
+      const int32_t cnt = as_int(o[28]);
+      for (int q = 0; q < cnt; ++q) {
+        const int32_t ref = as_int(o[24 + q]);
+        o[24 + q] = as_float(ref >= 0 ? ref + qbase : ~(~ref + tbase));
+      }
+    }
+    h.wide4.insert(h.wide4.end(), t.wide4.begin(), t.wide4.end());
+    h.ninner4 += t.ninner4;
     for (int64_t k = 0; k < t.nnodes; ++k) {
       float *o = &t.nodes[8 * (size_t)k];
       const int32_t miss = as_int(o[3]), info = as_int(o[7]);

@@ -11,7 +11,7 @@
 namespace illum {
 
 // Implemented in the kernel translation units.
-double time_visibility_sample(illum_bvh *bvh, int max_sources);
+double time_visibility_sample(illum_bvh *bvh, int max_sources, int layout);
 illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_vec,
                            double offset);
 void render(illum_bvh *bvh, int mode, int width, int height, double az, double el,
@@ -114,7 +114,7 @@ static void destroy_mesh(illum_mesh *m) {
   delete m;
 }
 static void destroy_bvh(illum_bvh *b) {
-  dfree(b->d_nodes); dfree(b->d_tris); dfree(b->d_wide); dfree(b->d_roots);
+  dfree(b->d_nodes); dfree(b->d_tris); dfree(b->d_wide); dfree(b->d_wide4); dfree(b->d_roots);
   delete b;
 }
 void destroy_sparse(illum_sparse *s) {
@@ -418,12 +418,16 @@ static illum_bvh *make_bvh(illum_mesh *mesh, int leaf_size) {
                            hipMemcpyHostToDevice, s));
   ILLUM_HIP(hipMemcpyAsync(b->d_tris, b->h.tris.data(), 4 * b->h.tris.size(),
                            hipMemcpyHostToDevice, s));
+  b->d_wide4 = dalloc<float>(b->h.wide4.size());
+  ILLUM_HIP(hipMemcpyAsync(b->d_wide4, b->h.wide4.data(), 4 * b->h.wide4.size(),
+                           hipMemcpyHostToDevice, s));
   b->nbody = (int)b->h.root_thr.size();
   b->body_off = mesh->g.body_off;
-  std::vector<int32_t> roots(2 * b->nbody);
+  std::vector<int32_t> roots(4 * b->nbody, 0);
   for (int k = 0; k < b->nbody; ++k) {
-    roots[2 * k] = b->h.root_thr[k];
-    roots[2 * k + 1] = b->h.root_wide[k];
+    roots[4 * k] = b->h.root_thr[k];
+    roots[4 * k + 1] = b->h.root_wide[k];
+    roots[4 * k + 2] = b->h.root_wide4[k];
   }
   b->d_roots = dalloc<int32_t>(roots.size());
   ILLUM_HIP(hipMemcpyAsync(b->d_roots, roots.data(), 4 * roots.size(), hipMemcpyHostToDevice, s));
@@ -441,38 +445,50 @@ int illum_bvh_build(illum_mesh *mesh, int leaf_size, illum_bvh **out) {
       return;
     }
     // Autotune: time the visibility kernel on a strided sample of up to 8192
-    // sources for each candidate leaf size and keep the fastest tree.
+    // sources for each candidate leaf size and traversal layout (two-wide,
+    // four-wide) and keep the fastest tree and layout.
     static const int cand[] = {2, 3, 4, 6, 8};
     std::unique_ptr<illum_bvh, void (*)(illum_bvh *)> best(nullptr, destroy_bvh);
     double best_ms = 0;
-    int leaves[8];
-    double times[8];
+    int best_layout = 2;
+    int leaves[16], layouts[16];
+    double times[16];
     int n = 0;
     for (int c : cand) {
       std::unique_ptr<illum_bvh, void (*)(illum_bvh *)> b(make_bvh(mesh, c), destroy_bvh);
-      const double ms = time_visibility_sample(b.get(), 8192);
-      leaves[n] = c;
-      times[n] = ms;
-      ++n;
-      if (!best || ms < best_ms) {
-        best = std::move(b);
-        best_ms = ms;
+      bool improved = false;
+      for (int layout : {2, 4}) {
+        const double ms = time_visibility_sample(b.get(), 8192, layout);
+        leaves[n] = c;
+        layouts[n] = layout;
+        times[n] = ms;
+        ++n;
+        if (n == 1 || ms < best_ms) {
+          best_ms = ms;
+          best_layout = layout;
+          improved = true;
+        }
       }
+      if (improved) best = std::move(b);
     }
     for (int k = 0; k < n; ++k) {
       best->tune_leaf[k] = leaves[k];
+      best->tune_layout[k] = layouts[k];
       best->tune_ms[k] = times[k];
     }
     best->ntuned = n;
+    best->ff_layout = best_layout;
     *out = best.release();
   });
 }
 
-int illum_bvh_tuning(const illum_bvh *bvh, int cap, int *leaf_sizes, double *ms, int *n) {
+int illum_bvh_tuning(const illum_bvh *bvh, int cap, int *leaf_sizes, int *layouts, double *ms,
+                     int *n) {
   return guarded([&] {
     CHECK_ARG(bvh && n, "null argument");
     for (int k = 0; k < bvh->ntuned && k < cap; ++k) {
       if (leaf_sizes) leaf_sizes[k] = bvh->tune_leaf[k];
+      if (layouts) layouts[k] = bvh->tune_layout[k];
       if (ms) ms[k] = bvh->tune_ms[k];
     }
     *n = bvh->ntuned;

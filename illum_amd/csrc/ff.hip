@@ -51,13 +51,14 @@ __device__ __forceinline__ double ff_value(float4 pi, float4 ni, float4 pj, floa
   return ((mij * mji) * Aj) / ((M_PI * r) * r);
 }
 
-// COUNT adds the work counters (pair, box and triangle tests) to stats[1..3];
-// stats[0] always receives the number of rays.
-template <bool WIDE, bool COUNT>
+// LAYOUT: 4 = near-first walk of the four-wide tree, 2 = of the two-wide
+// tree, 0 = stackless threaded walk.  COUNT adds the work counters (pair, box
+// and triangle tests) to stats[1..3]; stats[0] always receives the rays.
+template <int LAYOUT, bool COUNT>
 __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
     int nf, int nwords, const float4 *__restrict__ pr4, const float4 *__restrict__ n4,
     const float4 *__restrict__ nodes, const float4 *__restrict__ wide,
-    const float4 *__restrict__ tris, float offset,
+    const float4 *__restrict__ wide4, const float4 *__restrict__ tris, float offset,
     const double *__restrict__ area, const double *__restrict__ albedo_vec, double albedo,
     uint32_t *__restrict__ vis, uint32_t *__restrict__ mask, int32_t *__restrict__ counts,
     int *__restrict__ status, int nrows, int jstride, unsigned long long *__restrict__ stats,
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
   uint32_t *bits = smem;
   int *queue = (int *)(smem + nwords);
   // surviving receiver clusters (16-bit ids: the LDS bitset bounds nf far
-  // below 64 x 65536), then scap x FF_BLOCK stack slots (WIDE)
+  // below 64 x 65536), then scap x FF_BLOCK stack slots (LAYOUT 2 or 4)
   uint16_t *clist = (uint16_t *)(queue + FF_QCAP);
   int *stack = (int *)(clist + ((nclus + 1) & ~1));
   __shared__ int s_qn, s_rays, s_nc;
@@ -83,7 +84,8 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
   // The source's body: receivers [lo, hi), its trees' roots.
   const int body = body_of ? body_of[j] : 0;
   const int lo = body_off[body], hi = body_off[body + 1];
-  const int root_thr = roots[2 * body], root_wide = roots[2 * body + 1];
+  const int root_thr = roots[4 * body], root_wide = roots[4 * body + 1];
+  const int root_wide4 = roots[4 * body + 2];
   const int clo = lo >> 6, chi = (hi + 63) >> 6;  // clusters overlapping [lo, hi)
   const float4 pj = pr4[j], nj = n4[j];
   const float qx = pj.x + offset * nj.x, qy = pj.y + offset * nj.y, qz = pj.z + offset * nj.z;
@@ -153,14 +155,21 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
         const DRay r = make_ray(qx, qy, qz, pi.x - qx, pi.y - qy, pi.z - qz);
         float t;
         int h;
-        if (WIDE) {
+        if (LAYOUT == 4) {
+          h = trace_closest_wide4(wide4, tris, r, root_wide4, FF_TINIT, stack + tid, FF_BLOCK,
+                                  scap, &t, nbox, ntri);
+          if (h == -1)
+            h = trace_closest_wide4(wide4, tris, r, root_wide4, __builtin_huge_valf(),
+                                    stack + tid, FF_BLOCK, scap, &t, nbox, ntri);
+          // a walk deeper than the LDS stack: the stackless walk answers the
+          // same unbounded query (identical result)
+          if (h == -2) h = trace_closest(nodes, tris, r, root_thr, &t, nbox, ntri);
+        } else if (LAYOUT == 2) {
           h = trace_closest_wide(wide, tris, r, root_wide, FF_TINIT, stack + tid, FF_BLOCK, scap,
                                  &t, nbox, ntri);
           if (h == -1)
             h = trace_closest_wide(wide, tris, r, root_wide, __builtin_huge_valf(), stack + tid,
                                    FF_BLOCK, scap, &t, nbox, ntri);
-          // a walk deeper than the LDS stack: the stackless walk answers the
-          // same unbounded query (identical result)
           if (h == -2) h = trace_closest(nodes, tris, r, root_thr, &t, nbox, ntri);
         } else {
           h = trace_closest(nodes, tris, r, root_thr, &t, nbox, ntri);
@@ -343,7 +352,7 @@ __global__ void popcount_rows_kernel(int nrows, int nwords, const uint32_t *__re
 // unbounded and the (distance, index) minima are combined exactly.
 __global__ __launch_bounds__(256) void closest_hit_kernel(
     int64_t nray, const float *__restrict__ orig, const float *__restrict__ dir,
-    const float4 *__restrict__ nodes, const float4 *__restrict__ wide,
+    const float4 *__restrict__ nodes, const float4 *__restrict__ wide4,
     const float4 *__restrict__ tris, const int32_t *__restrict__ roots, int nbody,
     int32_t *__restrict__ hit, float *__restrict__ tout) {
   __shared__ int stack[FF_STACK * 256];
@@ -357,10 +366,10 @@ __global__ __launch_bounds__(256) void closest_hit_kernel(
   for (int b = 0; b < nbody; ++b) {
     float t;
     int h = -2;
-    if (wide)
-      h = trace_closest_wide(wide, tris, r, roots[2 * b + 1], __builtin_huge_valf(),
-                             stack + threadIdx.x, 256, FF_STACK, &t, nbox, ntri);
-    if (h == -2) h = trace_closest(nodes, tris, r, roots[2 * b], &t, nbox, ntri);  // deep walk
+    if (wide4)
+      h = trace_closest_wide4(wide4, tris, r, roots[4 * b + 2], __builtin_huge_valf(),
+                              stack + threadIdx.x, 256, FF_STACK, &t, nbox, ntri);
+    if (h == -2) h = trace_closest(nodes, tris, r, roots[4 * b], &t, nbox, ntri);  // deep walk
     if (h >= 0 && (t < tbest || (t == tbest && h < best))) {
       tbest = t;
       best = h;
@@ -370,13 +379,31 @@ __global__ __launch_bounds__(256) void closest_hit_kernel(
   tout[k] = tbest;
 }
 
-// The near-first layout is used unless ILLUM_FF_TRAVERSAL is "threaded".  Its
-// LDS stack holds min(depth + 1, FF_STACK) entries per lane; a walk that needs
-// more is answered by the stackless threaded walk (same result).
-static bool use_wide(const illum_bvh *bvh) {
+// Traversal layout: the BVH's (two-wide near-first unless the autotuner chose
+// four-wide), or ILLUM_FF_TRAVERSAL = "wide2", "wide4" or "threaded"
+// (stackless).  The near-first walks keep an LDS stack of at most FF_STACK
+// entries per lane; a walk that needs more is answered by the stackless
+// threaded walk (same result).
+static int ff_layout(const illum_bvh *bvh) {
   const char *e = getenv("ILLUM_FF_TRAVERSAL");
-  if (e && std::string(e) == "threaded") return false;
-  return bvh->d_wide != nullptr;
+  if (e && std::string(e) == "threaded") return 0;
+  if (e && std::string(e) == "wide2") return bvh->d_wide ? 2 : 0;
+  if (e && std::string(e) == "wide4") return bvh->d_wide4 ? 4 : 0;
+  return bvh->ff_layout == 4 && bvh->d_wide4 ? 4 : (bvh->d_wide ? 2 : 0);
+}
+
+// LDS stack entries a lane needs for a layout: one push per level (two-wide),
+// up to three per level (four-wide), capped at FF_STACK.
+static int ff_stack_slots(const illum_bvh *bvh, int layout) {
+  if (layout == 2) return std::min(bvh->h.depth + 1, FF_STACK);
+  if (layout == 4) return std::min(3 * bvh->h.depth4 + 1, FF_STACK);
+  return 0;
+}
+
+template <bool COUNT>
+static decltype(&ff_visibility_kernel<4, COUNT>) ff_kernel(int layout) {
+  return layout == 4 ? ff_visibility_kernel<4, COUNT>
+                     : layout == 2 ? ff_visibility_kernel<2, COUNT> : ff_visibility_kernel<0, COUNT>;
 }
 
 // Dynamic LDS of ff_visibility_kernel: bitset, ray queue, 16-bit cluster list
@@ -406,7 +433,7 @@ void bvh_closest_hit(illum_bvh *bvh, const float *orig, const float *dir, int64_
   const int blocks = (int)((nray + 255) / 256);
   closest_hit_kernel<<<blocks, 256, 0, s>>>(
       nray, o.p, d.p, (const float4 *)bvh->d_nodes,
-      use_wide(bvh) ? (const float4 *)bvh->d_wide : nullptr, (const float4 *)bvh->d_tris,
+      ff_layout(bvh) == 4 ? (const float4 *)bvh->d_wide4 : nullptr, (const float4 *)bvh->d_tris,
       bvh->d_roots, bvh->nbody, h.p, tt.p);
   ILLUM_LAUNCH_CHECK();
   ILLUM_HIP(hipMemcpyAsync(hit, h.p, 4 * nray, hipMemcpyDeviceToHost, s));
@@ -423,14 +450,13 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   const int64_t nf = mesh->g.nf;
   const int nwords = (int)(((nf + 511) / 512) * 16);
   const int64_t rows_p = (int64_t)nwords * 32;
-  const bool wide = use_wide(bvh);
+  const int layout = ff_layout(bvh);
   // traversal stack: one slot per tree level (a push happens at most once per level)
-  const int stack_slots = wide ? std::min(bvh->h.depth + 1, FF_STACK) : 0;
+  const int stack_slots = ff_stack_slots(bvh, layout);
   const size_t lds = ff_lds_bytes(nwords, mesh->nclus, stack_slots);
   if (lds > 160 * 1024)
     throw Error(ILLUM_ERR_CAPACITY, "too many facets for the LDS visibility bitset (max ~1.2M)");
-  const void *kfn = wide ? (const void *)ff_visibility_kernel<true, false>
-                         : (const void *)ff_visibility_kernel<false, false>;
+  const void *kfn = (const void *)ff_kernel<false>(layout);
   if (lds > 64 * 1024)
     ILLUM_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 
@@ -460,10 +486,11 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
 
   {
     StageTimer st(ctx, "ff_visibility");
-    auto launch = wide ? ff_visibility_kernel<true, false> : ff_visibility_kernel<false, false>;
+    auto launch = ff_kernel<false>(layout);
     launch<<<(int)nf, FF_BLOCK, lds, s>>>(
         (int)nf, nwords, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4,
-        (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_wide, (const float4 *)bvh->d_tris,
+        (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_wide, (const float4 *)bvh->d_wide4,
+        (const float4 *)bvh->d_tris,
         (float)offset, mesh->d_area, d_alb.p, albedo, sp->d_vis, M, counts, status.p,
         (int)nf, 1, sp->d_nrays, cluster_cull() ? (const float4 *)mesh->d_clus : nullptr,
         (int)mesh->nclus, stack_slots,
@@ -520,7 +547,7 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
 
 // Times ff_visibility on a strided sample of sources for one BVH (ms, from
 // hipEvents on the context stream).  Used by the leaf-size autotuner.
-double time_visibility_sample(illum_bvh *bvh, int max_sources) {
+double time_visibility_sample(illum_bvh *bvh, int max_sources, int layout_choice) {
   check_bvh_bodies(bvh);
   illum_mesh *mesh = bvh->mesh;
   illum_ctx *ctx = mesh->ctx;
@@ -529,12 +556,11 @@ double time_visibility_sample(illum_bvh *bvh, int max_sources) {
   const int jstride = (int)std::max<int64_t>(1, nf / std::max(1, max_sources));
   const int nrows = (int)((nf + jstride - 1) / jstride);
   const int nwords = (int)(((nf + 511) / 512) * 16);
-  const bool wide = use_wide(bvh);
-  const int stack_slots = wide ? std::min(bvh->h.depth + 1, FF_STACK) : 0;
+  const int layout = layout_choice == 4 && bvh->d_wide4 ? 4 : ff_layout(bvh);
+  const int stack_slots = ff_stack_slots(bvh, layout);
   const size_t lds = ff_lds_bytes(nwords, mesh->nclus, stack_slots);
   if (lds > 160 * 1024) throw Error(ILLUM_ERR_CAPACITY, "too many facets for the LDS bitset");
-  const void *kfn = wide ? (const void *)ff_visibility_kernel<true, false>
-                         : (const void *)ff_visibility_kernel<false, false>;
+  const void *kfn = (const void *)ff_kernel<false>(layout);
   if (lds > 64 * 1024)
     ILLUM_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   DBuf<uint32_t> vis((size_t)nrows * nwords), M((size_t)nrows * nwords);
@@ -543,13 +569,14 @@ double time_visibility_sample(illum_bvh *bvh, int max_sources) {
   hipEvent_t e0, e1;
   ILLUM_HIP(hipEventCreate(&e0));
   ILLUM_HIP(hipEventCreate(&e1));
-  auto launch = wide ? ff_visibility_kernel<true, false> : ff_visibility_kernel<false, false>;
+  auto launch = ff_kernel<false>(layout);
   float best = 1e30f;
   for (int rep = 0; rep < 2; ++rep) {  // first launch also loads the code object
     ILLUM_HIP(hipEventRecord(e0, s));
     launch<<<nrows, FF_BLOCK, lds, s>>>(
         (int)nf, nwords, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4,
-        (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_wide, (const float4 *)bvh->d_tris,
+        (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_wide, (const float4 *)bvh->d_wide4,
+        (const float4 *)bvh->d_tris,
         1e-5f, mesh->d_area, nullptr, 1.0, vis.p, M.p, counts.p, status.p, nrows, jstride, nullptr,
         cluster_cull() ? (const float4 *)mesh->d_clus : nullptr, (int)mesh->nclus, stack_slots,
         mesh->d_body, mesh->d_boff, bvh->d_roots);
@@ -575,12 +602,11 @@ void form_factor_work(illum_bvh *bvh, double offset, int64_t *out4) {
   hipStream_t s = stream_of(ctx);
   const int64_t nf = mesh->g.nf;
   const int nwords = (int)(((nf + 511) / 512) * 16);
-  const bool wide = use_wide(bvh);
-  const int stack_slots = wide ? std::min(bvh->h.depth + 1, FF_STACK) : 0;
+  const int layout = ff_layout(bvh);
+  const int stack_slots = ff_stack_slots(bvh, layout);
   const size_t lds = ff_lds_bytes(nwords, mesh->nclus, stack_slots);
   if (lds > 160 * 1024) throw Error(ILLUM_ERR_CAPACITY, "too many facets for the LDS bitset");
-  const void *kfn = wide ? (const void *)ff_visibility_kernel<true, true>
-                         : (const void *)ff_visibility_kernel<false, true>;
+  const void *kfn = (const void *)ff_kernel<true>(layout);
   if (lds > 64 * 1024)
     ILLUM_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const size_t mbytes = (size_t)nf * nwords * 4;
@@ -590,10 +616,11 @@ void form_factor_work(illum_bvh *bvh, double offset, int64_t *out4) {
   DBuf<int> status(1);
   DBuf<unsigned long long> stats(4);
   ILLUM_HIP(hipMemsetAsync(stats.p, 0, 32, s));
-  auto launch = wide ? ff_visibility_kernel<true, true> : ff_visibility_kernel<false, true>;
+  auto launch = ff_kernel<true>(layout);
   launch<<<(int)nf, FF_BLOCK, lds, s>>>(
       (int)nf, nwords, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4,
-      (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_wide, (const float4 *)bvh->d_tris,
+      (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_wide, (const float4 *)bvh->d_wide4,
+        (const float4 *)bvh->d_tris,
       (float)offset, mesh->d_area, nullptr, 1.0, vis, M, counts, status.p, (int)nf, 1, stats.p,
       cluster_cull() ? (const float4 *)mesh->d_clus : nullptr, (int)mesh->nclus, stack_slots,
         mesh->d_body, mesh->d_boff, bvh->d_roots);

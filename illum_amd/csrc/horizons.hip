@@ -20,16 +20,20 @@
 
 namespace illum {
 
-// WIDE: near-first any-hit over the two-wide layout with a per-lane stack in
-// dynamic LDS ((depth + 1) x 256 ints); otherwise the stackless threaded walk.
-template <bool WIDE>
+// LAYOUT 4: near-first any-hit over the four-wide layout with a per-lane
+// stack of `scap` entries in dynamic LDS (a walk needing more falls back to
+// the threaded walk, same answer); LAYOUT 2: the two-wide layout with a
+// (depth + 1)-entry stack; LAYOUT 0: the stackless threaded walk.
+template <int LAYOUT>
 __global__ __launch_bounds__(256) void horizons_kernel(
     int64_t nf, int nphi, const float4 *__restrict__ pr4, const float4 *__restrict__ n4,
     const double *__restrict__ frame, const float4 *__restrict__ nodes,
-    const float4 *__restrict__ wide, const float4 *__restrict__ tris, float offset,
+    const float4 *__restrict__ wide, const float4 *__restrict__ wide4,
+    const float4 *__restrict__ tris, float offset,
     const double *__restrict__ cphi, const double *__restrict__ sphi,
     const double *__restrict__ th, int tree_nodes, double *__restrict__ out,
-    const int32_t *__restrict__ roots, int nroots, const int32_t *__restrict__ body_of) {
+    const int32_t *__restrict__ roots, int nroots, const int32_t *__restrict__ body_of,
+    int scap) {
   extern __shared__ int hstack[];
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nf * nphi) return;
@@ -54,9 +58,17 @@ __global__ __launch_bounds__(256) void horizons_kernel(
     // (one body, or an occluder mesh).
     const int b0 = body_of ? body_of[j] : 0, b1 = body_of ? b0 + 1 : nroots;
     bool hit = false;
-    for (int b = b0; b < b1 && !hit; ++b)
-      hit = WIDE ? trace_any_wide(wide, tris, r, roots[2 * b + 1], hstack + threadIdx.x, 256)
-                 : trace_any(nodes, tris, r, roots[2 * b]);
+    for (int b = b0; b < b1 && !hit; ++b) {
+      if (LAYOUT == 4) {
+        const int a = trace_any_wide4(wide4, tris, r, roots[4 * b + 2], hstack + threadIdx.x,
+                                      256, scap);
+        hit = a == -2 ? trace_any(nodes, tris, r, roots[4 * b]) : a == 1;
+      } else if (LAYOUT == 2) {
+        hit = trace_any_wide(wide, tris, r, roots[4 * b + 1], hstack + threadIdx.x, 256);
+      } else {
+        hit = trace_any(nodes, tris, r, roots[4 * b]);
+      }
+    }
     m = 2 * m + (hit ? 0 : 1);
     if (m >= tree_nodes) { out[idx] = __builtin_nan(""); return; }
     if (th[4 * m + 3] != 0.0) break;
@@ -139,15 +151,23 @@ void horizons(illum_bvh *bvh, illum_bvh *occl, int nphi, double theta_eps, doubl
   {
     StageTimer st(ctx, "horizons");
     const char *e = getenv("ILLUM_HZ_TRAVERSAL");
-    const bool wide = !(e && std::string(e) == "threaded") && target->d_wide &&
-                      target->h.depth <= 32;
-    const size_t lds = wide ? (size_t)(target->h.depth + 1) * 256 * 4 : 0;
-    auto kern = wide ? horizons_kernel<true> : horizons_kernel<false>;
+    // ILLUM_HZ_TRAVERSAL: "threaded", "wide4", or the two-wide walk (default;
+    // faster than four-wide for these short bisection rays)
+    const std::string ev = e ? e : "";
+    const int layout = ev == "threaded" ? 0
+                       : ev == "wide4" ? (target->d_wide4 ? 4 : 0)
+                                       : (target->d_wide && target->h.depth <= 32 ? 2 : 0);
+    const int scap = layout == 4 ? std::min(3 * target->h.depth4 + 1, 16)
+                     : layout == 2 ? target->h.depth + 1 : 0;
+    const size_t lds = (size_t)scap * 256 * 4;
+    auto kern = layout == 4 ? horizons_kernel<4> : layout == 2 ? horizons_kernel<2>
+                                                               : horizons_kernel<0>;
     kern<<<(unsigned)((total + 255) / 256), 256, lds, s>>>(
         nf, nphi, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4, mesh->d_frame,
         (const float4 *)target->d_nodes, (const float4 *)target->d_wide,
-        (const float4 *)target->d_tris, (float)offset, d_c.p, d_s.p, d_tab.p, tree_nodes, dst,
-        target->d_roots, target->nbody, occl ? nullptr : mesh->d_body);
+        (const float4 *)target->d_wide4, (const float4 *)target->d_tris, (float)offset, d_c.p,
+        d_s.p, d_tab.p, tree_nodes, dst, target->d_roots, target->nbody,
+        occl ? nullptr : mesh->d_body, scap);
     ILLUM_LAUNCH_CHECK();
   }
   if (!(flags & ILLUM_OUT_ON_DEVICE))

@@ -73,12 +73,15 @@ void read_obj(const char *path, std::vector<double> &V, std::vector<int32_t> &F,
 struct HostBVH {
   std::vector<float> nodes;    // 8 floats per node
   std::vector<float> wide;     // 16 floats per inner node
+  std::vector<float> wide4;    // 32 floats per four-wide node
   std::vector<float> tris;     // 12 floats per triangle
   int64_t nnodes = 0;
   int64_t ninner = 0;
+  int64_t ninner4 = 0;
   int depth = 0;
+  int depth4 = 0;
   int leaf_size = 0;
-  std::vector<int32_t> root_thr, root_wide;  // per body (one entry for one body)
+  std::vector<int32_t> root_thr, root_wide, root_wide4;  // per body (one for one body)
 };
 HostBVH build_bvh(const HostGeometry &g, int leaf_size);
 
@@ -121,13 +124,17 @@ struct illum_bvh {
   float *d_nodes = nullptr;
   float *d_tris = nullptr;
   float *d_wide = nullptr;
-  int32_t *d_roots = nullptr;  // per body: threaded root, two-wide root
+  float *d_wide4 = nullptr;
+  int32_t *d_roots = nullptr;  // per body: threaded, two-wide, four-wide root, 0
   int nbody = 1;               // bodies of the mesh when the tree was built
   std::vector<int64_t> body_off;  // the partition the forest was built for
-  // leaf-size autotuning record (illum_bvh_build with leaf_size 0)
+  // form-factor traversal layout (2 = two-wide, 4 = four-wide near-first walk)
+  int ff_layout = 2;
+  // autotuning record (illum_bvh_build with leaf_size 0): leaf size x layout
   int ntuned = 0;
-  int tune_leaf[8] = {0};
-  double tune_ms[8] = {0};
+  int tune_leaf[16] = {0};
+  int tune_layout[16] = {0};
+  double tune_ms[16] = {0};
 };
 
 struct illum_sparse {

@@ -236,6 +236,141 @@ __device__ __forceinline__ int trace_closest_wide(const float4 *__restrict__ wid
   return sp < 0 ? -2 : best;
 }
 
+// Slab test of the four children of a four-wide node (SoA boxes); returns
+// the entry distance, +inf for a child that is absent or need not be visited
+// (same acceptance as box_enter against tbest).
+__device__ __forceinline__ void box4_enter(const DRay &r, const float4 *__restrict__ nd,
+                                           float tbest, float key[4], int ref[4]) {
+  const float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5];
+  const float4 rf = nd[6];
+  const int cnt = __float_as_int(nd[7].x);
+  const float lox[4] = {lx.x, lx.y, lx.z, lx.w}, loy[4] = {ly.x, ly.y, ly.z, ly.w},
+              loz[4] = {lz.x, lz.y, lz.z, lz.w}, hix[4] = {hx.x, hx.y, hx.z, hx.w},
+              hiy[4] = {hy.x, hy.y, hy.z, hy.w}, hiz[4] = {hz.x, hz.y, hz.z, hz.w};
+  const float rfa[4] = {rf.x, rf.y, rf.z, rf.w};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float t0x = (lox[q] - r.ox) * r.ix, t1x = (hix[q] - r.ox) * r.ix;
+    const float t0y = (loy[q] - r.oy) * r.iy, t1y = (hiy[q] - r.oy) * r.iy;
+    const float t0z = (loz[q] - r.oz) * r.iz, t1z = (hiz[q] - r.oz) * r.iz;
+    const float tn = fmaxf(fminf(t0x, t1x), fmaxf(fminf(t0y, t1y), fminf(t0z, t1z)));
+    const float tf = fminf(fmaxf(t0x, t1x), fminf(fmaxf(t0y, t1y), fmaxf(t0z, t1z)));
+    const bool ok = q < cnt && tn <= tf && tf > 0.0f && tn <= tbest;
+    key[q] = ok ? fminf(tn, 3.0e38f) : __builtin_huge_valf();
+    ref[q] = __float_as_int(rfa[q]);
+  }
+}
+
+__device__ __forceinline__ void cswap4(float *k, int *r, int a, int b) {
+  if (k[b] < k[a]) {
+    const float tk = k[a]; k[a] = k[b]; k[b] = tk;
+    const int tr = r[a]; r[a] = r[b]; r[b] = tr;
+  }
+}
+
+// Near-first closest hit over the four-wide tree rooted at `root`: the
+// entered children are sorted by entry distance (5-comparator network), the
+// nearest is visited next and the others pushed farthest first.  Returns the
+// facet index, -1 for no hit, or -2 when a push would exceed `cap` entries.
+// Exact for the same reason as the two-wide walk (ancestor boxes enter no
+// later than any hit inside them; ties go to the smaller index).
+__device__ __forceinline__ int trace_closest_wide4(const float4 *__restrict__ w4,
+                                                   const float4 *__restrict__ tris,
+                                                   const DRay &r, int root, float tinit,
+                                                   int *stack, int stride, int cap,
+                                                   float *tbest_out, unsigned &nbox,
+                                                   unsigned &ntri) {
+  float tbest = tinit;
+  int best = -1, sp = 0, cur = root;
+  for (;;) {
+    if (cur >= 0) {
+      float key[4];
+      int ref[4];
+      box4_enter(r, w4 + 8 * cur, tbest, key, ref);
+      nbox += 4;
+      cswap4(key, ref, 0, 1);
+      cswap4(key, ref, 2, 3);
+      cswap4(key, ref, 0, 2);
+      cswap4(key, ref, 1, 3);
+      cswap4(key, ref, 1, 2);
+      const int nh = (key[0] < __builtin_huge_valf()) + (key[1] < __builtin_huge_valf()) +
+                     (key[2] < __builtin_huge_valf()) + (key[3] < __builtin_huge_valf());
+      if (nh > 0) {
+        if (sp + nh - 1 > cap) {
+          sp = -1;
+          break;
+        }
+        if (nh > 3) stack[(sp++) * stride] = ref[3];
+        if (nh > 2) stack[(sp++) * stride] = ref[2];
+        if (nh > 1) stack[(sp++) * stride] = ref[1];
+        cur = ref[0];
+        continue;
+      }
+    } else {
+      const int info = ~cur;
+      const int first = info & 0xFFFFFF, cnt = info >> 24;
+      ntri += cnt;
+      for (int k = 0; k < cnt; ++k) {
+        const float4 a = tris[3 * (first + k)], b = tris[3 * (first + k) + 1],
+                     c = tris[3 * (first + k) + 2];
+        const int idx = __float_as_int(a.w);
+        float t;
+        if (tri_hit(r, a, b, c, &t) && (t < tbest || (t == tbest && idx < best))) {
+          tbest = t;
+          best = idx;
+        }
+      }
+    }
+    if (sp == 0) break;
+    --sp;
+    cur = stack[sp * stride];
+  }
+  *tbest_out = tbest;
+  return sp < 0 ? -2 : best;
+}
+
+// Any hit (t > 0) over the four-wide tree rooted at `root`: returns 1 on a
+// hit, 0 for none, -2 when a push would exceed `cap` stack entries.
+__device__ __forceinline__ int trace_any_wide4(const float4 *__restrict__ w4,
+                                               const float4 *__restrict__ tris, const DRay &r,
+                                               int root, int *stack, int stride, int cap) {
+  int sp = 0, cur = root;
+  for (;;) {
+    if (cur >= 0) {
+      float key[4];
+      int ref[4];
+      box4_enter(r, w4 + 8 * cur, __builtin_huge_valf(), key, ref);
+      cswap4(key, ref, 0, 1);
+      cswap4(key, ref, 2, 3);
+      cswap4(key, ref, 0, 2);
+      cswap4(key, ref, 1, 3);
+      cswap4(key, ref, 1, 2);
+      const int nh = (key[0] < __builtin_huge_valf()) + (key[1] < __builtin_huge_valf()) +
+                     (key[2] < __builtin_huge_valf()) + (key[3] < __builtin_huge_valf());
+      if (nh > 0) {
+        if (sp + nh - 1 > cap) return -2;
+        if (nh > 3) stack[(sp++) * stride] = ref[3];
+        if (nh > 2) stack[(sp++) * stride] = ref[2];
+        if (nh > 1) stack[(sp++) * stride] = ref[1];
+        cur = ref[0];
+        continue;
+      }
+    } else {
+      const int info = ~cur;
+      const int first = info & 0xFFFFFF, cnt = info >> 24;
+      for (int k = 0; k < cnt; ++k) {
+        float t;
+        if (tri_hit(r, tris[3 * (first + k)], tris[3 * (first + k) + 1],
+                    tris[3 * (first + k) + 2], &t))
+          return 1;
+      }
+    }
+    if (sp == 0) return 0;
+    --sp;
+    cur = stack[sp * stride];
+  }
+}
+
 // Any hit (t > 0) over the two-wide tree rooted at `root`, nearer child
 // first (stack as in trace_closest_wide; the tree depth must fit it).
 __device__ __forceinline__ bool trace_any_wide(const float4 *__restrict__ wide,

@@ -82,9 +82,9 @@ __global__ __launch_bounds__(256) void render_kernel(
     float t;
     int h = -2;
     if (wide)
-      h = trace_closest_wide(wide, tris, r, roots[2 * b + 1], __builtin_huge_valf(),
+      h = trace_closest_wide(wide, tris, r, roots[4 * b + 1], __builtin_huge_valf(),
                              stack + threadIdx.x, 256, RENDER_STACK, &t, nbox, ntri);
-    if (h == -2) h = trace_closest(nodes, tris, r, roots[2 * b], &t, nbox, ntri);
+    if (h == -2) h = trace_closest(nodes, tris, r, roots[4 * b], &t, nbox, ntri);
     if (h >= 0 && (t < tbest || (t == tbest && h < best))) {
       tbest = t;
       best = h;

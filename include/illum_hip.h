@@ -128,12 +128,14 @@ int illum_mesh_geometry(const illum_mesh *mesh, float *centroid, float *normal,
 /* FP32 vertices of every facet after scaling, nf x 9 floats (v0, v1, v2). */
 int illum_mesh_triangles(const illum_mesh *mesh, float *tri9);
 
-/* ---- BVH (host SAH build, flattened threaded layout uploaded to HBM) ----
+/* ---- BVH (host SAH build; threaded, two-wide and four-wide layouts in HBM)
  * leaf_size 0 autotunes: the visibility kernel is timed (hipEvents) on a
- * strided sample of up to 8192 source facets for leaf sizes 2, 3, 4, 6, 8 and
- * the fastest tree is kept; illum_bvh_tuning reports the trial times. */
+ * strided sample of up to 8192 source facets for leaf sizes 2, 3, 4, 6, 8
+ * and the two-wide and four-wide walks, and the fastest tree and walk are
+ * kept; illum_bvh_tuning reports the trials (leaf size, layout 2 or 4, ms). */
 int illum_bvh_build(illum_mesh *mesh, int leaf_size, illum_bvh **out);
-int illum_bvh_tuning(const illum_bvh *bvh, int cap, int *leaf_sizes, double *ms, int *n);
+int illum_bvh_tuning(const illum_bvh *bvh, int cap, int *leaf_sizes, int *layouts, double *ms,
+                     int *n);
 int illum_bvh_destroy(illum_bvh *bvh);
 int illum_bvh_info(const illum_bvh *bvh, int64_t *nodes, int *depth, int *leaf_size);
 /* Closest-hit query for arbitrary rays (testing/diagnostics): origins and

@@ -94,16 +94,19 @@ def test_form_factors_vs_oracle(api, subdiv, ncraters, albedo):
 
 def test_autotuned_bvh_and_traversal_choice_do_not_change_results(api, monkeypatch):
     # Visibility is a property of the triangles, not of the tree: the autotuned
-    # leaf size, leaf size 1, and the stackless threaded walk must all agree.
+    # leaf size, leaf size 1, the four-wide (default) and two-wide near-first
+    # walks and the stackless threaded walk must all agree.
     V, F = _mesh(api, 4, 40, seed=8)
     ref = api.IllumContext(V=V, F=F, leaf_size=4).compute_F(0.1)
     tuned = api.IllumContext(V=V, F=F, leaf_size=0)
     trials = tuned.bvh.tuning()
-    assert [l for l, _ in trials] == [2, 3, 4, 6, 8] and all(ms > 0 for _, ms in trials)
-    assert tuned.bvh.info()["leaf_size"] == min(trials, key=lambda x: x[1])[0]
+    assert [(l, w) for l, w, _ in trials] == [(l, w) for l in (2, 3, 4, 6, 8) for w in (2, 4)]
+    assert all(ms > 0 for _, _, ms in trials)
+    assert tuned.bvh.info()["leaf_size"] == min(trials, key=lambda x: x[2])[0]
     out = [tuned.compute_F(0.1), api.IllumContext(V=V, F=F, leaf_size=1).compute_F(0.1)]
-    monkeypatch.setenv("ILLUM_FF_TRAVERSAL", "threaded")
-    out.append(api.IllumContext(V=V, F=F, leaf_size=4).compute_F(0.1))
+    for walk in ("threaded", "wide2", "wide4"):
+        monkeypatch.setenv("ILLUM_FF_TRAVERSAL", walk)
+        out.append(api.IllumContext(V=V, F=F, leaf_size=4).compute_F(0.1))
     r = ref.csc()
     for m in out:
         assert np.array_equal(m.visibility(), ref.visibility())
@@ -142,9 +145,10 @@ def test_horizon_traversal_choice_does_not_change_results(api, monkeypatch):
     V, F = _mesh(api, 3, 40, seed=9)
     ctx = api.IllumContext(V=V, F=F)
     a = ctx.make_horizons(nphi=33).copy()
-    monkeypatch.setenv("ILLUM_HZ_TRAVERSAL", "threaded")
-    b = ctx.make_horizons(nphi=33)
-    assert np.array_equal(a, b)
+    for walk in ("threaded", "wide4"):
+        monkeypatch.setenv("ILLUM_HZ_TRAVERSAL", walk)
+        b = ctx.make_horizons(nphi=33)
+        assert np.array_equal(a, b), walk
 
 
 def test_form_factors_albedo_vector(api):
