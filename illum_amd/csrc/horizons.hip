@@ -175,6 +175,13 @@ void horizons(illum_bvh *bvh, illum_bvh *occl, int nphi, double theta_eps, doubl
   sync(ctx);
 }
 
+// ILLUM_DIRECT_EXACT=1 evaluates every disk sample (no whole-disk decision);
+// results are identical either way, the switch exists for comparison.
+static bool direct_exact_only() {
+  const char *e = getenv("ILLUM_DIRECT_EXACT");
+  return e && std::string(e) == "1";
+}
+
 // ---------------------------------------------------------------------------
 // Direct insolation.  One lane per (facet, sun direction); E is nf x S
 // column-major so consecutive lanes write consecutive facets.
@@ -185,7 +192,7 @@ __global__ __launch_bounds__(256) void direct_kernel(
     int64_t nf, int S, int nphi, const float4 *__restrict__ pr4,
     const double *__restrict__ frame, const double *__restrict__ hz,
     const double *__restrict__ sun, const double *__restrict__ disk, int ndisk,
-    double *__restrict__ E) {
+    double *__restrict__ E, int exact_only) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nf * S) return;
   const int64_t f = idx % nf;
@@ -214,7 +221,54 @@ __global__ __launch_bounds__(256) void direct_kernel(
   const double *F = frame + 9 * f;
   const double *h = hz + f * nphi;
   int count = 0;
-  for (int k = 0; k < ndisk - 1; ++k) {
+  // Whole-disk decision.  Every disk sample lies on the plane through the sun
+  // normal to N within R_sun of the sun, so its direction is within
+  // alpha = atan(R_sun / |sun - p|) of N: its zenith angle within alpha of
+  // N's, its azimuth within alpha / min sin(theta) over [theta_c - alpha,
+  // theta_c + alpha].  The thresholds h[i] + t dphi it can meet then lie in
+  // [min h, max h + dphi] over the azimuth bins covered.  When the whole disk
+  // is below (above) that band by a margin covering FP64 rounding, every
+  // (no) sample counts, exactly as the per-sample loop would decide; otherwise
+  // the loop runs.
+  bool decided = false;
+  if (!exact_only) {
+    const double alpha = atan(SUN_RADIUS_KM / nN);
+    const double cz = (F[6] * Nx + F[7] * Ny) + F[8] * Nz;
+    const double thc = acos(fmin(1.0, fmax(-1.0, cz)));
+    const double smin = fmin(sin(thc - alpha), sin(thc + alpha));
+    if (thc - alpha > 1e-3 && thc + alpha < M_PI - 1e-3 && smin > 0.0) {
+      const double dph = alpha / smin * 1.0001 + 1e-9;
+      if (dph < 0.25) {
+        const double cx = (F[0] * Nx + F[1] * Ny) + F[2] * Nz;
+        const double cy = (F[3] * Nx + F[4] * Ny) + F[5] * Nz;
+        double phc = atan2(cy, cx);
+        if (phc < 0) phc = phc + TWO_PI;
+        const int nb = nphi - 1;  // bins of [0, 2 pi)
+        const int i0 = (int)floor((phc - dph) / dphi) - 1, i1 = (int)floor((phc + dph) / dphi) + 1;
+        double hlo = __builtin_huge_valf(), hhi = -__builtin_huge_valf();
+        bool nan = false;
+        for (int i = i0; i <= i1 + 1; ++i) {
+          // i1 + 1: the azimuth 2 pi itself maps to h[nphi - 1] when the range
+          // reaches the wrap
+          const int ii = i <= i1 ? ((i % nb) + nb) % nb : ((i0 < 0 || i1 >= nb - 1) ? nb : -1);
+          if (ii < 0) continue;
+          const double hv = h[ii];
+          nan = nan || hv != hv;
+          hlo = fmin(hlo, hv);
+          hhi = fmax(hhi, hv);
+        }
+        const double eps = 1e-9;
+        if (!nan && thc + alpha + eps < hlo) {
+          count = ndisk - 1;
+          decided = true;
+        } else if (!nan && thc - alpha - eps >= hhi + dphi) {
+          count = 0;
+          decided = true;
+        }
+      }
+    }
+  }
+  for (int k = 0; k < ndisk - 1 && !decided; ++k) {
     const double xk = disk[k], yk = disk[ndisk + k];
     const double dx = sx + SUN_RADIUS_KM * (xk * Tx + yk * Bx);
     const double dy = sy + SUN_RADIUS_KM * (xk * Ty + yk * By);
@@ -264,7 +318,8 @@ void direct_radiosity(illum_mesh *mesh, const double *hz, int nphi, const double
     StageTimer st(ctx, "direct");
     const int64_t total = nf * S;
     direct_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(
-        nf, S, nphi, (const float4 *)mesh->d_pr4, mesh->d_frame, ph, psun, pdisk, ndisk, pE);
+        nf, S, nphi, (const float4 *)mesh->d_pr4, mesh->d_frame, ph, psun, pdisk, ndisk, pE,
+        direct_exact_only() ? 1 : 0);
     ILLUM_LAUNCH_CHECK();
   }
   if (!out_dev) ILLUM_HIP(hipMemcpyAsync(E, pE, 8 * nf * S, hipMemcpyDeviceToHost, s));

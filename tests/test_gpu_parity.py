@@ -191,6 +191,23 @@ def test_direct_radiosity(api):
     assert E.max() > 0
 
 
+def test_direct_whole_disk_decision_is_exact(api, monkeypatch):
+    # The whole-disk decision only skips samples whose outcome the bound fixes:
+    # bit-identical to evaluating every sample, suns at all elevations.
+    V, F = _mesh(api, 4, 60, seed=14)
+    ctx = api.IllumContext(V=V, F=F)
+    ctx.make_horizons(nphi=91)
+    disk = api.fib_spiral(100)
+    rng = np.random.default_rng(8)
+    sun = rng.normal(size=(3, 48))
+    sun = sun / np.linalg.norm(sun, axis=0) * 227390024.0
+    E = ctx.get_direct_radiosity(sun, disk)
+    monkeypatch.setenv("ILLUM_DIRECT_EXACT", "1")
+    Ex = ctx.get_direct_radiosity(sun, disk)
+    assert np.array_equal(E, Ex)
+    assert (E > 0).mean() > 0.2 and (E == 0).mean() > 0.2
+
+
 @pytest.mark.parametrize("S", [1, 5, 64])
 @pytest.mark.parametrize("method", ["fp64", "mixed"])
 def test_radiosity_vs_gauss_seidel(api, S, method):
