@@ -98,7 +98,7 @@ def bench_radiosity(api, illum, ctx, Fm, hz, S=512, tol=1e-8):
     sun = api.DeviceArray.from_host(ctx, np.ascontiguousarray(hemisphere_suns(S, 1).T))
     disk = api.DeviceArray.from_host(ctx, np.ascontiguousarray(api.fib_spiral(100).T))
     E = api.DeviceArray(ctx, (S, nf))  # nf x S column-major
-    dev = _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_OUT_ON_DEVICE
+    dev = _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_OUT_ON_DEVICE | _lib.ILLUM_ASYNC
     _lib.call("illum_direct_radiosity", illum.mesh.h, api._ptr(hz), nphi, api._ptr(sun), S,
               api._ptr(disk), 100, api._ptr(E), dev)
     out = {"S": S, "rel_tol": tol, "facets": nf, "nnz": Fm.nnz}
@@ -247,7 +247,7 @@ def bench_coupled(api, illum, ctx, Fm, hz, nsteps=32):
             for a in ang]
     out = {"steps": nsteps, "facets": nf, "layers": 40}
     finals = {}
-    dev = _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_OUT_ON_DEVICE
+    dev = _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_OUT_ON_DEVICE | _lib.ILLUM_ASYNC
     for prec in ("fp64", "mixed"):
         th = api.ThermalModel(nf, z, 200.0, 1.19177e6, 200.0, emissivity=0.95, precision=prec, ctx=ctx)
         E = api.DeviceArray(ctx, (nf,))

@@ -23,6 +23,7 @@ ILLUM_ERR_SHAPE = 9
 ILLUM_IN_ON_DEVICE = 1
 ILLUM_OUT_ON_DEVICE = 2
 ILLUM_ACCUMULATE = 4
+ILLUM_ASYNC = 8
 ILLUM_RAD_JACOBI_FP64 = 1
 ILLUM_RAD_MIXED = 2
 ILLUM_THERM_FP64 = 1

@@ -441,7 +441,8 @@ def radiosity_solve(F, E, method="mixed", rel_tol=1e-8, max_iter=200, B=None):
     if _is_dev(E):
         S = 1 if E.dim() == 1 else E.shape[1]
         call("illum_radiosity_solve", F.h, _ptr(E), S, m, rel_tol, max_iter, _ptr(B),
-             _ptr(resid), C.byref(iters), _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_OUT_ON_DEVICE)
+             _ptr(resid), C.byref(iters),
+             _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_OUT_ON_DEVICE | _lib.ILLUM_ASYNC)
         out = B
     else:
         E2 = np.asarray(E, np.float64)
@@ -518,7 +519,9 @@ class ThermalModel:
 
     def step(self, dt, Q):
         if _is_dev(Q):
-            call("illum_thermal_step", self.h, dt, _ptr(Q), _lib.ILLUM_IN_ON_DEVICE)
+            # device Q: queued without waiting (DeviceArray reads are ordered after it)
+            call("illum_thermal_step", self.h, dt, _ptr(Q),
+                 _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_ASYNC)
         else:
             q = np.ascontiguousarray(Q, np.float64)
             call("illum_thermal_step", self.h, dt, _ptr(q), 0)
@@ -528,7 +531,8 @@ class ThermalModel:
         """nsteps consecutive steps; Qseries has shape (nsteps, N)."""
         if _is_dev(Qseries):
             n = Qseries.shape[0]
-            call("illum_thermal_run", self.h, dt, _ptr(Qseries), n, _lib.ILLUM_IN_ON_DEVICE)
+            call("illum_thermal_run", self.h, dt, _ptr(Qseries), n,
+                 _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_ASYNC)
         else:
             q = np.ascontiguousarray(Qseries, np.float64)
             n = q.shape[0]

@@ -74,9 +74,21 @@ void destroy_sparse(illum_sparse *s);
 void destroy_thermal(illum_thermal *t);
 
 // Device scratch slot `slot` of the context, at least `bytes` long (grown by
-// reallocation after the stream drains; contents are not preserved).
+// reallocation after the stream drains; contents are not preserved).  Slots
+// live until the context is destroyed, so repeated calls (a time loop)
+// allocate nothing; a context is used by one host thread at a time.
 void *ctx_workspace(illum_ctx *ctx, int slot, size_t bytes);
-enum WorkspaceSlot { WS_FF_MASK = 0, WS_FF_MASK_T = 1, WS_FF_COUNTS = 2 };
+enum WorkspaceSlot {
+  WS_FF_MASK = 0,
+  WS_FF_MASK_T = 1,
+  WS_FF_COUNTS = 2,
+  WS_RAD_CTL = 3,   // solver control, residual history, column norms, partial sums
+  WS_RAD_E = 4,     // right-hand sides, row-major
+  WS_RAD_X0 = 5,    // iterates / residuals (nf x S each)
+  WS_RAD_X1 = 6,
+  WS_RAD_X2 = 7,
+  WS_RAD_D32 = 8,   // FP32 correction
+};
 
 // Drains pending stage events into the profile (after the stream is synced).
 void collect_profile(illum_ctx *ctx);

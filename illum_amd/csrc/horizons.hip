@@ -182,18 +182,39 @@ static bool direct_exact_only() {
   return e && std::string(e) == "1";
 }
 
+// Lanes per (facet, sun) pair: enough for about 2^20 lanes (4096 workgroups),
+// at most 16.  ILLUM_DIRECT_LANES=1|2|4|8|16 overrides.
+static int direct_lanes(int64_t pairs) {
+  if (const char *e = getenv("ILLUM_DIRECT_LANES")) {
+    const int v = atoi(e);
+    if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) return v;
+  }
+  int L = 1;
+  while (L < 16 && pairs * L < ((int64_t)1 << 20)) L *= 2;
+  return L;
+}
+
 // ---------------------------------------------------------------------------
-// Direct insolation.  One lane per (facet, sun direction); E is nf x S
-// column-major so consecutive lanes write consecutive facets.
+// Direct insolation.  L adjacent lanes per (facet, sun direction), each taking
+// every L-th disk sample, their counts summed by lane shuffles (an integer sum,
+// so E does not depend on L); L > 1 only when there are too few pairs to fill
+// the device (a single sun per time step).  E is nf x S column-major so
+// consecutive pairs write consecutive facets.
 constexpr double ONE_AU_KM = 149.5978707e6;   // constants.hpp:7
 constexpr double SUN_RADIUS_KM = 695508;      // constants.hpp:8
 
+template <int L>
 __global__ __launch_bounds__(256) void direct_kernel(
     int64_t nf, int S, int nphi, const float4 *__restrict__ pr4,
     const double *__restrict__ frame, const double *__restrict__ hz,
     const double *__restrict__ sun, const double *__restrict__ disk, int ndisk,
     double *__restrict__ E, int exact_only) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  static_assert(L >= 1 && L <= 64 && (L & (L - 1)) == 0, "L divides the wavefront");
+  const int64_t lane_id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t idx = lane_id / L;
+  const int sub = (int)(lane_id % L);
+  // a group of L lanes is all in range or all out of it, so the shuffles
+  // below only read lanes of the caller's own (active) group
   if (idx >= nf * S) return;
   const int64_t f = idx % nf;
   const int s = (int)(idx / nf);
@@ -259,7 +280,7 @@ __global__ __launch_bounds__(256) void direct_kernel(
         }
         const double eps = 1e-9;
         if (!nan && thc + alpha + eps < hlo) {
-          count = ndisk - 1;
+          count = sub == 0 ? ndisk - 1 : 0;
           decided = true;
         } else if (!nan && thc - alpha - eps >= hhi + dphi) {
           count = 0;
@@ -268,7 +289,7 @@ __global__ __launch_bounds__(256) void direct_kernel(
       }
     }
   }
-  for (int k = 0; k < ndisk - 1 && !decided; ++k) {
+  for (int k = sub; k < ndisk - 1 && !decided; k += L) {
     const double xk = disk[k], yk = disk[ndisk + k];
     const double dx = sx + SUN_RADIUS_KM * (xk * Tx + yk * Bx);
     const double dy = sy + SUN_RADIUS_KM * (xk * Ty + yk * By);
@@ -287,6 +308,8 @@ __global__ __launch_bounds__(256) void direct_kernel(
     const double t = (phi - pi_ * dphi) / dphi;
     if (theta < h[pi_] + t * dphi) ++count;
   }
+  for (int o = L / 2; o > 0; o >>= 1) count += __shfl_xor(count, o);
+  if (sub != 0) return;
   double E0 = (double)count / (nphi - 1);
   E0 = E0 * fmax(0.0, (Nx * F[6] + Ny * F[7]) + Nz * F[8]);
   const double ns = sqrt((sx * sx + sy * sy) + sz * sz);
@@ -316,14 +339,18 @@ void direct_radiosity(illum_mesh *mesh, const double *hz, int nphi, const double
   if (!out_dev) { d_E.reset(nf * S); pE = d_E.p; }
   {
     StageTimer st(ctx, "direct");
-    const int64_t total = nf * S;
-    direct_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(
+    const int64_t pairs = nf * S;
+    const int L = direct_lanes(pairs);
+    const int64_t lanes = pairs * L;
+    auto kern = L == 16 ? direct_kernel<16> : L == 8 ? direct_kernel<8>
+              : L == 4 ? direct_kernel<4> : L == 2 ? direct_kernel<2> : direct_kernel<1>;
+    kern<<<(unsigned)((lanes + 255) / 256), 256, 0, s>>>(
         nf, S, nphi, (const float4 *)mesh->d_pr4, mesh->d_frame, ph, psun, pdisk, ndisk, pE,
         direct_exact_only() ? 1 : 0);
     ILLUM_LAUNCH_CHECK();
   }
   if (!out_dev) ILLUM_HIP(hipMemcpyAsync(E, pE, 8 * nf * S, hipMemcpyDeviceToHost, s));
-  sync(ctx);
+  if (!(in_dev && out_dev && (flags & ILLUM_ASYNC))) sync(ctx);
 }
 
 }  // namespace illum

@@ -152,6 +152,8 @@ struct illum_sparse {
   uint32_t *d_vis = nullptr;   // raw visibility rows (union of first hits)
   // visibility-kernel work: rays, pair tests, box tests, triangle tests
   unsigned long long *d_nrays = nullptr;
+  // products the previous solve took: the next solve's first batch
+  int last_nprod = 0;
 };
 
 struct illum_thermal {

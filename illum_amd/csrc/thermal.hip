@@ -78,7 +78,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ)
     const int32_t *__restrict__ fidx, const int32_t *__restrict__ wgrid, int64_t block0,
     int64_t nf, int64_t nslot, const double *__restrict__ s2f, const double *__restrict__ hdr,
     const double *__restrict__ coef, const int64_t *__restrict__ goff, double emiss, double Fgeo,
-    int nsteps, const double *__restrict__ Q, const double *__restrict__ Qprev,
+    int nsteps, const double *__restrict__ Q, double *__restrict__ Qprev,
     double *__restrict__ T, double *__restrict__ Fs, double *__restrict__ G,
     double *__restrict__ Tsurf_series, int *__restrict__ status, int rows) {
   __shared__ double s_al[NZ], s_ga[NZ];
@@ -174,6 +174,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ)
     for (int l = 0; l < NZ + 2; ++l)
       if (l <= nz) T[(int64_t)l * nslot + slot] = Tl[l];
     Fs[f] = Fsurf;
+    Qprev[f] = Qn;  // the last step's Q, the next call's Q^n
     if (zero_pivot) atomicOr(status, 1);
   }
 }
@@ -206,7 +207,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ)
     int64_t nf, int64_t nslot, const double *__restrict__ s2f, const double *__restrict__ hdr,
     const double *__restrict__ coef, const float *__restrict__ coef32,
     const int64_t *__restrict__ goff, double emiss, double Fgeo, int nsteps,
-    const double *__restrict__ Q, const double *__restrict__ Qprev, double *__restrict__ T,
+    const double *__restrict__ Q, double *__restrict__ Qprev, double *__restrict__ T,
     double *__restrict__ Fs, double *__restrict__ Tsurf_series) {
   const int64_t blk = block0 + blockIdx.x;
   const int64_t slot = blk * 64 + threadIdx.x;
@@ -298,6 +299,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ)
     for (int l = 0; l <= NZ; ++l)
       if (l <= nz) T[(int64_t)l * nslot + slot] = Tl[l];
     Fs[f] = Fsurf;
+    Qprev[f] = Qn;
   }
 }
 
@@ -309,7 +311,7 @@ __global__ __launch_bounds__(64) void cn_stream_kernel(
     const int32_t *__restrict__ fidx, const int32_t *__restrict__ wgrid, int64_t block0,
     int64_t nf, int64_t nslot, const double *__restrict__ s2f, const double *__restrict__ hdr,
     const double *__restrict__ coef, const int64_t *__restrict__ goff, double emiss, double Fgeo,
-    int nsteps, const double *__restrict__ Q, const double *__restrict__ Qprev,
+    int nsteps, const double *__restrict__ Q, double *__restrict__ Qprev,
     double *__restrict__ T, double *__restrict__ Fs, double *__restrict__ G,
     float *__restrict__ G32, double *__restrict__ Tsurf_series, int *__restrict__ status) {
   const int64_t blk = block0 + blockIdx.x;
@@ -414,12 +416,7 @@ __global__ __launch_bounds__(64) void cn_stream_kernel(
     Qn = Qnp1;
   }
   Fs[f] = Fsurf;
-}
-
-__global__ void copy_qprev_kernel(int64_t nf, const double *__restrict__ Qlast,
-                                  double *__restrict__ Qprev) {
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f < nf) Qprev[f] = Qlast[f];
+  Qprev[f] = Qn;
 }
 
 // out[f * (nzmax + 1) + l] = T of facet f, layer l (NaN below its depth).
@@ -718,6 +715,13 @@ static void launch_all(illum_thermal *th, ThermalImpl *im, hipStream_t s, double
   }
 }
 
+static void check_status(illum_thermal *th) {
+  int st = 0;
+  ILLUM_HIP(hipMemcpyAsync(&st, th->d_status, 4, hipMemcpyDeviceToHost, stream_of(th->ctx)));
+  sync(th->ctx);
+  if (st) throw Error(ILLUM_ERR_ZERO_PIVOT, "tridag: rewrite equations (b(1) == 0)");
+}
+
 void thermal_run(illum_thermal *th, double dt, const double *Q, int nsteps, int flags,
                  double Fgeo, double *Tsurf_series) {
   if (nsteps < 1) throw Error(ILLUM_ERR_ARG, "nsteps must be >= 1");
@@ -744,13 +748,10 @@ void thermal_run(illum_thermal *th, double dt, const double *Q, int nsteps, int 
     if (th->precision == ILLUM_THERM_MIXED) launch_all<true>(th, im, s, Fgeo, nsteps, dQ, Tsurf_series);
     else launch_all<false>(th, im, s, Fgeo, nsteps, dQ, Tsurf_series);
   }
-  const unsigned blocks = (unsigned)((nf + 255) / 256);
-  copy_qprev_kernel<<<blocks, 256, 0, s>>>(nf, dQ + (int64_t)(nsteps - 1) * nf, th->d_Qprev);
-  ILLUM_LAUNCH_CHECK();
-  int st = 0;
-  ILLUM_HIP(hipMemcpyAsync(&st, th->d_status, 4, hipMemcpyDeviceToHost, s));
-  sync(ctx);
-  if (st) throw Error(ILLUM_ERR_ZERO_PIVOT, "tridag: rewrite equations (b(1) == 0)");
+  // A zero pivot is sticky in d_status; an asynchronous step leaves its check
+  // to the next synchronous step or thermal_get.
+  if ((flags & ILLUM_IN_ON_DEVICE) && (flags & ILLUM_ASYNC)) return;
+  check_status(th);
 }
 
 void thermal_get(illum_thermal *th, double *T, double *Tsurf, double *Fsurf, int flags) {
@@ -780,7 +781,7 @@ void thermal_get(illum_thermal *th, double *T, double *Tsurf, double *Fsurf, int
   }
   if (Fsurf)
     ILLUM_HIP(hipMemcpyAsync(Fsurf, th->d_F, 8 * nf, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
-  sync(ctx);
+  check_status(th);
 }
 
 void thermal_get_radiosity(illum_thermal *th, double *out, int flags) {

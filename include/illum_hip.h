@@ -49,6 +49,13 @@ extern "C" {
 #define ILLUM_IN_ON_DEVICE   1   /* input arrays are device pointers  */
 #define ILLUM_OUT_ON_DEVICE  2   /* output arrays are device pointers */
 #define ILLUM_ACCUMULATE     4   /* add into the output instead of overwriting it */
+/* With every array on the device: return once the work is queued on the
+ * context's stream instead of waiting for it.  Later calls on the same
+ * context (illum_dev_copy included) see the results; illum_ctx_synchronize
+ * waits for them.  A thermal zero pivot is then reported by the next
+ * illum_thermal_get or synchronous step of that model.  Ignored where an
+ * array is on the host. */
+#define ILLUM_ASYNC          8
 
 /* radiosity solve methods */
 #define ILLUM_RAD_JACOBI_FP64   1   /* FP64 Jacobi sweeps B <- E + F B                  */

@@ -202,6 +202,13 @@ def test_direct_whole_disk_decision_is_exact(api, monkeypatch):
     sun = rng.normal(size=(3, 48))
     sun = sun / np.linalg.norm(sun, axis=0) * 227390024.0
     E = ctx.get_direct_radiosity(sun, disk)
+    # the disk samples split over 1..16 lanes per (facet, sun) pair: an integer
+    # count, so the same E whatever the split (one sun and all 48 suns)
+    for lanes in ("1", "2", "4", "8", "16"):
+        monkeypatch.setenv("ILLUM_DIRECT_LANES", lanes)
+        assert np.array_equal(ctx.get_direct_radiosity(sun, disk), E), lanes
+        assert np.array_equal(ctx.get_direct_radiosity(sun[:, 7:8], disk)[:, 0], E[:, 7]), lanes
+    monkeypatch.delenv("ILLUM_DIRECT_LANES")
     monkeypatch.setenv("ILLUM_DIRECT_EXACT", "1")
     Ex = ctx.get_direct_radiosity(sun, disk)
     assert np.array_equal(E, Ex)
@@ -230,6 +237,30 @@ def test_radiosity_vs_gauss_seidel(api, S, method):
     rel = np.linalg.norm(r, axis=0) / np.linalg.norm(E, axis=0)
     assert rel.max() <= 1.2e-10
     assert res[-1] <= 1e-10
+
+
+@pytest.mark.parametrize("method", ["fp64", "mixed"])
+def test_radiosity_wide_products_chunking_is_exact(api, monkeypatch, method):
+    # S >= 128 runs the column-chunked product: every element sums its row in
+    # CSR order whatever the chunk width, so forcing narrower chunks (a small
+    # cache budget) changes nothing; odd S takes one value per lane.
+    V, F = _mesh(api, 4, 40, seed=6)
+    ctx = api.IllumContext(V=V, F=F)
+    Fm = ctx.compute_F(0.5)
+    nf = len(F)
+    cp, ri, v = Fm.csc()
+    for S in (128, 130, 131, 512):
+        E = np.random.default_rng(S).uniform(0, 500, (nf, S))
+        monkeypatch.delenv("ILLUM_SPMM_CHUNK_BYTES", raising=False)
+        B, res, it = api.radiosity_solve(Fm, E, method=method, rel_tol=1e-10, max_iter=400)
+        assert res[-1] <= 1e-10
+        for budget in (nf * 64 * 8 * 1.5, nf * 64 * 8 * 0.5):  # per-lane widths 1..4
+            monkeypatch.setenv("ILLUM_SPMM_CHUNK_BYTES", str(budget))
+            Bc, resc, itc = api.radiosity_solve(Fm, E, method=method, rel_tol=1e-10, max_iter=400)
+            assert itc == it and np.array_equal(Bc, B), (S, budget)
+        for s in (0, S - 1):
+            Bo, _, _ = O.gauss_seidel(nf, cp, ri, v, E[:, s], max_sweeps=200, rel_tol=1e-13)
+            np.testing.assert_allclose(B[:, s], Bo, rtol=1e-8, atol=1e-8 * np.abs(Bo).max())
 
 
 def _cn_case(nf, nz, seed):
@@ -318,6 +349,58 @@ def test_thermal_get_radiosity_accumulates(api):
     assert np.array_equal(Ed.to_host(), E - Fs)
     th.get_radiosity(out=Ed)
     assert np.array_equal(Ed.to_host(), -Fs)
+
+
+@pytest.mark.parametrize("method", ["fp64", "mixed"])
+def test_async_time_loop_matches_synchronous(api, method):
+    """The coupled time step (direct -> E -= F_surf -> solve -> CN step,
+    illum_cli.cpp:403-456) on device buffers: ILLUM_ASYNC calls, which queue
+    without waiting, give bit-identical temperatures and product counts to
+    the same calls made synchronously; the solve's B matches the host-buffer
+    solve of the same E."""
+    from illum_amd import _lib
+    V, F = _mesh(api, 3, 20, seed=21)
+    ctx = api.IllumContext(V=V, F=F)
+    Fm = ctx.compute_F(0.12)
+    hz = ctx.make_horizons(nphi=91)
+    nf = len(F)
+    c = api.default_context()
+    hzd = api.DeviceArray.from_host(c, np.ascontiguousarray(np.asarray(hz).T))
+    disk = api.DeviceArray.from_host(c, np.ascontiguousarray(api.fib_spiral(100).T))
+    z = 0.5 * (np.exp(np.linspace(0, 3, 30)) - 1 + 0.01) / (np.exp(3) - 1 + 0.01)
+    ang = 2 * np.pi * np.arange(6) / 16
+    suns = [api.DeviceArray.from_host(c, 2.27e8 * np.array([np.cos(a) * 0.94, np.sin(a) * 0.94, 0.34]))
+            for a in ang]
+    m = {"fp64": _lib.ILLUM_RAD_JACOBI_FP64, "mixed": _lib.ILLUM_RAD_MIXED}[method]
+    runs = {}
+    for asyn in (0, _lib.ILLUM_ASYNC):
+        dev = _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_OUT_ON_DEVICE | asyn
+        th = api.ThermalModel(nf, z, 200.0, 1.19177e6, 200.0, emissivity=0.95,
+                              precision=method)
+        E = api.DeviceArray(c, (nf,))
+        B = api.DeviceArray(c, (nf,))
+        counts, Es = [], []
+        for k in range(len(suns)):
+            _lib.call("illum_direct_radiosity", ctx.mesh.h, api._ptr(hzd), 91, api._ptr(suns[k]),
+                      1, api._ptr(disk), 100, api._ptr(E), dev)
+            th.get_radiosity(out=E, accumulate=True)
+            Es.append(E.to_host())
+            resid = np.zeros(101)
+            it = api.C.c_int()
+            _lib.call("illum_radiosity_solve", Fm.h, api._ptr(E), 1, m, 1e-8, 100, api._ptr(B),
+                      api._ptr(resid), api.C.byref(it), dev)
+            counts.append(it.value)
+            _lib.call("illum_thermal_step", th.h, 3600.0, api._ptr(B), _lib.ILLUM_IN_ON_DEVICE | asyn)
+        c.synchronize()
+        runs[asyn] = (th.T.copy(), counts, Es[-1], B.to_host())
+        th.close()
+    (T0, n0, E0, B0), (T1, n1, E1, B1) = runs[0], runs[_lib.ILLUM_ASYNC]
+    assert n0 == n1 and min(n0) >= 1
+    assert np.array_equal(T0, T1, equal_nan=True)
+    assert np.array_equal(B0, B1)
+    Bh, res, _ = api.radiosity_solve(Fm, E1, method=method, rel_tol=1e-8, max_iter=100)
+    assert np.array_equal(Bh, B1)
+    assert res[-1] <= 1e-8
 
 
 def test_conductionQ_dropin_vs_reference_fortran_golden(api):
