@@ -244,7 +244,8 @@ __global__ __launch_bounds__(256) void direct_kernel(
   int count = 0;
   // Whole-disk decision.  Every disk sample lies on the plane through the sun
   // normal to N within R_sun of the sun, so its direction is within
-  // alpha = atan(R_sun / |sun - p|) of N: its zenith angle within alpha of
+  // atan(R_sun / |sun - p|) <= alpha = R_sun / |sun - p| of N (the larger
+  // alpha only widens the bound): its zenith angle within alpha of
   // N's, its azimuth within alpha / min sin(theta) over [theta_c - alpha,
   // theta_c + alpha].  The thresholds h[i] + t dphi it can meet then lie in
   // [min h, max h + dphi] over the azimuth bins covered.  When the whole disk
@@ -253,7 +254,7 @@ __global__ __launch_bounds__(256) void direct_kernel(
   // the loop runs.
   bool decided = false;
   if (!exact_only) {
-    const double alpha = atan(SUN_RADIUS_KM / nN);
+    const double alpha = SUN_RADIUS_KM / nN;
     const double cz = (F[6] * Nx + F[7] * Ny) + F[8] * Nz;
     const double thc = acos(fmin(1.0, fmax(-1.0, cz)));
     const double smin = fmin(sin(thc - alpha), sin(thc + alpha));
@@ -271,7 +272,13 @@ __global__ __launch_bounds__(256) void direct_kernel(
         for (int i = i0; i <= i1 + 1; ++i) {
           // i1 + 1: the azimuth 2 pi itself maps to h[nphi - 1] when the range
           // reaches the wrap
-          const int ii = i <= i1 ? ((i % nb) + nb) % nb : ((i0 < 0 || i1 >= nb - 1) ? nb : -1);
+          int ii = i;
+          if (i <= i1) {
+            while (ii < 0) ii += nb;
+            while (ii >= nb) ii -= nb;
+          } else {
+            ii = (i0 < 0 || i1 >= nb - 1) ? nb : -1;
+          }
           if (ii < 0) continue;
           const double hv = h[ii];
           nan = nan || hv != hv;

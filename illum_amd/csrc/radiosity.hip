@@ -205,9 +205,27 @@ __device__ void block_colsum(int64_t r0, int64_t r1, int S, int c0, int cw, cons
   __syncthreads();  // red is free (a previous window's combine has finished)
   red[t] = acc;
   __syncthreads();
+  if (L <= 16) {
+    if (t < cw) {
+      double tot = 0;
+      for (int k = 0; k < L; ++k) tot += red[k * cw + t];
+      out[c0 + t] = tot;
+    }
+    return;
+  }
+  // Many groups (cw < 16): 16 sub-sums of consecutive groups, then their sum.
+  __shared__ double red2[256];
+  const int per = (L + 15) / 16;
+  if (t < 16 * cw) {
+    const int j = t / cw;
+    double u = 0;
+    for (int k = j * per; k < min(L, (j + 1) * per); ++k) u += red[k * cw + c];
+    red2[t] = u;
+  }
+  __syncthreads();
   if (t < cw) {
     double tot = 0;
-    for (int k = 0; k < L; ++k) tot += red[k * cw + t];
+    for (int j = 0; j < 16; ++j) tot += red2[j * cw + t];
     out[c0 + t] = tot;
   }
 }
