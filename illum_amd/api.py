@@ -238,7 +238,8 @@ class BVH:
 
     def tuning(self):
         """[(leaf_size, layout, ms)] of the autotuning trials (empty unless
-        leaf_size was 0); layout 2 = two-wide, 4 = four-wide walk."""
+        leaf_size was 0); layout 2 = two-wide, 3 = wavefront packet over the
+        two-wide tree, 4 = four-wide walk."""
         leaves = np.zeros(16, np.int32)
         layouts = np.zeros(16, np.int32)
         ms = np.zeros(16)

@@ -149,7 +149,25 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
     int qrem = s_qn;
     while (qrem >= FF_BLOCK || (last && qrem > 0)) {
       const int take = qrem < FF_BLOCK ? qrem : FF_BLOCK;
-      if (tid < take) {
+      if (LAYOUT == 3) {
+        // wavefront walk: the whole wavefront takes part, idle lanes masked;
+        // a wavefront with no work left skips it
+        const bool act = tid < take;
+        if (__ballot(act)) {
+          const int i = act ? queue[qrem - take + tid] : j;
+          const float4 pi = pr4[i];
+          const DRay r = make_ray(qx, qy, qz, pi.x - qx, pi.y - qy, pi.z - qz);
+          float t;
+          int h = trace_closest_packet(wide, tris, r, act, root_wide, FF_TINIT, &t, nbox, ntri);
+          const bool again = act && h == -1;
+          if (__ballot(again)) {
+            const int h2 = trace_closest_packet(wide, tris, r, again, root_wide,
+                                                __builtin_huge_valf(), &t, nbox, ntri);
+            if (again) h = h2;
+          }
+          if (act && h >= 0 && h != j) atomicOr(&bits[h >> 5], 1u << (h & 31));
+        }
+      } else if (tid < take) {
         const int i = queue[qrem - take + tid];
         const float4 pi = pr4[i];
         const DRay r = make_ray(qx, qy, qz, pi.x - qx, pi.y - qy, pi.z - qz);
@@ -384,26 +402,42 @@ __global__ __launch_bounds__(256) void closest_hit_kernel(
 // (stackless).  The near-first walks keep an LDS stack of at most FF_STACK
 // entries per lane; a walk that needs more is answered by the stackless
 // threaded walk (same result).
-static int ff_layout(const illum_bvh *bvh) {
+// The wavefront walk needs the two-wide tree and depth + 1 <= 64 stack
+// entries (one VGPR across the wavefront).
+static bool packet_ok(const illum_bvh *bvh) { return bvh->d_wide && bvh->h.depth + 1 <= 64; }
+
+// Walk for a requested layout (2 two-wide, 3 wavefront packet over the
+// two-wide tree, 4 four-wide, 0 threaded), falling back to what the tree
+// has.  ILLUM_FF_TRAVERSAL (threaded, wide2, packet, wide4) overrides.
+static int ff_resolve(const illum_bvh *bvh, int want) {
   const char *e = getenv("ILLUM_FF_TRAVERSAL");
-  if (e && std::string(e) == "threaded") return 0;
-  if (e && std::string(e) == "wide2") return bvh->d_wide ? 2 : 0;
-  if (e && std::string(e) == "wide4") return bvh->d_wide4 ? 4 : 0;
-  return bvh->ff_layout == 4 && bvh->d_wide4 ? 4 : (bvh->d_wide ? 2 : 0);
+  const std::string ev = e ? e : "";
+  if (ev == "threaded") want = 0;
+  else if (ev == "wide2") want = 2;
+  else if (ev == "packet") want = 3;
+  else if (ev == "wide4") want = 4;
+  if (want == 4 && bvh->d_wide4) return 4;
+  if (want == 3 && packet_ok(bvh)) return 3;
+  if (want == 0) return 0;
+  return bvh->d_wide ? 2 : 0;
 }
+
+static int ff_layout(const illum_bvh *bvh) { return ff_resolve(bvh, bvh->ff_layout); }
 
 // LDS stack entries a lane needs for a layout: one push per level (two-wide),
 // up to three per level (four-wide), capped at FF_STACK.
 static int ff_stack_slots(const illum_bvh *bvh, int layout) {
   if (layout == 2) return std::min(bvh->h.depth + 1, FF_STACK);
   if (layout == 4) return std::min(3 * bvh->h.depth4 + 1, FF_STACK);
-  return 0;
+  return 0;  // threaded walk; the wavefront walk keeps its stack in a VGPR
 }
 
 template <bool COUNT>
 static decltype(&ff_visibility_kernel<4, COUNT>) ff_kernel(int layout) {
-  return layout == 4 ? ff_visibility_kernel<4, COUNT>
-                     : layout == 2 ? ff_visibility_kernel<2, COUNT> : ff_visibility_kernel<0, COUNT>;
+  return layout == 4   ? ff_visibility_kernel<4, COUNT>
+         : layout == 3 ? ff_visibility_kernel<3, COUNT>
+         : layout == 2 ? ff_visibility_kernel<2, COUNT>
+                       : ff_visibility_kernel<0, COUNT>;
 }
 
 // Dynamic LDS of ff_visibility_kernel: bitset, ray queue, 16-bit cluster list
@@ -556,7 +590,7 @@ double time_visibility_sample(illum_bvh *bvh, int max_sources, int layout_choice
   const int jstride = (int)std::max<int64_t>(1, nf / std::max(1, max_sources));
   const int nrows = (int)((nf + jstride - 1) / jstride);
   const int nwords = (int)(((nf + 511) / 512) * 16);
-  const int layout = layout_choice == 4 && bvh->d_wide4 ? 4 : ff_layout(bvh);
+  const int layout = ff_resolve(bvh, layout_choice);
   const int stack_slots = ff_stack_slots(bvh, layout);
   const size_t lds = ff_lds_bytes(nwords, mesh->nclus, stack_slots);
   if (lds > 160 * 1024) throw Error(ILLUM_ERR_CAPACITY, "too many facets for the LDS bitset");

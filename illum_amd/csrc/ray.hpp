@@ -236,6 +236,76 @@ __device__ __forceinline__ int trace_closest_wide(const float4 *__restrict__ wid
   return sp < 0 ? -2 : best;
 }
 
+// Wavefront-cooperative closest hit over the two-wide tree rooted at inner
+// node `root`: all 64 lanes call it together (converged); lanes with
+// act = false take no part.  The wavefront walks one node sequence, the union
+// of its lanes' walks: a child is entered when any active lane's ray enters
+// its box no later than that lane's best distance; with both children
+// entered, the one more lanes enter first is visited first and the other
+// pushed.  Node and triangle records are read at wavefront-uniform addresses
+// (scalar loads, once per wavefront).  The stack is one VGPR across the
+// lanes (entry k in lane k), so it needs no LDS; a walk pushes at most once
+// per tree level, so depth + 1 <= 64 entries suffice (the caller checks).
+// Each lane's result equals its own near-first walk's: every node the lane
+// needs is visited (an entered child is visited by the wavefront), pruning
+// uses the lane's own best distance, and a node a lane does not need only
+// adds exact tests (its boxes lie inside the parent's, which the lane's ray
+// does not enter before its best distance).
+__device__ __forceinline__ int trace_closest_packet(const float4 *__restrict__ wide,
+                                                    const float4 *__restrict__ tris,
+                                                    const DRay &r, bool act, int root,
+                                                    float tinit, float *tbest_out,
+                                                    unsigned &nbox, unsigned &ntri) {
+  const int lane = __lane_id();
+  float tbest = tinit;
+  int best = -1;
+  int sp = 0;         // wavefront-uniform
+  int cur = root;     // wavefront-uniform
+  int stk = 0;        // lane k: stack entry k
+  for (;;) {
+    if (cur >= 0) {
+      const float4 a0 = wide[4 * cur], a1 = wide[4 * cur + 1];
+      const float4 b0 = wide[4 * cur + 2], b1 = wide[4 * cur + 3];
+      float tl = 0.0f, tr = 0.0f;
+      const bool hl = act && box_enter(r, a0, a1, tbest, &tl);
+      const bool hr = act && box_enter(r, b0, b1, tbest, &tr);
+      if (act) nbox += 2;
+      const uint64_t ml = __ballot(hl), mr = __ballot(hr);
+      const int cl = __float_as_int(a0.w), cr = __float_as_int(b0.w);
+      if (ml && mr) {
+        const uint64_t both = ml & mr;
+        const uint64_t lfirst = __ballot(hl && hr && tl <= tr);
+        const bool lnear = 2 * __popcll(lfirst) >= __popcll(both);
+        if (lane == sp) stk = lnear ? cr : cl;
+        ++sp;
+        cur = lnear ? cl : cr;
+        continue;
+      }
+      if (ml) { cur = cl; continue; }
+      if (mr) { cur = cr; continue; }
+    } else {
+      const int info = ~cur;
+      const int first = info & 0xFFFFFF, cnt = info >> 24;
+      if (act) ntri += cnt;
+      for (int k = 0; k < cnt; ++k) {
+        const float4 a = tris[3 * (first + k)], b = tris[3 * (first + k) + 1],
+                     c = tris[3 * (first + k) + 2];
+        const int idx = __float_as_int(a.w);
+        float t;
+        if (act && tri_hit(r, a, b, c, &t) && (t < tbest || (t == tbest && idx < best))) {
+          tbest = t;
+          best = idx;
+        }
+      }
+    }
+    if (sp == 0) break;
+    --sp;
+    cur = __builtin_amdgcn_readlane(stk, sp);
+  }
+  *tbest_out = tbest;
+  return best;
+}
+
 // Slab test of the four children of a four-wide node (SoA boxes); returns
 // the entry distance, +inf for a child that is absent or need not be visited
 // (same acceptance as box_enter against tbest).

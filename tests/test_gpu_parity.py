@@ -63,8 +63,11 @@ def test_closest_hit_random_rays_bitwise(api):
     assert np.all(hb[n:] >= 0)
 
 
+@pytest.mark.parametrize("walk", ["default", "packet"])
 @pytest.mark.parametrize("subdiv,ncraters,albedo", [(3, 40, 0.1), (4, 40, 0.1)])
-def test_form_factors_vs_oracle(api, subdiv, ncraters, albedo):
+def test_form_factors_vs_oracle(api, monkeypatch, subdiv, ncraters, albedo, walk):
+    if walk != "default":
+        monkeypatch.setenv("ILLUM_FF_TRAVERSAL", walk)
     V, F = _mesh(api, subdiv, ncraters, seed=subdiv)
     ctx = api.IllumContext(V=V, F=F, leaf_size=4)
     Fm = ctx.compute_F(albedo)
@@ -94,19 +97,20 @@ def test_form_factors_vs_oracle(api, subdiv, ncraters, albedo):
 
 def test_autotuned_bvh_and_traversal_choice_do_not_change_results(api, monkeypatch):
     # Visibility is a property of the triangles, not of the tree: the autotuned
-    # leaf size, leaf size 1, the four-wide (default) and two-wide near-first
-    # walks and the stackless threaded walk must all agree.
+    # leaf size, leaf size 1, the four-wide and two-wide near-first walks, the
+    # wavefront packet walk and the stackless threaded walk must all agree.
     V, F = _mesh(api, 4, 40, seed=8)
     ref = api.IllumContext(V=V, F=F, leaf_size=4).compute_F(0.1)
     tuned = api.IllumContext(V=V, F=F, leaf_size=0)
     trials = tuned.bvh.tuning()
-    assert [(l, w) for l, w, _ in trials] == [(l, w) for l in (2, 3, 4, 6, 8) for w in (2, 4)]
+    assert [(l, w) for l, w, _ in trials] == [(l, w) for l in (2, 3, 4, 6, 8) for w in (2, 4, 3)]
     assert all(ms > 0 for _, _, ms in trials)
     assert tuned.bvh.info()["leaf_size"] == min(trials, key=lambda x: x[2])[0]
     out = [tuned.compute_F(0.1), api.IllumContext(V=V, F=F, leaf_size=1).compute_F(0.1)]
-    for walk in ("threaded", "wide2", "wide4"):
+    for walk in ("threaded", "wide2", "wide4", "packet"):
         monkeypatch.setenv("ILLUM_FF_TRAVERSAL", walk)
         out.append(api.IllumContext(V=V, F=F, leaf_size=4).compute_F(0.1))
+        out.append(api.IllumContext(V=V, F=F, leaf_size=1).compute_F(0.1))
     r = ref.csc()
     for m in out:
         assert np.array_equal(m.visibility(), ref.visibility())
