@@ -323,7 +323,8 @@ def main():
         f"setup {time.perf_counter() - t0:.1f}s")
 
     tune = illum.bvh.tuning()
-    walk = ({2: "two-wide", 3: "wavefront packet (two-wide)", 4: "four-wide"}
+    walk = ({2: "two-wide", 3: "wavefront packet (two-wide)", 4: "four-wide",
+             5: "wavefront packet (four-wide)"}
             [min(tune, key=lambda x: x[2])[1]] if tune else "two-wide")
     for w in range(args.warmup):
         t = time.perf_counter()

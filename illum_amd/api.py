@@ -239,12 +239,13 @@ class BVH:
     def tuning(self):
         """[(leaf_size, layout, ms)] of the autotuning trials (empty unless
         leaf_size was 0); layout 2 = two-wide, 3 = wavefront packet over the
-        two-wide tree, 4 = four-wide walk."""
-        leaves = np.zeros(16, np.int32)
-        layouts = np.zeros(16, np.int32)
-        ms = np.zeros(16)
+        two-wide tree, 4 = four-wide, 5 = wavefront packet over the four-wide
+        tree."""
+        leaves = np.zeros(32, np.int32)
+        layouts = np.zeros(32, np.int32)
+        ms = np.zeros(32)
         n = C.c_int()
-        call("illum_bvh_tuning", self.h, 16, _ptr(leaves), _ptr(layouts), _ptr(ms), C.byref(n))
+        call("illum_bvh_tuning", self.h, 32, _ptr(leaves), _ptr(layouts), _ptr(ms), C.byref(n))
         return [(int(leaves[k]), int(layouts[k]), float(ms[k])) for k in range(n.value)]
 
     def closest_hit(self, orig, dirs):

@@ -446,19 +446,19 @@ int illum_bvh_build(illum_mesh *mesh, int leaf_size, illum_bvh **out) {
     }
     // Autotune: time the visibility kernel on a strided sample of up to 8192
     // sources for each candidate leaf size and traversal layout (two-wide,
-    // four-wide, wavefront packet over the two-wide tree) and keep the
-    // fastest tree and layout.
+    // four-wide, wavefront packets over the two-wide and four-wide trees) and
+    // keep the fastest tree and layout.
     static const int cand[] = {2, 3, 4, 6, 8};
     std::unique_ptr<illum_bvh, void (*)(illum_bvh *)> best(nullptr, destroy_bvh);
     double best_ms = 0;
     int best_layout = 2;
-    int leaves[16], layouts[16];
-    double times[16];
+    int leaves[32], layouts[32];
+    double times[32];
     int n = 0;
     for (int c : cand) {
       std::unique_ptr<illum_bvh, void (*)(illum_bvh *)> b(make_bvh(mesh, c), destroy_bvh);
       bool improved = false;
-      for (int layout : {2, 4, 3}) {
+      for (int layout : {2, 4, 3, 5}) {
         const double ms = time_visibility_sample(b.get(), 8192, layout);
         leaves[n] = c;
         layouts[n] = layout;

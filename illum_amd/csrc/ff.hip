@@ -149,7 +149,7 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
     int qrem = s_qn;
     while (qrem >= FF_BLOCK || (last && qrem > 0)) {
       const int take = qrem < FF_BLOCK ? qrem : FF_BLOCK;
-      if (LAYOUT == 3) {
+      if (LAYOUT == 3 || LAYOUT == 5) {
         // wavefront walk: the whole wavefront takes part, idle lanes masked;
         // a wavefront with no work left skips it
         const bool act = tid < take;
@@ -158,11 +158,18 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
           const float4 pi = pr4[i];
           const DRay r = make_ray(qx, qy, qz, pi.x - qx, pi.y - qy, pi.z - qz);
           float t;
-          int h = trace_closest_packet(wide, tris, r, act, root_wide, FF_TINIT, &t, nbox, ntri);
+          int h = LAYOUT == 3
+                      ? trace_closest_packet(wide, tris, r, act, root_wide, FF_TINIT, &t, nbox, ntri)
+                      : trace_closest_packet4(wide4, tris, r, act, root_wide4, FF_TINIT, &t, nbox,
+                                              ntri);
           const bool again = act && h == -1;
           if (__ballot(again)) {
-            const int h2 = trace_closest_packet(wide, tris, r, again, root_wide,
-                                                __builtin_huge_valf(), &t, nbox, ntri);
+            const float inf = __builtin_huge_valf();
+            const int h2 = LAYOUT == 3
+                               ? trace_closest_packet(wide, tris, r, again, root_wide, inf, &t,
+                                                      nbox, ntri)
+                               : trace_closest_packet4(wide4, tris, r, again, root_wide4, inf, &t,
+                                                       nbox, ntri);
             if (again) h = h2;
           }
           if (act && h >= 0 && h != j) atomicOr(&bits[h >> 5], 1u << (h & 31));
@@ -405,10 +412,14 @@ __global__ __launch_bounds__(256) void closest_hit_kernel(
 // The wavefront walk needs the two-wide tree and depth + 1 <= 64 stack
 // entries (one VGPR across the wavefront).
 static bool packet_ok(const illum_bvh *bvh) { return bvh->d_wide && bvh->h.depth + 1 <= 64; }
+static bool packet4_ok(const illum_bvh *bvh) {
+  return bvh->d_wide4 && 3 * bvh->h.depth4 + 1 <= 64;
+}
 
 // Walk for a requested layout (2 two-wide, 3 wavefront packet over the
-// two-wide tree, 4 four-wide, 0 threaded), falling back to what the tree
-// has.  ILLUM_FF_TRAVERSAL (threaded, wide2, packet, wide4) overrides.
+// two-wide tree, 4 four-wide, 5 wavefront packet over the four-wide tree, 0
+// threaded), falling back to what the tree has.  ILLUM_FF_TRAVERSAL
+// (threaded, wide2, packet, wide4, packet4) overrides.
 static int ff_resolve(const illum_bvh *bvh, int want) {
   const char *e = getenv("ILLUM_FF_TRAVERSAL");
   const std::string ev = e ? e : "";
@@ -416,6 +427,8 @@ static int ff_resolve(const illum_bvh *bvh, int want) {
   else if (ev == "wide2") want = 2;
   else if (ev == "packet") want = 3;
   else if (ev == "wide4") want = 4;
+  else if (ev == "packet4") want = 5;
+  if (want == 5 && packet4_ok(bvh)) return 5;
   if (want == 4 && bvh->d_wide4) return 4;
   if (want == 3 && packet_ok(bvh)) return 3;
   if (want == 0) return 0;
@@ -434,7 +447,8 @@ static int ff_stack_slots(const illum_bvh *bvh, int layout) {
 
 template <bool COUNT>
 static decltype(&ff_visibility_kernel<4, COUNT>) ff_kernel(int layout) {
-  return layout == 4   ? ff_visibility_kernel<4, COUNT>
+  return layout == 5   ? ff_visibility_kernel<5, COUNT>
+         : layout == 4 ? ff_visibility_kernel<4, COUNT>
          : layout == 3 ? ff_visibility_kernel<3, COUNT>
          : layout == 2 ? ff_visibility_kernel<2, COUNT>
                        : ff_visibility_kernel<0, COUNT>;

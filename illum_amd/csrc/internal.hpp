@@ -128,13 +128,14 @@ struct illum_bvh {
   int32_t *d_roots = nullptr;  // per body: threaded, two-wide, four-wide root, 0
   int nbody = 1;               // bodies of the mesh when the tree was built
   std::vector<int64_t> body_off;  // the partition the forest was built for
-  // form-factor traversal layout (2 = two-wide, 4 = four-wide near-first walk)
+  // form-factor traversal (2 / 4 = two- / four-wide near-first walk per lane,
+  // 3 / 5 = wavefront packet walk over the two- / four-wide tree)
   int ff_layout = 2;
   // autotuning record (illum_bvh_build with leaf_size 0): leaf size x layout
   int ntuned = 0;
-  int tune_leaf[16] = {0};
-  int tune_layout[16] = {0};
-  double tune_ms[16] = {0};
+  int tune_leaf[32] = {0};
+  int tune_layout[32] = {0};
+  double tune_ms[32] = {0};
 };
 
 struct illum_sparse {

@@ -338,6 +338,77 @@ __device__ __forceinline__ void cswap4(float *k, int *r, int a, int b) {
   }
 }
 
+// Wavefront-cooperative closest hit over the four-wide tree rooted at `root`
+// (same contract and exactness argument as trace_closest_packet).  A child is
+// entered when any active lane enters it; entered children are visited in
+// order of the entry distance seen by the first lane that enters each (the
+// lanes share an origin and aim at neighbouring points, so one lane's order
+// is representative), the nearest next and the others pushed farthest first.
+// A walk pushes at most three entries per level, so 3 * depth4 + 1 <= 64
+// entries suffice (the caller checks).
+__device__ __forceinline__ int trace_closest_packet4(const float4 *__restrict__ w4,
+                                                     const float4 *__restrict__ tris,
+                                                     const DRay &r, bool act, int root,
+                                                     float tinit, float *tbest_out,
+                                                     unsigned &nbox, unsigned &ntri) {
+  const int lane = __lane_id();
+  float tbest = tinit;
+  int best = -1;
+  int sp = 0;
+  int cur = root;
+  int stk = 0;
+  for (;;) {
+    if (cur >= 0) {
+      float key[4];
+      int ref[4];
+      box4_enter(r, w4 + 8 * cur, tbest, key, ref);
+      if (act) nbox += 4;
+      float k[4];
+      int n = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint64_t m = __ballot(act && key[q] < __builtin_huge_valf());
+        // representative entry distance: the first entering lane's
+        k[q] = m ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(key[q]),
+                                                              (int)__builtin_ctzll(m)))
+                 : __builtin_huge_valf();
+        n += m ? 1 : 0;
+      }
+      if (n > 0) {
+        cswap4(k, ref, 0, 1);
+        cswap4(k, ref, 2, 3);
+        cswap4(k, ref, 0, 2);
+        cswap4(k, ref, 1, 3);
+        cswap4(k, ref, 1, 2);
+        if (n > 3) { if (lane == sp) stk = ref[3]; ++sp; }
+        if (n > 2) { if (lane == sp) stk = ref[2]; ++sp; }
+        if (n > 1) { if (lane == sp) stk = ref[1]; ++sp; }
+        cur = ref[0];
+        continue;
+      }
+    } else {
+      const int info = ~cur;
+      const int first = info & 0xFFFFFF, cnt = info >> 24;
+      if (act) ntri += cnt;
+      for (int q = 0; q < cnt; ++q) {
+        const float4 a = tris[3 * (first + q)], b = tris[3 * (first + q) + 1],
+                     c = tris[3 * (first + q) + 2];
+        const int idx = __float_as_int(a.w);
+        float t;
+        if (act && tri_hit(r, a, b, c, &t) && (t < tbest || (t == tbest && idx < best))) {
+          tbest = t;
+          best = idx;
+        }
+      }
+    }
+    if (sp == 0) break;
+    --sp;
+    cur = __builtin_amdgcn_readlane(stk, sp);
+  }
+  *tbest_out = tbest;
+  return best;
+}
+
 // Near-first closest hit over the four-wide tree rooted at `root`: the
 // entered children are sorted by entry distance (5-comparator network), the
 // nearest is visited next and the others pushed farthest first.  Returns the

@@ -63,7 +63,7 @@ def test_closest_hit_random_rays_bitwise(api):
     assert np.all(hb[n:] >= 0)
 
 
-@pytest.mark.parametrize("walk", ["default", "packet"])
+@pytest.mark.parametrize("walk", ["default", "packet", "packet4"])
 @pytest.mark.parametrize("subdiv,ncraters,albedo", [(3, 40, 0.1), (4, 40, 0.1)])
 def test_form_factors_vs_oracle(api, monkeypatch, subdiv, ncraters, albedo, walk):
     if walk != "default":
@@ -103,11 +103,11 @@ def test_autotuned_bvh_and_traversal_choice_do_not_change_results(api, monkeypat
     ref = api.IllumContext(V=V, F=F, leaf_size=4).compute_F(0.1)
     tuned = api.IllumContext(V=V, F=F, leaf_size=0)
     trials = tuned.bvh.tuning()
-    assert [(l, w) for l, w, _ in trials] == [(l, w) for l in (2, 3, 4, 6, 8) for w in (2, 4, 3)]
+    assert [(l, w) for l, w, _ in trials] == [(l, w) for l in (2, 3, 4, 6, 8) for w in (2, 4, 3, 5)]
     assert all(ms > 0 for _, _, ms in trials)
     assert tuned.bvh.info()["leaf_size"] == min(trials, key=lambda x: x[2])[0]
     out = [tuned.compute_F(0.1), api.IllumContext(V=V, F=F, leaf_size=1).compute_F(0.1)]
-    for walk in ("threaded", "wide2", "wide4", "packet"):
+    for walk in ("threaded", "wide2", "wide4", "packet", "packet4"):
         monkeypatch.setenv("ILLUM_FF_TRAVERSAL", walk)
         out.append(api.IllumContext(V=V, F=F, leaf_size=4).compute_F(0.1))
         out.append(api.IllumContext(V=V, F=F, leaf_size=1).compute_F(0.1))
