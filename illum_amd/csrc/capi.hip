@@ -77,32 +77,97 @@ void sync(illum_ctx *ctx) {
   if (!ctx->pending.empty()) collect_profile(ctx);
 }
 
-__global__ void scan_kernel(const int32_t *__restrict__ counts, int64_t n,
-                            int64_t *__restrict__ out) {
-  __shared__ int64_t s[1024];
-  const int tid = threadIdx.x;
-  const int64_t per = (n + 1023) / 1024;
-  const int64_t a = tid * per, b = a + per < n ? a + per : n;
-  int64_t sum = 0;
-  for (int64_t i = a; i < b; ++i) sum += counts[i];
-  s[tid] = sum;
+// Exclusive scan of int32 counts into int64 offsets, reduce-then-scan over
+// chunks of SCAN_CHUNK counts: block totals, a one-block scan of the totals
+// (which also writes out[n]), then each block rescans its chunk from its
+// offset.  Every thread handles SCAN_PER consecutive counts.
+constexpr int SCAN_PER = 16;
+constexpr int SCAN_CHUNK = 256 * SCAN_PER;
+
+// Block-wide exclusive scan of one int64 per thread (256 threads); *total
+// receives the block sum.
+__device__ int64_t block_exscan256_i64(int64_t v, int64_t *s_wave, int64_t *total) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  int64_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int64_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_wave[wid] = x;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int64_t v = tid >= off ? s[tid - off] : 0;
-    __syncthreads();
-    s[tid] += v;
-    __syncthreads();
+  int64_t off = 0, tot = 0;
+  for (int w = 0; w < 4; ++w) {
+    const int64_t sw = s_wave[w];
+    if (w < wid) off += sw;
+    tot += sw;
   }
-  int64_t run = s[tid] - sum;
-  for (int64_t i = a; i < b; ++i) {
-    out[i] = run;
-    run += counts[i];
+  __syncthreads();
+  *total = tot;
+  return off + x - v;
+}
+
+__global__ __launch_bounds__(256) void scan_reduce_kernel(const int32_t *__restrict__ counts,
+                                                          int64_t n, int64_t *__restrict__ part) {
+  __shared__ int64_t s_wave[4];
+  const int64_t a = (int64_t)blockIdx.x * SCAN_CHUNK + (int64_t)threadIdx.x * SCAN_PER;
+  int64_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_PER; ++k)
+    if (a + k < n) sum += counts[a + k];
+  int64_t total;
+  block_exscan256_i64(sum, s_wave, &total);
+  if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+// One block: part[0..nb) becomes its exclusive scan; out_n = the grand total.
+__global__ __launch_bounds__(256) void scan_part_kernel(int64_t *__restrict__ part, int nb,
+                                                        int64_t *__restrict__ out_n) {
+  __shared__ int64_t s_wave[4];
+  const int per = (nb + 255) / 256;
+  const int a = threadIdx.x * per, b = min(nb, a + per);
+  int64_t sum = 0;
+  for (int i = a; i < b; ++i) sum += part[i];
+  int64_t total;
+  int64_t run = block_exscan256_i64(sum, s_wave, &total);
+  for (int i = a; i < b; ++i) {
+    const int64_t v = part[i];
+    part[i] = run;
+    run += v;
   }
-  if (tid == 1023) out[n] = s[1023];
+  if (threadIdx.x == 0) *out_n = total;
+}
+
+__global__ __launch_bounds__(256) void scan_apply_kernel(const int32_t *__restrict__ counts,
+                                                         int64_t n,
+                                                         const int64_t *__restrict__ part,
+                                                         int64_t *__restrict__ out) {
+  __shared__ int64_t s_wave[4];
+  const int64_t a = (int64_t)blockIdx.x * SCAN_CHUNK + (int64_t)threadIdx.x * SCAN_PER;
+  int32_t c[SCAN_PER];
+  int64_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_PER; ++k) {
+    c[k] = a + k < n ? counts[a + k] : 0;
+    sum += c[k];
+  }
+  int64_t total;
+  int64_t run = part[blockIdx.x] + block_exscan256_i64(sum, s_wave, &total);
+#pragma unroll
+  for (int k = 0; k < SCAN_PER; ++k) {
+    if (a + k < n) out[a + k] = run;
+    run += c[k];
+  }
 }
 
 void scan_counts(illum_ctx *ctx, const int32_t *counts, int64_t n, int64_t *offsets) {
-  scan_kernel<<<1, 1024, 0, stream_of(ctx)>>>(counts, n, offsets);
+  hipStream_t s = stream_of(ctx);
+  const int nb = (int)std::max<int64_t>(1, (n + SCAN_CHUNK - 1) / SCAN_CHUNK);
+  int64_t *part = (int64_t *)ctx_workspace(ctx, WS_SCAN_PART, 8 * (size_t)nb);
+  scan_reduce_kernel<<<nb, 256, 0, s>>>(counts, n, part);
+  ILLUM_LAUNCH_CHECK();
+  scan_part_kernel<<<1, 256, 0, s>>>(part, nb, offsets + n);
+  ILLUM_LAUNCH_CHECK();
+  scan_apply_kernel<<<nb, 256, 0, s>>>(counts, n, part, offsets);
   ILLUM_LAUNCH_CHECK();
 }
 

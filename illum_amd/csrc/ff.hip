@@ -63,7 +63,8 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
     uint32_t *__restrict__ vis, uint32_t *__restrict__ mask, int32_t *__restrict__ counts,
     int *__restrict__ status, int nrows, int jstride, unsigned long long *__restrict__ stats,
     const float4 *__restrict__ clus, int nclus, int scap, const int32_t *__restrict__ body_of,
-    const int32_t *__restrict__ body_off, const int32_t *__restrict__ roots) {
+    const int32_t *__restrict__ body_off, const int32_t *__restrict__ roots,
+    int32_t *__restrict__ col_counts) {
   extern __shared__ uint32_t smem[];
   uint32_t *bits = smem;
   int *queue = (int *)(smem + nwords);
@@ -223,7 +224,12 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
       const double rho = albedo_vec ? albedo_vec[i] : albedo;
       const double v = ff_value(pr4[i], n4[i], pj, nj, Aj) * rho;
       if (v != v) nan = true;
-      if (v != 0.0) keep |= 1u << b;
+      if (v != 0.0) {
+        keep |= 1u << b;
+        // receiver i's row count of the transposed matrix (the CSR row
+        // lengths), so no pass over the transposed bits is needed for them
+        if (col_counts) atomicAdd(&col_counts[i], 1);
+      }
     }
     vis[(size_t)row * nwords + w] = word;
     mask[(size_t)row * nwords + w] = keep;
@@ -272,6 +278,8 @@ __device__ __forceinline__ int block_exscan256(int v, int *s_wave, int *total) {
   return off + x - v;
 }
 
+constexpr int FILL_MAXW = 16;
+
 // Fills one CSC column (csr == 0: row of M is source j, bits are receivers i)
 // or one CSR row (csr == 1: row of M^T is receiver i, bits are sources j).
 __global__ __launch_bounds__(256) void ff_fill_kernel(
@@ -286,31 +294,63 @@ __global__ __launch_bounds__(256) void ff_fill_kernel(
   const int per = (nwords + 255) / 256;
   const int w0 = tid * per, w1 = min(nwords, w0 + per);
   const uint32_t *mr = m + (size_t)row * nwords;
+  // The thread's words stay in registers between the count and the listing
+  // when there are at most FILL_MAXW of them (rows of up to 131,072 bits).
+  const bool cached = per <= FILL_MAXW;
+  uint32_t wv[FILL_MAXW];
   int c = 0;
-  for (int w = w0; w < w1; ++w) c += __popc(mr[w]);
+  if (cached) {
+#pragma unroll
+    for (int k = 0; k < FILL_MAXW; ++k) {
+      wv[k] = w0 + k < w1 ? mr[w0 + k] : 0u;
+      c += __popc(wv[k]);
+    }
+  } else {
+    for (int w = w0; w < w1; ++w) c += __popc(mr[w]);
+  }
   int total;
   int off = block_exscan256(c, s_wave, &total);
-  int64_t pos = ptr[row] + off;
-  const float4 prow = pr4[row], nrow = n4[row];
-  for (int w = w0; w < w1; ++w) {
-    uint32_t rest = mr[w];
-    while (rest) {
-      const int b = __ffs(rest) - 1;
-      rest &= rest - 1u;
-      const int col = 32 * w + b;
-      double v;
-      if (csr) {  // row = receiver i, col = source j
-        const double rho = albedo_vec ? albedo_vec[row] : albedo;
-        v = ff_value(prow, nrow, pr4[col], n4[col], area[col]) * rho;
-      } else {    // row = source j, col = receiver i
-        const double rho = albedo_vec ? albedo_vec[col] : albedo;
-        v = ff_value(pr4[col], n4[col], prow, nrow, area[row]) * rho;
+  const int64_t base = ptr[row];
+  // Phase 1: the row's set bits, in order, become its index list.
+  int64_t pos = base + off;
+  if (cached) {
+#pragma unroll
+    for (int k = 0; k < FILL_MAXW; ++k) {
+      uint32_t rest = wv[k];
+      while (rest) {
+        const int b = __ffs(rest) - 1;
+        rest &= rest - 1u;
+        idx_out[pos++] = 32 * (w0 + k) + b;
       }
-      idx_out[pos] = col;
-      val_out[pos] = v;
-      if (val32_out) val32_out[pos] = (float)v;
-      ++pos;
     }
+  } else {
+    for (int w = w0; w < w1; ++w) {
+      uint32_t rest = mr[w];
+      while (rest) {
+        const int b = __ffs(rest) - 1;
+        rest &= rest - 1u;
+        idx_out[pos++] = 32 * w + b;
+      }
+    }
+  }
+  // Phase 2: the form factors, one entry per thread (the set bits of a row
+  // cluster in a few words, so phase 1 alone would leave them to one or two
+  // threads).  The block reads back its own writes after the barrier, from
+  // L2 (an agent-scope load passes the CU's L1).
+  __syncthreads();
+  const float4 prow = pr4[row], nrow = n4[row];
+  for (int k = tid; k < total; k += 256) {
+    const int col = __hip_atomic_load(&idx_out[base + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    double v;
+    if (csr) {  // row = receiver i, col = source j
+      const double rho = albedo_vec ? albedo_vec[row] : albedo;
+      v = ff_value(prow, nrow, pr4[col], n4[col], area[col]) * rho;
+    } else {    // row = source j, col = receiver i
+      const double rho = albedo_vec ? albedo_vec[col] : albedo;
+      v = ff_value(pr4[col], n4[col], prow, nrow, area[row]) * rho;
+    }
+    val_out[base + k] = v;
+    if (val32_out) val32_out[base + k] = (float)v;
   }
 }
 
@@ -362,16 +402,6 @@ __global__ __launch_bounds__(256) void bit_transpose_kernel(const uint32_t *__re
   }
 }
 
-__global__ void popcount_rows_kernel(int nrows, int nwords, const uint32_t *__restrict__ m,
-                                     int32_t *__restrict__ counts) {
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (row >= nrows) return;
-  int c = 0;
-  for (int w = lane; w < nwords; w += 64) c += __popc(m[(size_t)row * nwords + w]);
-  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-  if (lane == 0) counts[row] = c;
-}
 
 // Closest hit of arbitrary rays over every body's tree: each tree is walked
 // unbounded and the (distance, index) minima are combined exactly.
@@ -524,7 +554,10 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   const size_t mbytes = (size_t)rows_p * nwords * 4;
   uint32_t *M = (uint32_t *)ctx_workspace(ctx, WS_FF_MASK, mbytes);
   uint32_t *MT = (uint32_t *)ctx_workspace(ctx, WS_FF_MASK_T, mbytes);
-  int32_t *counts = (int32_t *)ctx_workspace(ctx, WS_FF_COUNTS, 4 * rows_p);
+  // counts: CSC column lengths (rows of M), then CSR row lengths (rows of M^T)
+  int32_t *counts = (int32_t *)ctx_workspace(ctx, WS_FF_COUNTS, 8 * rows_p);
+  int32_t *rcounts = counts + rows_p;
+  ILLUM_HIP(hipMemsetAsync(rcounts, 0, 4 * nf, s));
   DBuf<int> status(1);
   if (rows_p > nf)
     ILLUM_HIP(hipMemsetAsync(M + (size_t)nf * nwords, 0, (size_t)(rows_p - nf) * nwords * 4, s));
@@ -542,7 +575,7 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
         (float)offset, mesh->d_area, d_alb.p, albedo, sp->d_vis, M, counts, status.p,
         (int)nf, 1, sp->d_nrays, cluster_cull() ? (const float4 *)mesh->d_clus : nullptr,
         (int)mesh->nclus, stack_slots,
-        mesh->d_body, mesh->d_boff, bvh->d_roots);
+        mesh->d_body, mesh->d_boff, bvh->d_roots, rcounts);
     ILLUM_LAUNCH_CHECK();
   }
   // CSC
@@ -580,9 +613,7 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
     dim3 g(nwords / 16, (unsigned)(rows_p / 512));
     bit_transpose_kernel<<<g, 256, 0, s>>>(M, MT, nwords);
     ILLUM_LAUNCH_CHECK();
-    popcount_rows_kernel<<<(int)((nf + 3) / 4), 256, 0, s>>>((int)nf, nwords, MT, counts);
-    ILLUM_LAUNCH_CHECK();
-    scan_counts(ctx, counts, nf, sp->d_rowptr);
+    scan_counts(ctx, rcounts, nf, sp->d_rowptr);
     ff_fill_kernel<<<(int)nf, 256, 0, s>>>((int)nf, nwords, MT, sp->d_rowptr, 1,
                                            (const float4 *)mesh->d_pr4,
                                            (const float4 *)mesh->d_n4, mesh->d_area, d_alb.p,
@@ -627,7 +658,7 @@ double time_visibility_sample(illum_bvh *bvh, int max_sources, int layout_choice
         (const float4 *)bvh->d_tris,
         1e-5f, mesh->d_area, nullptr, 1.0, vis.p, M.p, counts.p, status.p, nrows, jstride, nullptr,
         cluster_cull() ? (const float4 *)mesh->d_clus : nullptr, (int)mesh->nclus, stack_slots,
-        mesh->d_body, mesh->d_boff, bvh->d_roots);
+        mesh->d_body, mesh->d_boff, bvh->d_roots, nullptr);
     ILLUM_LAUNCH_CHECK();
     ILLUM_HIP(hipEventRecord(e1, s));
     ILLUM_HIP(hipEventSynchronize(e1));
@@ -671,7 +702,7 @@ void form_factor_work(illum_bvh *bvh, double offset, int64_t *out4) {
         (const float4 *)bvh->d_tris,
       (float)offset, mesh->d_area, nullptr, 1.0, vis, M, counts, status.p, (int)nf, 1, stats.p,
       cluster_cull() ? (const float4 *)mesh->d_clus : nullptr, (int)mesh->nclus, stack_slots,
-        mesh->d_body, mesh->d_boff, bvh->d_roots);
+        mesh->d_body, mesh->d_boff, bvh->d_roots, nullptr);
   ILLUM_LAUNCH_CHECK();
   unsigned long long h[4];
   ILLUM_HIP(hipMemcpyAsync(h, stats.p, 32, hipMemcpyDeviceToHost, s));

@@ -88,6 +88,7 @@ enum WorkspaceSlot {
   WS_RAD_X1 = 6,
   WS_RAD_X2 = 7,
   WS_RAD_D32 = 8,   // FP32 correction
+  WS_SCAN_PART = 9, // scan_counts block totals
 };
 
 // Drains pending stage events into the profile (after the stream is synced).
