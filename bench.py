@@ -161,6 +161,9 @@ def insolation_chunks(normals, steps_per_rot, chunk):
 
 
 def bench_thermal(api, nsteps=4096, subdiv=7, seed=0, chunk=256):
+    # whole launches of `chunk` steps, at least one
+    chunk = max(4, min(chunk, nsteps))
+    nsteps = max(chunk, nsteps - nsteps % chunk)
     ctx = api.default_context()
     V, F = api.synth_ellipsoid(subdiv, ncraters=150, seed=seed)
     nf = len(F)

@@ -76,6 +76,71 @@ __global__ __launch_bounds__(256) void horizons_kernel(
   out[idx] = th[4 * m];
 }
 
+// Wavefront packet variant (trace_any_packet).  A wavefront's lanes are 64
+// consecutive (facet, azimuth) pairs, mostly of one facet, so their rays
+// share an origin.  The lanes step their bisections together; a lane whose
+// search has ended, or that is past the end, takes no part.  The wavefront
+// walks the trees of the union of its lanes' bodies in turn, each lane
+// testing only its own.  Per lane this is the same sequence of queries as
+// horizons_kernel, so the same result.
+__global__ __launch_bounds__(256) void horizons_packet_kernel(
+    int64_t nf, int nphi, const float4 *__restrict__ pr4, const float4 *__restrict__ n4,
+    const double *__restrict__ frame, const float4 *__restrict__ wide,
+    const float4 *__restrict__ tris, float offset, const double *__restrict__ cphi,
+    const double *__restrict__ sphi, const double *__restrict__ th, int tree_nodes,
+    double *__restrict__ out, const int32_t *__restrict__ roots, int nroots,
+    const int32_t *__restrict__ body_of) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = idx < nf * nphi;
+  const int64_t j = valid ? idx / nphi : 0;
+  const int k = (int)(idx - j * nphi);
+  const float4 p = pr4[j], n = n4[j];
+  const float ox = p.x + offset * n.x, oy = p.y + offset * n.y, oz = p.z + offset * n.z;
+  const double *F = frame + 9 * j;
+  const double F00 = F[0], F10 = F[1], F20 = F[2];  // column b
+  const double F01 = F[3], F11 = F[4], F21 = F[5];  // column t
+  const double F02 = F[6], F12 = F[7], F22 = F[8];  // column n
+  const double cp = valid ? cphi[k] : 1.0, sp = valid ? sphi[k] : 0.0;
+  // the lane's trees [b0, b1); the wavefront's union [ulo, uhi)
+  const int b0 = body_of ? body_of[j] : 0, b1 = body_of ? b0 + 1 : nroots;
+  int ulo = b0, uhi = b1;
+  if (body_of) {
+    for (int o = 32; o > 0; o >>= 1) {
+      ulo = min(ulo, __shfl_xor(ulo, o));
+      uhi = max(uhi, __shfl_xor(uhi, o));
+    }
+  }
+  int m = 1;
+  bool done = !valid;
+  double res = 0.0;
+  while (__ballot(!done)) {
+    const double st = th[4 * m + 1], ct = th[4 * m + 2];
+    const double vx = cp * st, vy = sp * st, vz = ct;
+    const float dx = (float)((F00 * vx + F01 * vy) + F02 * vz);
+    const float dy = (float)((F10 * vx + F11 * vy) + F12 * vz);
+    const float dz = (float)((F20 * vx + F21 * vy) + F22 * vz);
+    const DRay r = make_ray(ox, oy, oz, dx, dy, dz);
+    bool hit = false;
+    for (int b = ulo; b < uhi; ++b) {
+      const bool act = !done && !hit && b >= b0 && b < b1;
+      if (!__ballot(act)) continue;
+      const bool h = trace_any_packet(wide, tris, r, act, roots[4 * b + 1]);
+      if (act && h) hit = true;
+    }
+    if (!done) {
+      m = 2 * m + (hit ? 0 : 1);
+      if (m >= tree_nodes) {
+        res = __builtin_nan("");
+        done = true;
+      } else if (th[4 * m + 3] != 0.0) {
+        res = th[4 * m];
+        done = true;
+      }
+    }
+  }
+  if (valid) out[idx] = res;
+}
+
 // Tabulates the bisection tree: node m (root 1) holds theta (the midpoint
 // tested at m), sin, cos and whether the search stops on arriving at m.
 static std::vector<double> bisection_table(double theta_eps, int *tree_nodes) {
@@ -151,23 +216,33 @@ void horizons(illum_bvh *bvh, illum_bvh *occl, int nphi, double theta_eps, doubl
   {
     StageTimer st(ctx, "horizons");
     const char *e = getenv("ILLUM_HZ_TRAVERSAL");
-    // ILLUM_HZ_TRAVERSAL: "threaded", "wide4", or the two-wide walk (default;
-    // faster than four-wide for these short bisection rays)
+    // ILLUM_HZ_TRAVERSAL: "threaded", "wide4", "packet" (wavefront walk over
+    // the two-wide tree), or the per-lane two-wide walk (default; faster than
+    // four-wide for these short bisection rays)
     const std::string ev = e ? e : "";
     const int layout = ev == "threaded" ? 0
                        : ev == "wide4" ? (target->d_wide4 ? 4 : 0)
-                                       : (target->d_wide && target->h.depth <= 32 ? 2 : 0);
+                       : ev == "packet" ? (target->d_wide && target->h.depth + 1 <= 64 ? 3 : 0)
+                                        : (target->d_wide && target->h.depth <= 32 ? 2 : 0);
     const int scap = layout == 4 ? std::min(3 * target->h.depth4 + 1, 16)
                      : layout == 2 ? target->h.depth + 1 : 0;
     const size_t lds = (size_t)scap * 256 * 4;
-    auto kern = layout == 4 ? horizons_kernel<4> : layout == 2 ? horizons_kernel<2>
-                                                               : horizons_kernel<0>;
-    kern<<<(unsigned)((total + 255) / 256), 256, lds, s>>>(
-        nf, nphi, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4, mesh->d_frame,
-        (const float4 *)target->d_nodes, (const float4 *)target->d_wide,
-        (const float4 *)target->d_wide4, (const float4 *)target->d_tris, (float)offset, d_c.p,
-        d_s.p, d_tab.p, tree_nodes, dst, target->d_roots, target->nbody,
-        occl ? nullptr : mesh->d_body, scap);
+    if (layout == 3) {
+      horizons_packet_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(
+          nf, nphi, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4, mesh->d_frame,
+          (const float4 *)target->d_wide, (const float4 *)target->d_tris, (float)offset, d_c.p,
+          d_s.p, d_tab.p, tree_nodes, dst, target->d_roots, target->nbody,
+          occl ? nullptr : mesh->d_body);
+    } else {
+      auto kern = layout == 4 ? horizons_kernel<4> : layout == 2 ? horizons_kernel<2>
+                                                                 : horizons_kernel<0>;
+      kern<<<(unsigned)((total + 255) / 256), 256, lds, s>>>(
+          nf, nphi, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4, mesh->d_frame,
+          (const float4 *)target->d_nodes, (const float4 *)target->d_wide,
+          (const float4 *)target->d_wide4, (const float4 *)target->d_tris, (float)offset, d_c.p,
+          d_s.p, d_tab.p, tree_nodes, dst, target->d_roots, target->nbody,
+          occl ? nullptr : mesh->d_body, scap);
+    }
     ILLUM_LAUNCH_CHECK();
   }
   if (!(flags & ILLUM_OUT_ON_DEVICE))

@@ -129,8 +129,9 @@ struct illum_bvh {
   int nbody = 1;               // bodies of the mesh when the tree was built
   std::vector<int64_t> body_off;  // the partition the forest was built for
   // form-factor traversal (2 / 4 = two- / four-wide near-first walk per lane,
-  // 3 / 5 = wavefront packet walk over the two- / four-wide tree)
-  int ff_layout = 2;
+  // 3 / 5 = wavefront packet walk over the two- / four-wide tree); without
+  // autotuning the two-wide packet walk
+  int ff_layout = 3;
   // autotuning record (illum_bvh_build with leaf_size 0): leaf size x layout
   int ntuned = 0;
   int tune_leaf[32] = {0};

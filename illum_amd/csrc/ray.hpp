@@ -554,6 +554,64 @@ __device__ __forceinline__ bool trace_any_wide(const float4 *__restrict__ wide,
   return false;
 }
 
+// Wavefront-cooperative any hit (t > 0) over the two-wide tree rooted at
+// `root`, with the contract of trace_closest_packet: all 64 lanes call it
+// together, lanes with act = false take no part.  A lane drops out of the
+// walk once it has a hit; the walk ends when no lane is left searching or the
+// tree is exhausted.  A child is entered when any searching lane's ray enters
+// its box; with both entered, the one more lanes enter first goes first.
+// Exact: a lane's answer is whether any triangle is hit, every node whose box
+// its ray enters is visited until it has one, and extra tests only add exact
+// hits.
+__device__ __forceinline__ bool trace_any_packet(const float4 *__restrict__ wide,
+                                                 const float4 *__restrict__ tris,
+                                                 const DRay &r, bool act, int root) {
+  const int lane = __lane_id();
+  const float inf = __builtin_huge_valf();
+  bool hit = false;
+  int sp = 0;
+  int cur = root;
+  int stk = 0;
+  for (;;) {
+    const bool live = act && !hit;
+    if (!__ballot(live)) break;
+    if (cur >= 0) {
+      const float4 a0 = wide[4 * cur], a1 = wide[4 * cur + 1];
+      const float4 b0 = wide[4 * cur + 2], b1 = wide[4 * cur + 3];
+      float tl, tr;
+      const bool el = box_enter(r, a0, a1, inf, &tl);
+      const bool er = box_enter(r, b0, b1, inf, &tr);
+      const bool hl = live && el, hr = live && er;
+      const uint64_t ml = __ballot(hl), mr = __ballot(hr);
+      const int cl = __float_as_int(a0.w), cr = __float_as_int(b0.w);
+      if (ml && mr) {
+        const uint64_t both = ml & mr;
+        const uint64_t lfirst = __ballot(hl && hr && tl <= tr);
+        const bool lnear = 2 * __popcll(lfirst) >= __popcll(both);
+        if (lane == sp) stk = lnear ? cr : cl;
+        ++sp;
+        cur = lnear ? cl : cr;
+        continue;
+      }
+      if (ml) { cur = cl; continue; }
+      if (mr) { cur = cr; continue; }
+    } else {
+      const int info = ~cur;
+      const int first = info & 0xFFFFFF, cnt = info >> 24;
+      for (int k = 0; k < cnt; ++k) {
+        float t;
+        const bool h = tri_hit(r, tris[3 * (first + k)], tris[3 * (first + k) + 1],
+                               tris[3 * (first + k) + 2], &t);
+        if (live && h) hit = true;
+      }
+    }
+    if (sp == 0) break;
+    --sp;
+    cur = __builtin_amdgcn_readlane(stk, sp);
+  }
+  return hit;
+}
+
 // Any hit (t > 0) over the threaded tree starting at node `root`.
 __device__ __forceinline__ bool trace_any(const float4 *__restrict__ nodes,
                                           const float4 *__restrict__ tris, const DRay &r,

@@ -138,8 +138,11 @@ int illum_mesh_triangles(const illum_mesh *mesh, float *tri9);
 /* ---- BVH (host SAH build; threaded, two-wide and four-wide layouts in HBM)
  * leaf_size 0 autotunes: the visibility kernel is timed (hipEvents) on a
  * strided sample of up to 8192 source facets for leaf sizes 2, 3, 4, 6, 8
- * and the two-wide and four-wide walks, and the fastest tree and walk are
- * kept; illum_bvh_tuning reports the trials (leaf size, layout 2 or 4, ms). */
+ * and four walks -- per-lane near-first over the two-wide (layout 2) and
+ * four-wide (4) trees, wavefront packets over the two-wide (3) and four-wide
+ * (5) trees -- and the fastest tree and walk are kept; illum_bvh_tuning
+ * reports the trials (leaf size, layout, ms).  A given leaf size uses the
+ * two-wide packet walk.  Every walk gives the same visibility. */
 int illum_bvh_build(illum_mesh *mesh, int leaf_size, illum_bvh **out);
 int illum_bvh_tuning(const illum_bvh *bvh, int cap, int *leaf_sizes, int *layouts, double *ms,
                      int *n);

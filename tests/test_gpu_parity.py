@@ -149,7 +149,7 @@ def test_horizon_traversal_choice_does_not_change_results(api, monkeypatch):
     V, F = _mesh(api, 3, 40, seed=9)
     ctx = api.IllumContext(V=V, F=F)
     a = ctx.make_horizons(nphi=33).copy()
-    for walk in ("threaded", "wide4"):
+    for walk in ("threaded", "wide4", "packet"):
         monkeypatch.setenv("ILLUM_HZ_TRAVERSAL", walk)
         b = ctx.make_horizons(nphi=33)
         assert np.array_equal(a, b), walk
@@ -168,8 +168,11 @@ def test_form_factors_albedo_vector(api):
     np.testing.assert_allclose(gv, v, rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("walk", ["default", "packet"])
 @pytest.mark.parametrize("nphi,theta_eps", [(65, 1e-3), (17, 1e-2)])
-def test_horizons_bitwise(api, nphi, theta_eps):
+def test_horizons_bitwise(api, monkeypatch, nphi, theta_eps, walk):
+    if walk != "default":
+        monkeypatch.setenv("ILLUM_HZ_TRAVERSAL", walk)
     V, F = _mesh(api, 3, 40, seed=2)
     ctx = api.IllumContext(V=V, F=F)
     hz = ctx.make_horizons(nphi=nphi, theta_eps=theta_eps, offset=1e-5)
@@ -425,7 +428,7 @@ def test_conductionQ_dropin_vs_reference_fortran_golden(api):
             assert abs(Fs - gold[f"{n}.Fsurf"][k]) <= 1e-8 * max(1.0, abs(Fs))
 
 
-def test_multibody_batch_vs_oracle_per_body(api):
+def test_multibody_batch_vs_oracle_per_body(api, monkeypatch):
     """Bodies of uneven size in one launch sequence (the multi-body batch
     configuration, scaled down): each body's visibility, form factors,
     horizons and scattered radiosity equal the oracle run on that body alone,
@@ -443,7 +446,11 @@ def test_multibody_batch_vs_oracle_per_body(api):
     vis = _bits_to_sets(Fm.visibility(), nf)
     cp, ri, v = Fm.csc()
     A = sps.csc_matrix((v, ri, cp), shape=(nf, nf))
-    hz = ctx.make_horizons(nphi=17)
+    hz = ctx.make_horizons(nphi=17).copy()
+    # the wavefront walk: lanes of one wavefront span body boundaries here
+    monkeypatch.setenv("ILLUM_HZ_TRAVERSAL", "packet")
+    assert np.array_equal(ctx.make_horizons(nphi=17), hz)
+    monkeypatch.delenv("ILLUM_HZ_TRAVERSAL")
     E = rng.uniform(0, 500, (nf, 3))
     B, res, _ = api.radiosity_solve(Fm, E, method="mixed", rel_tol=1e-10, max_iter=400)
     rays = 0
