@@ -286,7 +286,8 @@ __global__ __launch_bounds__(256) void ff_fill_kernel(
     int nrows, int nwords, const uint32_t *__restrict__ m, const int64_t *__restrict__ ptr,
     int csr, const float4 *__restrict__ pr4, const float4 *__restrict__ n4,
     const double *__restrict__ area, const double *__restrict__ albedo_vec, double albedo,
-    int32_t *__restrict__ idx_out, double *__restrict__ val_out, float *__restrict__ val32_out) {
+    int32_t *__restrict__ idx_out, double *__restrict__ val_out, float *__restrict__ val32_out,
+    const uint8_t *__restrict__ tile_nz) {
   __shared__ int s_wave[4];
   const int row = blockIdx.x;
   if (row >= nrows) return;
@@ -297,16 +298,22 @@ __global__ __launch_bounds__(256) void ff_fill_kernel(
   // The thread's words stay in registers between the count and the listing
   // when there are at most FILL_MAXW of them (rows of up to 131,072 bits).
   const bool cached = per <= FILL_MAXW;
+  // tile_nz (rows of the bit transpose): words of an all-zero 512 x 512 tile
+  // were not written and read as zero
+  const int ntile = nwords >> 4, tcol = row >> 9;
+  auto word = [&](int w) -> uint32_t {
+    return (!tile_nz || tile_nz[(w >> 4) * ntile + tcol]) ? mr[w] : 0u;
+  };
   uint32_t wv[FILL_MAXW];
   int c = 0;
   if (cached) {
 #pragma unroll
     for (int k = 0; k < FILL_MAXW; ++k) {
-      wv[k] = w0 + k < w1 ? mr[w0 + k] : 0u;
+      wv[k] = w0 + k < w1 ? word(w0 + k) : 0u;
       c += __popc(wv[k]);
     }
   } else {
-    for (int w = w0; w < w1; ++w) c += __popc(mr[w]);
+    for (int w = w0; w < w1; ++w) c += __popc(word(w));
   }
   int total;
   int off = block_exscan256(c, s_wave, &total);
@@ -325,7 +332,7 @@ __global__ __launch_bounds__(256) void ff_fill_kernel(
     }
   } else {
     for (int w = w0; w < w1; ++w) {
-      uint32_t rest = mr[w];
+      uint32_t rest = word(w);
       while (rest) {
         const int b = __ffs(rest) - 1;
         rest &= rest - 1u;
@@ -362,15 +369,23 @@ __global__ __launch_bounds__(256) void ff_fill_kernel(
 // section 7-3) with cross-lane shuffles.
 __global__ __launch_bounds__(256) void bit_transpose_kernel(const uint32_t *__restrict__ in,
                                                             uint32_t *__restrict__ out,
-                                                            int nwords) {
+                                                            int nwords,
+                                                            uint8_t *__restrict__ nz) {
   __shared__ uint32_t tile[512][17];
   const int tc = blockIdx.x, tr = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  uint32_t any = 0;
   for (int k = 0; k < 32; ++k) {
     const int idx = tid + 256 * k, r = idx >> 4, w = idx & 15;
-    tile[r][w] = in[(size_t)(512 * tr + r) * nwords + 16 * tc + w];
+    const uint32_t x = in[(size_t)(512 * tr + r) * nwords + 16 * tc + w];
+    tile[r][w] = x;
+    any |= x;
   }
-  __syncthreads();
+  // An all-zero tile (most of them: a facet sees few others, mostly near it
+  // in facet order) is only flagged; its output words are never read.
+  const int nonzero = __syncthreads_or(any != 0);
+  if (tid == 0) nz[tr * gridDim.x + tc] = nonzero ? 1 : 0;
+  if (!nonzero) return;
   uint32_t res[32];
 #pragma unroll
   for (int c = 0; c < 32; ++c) {
@@ -600,7 +615,7 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
     ff_fill_kernel<<<(int)nf, 256, 0, s>>>((int)nf, nwords, M, sp->d_colptr, 0,
                                            (const float4 *)mesh->d_pr4,
                                            (const float4 *)mesh->d_n4, mesh->d_area, d_alb.p,
-                                           albedo, sp->d_rowidx, sp->d_cval, nullptr);
+                                           albedo, sp->d_rowidx, sp->d_cval, nullptr, nullptr);
     ILLUM_LAUNCH_CHECK();
   }
   // CSR via the bit transpose.
@@ -611,13 +626,15 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   {
     StageTimer st(ctx, "ff_csr");
     dim3 g(nwords / 16, (unsigned)(rows_p / 512));
-    bit_transpose_kernel<<<g, 256, 0, s>>>(M, MT, nwords);
+    uint8_t *tile_nz = (uint8_t *)ctx_workspace(ctx, WS_FF_TILES, (size_t)g.x * g.y);
+    bit_transpose_kernel<<<g, 256, 0, s>>>(M, MT, nwords, tile_nz);
     ILLUM_LAUNCH_CHECK();
     scan_counts(ctx, rcounts, nf, sp->d_rowptr);
     ff_fill_kernel<<<(int)nf, 256, 0, s>>>((int)nf, nwords, MT, sp->d_rowptr, 1,
                                            (const float4 *)mesh->d_pr4,
                                            (const float4 *)mesh->d_n4, mesh->d_area, d_alb.p,
-                                           albedo, sp->d_colidx, sp->d_rval, sp->d_rval32);
+                                           albedo, sp->d_colidx, sp->d_rval, sp->d_rval32,
+                                           tile_nz);
     ILLUM_LAUNCH_CHECK();
   }
   sync(ctx);

@@ -89,6 +89,7 @@ enum WorkspaceSlot {
   WS_RAD_X2 = 7,
   WS_RAD_D32 = 8,   // FP32 correction
   WS_SCAN_PART = 9, // scan_counts block totals
+  WS_FF_TILES = 10, // nonzero flags of the bit transpose's 512 x 512 tiles
 };
 
 // Drains pending stage events into the profile (after the stream is synced).
