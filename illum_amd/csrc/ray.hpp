@@ -6,9 +6,11 @@
 // Ray/Triangle Intersection", JCGT 2(1), 2013, two-sided, with the paper's
 // FP64 recomputation of an edge function that evaluates to exactly zero.
 // The reported distance is max(t, e), where e is the entry distance of the ray
-// into the triangle's own bounding box; because correctly rounded arithmetic
-// is monotone, every ancestor box then has an entry distance <= the reported
-// distance, so pruning boxes whose entry lies beyond the best hit never
+// into the triangle's own bounding box.  Node boxes and the triangle's box use
+// one slab function, s(x) = fma(x, inv, -o * inv) per axis (one correctly
+// rounded multiply-add of the coordinate with per-ray constants), which is
+// monotone in x; so every ancestor box has an entry distance <= the reported
+// distance, and pruning boxes whose entry lies beyond the best hit never
 // changes the closest hit: a traversal returns exactly what a brute-force
 // loop over all triangles returns, whatever the tree.
 // Closest hit = smallest reported distance, ties to the smaller facet index.
@@ -25,6 +27,7 @@ namespace illum {
 struct DRay {
   float ox, oy, oz;   // origin
   float ix, iy, iz;   // per-axis inverse direction for box slabs (finite)
+  float nx, ny, nz;   // -o * inv per axis (the slab function's offset)
   float Sx, Sy, Sz;   // shear constants
   int kx, ky, kz;     // axis permutation (kz = dominant axis)
 };
@@ -58,15 +61,16 @@ __device__ __forceinline__ DRay make_ray(float ox, float oy, float oz, float dx,
   r.Sy = sel3(dx, dy, dz, ky) / dkz;
   r.Sz = 1.0f / dkz;
   r.ix = slab_inv(dx); r.iy = slab_inv(dy); r.iz = slab_inv(dz);
+  r.nx = -(ox * r.ix); r.ny = -(oy * r.iy); r.nz = -(oz * r.iz);
   return r;
 }
 
 // Slab test against a node box. Returns the entry distance in *tn and
 // whether the box must be visited for a best distance tbest.
 __device__ __forceinline__ bool box_visit(const DRay &r, float4 lo, float4 hi, float tbest) {
-  float t0x = (lo.x - r.ox) * r.ix, t1x = (hi.x - r.ox) * r.ix;
-  float t0y = (lo.y - r.oy) * r.iy, t1y = (hi.y - r.oy) * r.iy;
-  float t0z = (lo.z - r.oz) * r.iz, t1z = (hi.z - r.oz) * r.iz;
+  float t0x = __builtin_fmaf(lo.x, r.ix, r.nx), t1x = __builtin_fmaf(hi.x, r.ix, r.nx);
+  float t0y = __builtin_fmaf(lo.y, r.iy, r.ny), t1y = __builtin_fmaf(hi.y, r.iy, r.ny);
+  float t0z = __builtin_fmaf(lo.z, r.iz, r.nz), t1z = __builtin_fmaf(hi.z, r.iz, r.nz);
   float tn = fmaxf(fminf(t0x, t1x), fmaxf(fminf(t0y, t1y), fminf(t0z, t1z)));
   float tf = fminf(fmaxf(t0x, t1x), fminf(fmaxf(t0y, t1y), fmaxf(t0z, t1z)));
   return tn <= tf && tf > 0.0f && tn <= tbest;
@@ -105,10 +109,13 @@ __device__ __forceinline__ bool tri_hit(const DRay &r, float4 a, float4 b, float
   const float T = (U * Az + V * Bz) + W * Cz;
   const float t = T / det;
   if (!(t > 0.0f)) return false;
-  // Entry distance into the triangle's own bounding box.
-  const float ex = fminf(Ax0 * r.ix, fminf(Bx0 * r.ix, Cx0 * r.ix));
-  const float ey = fminf(Ay0 * r.iy, fminf(By0 * r.iy, Cy0 * r.iy));
-  const float ez = fminf(Az0 * r.iz, fminf(Bz0 * r.iz, Cz0 * r.iz));
+  // Entry distance into the triangle's own bounding box (the slab function).
+  const float ex = fminf(__builtin_fmaf(a.x, r.ix, r.nx),
+                         fminf(__builtin_fmaf(b.x, r.ix, r.nx), __builtin_fmaf(c.x, r.ix, r.nx)));
+  const float ey = fminf(__builtin_fmaf(a.y, r.iy, r.ny),
+                         fminf(__builtin_fmaf(b.y, r.iy, r.ny), __builtin_fmaf(c.y, r.iy, r.ny)));
+  const float ez = fminf(__builtin_fmaf(a.z, r.iz, r.nz),
+                         fminf(__builtin_fmaf(b.z, r.iz, r.nz), __builtin_fmaf(c.z, r.iz, r.nz)));
   const float e = fmaxf(ex, fmaxf(ey, ez));
   *tout = fmaxf(t, e);
   return true;
@@ -150,9 +157,9 @@ __device__ __forceinline__ int trace_closest(const float4 *__restrict__ nodes,
 // Slab test returning the entry distance.
 __device__ __forceinline__ bool box_enter(const DRay &r, float4 lo, float4 hi, float tbest,
                                           float *tn_out) {
-  float t0x = (lo.x - r.ox) * r.ix, t1x = (hi.x - r.ox) * r.ix;
-  float t0y = (lo.y - r.oy) * r.iy, t1y = (hi.y - r.oy) * r.iy;
-  float t0z = (lo.z - r.oz) * r.iz, t1z = (hi.z - r.oz) * r.iz;
+  float t0x = __builtin_fmaf(lo.x, r.ix, r.nx), t1x = __builtin_fmaf(hi.x, r.ix, r.nx);
+  float t0y = __builtin_fmaf(lo.y, r.iy, r.ny), t1y = __builtin_fmaf(hi.y, r.iy, r.ny);
+  float t0z = __builtin_fmaf(lo.z, r.iz, r.nz), t1z = __builtin_fmaf(hi.z, r.iz, r.nz);
   float tn = fmaxf(fminf(t0x, t1x), fmaxf(fminf(t0y, t1y), fminf(t0z, t1z)));
   float tf = fminf(fmaxf(t0x, t1x), fminf(fmaxf(t0y, t1y), fmaxf(t0z, t1z)));
   *tn_out = tn;
@@ -320,9 +327,9 @@ __device__ __forceinline__ void box4_enter(const DRay &r, const float4 *__restri
   const float rfa[4] = {rf.x, rf.y, rf.z, rf.w};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const float t0x = (lox[q] - r.ox) * r.ix, t1x = (hix[q] - r.ox) * r.ix;
-    const float t0y = (loy[q] - r.oy) * r.iy, t1y = (hiy[q] - r.oy) * r.iy;
-    const float t0z = (loz[q] - r.oz) * r.iz, t1z = (hiz[q] - r.oz) * r.iz;
+    const float t0x = __builtin_fmaf(lox[q], r.ix, r.nx), t1x = __builtin_fmaf(hix[q], r.ix, r.nx);
+    const float t0y = __builtin_fmaf(loy[q], r.iy, r.ny), t1y = __builtin_fmaf(hiy[q], r.iy, r.ny);
+    const float t0z = __builtin_fmaf(loz[q], r.iz, r.nz), t1z = __builtin_fmaf(hiz[q], r.iz, r.nz);
     const float tn = fmaxf(fminf(t0x, t1x), fmaxf(fminf(t0y, t1y), fminf(t0z, t1z)));
     const float tf = fminf(fmaxf(t0x, t1x), fminf(fmaxf(t0y, t1y), fmaxf(t0z, t1z)));
     const bool ok = q < cnt && tn <= tf && tf > 0.0f && tn <= tbest;

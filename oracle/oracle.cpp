@@ -117,6 +117,7 @@ namespace {
 struct Ray {
   float o[3];
   float inv[3];
+  float no[3];  // -o * inv: slab function s(x) = fma(x, inv, no)
   float S[3];
   int k[3];  // kx, ky, kz
 };
@@ -140,6 +141,7 @@ Ray setup(const float *o, const float *d) {
   r.S[1] = d[ky] / d[kz];
   r.S[2] = 1.0f / d[kz];
   for (int c = 0; c < 3; ++c) r.inv[c] = axis_inv(d[c]);
+  for (int c = 0; c < 3; ++c) r.no[c] = -(r.o[c] * r.inv[c]);
   return r;
 }
 
@@ -172,7 +174,9 @@ bool hit_tri(const Ray &r, const float *v0, const float *v1, const float *v2, fl
   if (!(t > 0.0f)) return false;
   float e = -FLT_MAX;
   for (int c = 0; c < 3; ++c) {
-    const float lo = std::fmin(A[c] * r.inv[c], std::fmin(B[c] * r.inv[c], C[c] * r.inv[c]));
+    const float lo = std::fmin(std::fma(v0[c], r.inv[c], r.no[c]),
+                               std::fmin(std::fma(v1[c], r.inv[c], r.no[c]),
+                                         std::fma(v2[c], r.inv[c], r.no[c])));
     e = std::fmax(e, lo);
   }
   *tout = std::fmax(t, e);
@@ -260,7 +264,8 @@ struct BVH {
   static bool box(const Ray &r, const Node &n, float tbest, float *tn_out) {
     float tn = -FLT_MAX, tf = FLT_MAX;
     for (int c = 0; c < 3; ++c) {
-      const float t0 = (n.lo[c] - r.o[c]) * r.inv[c], t1 = (n.hi[c] - r.o[c]) * r.inv[c];
+      const float t0 = std::fma(n.lo[c], r.inv[c], r.no[c]);
+      const float t1 = std::fma(n.hi[c], r.inv[c], r.no[c]);
       tn = std::fmax(tn, std::fmin(t0, t1));
       tf = std::fmin(tf, std::fmax(t0, t1));
     }
