@@ -183,9 +183,11 @@ static void destroy_bvh(illum_bvh *b) {
   delete b;
 }
 void destroy_sparse(illum_sparse *s) {
-  dfree(s->d_colptr); dfree(s->d_rowidx); dfree(s->d_cval);
-  dfree(s->d_rowptr); dfree(s->d_colidx); dfree(s->d_rval); dfree(s->d_rval32);
-  dfree(s->d_vis); dfree(s->d_nrays);
+  // the context's reuse pool takes the blocks (no hipFree, no device sync)
+  for (void *p : {(void *)s->d_colptr, (void *)s->d_rowidx, (void *)s->d_cval,
+                  (void *)s->d_rowptr, (void *)s->d_colidx, (void *)s->d_rval,
+                  (void *)s->d_rval32, (void *)s->d_vis, (void *)s->d_nrays})
+    pool_free(s->ctx, p);
   delete s;
 }
 void destroy_thermal(illum_thermal *t) {
@@ -268,6 +270,39 @@ static illum_ctx *default_ctx() {
   return ctx;
 }
 
+constexpr size_t POOL_MAX_BYTES = (size_t)8 << 30;
+
+void *pool_alloc(illum_ctx *ctx, size_t bytes) {
+  if (bytes == 0) bytes = 1;
+  auto it = ctx->pool.lower_bound(bytes);
+  if (it != ctx->pool.end() && it->first <= bytes + bytes / 2) {
+    void *p = it->second;
+    ctx->pool_bytes -= it->first;
+    ctx->pool.erase(it);
+    return p;
+  }
+  void *p = nullptr;
+  ILLUM_HIP(hipMalloc(&p, bytes));
+  ctx->pool_cap[p] = bytes;
+  return p;
+}
+
+void pool_free(illum_ctx *ctx, void *p) {
+  if (!p) return;
+  auto c = ctx->pool_cap.find(p);
+  if (c == ctx->pool_cap.end()) {
+    (void)hipFree(p);
+    return;
+  }
+  if (ctx->pool_bytes + c->second > POOL_MAX_BYTES) {
+    (void)hipFree(p);
+    ctx->pool_cap.erase(c);
+    return;
+  }
+  ctx->pool.emplace(c->second, p);
+  ctx->pool_bytes += c->second;
+}
+
 void *ctx_workspace(illum_ctx *ctx, int slot, size_t bytes) {
   if ((int)ctx->workspace.size() <= slot) ctx->workspace.resize(slot + 1, {nullptr, 0});
   auto &w = ctx->workspace[slot];
@@ -317,6 +352,7 @@ int illum_ctx_destroy(illum_ctx *ctx) {
     collect_profile(ctx);
     (void)hipStreamDestroy(stream_of(ctx));
     for (auto &w : ctx->workspace) (void)hipFree(w.first);
+    for (auto &b : ctx->pool) (void)hipFree(b.second);
     delete ctx;
   });
 }
@@ -630,10 +666,10 @@ int illum_sparse_create_csr(illum_ctx *ctx, int64_t n, int64_t nnz, const int64_
     sp->n = n;
     sp->nnz = nnz;
     hipStream_t s = stream_of(ctx);
-    sp->d_rowptr = dalloc<int64_t>(n + 1);
-    sp->d_colidx = dalloc<int32_t>(nnz);
-    sp->d_rval = dalloc<double>(nnz);
-    sp->d_rval32 = dalloc<float>(nnz);
+    sp->d_rowptr = palloc<int64_t>(ctx, n + 1);
+    sp->d_colidx = palloc<int32_t>(ctx, nnz);
+    sp->d_rval = palloc<double>(ctx, nnz);
+    sp->d_rval32 = palloc<float>(ctx, nnz);
     std::vector<float> v32(nnz);
     for (int64_t k = 0; k < nnz; ++k) v32[k] = (float)val[k];
     ILLUM_HIP(hipMemcpyAsync(sp->d_rowptr, rowptr, 8 * (n + 1), hipMemcpyHostToDevice, s));

@@ -557,13 +557,15 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   sp->ctx = ctx;
   sp->n = nf;
   sp->vis_words = nwords;
-  sp->d_vis = dalloc<uint32_t>((size_t)nf * nwords);
+  sp->d_vis = palloc<uint32_t>(ctx, (size_t)nf * nwords);
 
-  DBuf<double> d_alb;
+  // per-facet albedo and the NaN status word in the context's workspace
+  double *d_alb = nullptr;
   if (albedo_vec) {
-    d_alb.reset(nf);
-    ILLUM_HIP(hipMemcpyAsync(d_alb.p, albedo_vec, 8 * nf, hipMemcpyHostToDevice, s));
+    d_alb = (double *)ctx_workspace(ctx, WS_FF_ALBEDO, 8 * nf);
+    ILLUM_HIP(hipMemcpyAsync(d_alb, albedo_vec, 8 * nf, hipMemcpyHostToDevice, s));
   }
+  int *status = (int *)ctx_workspace(ctx, WS_FF_STATUS, 4);
   // M and its transpose live in the context's workspace.  Rows >= nf of M
   // (padding to whole 512-row tiles) must read as zero for the transpose.
   const size_t mbytes = (size_t)rows_p * nwords * 4;
@@ -573,11 +575,10 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   int32_t *counts = (int32_t *)ctx_workspace(ctx, WS_FF_COUNTS, 8 * rows_p);
   int32_t *rcounts = counts + rows_p;
   ILLUM_HIP(hipMemsetAsync(rcounts, 0, 4 * nf, s));
-  DBuf<int> status(1);
   if (rows_p > nf)
     ILLUM_HIP(hipMemsetAsync(M + (size_t)nf * nwords, 0, (size_t)(rows_p - nf) * nwords * 4, s));
-  ILLUM_HIP(hipMemsetAsync(status.p, 0, 4, s));
-  sp->d_nrays = dalloc<unsigned long long>(1);
+  ILLUM_HIP(hipMemsetAsync(status, 0, 4, s));
+  sp->d_nrays = palloc<unsigned long long>(ctx, 1);
   ILLUM_HIP(hipMemsetAsync(sp->d_nrays, 0, 8, s));
 
   {
@@ -587,14 +588,14 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
         (int)nf, nwords, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4,
         (const float4 *)bvh->d_nodes, (const float4 *)bvh->d_wide, (const float4 *)bvh->d_wide4,
         (const float4 *)bvh->d_tris,
-        (float)offset, mesh->d_area, d_alb.p, albedo, sp->d_vis, M, counts, status.p,
+        (float)offset, mesh->d_area, d_alb, albedo, sp->d_vis, M, counts, status,
         (int)nf, 1, sp->d_nrays, cluster_cull() ? (const float4 *)mesh->d_clus : nullptr,
         (int)mesh->nclus, stack_slots,
         mesh->d_body, mesh->d_boff, bvh->d_roots, rcounts);
     ILLUM_LAUNCH_CHECK();
   }
   // CSC
-  sp->d_colptr = dalloc<int64_t>(nf + 1);
+  sp->d_colptr = palloc<int64_t>(ctx, nf + 1);
   {
     StageTimer st(ctx, "ff_csc");
     scan_counts(ctx, counts, nf, sp->d_colptr);
@@ -602,27 +603,27 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   int64_t nnz = 0;
   int hstatus = 0;
   ILLUM_HIP(hipMemcpyAsync(&nnz, sp->d_colptr + nf, 8, hipMemcpyDeviceToHost, s));
-  ILLUM_HIP(hipMemcpyAsync(&hstatus, status.p, 4, hipMemcpyDeviceToHost, s));
+  ILLUM_HIP(hipMemcpyAsync(&hstatus, status, 4, hipMemcpyDeviceToHost, s));
   sync(ctx);
   if (hstatus) throw Error(ILLUM_ERR_NAN, "computed a NaN form factor");
   if (nnz >= ((int64_t)1 << 40)) throw Error(ILLUM_ERR_CAPACITY, "nnz out of range");
   sp->nnz = nnz;
   sp->has_csc = true;
-  sp->d_rowidx = dalloc<int32_t>(nnz);
-  sp->d_cval = dalloc<double>(nnz);
+  sp->d_rowidx = palloc<int32_t>(ctx, nnz);
+  sp->d_cval = palloc<double>(ctx, nnz);
   {
     StageTimer st(ctx, "ff_csc");
     ff_fill_kernel<<<(int)nf, 256, 0, s>>>((int)nf, nwords, M, sp->d_colptr, 0,
                                            (const float4 *)mesh->d_pr4,
-                                           (const float4 *)mesh->d_n4, mesh->d_area, d_alb.p,
+                                           (const float4 *)mesh->d_n4, mesh->d_area, d_alb,
                                            albedo, sp->d_rowidx, sp->d_cval, nullptr, nullptr);
     ILLUM_LAUNCH_CHECK();
   }
   // CSR via the bit transpose.
-  sp->d_rowptr = dalloc<int64_t>(nf + 1);
-  sp->d_colidx = dalloc<int32_t>(nnz);
-  sp->d_rval = dalloc<double>(nnz);
-  sp->d_rval32 = dalloc<float>(nnz);
+  sp->d_rowptr = palloc<int64_t>(ctx, nf + 1);
+  sp->d_colidx = palloc<int32_t>(ctx, nnz);
+  sp->d_rval = palloc<double>(ctx, nnz);
+  sp->d_rval32 = palloc<float>(ctx, nnz);
   {
     StageTimer st(ctx, "ff_csr");
     dim3 g(nwords / 16, (unsigned)(rows_p / 512));
@@ -632,7 +633,7 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
     scan_counts(ctx, rcounts, nf, sp->d_rowptr);
     ff_fill_kernel<<<(int)nf, 256, 0, s>>>((int)nf, nwords, MT, sp->d_rowptr, 1,
                                            (const float4 *)mesh->d_pr4,
-                                           (const float4 *)mesh->d_n4, mesh->d_area, d_alb.p,
+                                           (const float4 *)mesh->d_n4, mesh->d_area, d_alb,
                                            albedo, sp->d_colidx, sp->d_rval, sp->d_rval32,
                                            tile_nz);
     ILLUM_LAUNCH_CHECK();

@@ -90,7 +90,20 @@ enum WorkspaceSlot {
   WS_RAD_D32 = 8,   // FP32 correction
   WS_SCAN_PART = 9, // scan_counts block totals
   WS_FF_TILES = 10, // nonzero flags of the bit transpose's 512 x 512 tiles
+  WS_FF_ALBEDO = 11,
+  WS_FF_STATUS = 12,
 };
+
+// Device blocks from the context's reuse pool (see illum_ctx::pool): a free
+// block of capacity in [bytes, 1.5 bytes] if there is one, else a new one.
+// pool_free returns a block to the pool (or frees it past the pool's cap;
+// blocks not from pool_alloc are freed).
+void *pool_alloc(illum_ctx *ctx, size_t bytes);
+void pool_free(illum_ctx *ctx, void *p);
+template <class T>
+T *palloc(illum_ctx *ctx, size_t n) {
+  return (T *)pool_alloc(ctx, (n ? n : 1) * sizeof(T));
+}
 
 // Drains pending stage events into the profile (after the stream is synced).
 void collect_profile(illum_ctx *ctx);

@@ -99,6 +99,13 @@ struct illum_ctx {
   // repeated calls skip large allocations.  Calls on one context are issued
   // from one host thread at a time, in stream order.
   std::vector<std::pair<void *, size_t>> workspace;
+  // Reuse pool for the buffers of form-factor matrices: a destroyed
+  // matrix's blocks stay here and the next assembly takes them (best fit),
+  // so a loop of assemblies allocates and frees nothing.  Blocks are reused
+  // in stream order on this context's stream.
+  std::multimap<size_t, void *> pool;   // capacity -> free block
+  std::map<void *, size_t> pool_cap;    // every block from pool_alloc -> capacity
+  size_t pool_bytes = 0;                // bytes held free in `pool`
 };
 
 struct illum_mesh {

@@ -74,7 +74,11 @@ typedef struct illum_thermal illum_thermal;
 const char *illum_last_error(void);
 int illum_version(void);
 
-/* ---- context: one HIP device + one stream; calls on a context are serialised */
+/* ---- context: one HIP device + one stream; calls on a context are serialised.
+ * Handles made on a context (meshes, trees, matrices, thermal models) are
+ * destroyed before it.  The context keeps the device buffers of destroyed
+ * sparse matrices for reuse by the next assembly (up to 8 GiB), and its
+ * scratch workspaces, until illum_ctx_destroy. */
 int illum_ctx_create(int device, illum_ctx **out);
 int illum_ctx_destroy(illum_ctx *ctx);
 int illum_ctx_synchronize(illum_ctx *ctx);
