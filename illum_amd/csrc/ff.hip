@@ -209,29 +209,59 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
     __syncthreads();
   }
 
-  // Write the raw visibility row and its nonzero-form-factor filter.
+  // Write the raw visibility row and its nonzero-form-factor filter.  A
+  // row's set bits cluster in a few words, so their form factors are
+  // evaluated one per thread from a list (in the now idle ray queue) rather
+  // than by the words' owners.
+  //   1. Each thread reads and clears its words, writes the raw row and lists
+  //      the set bits; bits beyond the queue's capacity are evaluated by the
+  //      owner at once, into its own (cleared) word.
+  //   2. The listed receivers are evaluated, setting their filter bits by LDS
+  //      atomics (no word is read as a visibility word any more).
+  //   3. The filtered row is written.
+  // The filter bits and counts do not depend on who evaluates what.
   const double Aj = area[j];
-  int cnt = 0;
   bool nan = false;
+  auto filter = [&](int i) -> bool {
+    const double rho = albedo_vec ? albedo_vec[i] : albedo;
+    const double v = ff_value(pr4[i], n4[i], pj, nj, Aj) * rho;
+    if (v != v) nan = true;
+    if (v == 0.0) return false;
+    // receiver i's row count of the transposed matrix (the CSR row
+    // lengths), so no pass over the transposed bits is needed for them
+    if (col_counts) atomicAdd(&col_counts[i], 1);
+    return true;
+  };
+  if (tid == 0) s_qn = 0;
+  __syncthreads();
   for (int w = tid; w < nwords; w += FF_BLOCK) {
     const uint32_t word = bits[w];
+    vis[(size_t)row * nwords + w] = word;
     uint32_t keep = 0u;
-    uint32_t rest = word;
-    while (rest) {
-      const int b = __ffs(rest) - 1;
-      rest &= rest - 1u;
-      const int i = 32 * w + b;
-      const double rho = albedo_vec ? albedo_vec[i] : albedo;
-      const double v = ff_value(pr4[i], n4[i], pj, nj, Aj) * rho;
-      if (v != v) nan = true;
-      if (v != 0.0) {
-        keep |= 1u << b;
-        // receiver i's row count of the transposed matrix (the CSR row
-        // lengths), so no pass over the transposed bits is needed for them
-        if (col_counts) atomicAdd(&col_counts[i], 1);
+    if (word) {
+      int slot = atomicAdd(&s_qn, __popc(word));
+      uint32_t rest = word;
+      while (rest) {
+        const int b = __ffs(rest) - 1;
+        rest &= rest - 1u;
+        const int i = 32 * w + b;
+        if (slot < FF_QCAP) queue[slot] = i;
+        else if (filter(i)) keep |= 1u << b;
+        ++slot;
       }
     }
-    vis[(size_t)row * nwords + w] = word;
+    bits[w] = keep;
+  }
+  __syncthreads();
+  const int nlist = min(s_qn, FF_QCAP);
+  for (int e = tid; e < nlist; e += FF_BLOCK) {
+    const int i = queue[e];
+    if (filter(i)) atomicOr(&bits[i >> 5], 1u << (i & 31));
+  }
+  __syncthreads();
+  int cnt = 0;
+  for (int w = tid; w < nwords; w += FF_BLOCK) {
+    const uint32_t keep = bits[w];
     mask[(size_t)row * nwords + w] = keep;
     cnt += __popc(keep);
   }
