@@ -270,6 +270,36 @@ static illum_ctx *default_ctx() {
   return ctx;
 }
 
+std::vector<hipStream_t> fork_streams(illum_ctx *ctx, int n) {
+  if (!ctx->fork_event) {
+    hipEvent_t e;
+    ILLUM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    ctx->fork_event = e;
+  }
+  while ((int)ctx->aux_streams.size() < n) {
+    hipStream_t s;
+    ILLUM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    ctx->aux_streams.push_back(s);
+    hipEvent_t e;
+    ILLUM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    ctx->aux_events.push_back(e);
+  }
+  ILLUM_HIP(hipEventRecord((hipEvent_t)ctx->fork_event, stream_of(ctx)));
+  std::vector<hipStream_t> out(n);
+  for (int k = 0; k < n; ++k) {
+    out[k] = (hipStream_t)ctx->aux_streams[k];
+    ILLUM_HIP(hipStreamWaitEvent(out[k], (hipEvent_t)ctx->fork_event, 0));
+  }
+  return out;
+}
+
+void join_streams(illum_ctx *ctx, int n) {
+  for (int k = 0; k < n; ++k) {
+    ILLUM_HIP(hipEventRecord((hipEvent_t)ctx->aux_events[k], (hipStream_t)ctx->aux_streams[k]));
+    ILLUM_HIP(hipStreamWaitEvent(stream_of(ctx), (hipEvent_t)ctx->aux_events[k], 0));
+  }
+}
+
 constexpr size_t POOL_MAX_BYTES = (size_t)8 << 30;
 
 void *pool_alloc(illum_ctx *ctx, size_t bytes) {
@@ -350,7 +380,11 @@ int illum_ctx_destroy(illum_ctx *ctx) {
     if (!ctx) return;
     (void)hipStreamSynchronize(stream_of(ctx));
     collect_profile(ctx);
+    for (void *a : ctx->aux_streams) (void)hipStreamSynchronize((hipStream_t)a);
     (void)hipStreamDestroy(stream_of(ctx));
+    for (void *a : ctx->aux_streams) (void)hipStreamDestroy((hipStream_t)a);
+    for (void *e : ctx->aux_events) (void)hipEventDestroy((hipEvent_t)e);
+    if (ctx->fork_event) (void)hipEventDestroy((hipEvent_t)ctx->fork_event);
     for (auto &w : ctx->workspace) (void)hipFree(w.first);
     for (auto &b : ctx->pool) (void)hipFree(b.second);
     delete ctx;

@@ -105,6 +105,13 @@ T *palloc(illum_ctx *ctx, size_t n) {
   return (T *)pool_alloc(ctx, (n ? n : 1) * sizeof(T));
 }
 
+// Fork / join of independent launches: fork_streams makes n auxiliary
+// streams wait for the work queued so far on the context stream and returns
+// them; join_streams makes the context stream wait for everything queued on
+// them since.  A call that forks joins before it returns.
+std::vector<hipStream_t> fork_streams(illum_ctx *ctx, int n);
+void join_streams(illum_ctx *ctx, int n);
+
 // Drains pending stage events into the profile (after the stream is synced).
 void collect_profile(illum_ctx *ctx);
 void sync(illum_ctx *ctx);

@@ -91,6 +91,11 @@ HostBVH build_bvh(const HostGeometry &g, int leaf_size);
 struct illum_ctx {
   int device = 0;
   void *stream = nullptr;              // hipStream_t
+  // Auxiliary streams (default priority) and their join events, made on
+  // first use, for independent launches forked from `stream` and joined
+  // back into it within one call (the conduction depth buckets).
+  std::vector<void *> aux_streams, aux_events;
+  void *fork_event = nullptr;
   bool profiling = false;
   struct Stage { double ms = 0; int64_t launches = 0; };
   std::map<std::string, Stage> profile;

@@ -205,10 +205,10 @@ template <int NZ>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ), cn_waves(NZ)))) void cn_mixed_kernel(
     const int32_t *__restrict__ fidx, const int32_t *__restrict__ wgrid, int64_t block0,
     int64_t nf, int64_t nslot, const double *__restrict__ s2f, const double *__restrict__ hdr,
-    const double *__restrict__ coef, const float *__restrict__ coef32,
-    const int64_t *__restrict__ goff, double emiss, double Fgeo, int nsteps,
-    const double *__restrict__ Q, double *__restrict__ Qprev, double *__restrict__ T,
-    double *__restrict__ Fs, double *__restrict__ Tsurf_series) {
+    const double *__restrict__ coef, const float4 *__restrict__ coefp,
+    const int64_t *__restrict__ goff, const int64_t *__restrict__ goffp, double emiss,
+    double Fgeo, int nsteps, const double *__restrict__ Q, double *__restrict__ Qprev,
+    double *__restrict__ T, double *__restrict__ Fs, double *__restrict__ Tsurf_series) {
   const int64_t blk = block0 + blockIdx.x;
   const int64_t slot = blk * 64 + threadIdx.x;
   const int f = fidx[slot];
@@ -217,7 +217,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ)
   const GridHdr H = *(const GridHdr *)(hdr + 6 * g);
   const int nz = (int)H.nz;  // uniform across the wavefront, <= NZ
   const double *al = coef + 2 * goff[g];
-  const float *c32 = coef32 + 3 * goff[g];  // per layer: alpha, gamma, alpha + gamma
+  // per layer j (entry j - 1): alpha, gamma, alpha + gamma, 1 on the bottom row;
+  // zeros past the grid's nz up to NZ (the bucket's depth)
+  const float4 *cq = coefp + goffp[g];
   const double s2 = (act && s2f) ? s2f[f] : 1.0;
   const float s2s = (float)s2;
   const double k1 = s2 * H.k1, kfirst = s2 * H.kfirst;
@@ -239,11 +241,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ)
 #pragma unroll
     for (int j = NZ; j >= 2; --j) {
       if (j <= nz) {
-        const float al_j = c32[3 * (j - 1)], ga_j = c32[3 * (j - 1) + 1];
-        const float bj = __builtin_fmaf(s2s, c32[3 * (j - 1) + 2], 1.0f);
-        const float beta = __builtin_fmaf(-s2s * al_j, Mn, bj);
+        const float4 c = cq[j - 1];
+        const float bj = __builtin_fmaf(s2s, c.z, 1.0f);
+        const float beta = __builtin_fmaf(-s2s * c.x, Mn, bj);
         Pp[j] = s2s * __builtin_amdgcn_rcpf(beta);
-        Mn = ga_j * Pp[j];
+        Mn = c.y * Pp[j];
+      } else {
+        Pp[j] = 0.0f;  // rows past nz: z_j = d_j = 0, T_j stays 0
       }
     }
     M2 = Mn;
@@ -254,20 +258,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ)
     const double Qnp1 = act ? Q[(int64_t)step * nf + f] : 0.0;
     double ann, annp1, bn;
     surface_bc(Tl[0], Qn, Qnp1, k1, emiss, &ann, &annp1, &bn);
+    // The sweeps run over all NZ rows without branches: rows past nz have
+    // zero coefficients and pivots, so they contribute exact zeros, and the
+    // bottom row's alpha is 0, so the row below it does not reach it.  The
+    // coefficients are reloaded every step (scalar loads) rather than held in
+    // scalar registers across the time loop.
+    __asm__ volatile("" ::: "memory");
     float z[NZ + 1];
     float znext = 0.0f, enext = 0.0f;  // z and T difference of row j+1
 #pragma unroll
     for (int j = NZ; j >= 2; --j) {
-      if (j <= nz) {
-        const float al_j = c32[3 * (j - 1)], ga_j = c32[3 * (j - 1) + 1];
-        const float ej = (float)(Tl[j] - Tl[j - 1]);
-        float w = __builtin_fmaf(al_j, enext, -ga_j * ej);
-        if (j == nz) w += geo_z;
-        // one fused multiply-add on the z recurrence; the rest overlaps
-        z[j] = __builtin_fmaf(al_j * Pp[j], znext, w * Pp[j]);
-        znext = z[j];
-        enext = ej;
-      }
+      const float4 c = cq[j - 1];
+      const float ej = (float)(Tl[j] - Tl[j - 1]);
+      float w = __builtin_fmaf(c.x, enext, -c.y * ej);
+      w = __builtin_fmaf(geo_z, c.w, w);  // geothermal term on the bottom row
+      // one fused multiply-add on the z recurrence; the rest overlaps
+      z[j] = __builtin_fmaf(c.x * Pp[j], znext, w * Pp[j]);
+      znext = z[j];
+      enext = ej;
       if ((j & 7) == 0) __builtin_amdgcn_sched_barrier(0);
     }
     {
@@ -281,10 +289,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ)
     Tl[1] = __builtin_fma(2.0, (double)d, Tl[1]);
 #pragma unroll
     for (int j = 2; j <= NZ; ++j) {
-      if (j <= nz) {
-        d = __builtin_fmaf(c32[3 * (j - 1) + 1] * Pp[j], d, z[j]);
-        Tl[j] = __builtin_fma(2.0, (double)d, Tl[j]);
-      }
+      d = __builtin_fmaf(cq[j - 1].y * Pp[j], d, z[j]);
+      Tl[j] = __builtin_fma(2.0, (double)d, Tl[j]);
       if ((j & 7) == 0) __builtin_amdgcn_sched_barrier(0);
     }
     const double T1 = Tl[1];
@@ -313,11 +319,16 @@ __global__ __launch_bounds__(64) void cn_stream_kernel(
     const double *__restrict__ coef, const int64_t *__restrict__ goff, double emiss, double Fgeo,
     int nsteps, const double *__restrict__ Q, double *__restrict__ Qprev,
     double *__restrict__ T, double *__restrict__ Fs, double *__restrict__ G,
-    float *__restrict__ G32, double *__restrict__ Tsurf_series, int *__restrict__ status) {
+    float *__restrict__ G32, double *__restrict__ Tsurf_series, int *__restrict__ status,
+    int rows) {
   const int64_t blk = block0 + blockIdx.x;
   const int64_t slot = blk * 64 + threadIdx.x;
   const int f = fidx[slot];
   if (f < 0) return;
+  // FP64 multipliers: this wavefront's own block of the scratch, element
+  // [row][lane] (the layout of cn_fp64_kernel's scratch, so concurrently
+  // running depth buckets never share scratch words)
+  double *Gw = G ? G + blk * (int64_t)rows * 64 + threadIdx.x : nullptr;
   const int g = wgrid[blk];
   const GridHdr H = *(const GridHdr *)(hdr + 6 * g);
   const int nz = (int)H.nz;
@@ -364,7 +375,7 @@ __global__ __launch_bounds__(64) void cn_stream_kernel(
           u = r / bet;
         } else {
           const double gm = cprev / bet;
-          G[(int64_t)j * nslot + slot] = gm;
+          Gw[(int64_t)j * 64] = gm;
           bet = bj - a_j * gm;
           u = (r - a_j * uprev) / bet;
         }
@@ -393,7 +404,7 @@ __global__ __launch_bounds__(64) void cn_stream_kernel(
     if (!MIXED) {
       double unext = uprev;
       for (int j = nz - 1; j >= 1; --j) {
-        const double u = Tat(j) - G[(int64_t)(j + 1) * nslot + slot] * unext;
+        const double u = Tat(j) - Gw[(int64_t)(j + 1) * 64] * unext;
         Tat(j) = u;
         unext = u;
       }
@@ -463,13 +474,33 @@ __global__ void fill_kernel(int64_t n, double v, double *__restrict__ p) {
 namespace {
 
 // Host tables for a given dt, conductionQ.f:50-65 operation by operation.
+// Register buckets (layer capacity of the cn_*_kernel<NZ> instantiations); 0
+// means the streaming kernel.
+constexpr int BUCKETS[] = {32, 48, 64, 80, 96, 128};
+constexpr int NBUCKET = (int)(sizeof(BUCKETS) / sizeof(BUCKETS[0]));
+
+int bucket_of(int nz) {
+  for (int b : BUCKETS)
+    if (nz <= b) return b;
+  return 0;
+}
+
 void make_tables(const std::vector<int64_t> &goff, const double *z, const double *ti,
                  const double *rhoc, double dt, std::vector<double> &hdr,
-                 std::vector<double> &coef, std::vector<float> &coef32) {
+                 std::vector<double> &coef, std::vector<float> &coefp,
+                 std::vector<int64_t> &goffp) {
   const int ng = (int)goff.size() - 1;
   hdr.assign(6 * ng, 0.0);
   coef.assign(2 * goff[ng], 0.0);
-  coef32.assign(3 * goff[ng], 0.0f);
+  // FP32 table of the mixed kernel: 4 floats per row, each grid padded with
+  // zero rows to its register bucket's depth
+  goffp.assign(ng + 1, 0);
+  for (int g = 0; g < ng; ++g) {
+    const int nz = (int)(goff[g + 1] - goff[g]);
+    const int nb = bucket_of(nz);
+    goffp[g + 1] = goffp[g] + (nb ? nb : nz);
+  }
+  coefp.assign(4 * goffp[ng], 0.0f);
   for (int g = 0; g < ng; ++g) {
     const int64_t o = goff[g];
     const int nz = (int)(goff[g + 1] - o);
@@ -491,9 +522,11 @@ void make_tables(const std::vector<int64_t> &goff, const double *z, const double
     for (int i = 0; i < nz; ++i) {
       coef[2 * (o + i)] = alpha[i];
       coef[2 * (o + i) + 1] = gamma[i];
-      coef32[3 * (o + i)] = (float)alpha[i];
-      coef32[3 * (o + i) + 1] = (float)gamma[i];
-      coef32[3 * (o + i) + 2] = (float)(alpha[i] + gamma[i]);
+      float *cp = &coefp[4 * (goffp[g] + i)];
+      cp[0] = (float)alpha[i];  // 0 on the bottom row (no row below it)
+      cp[1] = (float)gamma[i];
+      cp[2] = (float)(alpha[i] + gamma[i]);
+      cp[3] = i == nz - 1 ? 1.0f : 0.0f;
     }
     double *h = &hdr[6 * g];
     h[0] = k[0] / dz;
@@ -505,15 +538,6 @@ void make_tables(const std::vector<int64_t> &goff, const double *z, const double
   }
 }
 
-// Register buckets (layer capacity of cn_reg_kernel instantiations); 0 means
-// the streaming kernel.
-constexpr int BUCKETS[] = {32, 64, 96, 128};
-
-int bucket_of(int nz) {
-  for (int b : BUCKETS)
-    if (nz <= b) return b;
-  return 0;
-}
 
 }  // namespace
 
@@ -521,8 +545,9 @@ struct ThermalImpl {
   std::vector<double> z, ti, rhoc;
   double dt = -1;
   double *d_hdr = nullptr, *d_coef = nullptr;
-  float *d_coef32 = nullptr;
-  ~ThermalImpl() { dfree(d_hdr); dfree(d_coef); dfree(d_coef32); }
+  float *d_coefp = nullptr;
+  int64_t *d_goffp = nullptr;
+  ~ThermalImpl() { dfree(d_hdr); dfree(d_coef); dfree(d_coefp); dfree(d_goffp); }
 };
 
 static ThermalImpl *impl_of(illum_thermal *th) { return (ThermalImpl *)th->impl; }
@@ -579,8 +604,8 @@ illum_thermal *thermal_create(illum_ctx *ctx, int64_t nf, int ngrid, const int64
   // padded to whole wavefronts.
   std::vector<int32_t> fidx, wgrid, pos(nf, -1);
   bool need_stream = false;
-  for (int bi = 0; bi <= 4; ++bi) {
-    const int bucket = bi < 4 ? BUCKETS[bi] : 0;
+  for (int bi = 0; bi <= NBUCKET; ++bi) {
+    const int bucket = bi < NBUCKET ? BUCKETS[bi] : 0;
     const int64_t block0 = (int64_t)wgrid.size();
     for (int g = 0; g < ngrid; ++g) {
       if (members[g].empty() || bucket_of((int)(goff[g + 1] - goff[g])) != bucket) continue;
@@ -663,28 +688,49 @@ static void ensure_tables(illum_thermal *th, double dt) {
   ThermalImpl *im = impl_of(th);
   if (im->dt == dt && im->d_hdr) return;
   std::vector<double> hdr, coef;
-  std::vector<float> coef32;
-  make_tables(th->goff, im->z.data(), im->ti.data(), im->rhoc.data(), dt, hdr, coef, coef32);
+  std::vector<float> coefp;
+  std::vector<int64_t> goffp;
+  make_tables(th->goff, im->z.data(), im->ti.data(), im->rhoc.data(), dt, hdr, coef, coefp,
+              goffp);
   dfree(im->d_hdr);
   dfree(im->d_coef);
-  dfree(im->d_coef32);
+  dfree(im->d_coefp);
+  dfree(im->d_goffp);
   im->d_hdr = dalloc<double>(hdr.size());
   im->d_coef = dalloc<double>(coef.size());
-  im->d_coef32 = dalloc<float>(coef32.size());
+  im->d_coefp = dalloc<float>(coefp.size());
+  im->d_goffp = dalloc<int64_t>(goffp.size());
   hipStream_t s = stream_of(th->ctx);
   ILLUM_HIP(hipMemcpyAsync(im->d_hdr, hdr.data(), 8 * hdr.size(), hipMemcpyHostToDevice, s));
   ILLUM_HIP(hipMemcpyAsync(im->d_coef, coef.data(), 8 * coef.size(), hipMemcpyHostToDevice, s));
-  ILLUM_HIP(hipMemcpyAsync(im->d_coef32, coef32.data(), 4 * coef32.size(), hipMemcpyHostToDevice, s));
+  ILLUM_HIP(hipMemcpyAsync(im->d_coefp, coefp.data(), 4 * coefp.size(), hipMemcpyHostToDevice, s));
+  ILLUM_HIP(hipMemcpyAsync(im->d_goffp, goffp.data(), 8 * goffp.size(), hipMemcpyHostToDevice, s));
   sync(th->ctx);
   im->dt = dt;
 }
 
 template <bool MIXED>
-static void launch_all(illum_thermal *th, ThermalImpl *im, hipStream_t s, double Fgeo, int nsteps,
-                       const double *dQ, double *Tss) {
+static void launch_all(illum_thermal *th, ThermalImpl *im, hipStream_t s0, double Fgeo,
+                       int nsteps, const double *dQ, double *Tss) {
   const char *eng = getenv("ILLUM_CN_ENGINE");
   const bool force_stream = eng && std::string(eng) == "stream";
-  for (const auto &L : th->launches) {
+  // The depth buckets touch disjoint slots and facets.  At one or two
+  // wavefronts per SIMD a bucket's launch ends with a partly idle last round,
+  // so with several buckets they run concurrently on up to three forked
+  // streams (largest first), each filling the others' tails.
+  const int nl = (int)th->launches.size();
+  const int nfork = nl > 1 ? std::min(nl, 3) : 0;
+  std::vector<hipStream_t> fs;
+  if (nfork) fs = fork_streams(th->ctx, nfork);
+  std::vector<int> order(nl);
+  for (int i = 0; i < nl; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int a, int b) {
+    const auto &A = th->launches[a], &B = th->launches[b];
+    return A.nblocks * std::max(A.nz_bucket, 128) > B.nblocks * std::max(B.nz_bucket, 128);
+  });
+  for (int oi = 0; oi < nl; ++oi) {
+    const auto &L = th->launches[order[oi]];
+    const hipStream_t s = nfork ? fs[oi % nfork] : s0;
 #define ILLUM_CN_ARGS                                                                        \
   th->d_fidx, th->d_wgrid, L.block0, th->nf, th->nslot, th->d_s2, im->d_hdr, im->d_coef,     \
       th->d_goff, th->emiss, Fgeo, nsteps, dQ, th->d_Qprev, th->d_T, th->d_F
@@ -695,24 +741,27 @@ static void launch_all(illum_thermal *th, ThermalImpl *im, hipStream_t s, double
     if (MIXED)                                                                               \
       cn_mixed_kernel<NZ><<<nb, 64, 0, s>>>(                                                 \
           th->d_fidx, th->d_wgrid, L.block0, th->nf, th->nslot, th->d_s2, im->d_hdr,         \
-          im->d_coef, im->d_coef32, th->d_goff, th->emiss, Fgeo, nsteps, dQ, th->d_Qprev,    \
-          th->d_T, th->d_F, Tss);                                                            \
+          im->d_coef, (const float4 *)im->d_coefp, th->d_goff, im->d_goffp, th->emiss, Fgeo, \
+          nsteps, dQ, th->d_Qprev, th->d_T, th->d_F, Tss);                                   \
     else                                                                                     \
       cn_fp64_kernel<NZ, GMREG><<<nb, 64, 0, s>>>(ILLUM_CN_ARGS, th->d_work, Tss,            \
                                                   th->d_status, th->nzmax + 1);              \
     break;
       ILLUM_CN_REG(32, true)
+      ILLUM_CN_REG(48, true)
       ILLUM_CN_REG(64, true)
+      ILLUM_CN_REG(80, true)
       ILLUM_CN_REG(96, true)
       ILLUM_CN_REG(128, false)
 #undef ILLUM_CN_REG
       default:
         cn_stream_kernel<MIXED><<<nb, 64, 0, s>>>(ILLUM_CN_ARGS, th->d_work, th->d_work32, Tss,
-                                                  th->d_status);
+                                                  th->d_status, th->nzmax + 1);
     }
 #undef ILLUM_CN_ARGS
     ILLUM_LAUNCH_CHECK();
   }
+  if (nfork) join_streams(th->ctx, nfork);
 }
 
 static void check_status(illum_thermal *th) {
