@@ -81,7 +81,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ)
     int nsteps, const double *__restrict__ Q, double *__restrict__ Qprev,
     double *__restrict__ T, double *__restrict__ Fs, double *__restrict__ G,
     double *__restrict__ Tsurf_series, int *__restrict__ status, int rows) {
-  __shared__ double s_al[NZ], s_ga[NZ];
+  __shared__ double s_al[NZ], s_ga[NZ], s_geo[NZ];
   const int64_t blk = block0 + blockIdx.x;
   const int64_t slot = blk * 64 + threadIdx.x;
   const int f = fidx[slot];
@@ -90,14 +90,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ)
   const GridHdr H = *(const GridHdr *)(hdr + 6 * g);
   const int nz = (int)H.nz;  // uniform across the wavefront, <= NZ
   const double *al = coef + 2 * goff[g];
-  for (int j = threadIdx.x; j < nz; j += 64) {
-    s_al[j] = al[2 * j];
-    s_ga[j] = al[2 * j + 1];
+  const double geo_term = (H.geo * Fgeo) / H.dzlast;
+  // rows past nz: zero coefficients (and the geothermal term on row nz only)
+  for (int j = threadIdx.x; j < NZ; j += 64) {
+    s_al[j] = j < nz ? al[2 * j] : 0.0;
+    s_ga[j] = j < nz ? al[2 * j + 1] : 0.0;
+    s_geo[j] = j == nz - 1 ? geo_term : 0.0;
   }
   __syncthreads();
   const double s2 = (act && s2f) ? s2f[f] : 1.0;
   const double k1 = s2 * H.k1, kfirst = s2 * H.kfirst;
-  const double geo_term = (H.geo * Fgeo) / H.dzlast;
   // Wavefront-private scratch block, element [row][lane]: the per-layer offset
   // is a compile-time immediate on one base address.
   double *Gw = GMREG ? nullptr : G + blk * (int64_t)rows * 64 + threadIdx.x;
@@ -118,17 +120,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ)
     const double b1 = ((1.0 + a1) + g1) - g1 * bn;
     if (b1 == 0.0) zero_pivot = true;
     double bet = 0.0, uprev = 0.0, cprev = 0.0, Told = 0.0;
+    // With the multipliers in registers (NZ <= 96) the sweeps run over all NZ
+    // rows without branches.  Rows past nz have zero coefficients and T = 0,
+    // so they stay 0 and, the bottom row's alpha being 0, never reach row nz.
+    // The interior formula with alpha = 0 plus the geothermal term is the
+    // bottom row's r and b term for term, so every real row is computed
+    // exactly as the reference's tridag step computes it.
 #pragma unroll
     for (int j = 1; j <= NZ; ++j) {
-      if (j <= nz) {
+      if (GMREG || j <= nz) {
         const double Tc = Tl[j];
-        const double Tn = j < nz ? Tl[j + 1] : 0.0;
+        const double Tn = (GMREG || j < nz) ? Tl[j + 1] : 0.0;
         const double aj = j == 1 ? a1 : s2 * s_al[j - 1];
         const double gj = j == 1 ? g1 : s2 * s_ga[j - 1];
         double r, bj;
         if (j == 1) {
           r = (g1 * (annp1 + ann) + (((1.0 - a1) - g1) + g1 * bn) * Tc) + a1 * Tn;
           bj = b1;
+        } else if (GMREG) {
+          r = ((gj * Told + ((1.0 - aj) - gj) * Tc) + aj * Tn) + s_geo[j - 1];
+          bj = (1.0 + aj) + gj;
         } else if (j < nz) {
           r = (gj * Told + ((1.0 - aj) - gj) * Tc) + aj * Tn;
           bj = (1.0 + aj) + gj;
@@ -159,7 +170,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ)
     }
 #pragma unroll
     for (int j = NZ - 1; j >= 1; --j) {
-      if (j < nz) Tl[j] = Tl[j] - (GMREG ? Gm[j + 1] : Gw[(j + 1) * 64]) * Tl[j + 1];
+      if (GMREG || j < nz) Tl[j] = Tl[j] - (GMREG ? Gm[j + 1] : Gw[(j + 1) * 64]) * Tl[j + 1];
       if ((j & 7) == 0) __builtin_amdgcn_sched_barrier(0);
     }
     const double T1 = Tl[1];
