@@ -10,7 +10,7 @@ Secondary measurements on the same JSON line (skip with --quick):
   horizons   360 azimuths (nphi 361), theta_eps 1e-3 on the same mesh: rays/s
   radiosity  512 sun directions on the +z hemisphere, E from the direct
              insolation, (I - F) B = E to 1e-8 relative residual, FP64 Jacobi
-             vs mixed (FP32-gathered corrections, FP64 residual refinement)
+             vs mixed (FP16-gathered corrections, FP64 residual refinement)
   thermal    327,680 facets, 4,096 CN steps over 4 rotations, layer counts
              U{32..96}, thermal inertia lognormal (median 200, sigma 0.5),
              FP64 vs mixed

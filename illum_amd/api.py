@@ -41,7 +41,13 @@ def _is_dev(a):
 
 
 class Context:
-    """One GPU + one HIP stream (illum_ctx)."""
+    """One GPU + one HIP stream (illum_ctx).
+
+    Objects made on a context (meshes, BVHs, matrices, thermal models, device
+    arrays) must be closed before it.  One closed after its context (the
+    interpreter's shutdown finalizes objects in any order) only drops its
+    handle: destroying it would touch the freed context, so its buffers are
+    left to the end of the process."""
 
     def __init__(self, device=0):
         h = C.c_void_p()
@@ -114,9 +120,9 @@ class DeviceArray:
         return len(self.shape)
 
     def close(self):
-        if self.ptr:
+        if self.ptr and self.ctx.h:
             call("illum_dev_free", self.ctx.h, self.ptr)
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:
@@ -185,9 +191,9 @@ class Mesh:
             call("illum_mesh_set_bodies", h, _ptr(off), len(off) - 1)
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             call("illum_mesh_destroy", self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -219,9 +225,9 @@ class BVH:
         self.h = h
 
     def close(self):
-        if self.h:
+        if self.h and self.mesh.h and self.mesh.ctx.h:
             call("illum_bvh_destroy", self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -281,9 +287,9 @@ class SparseMatrix:
         return cls(ctx, h)
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             call("illum_sparse_destroy", self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -509,9 +515,9 @@ class ThermalModel:
         self.t = 0.0
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             call("illum_thermal_destroy", self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

@@ -186,7 +186,7 @@ void destroy_sparse(illum_sparse *s) {
   // the context's reuse pool takes the blocks (no hipFree, no device sync)
   for (void *p : {(void *)s->d_colptr, (void *)s->d_rowidx, (void *)s->d_cval,
                   (void *)s->d_rowptr, (void *)s->d_colidx, (void *)s->d_rval,
-                  (void *)s->d_rval32, (void *)s->d_vis, (void *)s->d_nrays})
+                  (void *)s->d_vis, (void *)s->d_nrays})
     pool_free(s->ctx, p);
   delete s;
 }
@@ -701,16 +701,14 @@ int illum_sparse_create_csr(illum_ctx *ctx, int64_t n, int64_t nnz, const int64_
     sp->nnz = nnz;
     hipStream_t s = stream_of(ctx);
     sp->d_rowptr = palloc<int64_t>(ctx, n + 1);
-    sp->d_colidx = palloc<int32_t>(ctx, nnz);
-    sp->d_rval = palloc<double>(ctx, nnz);
-    sp->d_rval32 = palloc<float>(ctx, nnz);
-    std::vector<float> v32(nnz);
-    for (int64_t k = 0; k < nnz; ++k) v32[k] = (float)val[k];
+    sp->d_colidx = palloc<int32_t>(ctx, nnz + CSR_PAD);
+    sp->d_rval = palloc<double>(ctx, nnz + CSR_PAD);
+    ILLUM_HIP(hipMemsetAsync(sp->d_colidx + nnz, 0, 4 * CSR_PAD, s));
+    ILLUM_HIP(hipMemsetAsync(sp->d_rval + nnz, 0, 8 * CSR_PAD, s));
     ILLUM_HIP(hipMemcpyAsync(sp->d_rowptr, rowptr, 8 * (n + 1), hipMemcpyHostToDevice, s));
     if (nnz) {
       ILLUM_HIP(hipMemcpyAsync(sp->d_colidx, colidx, 4 * nnz, hipMemcpyHostToDevice, s));
       ILLUM_HIP(hipMemcpyAsync(sp->d_rval, val, 8 * nnz, hipMemcpyHostToDevice, s));
-      ILLUM_HIP(hipMemcpyAsync(sp->d_rval32, v32.data(), 4 * nnz, hipMemcpyHostToDevice, s));
     }
     sync(ctx);
     *out = sp.release();

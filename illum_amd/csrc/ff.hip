@@ -316,7 +316,7 @@ __global__ __launch_bounds__(256) void ff_fill_kernel(
     int nrows, int nwords, const uint32_t *__restrict__ m, const int64_t *__restrict__ ptr,
     int csr, const float4 *__restrict__ pr4, const float4 *__restrict__ n4,
     const double *__restrict__ area, const double *__restrict__ albedo_vec, double albedo,
-    int32_t *__restrict__ idx_out, double *__restrict__ val_out, float *__restrict__ val32_out,
+    int32_t *__restrict__ idx_out, double *__restrict__ val_out,
     const uint8_t *__restrict__ tile_nz) {
   __shared__ int s_wave[4];
   const int row = blockIdx.x;
@@ -387,7 +387,6 @@ __global__ __launch_bounds__(256) void ff_fill_kernel(
       v = ff_value(pr4[col], n4[col], prow, nrow, area[row]) * rho;
     }
     val_out[base + k] = v;
-    if (val32_out) val32_out[base + k] = (float)v;
   }
 }
 
@@ -646,14 +645,15 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
     ff_fill_kernel<<<(int)nf, 256, 0, s>>>((int)nf, nwords, M, sp->d_colptr, 0,
                                            (const float4 *)mesh->d_pr4,
                                            (const float4 *)mesh->d_n4, mesh->d_area, d_alb,
-                                           albedo, sp->d_rowidx, sp->d_cval, nullptr, nullptr);
+                                           albedo, sp->d_rowidx, sp->d_cval, nullptr);
     ILLUM_LAUNCH_CHECK();
   }
   // CSR via the bit transpose.
   sp->d_rowptr = palloc<int64_t>(ctx, nf + 1);
-  sp->d_colidx = palloc<int32_t>(ctx, nnz);
-  sp->d_rval = palloc<double>(ctx, nnz);
-  sp->d_rval32 = palloc<float>(ctx, nnz);
+  sp->d_colidx = palloc<int32_t>(ctx, nnz + CSR_PAD);
+  sp->d_rval = palloc<double>(ctx, nnz + CSR_PAD);
+  ILLUM_HIP(hipMemsetAsync(sp->d_colidx + nnz, 0, 4 * CSR_PAD, s));
+  ILLUM_HIP(hipMemsetAsync(sp->d_rval + nnz, 0, 8 * CSR_PAD, s));
   {
     StageTimer st(ctx, "ff_csr");
     dim3 g(nwords / 16, (unsigned)(rows_p / 512));
@@ -664,8 +664,7 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
     ff_fill_kernel<<<(int)nf, 256, 0, s>>>((int)nf, nwords, MT, sp->d_rowptr, 1,
                                            (const float4 *)mesh->d_pr4,
                                            (const float4 *)mesh->d_n4, mesh->d_area, d_alb,
-                                           albedo, sp->d_colidx, sp->d_rval, sp->d_rval32,
-                                           tile_nz);
+                                           albedo, sp->d_colidx, sp->d_rval, tile_nz);
     ILLUM_LAUNCH_CHECK();
   }
   sync(ctx);

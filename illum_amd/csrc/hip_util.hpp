@@ -87,7 +87,7 @@ enum WorkspaceSlot {
   WS_RAD_X0 = 5,    // iterates / residuals (nf x S each)
   WS_RAD_X1 = 6,
   WS_RAD_X2 = 7,
-  WS_RAD_D32 = 8,   // FP32 correction
+  WS_RAD_D32 = 8,   // FP16 corrections (ring)
   WS_SCAN_PART = 9, // scan_counts block totals
   WS_FF_TILES = 10, // nonzero flags of the bit transpose's 512 x 512 tiles
   WS_FF_ALBEDO = 11,

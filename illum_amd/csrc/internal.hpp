@@ -151,6 +151,8 @@ struct illum_bvh {
   double tune_ms[32] = {0};
 };
 
+constexpr int64_t CSR_PAD = 64;
+
 struct illum_sparse {
   illum_ctx *ctx = nullptr;
   int64_t n = 0, nnz = 0;
@@ -158,10 +160,11 @@ struct illum_sparse {
   int64_t *d_colptr = nullptr;
   int32_t *d_rowidx = nullptr;
   double *d_cval = nullptr;
+  // CSR; colidx and rval hold CSR_PAD entries past nnz (zero), so a kernel
+  // may read a whole group of nonzeros past a row's end
   int64_t *d_rowptr = nullptr;
   int32_t *d_colidx = nullptr;
   double *d_rval = nullptr;
-  float *d_rval32 = nullptr;
   int64_t vis_words = 0;
   uint32_t *d_vis = nullptr;   // raw visibility rows (union of first hits)
   // visibility-kernel work: rays, pair tests, box tests, triangle tests

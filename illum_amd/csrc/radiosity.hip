@@ -9,10 +9,15 @@
 // so every nonzero F(i,j) gathers one contiguous row of B).
 //
 //   ILLUM_RAD_JACOBI_FP64: FP64 values, FP64 vectors.
-//   ILLUM_RAD_MIXED: iterative refinement with a one-sweep correction.  The
-//     FP64 residual r = E - (I-F)B is carried from sweep to sweep; each sweep
-//     rounds it to d = fp32(r), adds d to B (FP64) and updates r <- (r - d) +
-//     F d with FP64 values and accumulation, gathering d in FP32.
+//   ILLUM_RAD_MIXED: iterative refinement with one-sweep corrections.  The
+//     FP64 residual r = E - (I-F)B is carried from sweep to sweep.  Each
+//     sweep rounds it to a correction d = fp16(r 2^e), with a per-column
+//     power of two 2^e that puts the column's 2-norm at or below 2^13, and
+//     adds delta = d 2^-e (exact in FP64) to B.  It updates
+//     r <- (r - delta) + F delta with FP64 values and accumulation, gathering
+//     d in FP16.  So r stays the residual of the B built so far: rounding d
+//     only slows the fall of r (by about 2^-11 per sweep), never its
+//     accuracy.
 // Both report max_s ||E_s - (I-F)B_s||_2 / ||E_s||_2, measured in FP64.
 #include "hip_util.hpp"
 
@@ -20,44 +25,62 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace illum {
 
-// Sparse products over the CSR rows of F (values TV) with a gathered block X
-// (TX) and right-hand side R / output Y (FP64 accumulation):
-//   Y = R + F X            (subtract_x = 0)
-//   Y = (R - X) + F X      (subtract_x = 1: the residual E - (I - F) X for R = E)
-// Rows of X, R and Y hold S contiguous values (row-major [nf][S]).  The rows
-// and wide kernels sum a row's nonzeros in CSR order (identical results); the
-// small-S kernel uses a fixed lane-strided order and tree.  Rows go to XCDs in
-// contiguous ranges: neighbouring facets share most of their visible set, so
-// gathered rows of X are reused from the XCD's L2.
-//
-// Each product also does the per-element work of the sweep around it, on the
-// rows it owns (no other launch touches the vectors in between):
-//   B  != null: B += X at the row (the mixed method's B += d);
-//   Dn != null: Dn = fp32(Y)      (the mixed method's next correction d);
-//   part != null (small-S kernel): per-block column sums of (Y - X)^2
-//               (norm_sub_x) or Y^2, for the residual norm.
-struct SweepOut {
-  double *B;
-  float *Dn;
-  double *part;
-  int norm_sub_x;
-};
-
-__device__ __forceinline__ void sweep_elem(const SweepOut &o, int64_t at, double x, double y) {
-  if (o.B) o.B[at] = o.B[at] + x;
-  if (o.Dn) o.Dn[at] = (float)y;
+// Scale exponent of the mixed method's FP16 corrections for a column whose
+// residual has squared 2-norm nrm2: d = fp16(r 2^e), ||r||_2 2^e <= 2^13.
+// A column's next residual is bounded element-wise by ||F||_inf times this
+// norm, so elements stay below FP16's 65504 while ||F||_inf < 8 (row sums of
+// F are at most the albedo for physical form factors).
+__device__ __forceinline__ int half_scale_exp(double nrm2) {
+  const double nrm = sqrt(nrm2);
+  if (!(nrm > 0.0) || nrm == (double)INFINITY) return 0;  // zero, NaN: any scale
+  int q;
+  (void)frexp(nrm, &q);  // nrm < 2^q
+  return max(-1000, min(1000, 13 - q));
 }
 
+// fp16(y scale), clamped to FP16's finite range: a clamped element leaves a
+// larger remainder in the residual, which the next sweeps remove.
+__device__ __forceinline__ _Float16 to_half_scaled(double y, double scale) {
+  const float v = (float)(y * scale);
+  return (_Float16)fminf(fmaxf(v, -65504.0f), 65504.0f);
+}
+
+// Sparse products over the CSR rows of F (FP64 values) with a gathered block
+// X and right-hand side R / output Y (FP64 accumulation).  Rows of X, R and Y
+// hold S contiguous values (row-major [nf][S]).
+//   FP64 Jacobi (X FP64):  Y = R + F X; the residual element is Y - X.
+//   mixed (X = d_k, FP16): with delta_k = d_k 2^-ek[s] at the row,
+//                          Y = r_{k+1} = (R - delta_k) + (F d_k) 2^-ek[s]
+//                          (R = r_k), Dn = d_{k+1} = fp16(Y 2^en[s]), and in
+//                          B mode B += delta_k; the residual element is Y.
+// The rows and wide kernels sum a row's nonzeros in CSR order (identical
+// results whatever the lane width); the small-S kernel uses a fixed
+// lane-strided order and tree.  Rows go to XCDs in contiguous ranges:
+// neighbouring facets share most of their visible set, so gathered rows of X
+// are reused from the XCD's L2.  Each kernel does the per-element work of the
+// sweep on the rows it owns (no other launch touches the vectors in between).
+struct MixedIO {
+  const int *ek;  // exponents of the gathered correction d_k
+  const int *en;  // exponents of the next correction d_{k+1}
+  _Float16 *Dn;   // d_{k+1}
+  double *B;      // B mode: B += delta_k; null: the corrections stay in a ring
+};
+
+template <class TX>
+constexpr bool is_mixed() { return std::is_same<TX, _Float16>::value; }
+
 // One wave per row, lane s covers one right-hand side per 64-wide chunk.
-template <class TV, class TX>
-__global__ __launch_bounds__(256) void spmm_rows_kernel(
+// The residual norm follows in colnorm_partial_kernel.
+template <class TX>
+__global__ __launch_bounds__(256) void sweep_rows_kernel(
     int64_t nf, int S, const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colidx,
-    const TV *__restrict__ val, const TX *__restrict__ X, const double *__restrict__ R,
-    double *__restrict__ Y, int subtract_x, const int *__restrict__ done, SweepOut o) {
+    const double *__restrict__ val, const TX *__restrict__ X, const double *__restrict__ R,
+    double *__restrict__ Y, const int *__restrict__ done, MixedIO m) {
   if (*done) return;
   const int64_t row = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -68,67 +91,146 @@ __global__ __launch_bounds__(256) void spmm_rows_kernel(
     if (s >= S) break;
     double acc = 0;
     for (int64_t k = k0; k < k1; ++k)
-      acc = acc + (double)val[k] * (double)X[(int64_t)colidx[k] * S + s];
+      acc = __builtin_fma(val[k], (double)X[(int64_t)colidx[k] * S + s], acc);
     const int64_t at = row * S + s;
-    const double x = (double)X[at];
-    const double y = subtract_x ? (R[at] - x) + acc : R[at] + acc;
-    Y[at] = y;
-    sweep_elem(o, at, x, y);
+    if constexpr (is_mixed<TX>()) {
+      const double ik = ldexp(1.0, -m.ek[s]);
+      const double x = (double)X[at] * ik;
+      const double y = (R[at] - x) + acc * ik;
+      Y[at] = y;
+      if (m.B) m.B[at] = m.B[at] + x;
+      m.Dn[at] = to_half_scaled(y, ldexp(1.0, m.en[s]));
+    } else {
+      Y[at] = R[at] + acc;
+    }
   }
 }
 
-// Wide form: lane l covers PER consecutive right-hand sides (one vector load
-// of PER values per nonzero).  The grid is column-chunk-major: chunk ch
-// (columns [64 PER ch, 64 PER (ch + 1))) runs over all rows before the next
-// chunk starts, so the gathered block of X in use is nf x 64 PER values,
-// sized by the host to stay resident in the memory-side cache; per chunk
-// nrb8 blocks (a multiple of 8, so each XCD's blocks take a contiguous row
-// range).  Requires S % PER == 0 (vector alignment).  Each output element
-// sums its row's nonzeros in CSR order, whatever PER.
-template <class TV, class TX, int PER>
-__global__ __launch_bounds__(256) void spmm_wide_kernel(
-    int64_t nf, int S, int nrb8, const int64_t *__restrict__ rowptr,
-    const int32_t *__restrict__ colidx, const TV *__restrict__ val, const TX *__restrict__ X,
-    const double *__restrict__ R, double *__restrict__ Y, int subtract_x,
-    const int *__restrict__ done, SweepOut o) {
-  if (*done) return;
+// Wide form (S >= 128): lane l covers PER consecutive right-hand sides (one
+// vector load of PER values per nonzero).  A block is a group of
+// 4 SWEEP_RPW rows, interleaved over its four waves (wave w takes rows
+// 4 i + w), so the waves gather neighbouring rows together.  The grid is
+// column-chunk-major: chunk ch (columns [64 PER ch, 64 PER (ch + 1))) runs
+// over all groups before the next chunk starts, so the gathered block of X in
+// use is nf x 64 PER values, sized by the host to stay resident in the
+// memory-side cache; per chunk ngrp8 blocks (a multiple of 8, so each XCD's
+// blocks take a contiguous row range).  Requires S % PER == 0.
+// The block's column sums of squares of the residual elements go to
+// part[group][S]: each wave sums its rows in order, then waves 0..3 in order,
+// so the sums do not depend on PER either.
+constexpr int SWEEP_RPW = 4;
+constexpr int SWEEP_UNROLL = 8;  // gathers in flight per wave
+
+// acc[p] += f x[p] with one rounding (a fused multiply-add per element).
+template <int PER, class AV, class XV>
+__device__ __forceinline__ void fma_acc(AV &acc, double f, const XV &x) {
+#pragma unroll
+  for (int p = 0; p < PER; ++p) acc[p] = __builtin_fma(f, (double)x[p], acc[p]);
+}
+
+// A row's nonzeros are taken 64 at a time: the whole wave loads 64 column
+// indices (one per lane) and the gathers read them back with readlane, so
+// the SWEEP_UNROLL gathers of a group issue back to back with no scalar-load
+// round trip before their addresses; the values come by scalar loads.  Group
+// g + 1's gathers are issued before group g is accumulated (two register
+// sets), so a row costs about one memory round trip, not one per group; the
+// last group is padded with zero coefficients (the CSR arrays are padded, so
+// its indices and values are in bounds).  The row's own operands are loaded
+// before its gathers.  Each element is accumulated with fused multiply-adds
+// in CSR order.
+template <class TX, int PER>
+__global__ __launch_bounds__(256) void sweep_wide_kernel(
+    int64_t nf, int S, int ngrp8, const int64_t *__restrict__ rowptr,
+    const int32_t *__restrict__ colidx, const double *__restrict__ val, const TX *__restrict__ X,
+    const double *__restrict__ R, double *__restrict__ Y, const int *__restrict__ done,
+    double *__restrict__ part, MixedIO m) {
+  if (*done) return;  // uniform: the whole grid returns or none of it
+  constexpr int U = SWEEP_UNROLL;
   typedef TX xvec __attribute__((ext_vector_type(PER)));
   typedef double yvec __attribute__((ext_vector_type(PER)));
-  typedef float fvec __attribute__((ext_vector_type(PER)));
-  const int ch = blockIdx.x / nrb8, rb = blockIdx.x - ch * nrb8;
-  const int64_t row = (int64_t)xcd_remap(rb, nrb8) * 4 + (threadIdx.x >> 6);
+  __shared__ yvec red[3][64];
+  const int ch = blockIdx.x / ngrp8, gb = blockIdx.x - ch * ngrp8;
+  const int64_t g = xcd_remap(gb, ngrp8);
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   const int s = (ch * 64 + lane) * PER;
-  if (row >= nf || s >= S) return;
-  const int64_t k0 = rowptr[row], k1 = rowptr[row + 1];
-  yvec acc = (yvec)0.0;
-#pragma unroll 4
-  for (int64_t k = k0; k < k1; ++k) {
-    const double f = (double)val[k];
-    const xvec x = *(const xvec *)(X + (int64_t)colidx[k] * S + s);
-    acc = acc + f * __builtin_convertvector(x, yvec);
+  const bool on = s < S;
+  yvec sq = (yvec)0.0;
+  for (int i = 0; i < SWEEP_RPW; ++i) {
+    const int64_t row = (g * SWEEP_RPW + i) * 4 + w;
+    if (row >= nf) break;
+    const int64_t k0 = rowptr[row], k1 = rowptr[row + 1];
+    const int64_t at = row * S + s;
+    yvec acc = (yvec)0.0;
+    // Every lane runs the gathers (readlane needs the index lanes active);
+    // a lane past S reads column 0 of the same rows and discards the sums.
+    const int sx = on ? s : 0;
+    for (int64_t kb = k0; kb < k1; kb += 64) {
+      const int cnt = (int)std::min<int64_t>(64, k1 - kb);
+      const int cl = colidx[kb + lane];  // past the row: another valid index or the pad
+      const double *vk = val + kb;  // wave-uniform: scalar loads
+      int j = 0;
+      for (; j + U <= cnt; j += U) {
+        xvec x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          x[u] = *(const xvec *)(X + (int64_t)__builtin_amdgcn_readlane(cl, j + u) * S + sx);
+#pragma unroll
+        for (int u = 0; u < U; ++u) fma_acc<PER>(acc, vk[j + u], x[u]);
+      }
+      for (; j < cnt; ++j)
+        fma_acc<PER>(acc, vk[j],
+                     *(const xvec *)(X + (int64_t)__builtin_amdgcn_readlane(cl, j) * S + sx));
+    }
+    if (!on) continue;
+    const yvec r = *(const yvec *)(R + at);
+    const yvec x = __builtin_convertvector(*(const xvec *)(X + at), yvec);
+    if constexpr (is_mixed<TX>()) {
+      yvec ik, sn;
+#pragma unroll
+      for (int p = 0; p < PER; ++p) {
+        ik[p] = ldexp(1.0, -m.ek[s + p]);
+        sn[p] = ldexp(1.0, m.en[s + p]);
+      }
+      const yvec xs = x * ik;
+      const yvec y = (r - xs) + acc * ik;
+      *(yvec *)(Y + at) = y;
+      sq = sq + y * y;
+      xvec dn;
+#pragma unroll
+      for (int p = 0; p < PER; ++p) dn[p] = to_half_scaled(y[p], sn[p]);
+      *(xvec *)(m.Dn + at) = dn;
+      if (m.B) *(yvec *)(m.B + at) = *(const yvec *)(m.B + at) + xs;
+    } else {
+      const yvec y = r + acc;
+      *(yvec *)(Y + at) = y;
+      const yvec d = y - x;
+      sq = sq + d * d;
+    }
   }
-  const int64_t at = row * S + s;
-  const yvec r = *(const yvec *)(R + at);
-  const yvec x = __builtin_convertvector(*(const xvec *)(X + at), yvec);
-  const yvec y = subtract_x ? (r - x) + acc : r + acc;
-  *(yvec *)(Y + at) = y;
-  if (o.B) *(yvec *)(o.B + at) = *(const yvec *)(o.B + at) + x;
-  if (o.Dn) *(fvec *)(o.Dn + at) = __builtin_convertvector(y, fvec);
+  if (w > 0) red[w - 1][lane] = sq;
+  __syncthreads();
+  if (w == 0 && on) {
+    yvec t = sq + red[0][lane];
+    t = t + red[1][lane];
+    t = t + red[2][lane];
+    *(yvec *)(part + g * S + s) = t;
+  }
 }
 
 // Small S (< 32): LPR lanes per row (64 / LPR rows per wave), lanes strided
 // over the row's nonzeros, fixed-shape reduction per right-hand side; the
-// block's column sums of squares go to o.part[block][S] (rows in order).
+// block's column sums of squares of the residual elements go to
+// part[block][S] (rows in order).
 constexpr int SPMM_NNZ_LPR = 16;
 constexpr int SPMM_NNZ_ROWS = 4 * 64 / SPMM_NNZ_LPR;  // rows per 256-thread block
 constexpr int SPMM_NNZ_MAXS = 31;
 
-template <class TV, class TX>
-__global__ __launch_bounds__(256) void spmm_nnz_kernel(
+template <class TX>
+__global__ __launch_bounds__(256) void sweep_nnz_kernel(
     int64_t nf, int S, const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colidx,
-    const TV *__restrict__ val, const TX *__restrict__ X, const double *__restrict__ R,
-    double *__restrict__ Y, int subtract_x, const int *__restrict__ done, SweepOut o) {
+    const double *__restrict__ val, const TX *__restrict__ X, const double *__restrict__ R,
+    double *__restrict__ Y, const int *__restrict__ done, double *__restrict__ part, MixedIO m) {
   constexpr int LPR = SPMM_NNZ_LPR;
   __shared__ double sq[SPMM_NNZ_ROWS][SPMM_NNZ_MAXS + 1];
   if (*done) return;  // uniform: the whole grid returns or none of it
@@ -142,31 +244,38 @@ __global__ __launch_bounds__(256) void spmm_nnz_kernel(
     const int64_t at = row * S + s;
     const bool own = valid && sub == 0;
     const double r_at = own ? R[at] : 0.0, x_at = own ? (double)X[at] : 0.0;
-    const double b_at = own && o.B ? o.B[at] : 0.0;
+    const double b_at = own && m.B ? m.B[at] : 0.0;
     double acc = 0;
 #pragma unroll 4
     for (int64_t k = k0 + sub; k < k1; k += LPR)
-      acc = acc + (double)val[k] * (double)X[(int64_t)colidx[k] * S + s];
+      acc = __builtin_fma(val[k], (double)X[(int64_t)colidx[k] * S + s], acc);
     for (int w = LPR / 2; w > 0; w >>= 1) acc = acc + __shfl_xor(acc, w);
     if (sub == 0) {
       double d2 = 0.0;
       if (valid) {
-        const double y = subtract_x ? (r_at - x_at) + acc : r_at + acc;
-        Y[at] = y;
-        if (o.B) o.B[at] = b_at + x_at;
-        if (o.Dn) o.Dn[at] = (float)y;
-        const double d = o.norm_sub_x ? y - x_at : y;
-        d2 = d * d;
+        if constexpr (is_mixed<TX>()) {
+          const double ik = ldexp(1.0, -m.ek[s]);
+          const double x = x_at * ik;
+          const double y = (r_at - x) + acc * ik;
+          Y[at] = y;
+          if (m.B) m.B[at] = b_at + x;
+          m.Dn[at] = to_half_scaled(y, ldexp(1.0, m.en[s]));
+          d2 = y * y;
+        } else {
+          const double y = r_at + acc;
+          Y[at] = y;
+          const double d = y - x_at;
+          d2 = d * d;
+        }
       }
       sq[rib][s] = d2;
     }
   }
-  if (!o.part) return;
   __syncthreads();
   if ((int)threadIdx.x < S) {
     double t = 0.0;
     for (int r = 0; r < SPMM_NNZ_ROWS; ++r) t += sq[r][threadIdx.x];
-    o.part[(int64_t)blockIdx.x * S + threadIdx.x] = t;
+    part[(int64_t)blockIdx.x * S + threadIdx.x] = t;
   }
 }
 
@@ -195,7 +304,7 @@ __device__ void block_colsum(int64_t r0, int64_t r1, int S, int c0, int cw, cons
   double acc = 0;
   if (q < L) {
     const int64_t s = c0 + c;
-#pragma unroll 4
+#pragma unroll 16
     for (int64_t i = r0 + q; i < r1; i += L) {
       double d = (double)A[i * S + s];
       if (SQ && B) d = d - (double)B[i * S + s];
@@ -255,13 +364,18 @@ __global__ __launch_bounds__(256) void colnorm_partial_kernel(int64_t nf, int S,
 
 // The stop decision: rel = max_s sqrt(col_s / e2_s) (sqrt(col_s) where e2_s =
 // 0) is stored as hist[nprod], the product counted, done set when rel <= tol.
+// ex (nullable, the mixed method): col is the residual r_{k+1} of product k =
+// nprod, and ex[k + 2][s] receives the scale exponent of d_{k+2}, which
+// product k + 1 writes (r_{k+2} is bounded by r_{k+1}).
 // Called by one whole block after col[] is complete.
 __device__ void decide(int S, const double *col, const double *e2, SolveCtl *ctl, double *hist,
-                       double tol, double *red) {
+                       double tol, double *red, int *ex) {
+  const int k = ctl->nprod;
   double m = 0.0;
   for (int s = threadIdx.x; s < S; s += 256) {
     const double r2 = col[s];
     m = max_nan(m, e2[s] > 0 ? sqrt(r2 / e2[s]) : sqrt(r2));
+    if (ex) ex[(int64_t)(k + 2) * S + s] = half_scale_exp(r2);
   }
   __syncthreads();
   red[threadIdx.x] = m;
@@ -272,44 +386,50 @@ __device__ void decide(int S, const double *col, const double *e2, SolveCtl *ctl
   }
   if (threadIdx.x == 0) {
     const double rel = red[0];
-    const int k = ctl->nprod;
     hist[k] = rel;
     ctl->nprod = k + 1;
     if (rel <= tol) ctl->done = 1;
   }
 }
 
-// Adds the G partials of columns [b * COLNORM_CW, +COLNORM_CW) (block b, all
-// columns when S <= COLNORM_CW) in a fixed order into col[].  record = 1:
-// nothing once done is set, and a single block also takes the stop decision
-// (with several blocks decide_kernel follows).
+// Adds the G partials of columns [b cw, (b + 1) cw) (block b) in a fixed
+// order into col[].  record = 1: nothing once done is set, and a single block
+// also takes the stop decision (with several blocks decide_kernel follows).
+// record = 0 with ex (the mixed method's start, col = ||E_s||^2): ex[0][s]
+// and ex[1][s] receive the scale exponents of d_0 and d_1.
 constexpr int COLNORM_CW = 32;
 
-__global__ __launch_bounds__(256) void colnorm_final_kernel(int G, int S,
+__global__ __launch_bounds__(256) void colnorm_final_kernel(int G, int S, int cw,
                                                             const double *__restrict__ part,
                                                             double *__restrict__ col,
                                                             const double *__restrict__ e2,
                                                             SolveCtl *__restrict__ ctl,
                                                             double *__restrict__ hist, double tol,
-                                                            int record) {
+                                                            int record, int *__restrict__ ex) {
   if (record && ctl->done) return;
   __shared__ double red[256];
-  const int c0 = blockIdx.x * COLNORM_CW;
-  block_colsum<false>(0, G, S, c0, std::min(COLNORM_CW, S - c0), part, (const double *)nullptr,
-                      red, col);
+  const int c0 = blockIdx.x * cw;
+  const int cwb = std::min(cw, S - c0);
+  block_colsum<false>(0, G, S, c0, cwb, part, (const double *)nullptr, red, col);
   if (record && gridDim.x == 1) {
     __syncthreads();
-    decide(S, col, e2, ctl, hist, tol, red);
+    decide(S, col, e2, ctl, hist, tol, red, ex);
+  } else if (!record && ex && (int)threadIdx.x < cwb) {
+    // thread t wrote col[c0 + t] itself
+    const int e = half_scale_exp(col[c0 + threadIdx.x]);
+    ex[c0 + threadIdx.x] = e;
+    ex[S + c0 + threadIdx.x] = e;
   }
 }
 
 __global__ __launch_bounds__(256) void decide_kernel(int S, const double *__restrict__ col,
                                                      const double *__restrict__ e2,
                                                      SolveCtl *__restrict__ ctl,
-                                                     double *__restrict__ hist, double tol) {
+                                                     double *__restrict__ hist, double tol,
+                                                     int *__restrict__ ex) {
   if (ctl->done) return;
   __shared__ double red[256];
-  decide(S, col, e2, ctl, hist, tol, red);
+  decide(S, col, e2, ctl, hist, tol, red, ex);
 }
 
 __global__ void solve_init_kernel(SolveCtl *ctl) {
@@ -317,13 +437,184 @@ __global__ void solve_init_kernel(SolveCtl *ctl) {
   ctl->nprod = 0;
 }
 
-// Mixed method start: B = 0, d_0 = fp32(E).
-__global__ void mixed_init_kernel(int64_t n, const double *__restrict__ E,
-                                  float *__restrict__ D, double *__restrict__ B) {
+// Mixed method start (row-major E): d_0 = fp16(E 2^ex[0][s]), B = 0 (B
+// nullable).  r_0 = E is read from E itself by the first product.
+__global__ void mixed_init_kernel(int64_t n, int S, const double *__restrict__ E,
+                                  const int *__restrict__ ex, _Float16 *__restrict__ D,
+                                  double *__restrict__ B) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  D[k] = (float)E[k];
-  B[k] = 0.0;
+  D[k] = to_half_scaled(E[k], ldexp(1.0, ex[k % S]));
+  if (B) B[k] = 0.0;
+}
+
+// Ring mode (S >= 128): the corrections d_j stay in ring slot j % R and B is
+// their sum.  fold_kernel adds delta_j for j in [a, min(b, nprod)) to Bacc in
+// order of j (from zero when init), before product b overwrites slot
+// (b + 1) % R; it does nothing for sweeps that never ran.
+__global__ __launch_bounds__(256) void fold_kernel(int64_t n, int S, int R,
+                                                   const _Float16 *__restrict__ ring,
+                                                   const int *__restrict__ ex,
+                                                   const SolveCtl *__restrict__ ctl, int a, int b,
+                                                   int init, double *__restrict__ Bacc) {
+  const int j1 = min(b, ctl->nprod);
+  if (a >= j1) return;
+  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * 256) {
+    const int s = (int)(k % S);
+    double acc = init ? 0.0 : Bacc[k];
+    for (int j = a; j < j1; ++j)
+      acc += (double)ring[(int64_t)(j % R) * n + k] * ldexp(1.0, -ex[(int64_t)j * S + s]);
+    Bacc[k] = acc;
+  }
+}
+
+// Ring mode output: out (column-major nf x S) = [Bacc +] sum of delta_j over
+// j in [min(folded, nprod), nprod), in order of j (nprod read on the device,
+// so the output is queued before the host reads the control block).  A
+// block takes 32 rows x 64 columns: with
+// S % 4 == 0 a thread reads 4 consecutive values of a row per slot (16
+// threads per row, two passes of 16 rows), else one value at a time; the
+// tile is written transposed, 32 consecutive rows per column.
+__global__ __launch_bounds__(256) void ring_out_kernel(int64_t nf, int S, int R,
+                                                       const _Float16 *__restrict__ ring,
+                                                       const int *__restrict__ ex, int folded,
+                                                       const SolveCtl *__restrict__ ctl,
+                                                       const double *__restrict__ Bacc,
+                                                       double *__restrict__ out) {
+  __shared__ double tile[32][65];
+  const int j1 = ctl->nprod, j0 = min(folded, j1);
+  const int64_t n = nf * S;
+  const int64_t i0 = (int64_t)blockIdx.x * 32;
+  const int s0 = blockIdx.y * 64;
+  const int t = threadIdx.x;
+  if (S % 4 == 0) {
+    typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+    const int c = (t & 15) * 4;  // columns s0 + c .. + 3
+    const int s = s0 + c;
+    for (int k = t >> 4; k < 32; k += 16) {
+      const int64_t i = i0 + k;
+      if (i >= nf || s >= S) continue;
+      const int64_t at = i * S + s;
+      double v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[q] = Bacc ? Bacc[at + q] : 0.0;
+      for (int j = j0; j < j1; ++j) {
+        const h4 d = *(const h4 *)(ring + (int64_t)(j % R) * n + at);
+        const int *e = ex + (int64_t)j * S + s;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] += (double)d[q] * ldexp(1.0, -e[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) tile[k][c + q] = v[q];
+    }
+  } else {
+    const int c = t & 63;
+    const int s = s0 + c;
+    for (int k = t >> 6; k < 32; k += 4) {
+      const int64_t i = i0 + k;
+      if (i >= nf || s >= S) continue;
+      const int64_t at = i * S + s;
+      double v = Bacc ? Bacc[at] : 0.0;
+      for (int j = j0; j < j1; ++j)
+        v += (double)ring[(int64_t)(j % R) * n + at] * ldexp(1.0, -ex[(int64_t)j * S + s]);
+      tile[k][c] = v;
+    }
+  }
+  __syncthreads();
+  const int i = t & 31;
+  for (int c = t >> 5; c < 64; c += 8) {
+    const int s = s0 + c;
+    if (i0 + i < nf && s < S) out[(int64_t)s * nf + i0 + i] = tile[i][c];
+  }
+}
+
+// Column sums of squares of a column-major (nf x S) array: block (s, p)
+// sums rows [p rows_per, (p + 1) rows_per) of column s in a fixed order into
+// part[p][s] (colnorm_final_kernel adds the partials).
+__global__ __launch_bounds__(256) void colnorm_cm_partial_kernel(int64_t nf, int S,
+                                                                 int64_t rows_per,
+                                                                 const double *__restrict__ A,
+                                                                 double *__restrict__ part) {
+  __shared__ double red[256];
+  const int s = blockIdx.x, p = blockIdx.y;
+  const int64_t r0 = std::min<int64_t>(nf, p * rows_per), r1 = std::min<int64_t>(nf, r0 + rows_per);
+  const double *c = A + (int64_t)s * nf;
+  double acc = 0.0;
+#pragma unroll 4
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) {
+    const double v = c[i];
+    acc += v * v;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(int64_t)p * S + s] = red[0];
+}
+
+// Right-hand sides to row-major: Er = E (column-major nf x S in), and for
+// the mixed method d_0 = fp16(E 2^ex[0][s]) (D0 nullable) and B = 0 (Bacc
+// nullable), through a 32 x 32 tile transpose.
+__global__ __launch_bounds__(256) void load_rhs_kernel(int64_t nf, int S,
+                                                       const double *__restrict__ E,
+                                                       double *__restrict__ Er,
+                                                       const int *__restrict__ ex,
+                                                       _Float16 *__restrict__ D0,
+                                                       double *__restrict__ Bacc) {
+  __shared__ double tile[32][33];
+  const int64_t i0 = (int64_t)blockIdx.x * 32;
+  const int s0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t i = i0 + tx;
+    const int s = s0 + k;
+    if (i < nf && s < S) tile[k][tx] = E[(int64_t)s * nf + i];
+  }
+  __syncthreads();
+  const int s = s0 + tx;
+  const double scale = D0 && s < S ? ldexp(1.0, ex[s]) : 1.0;
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t i = i0 + k;
+    if (i < nf && s < S) {
+      const double v = tile[tx][k];
+      const int64_t at = i * S + s;
+      Er[at] = v;
+      if (D0) D0[at] = to_half_scaled(v, scale);
+      if (Bacc) Bacc[at] = 0.0;
+    }
+  }
+}
+
+// FP64 output: B = cur(k), k = nprod - 1 once done, else nprod (the newest
+// iterate), where cur(0) = E and then X1, X0, X1, ... (row-major in,
+// column-major out).  The source is chosen on the device, so the output is
+// queued before the host reads the control block.
+__global__ __launch_bounds__(256) void fp64_out_kernel(int64_t nf, int S,
+                                                       const double *__restrict__ Er,
+                                                       const double *__restrict__ X0,
+                                                       const double *__restrict__ X1,
+                                                       const SolveCtl *__restrict__ ctl,
+                                                       double *out) {
+  __shared__ double tile[32][33];
+  const int k = ctl->done ? ctl->nprod - 1 : ctl->nprod;
+  const double *in = k == 0 ? Er : (k & 1 ? X1 : X0);
+  const int64_t i0 = (int64_t)blockIdx.x * 32;
+  const int s0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int q = ty; q < 32; q += 8) {
+    const int64_t i = i0 + q;
+    const int s = s0 + tx;
+    if (i < nf && s < S) tile[q][tx] = in[i * S + s];
+  }
+  __syncthreads();  // (with S = 1 and out == Er a block rewrites only what it read)
+  for (int q = ty; q < 32; q += 8) {
+    const int64_t i = i0 + tx;
+    const int s = s0 + q;
+    if (i < nf && s < S) out[(int64_t)s * nf + i] = tile[tx][q];
+  }
 }
 
 // Column-major (nf x S) <-> row-major ([nf][S]) copies with optional cast.
@@ -361,14 +652,15 @@ __global__ void transpose_kernel(int64_t nf, int S, const TI *__restrict__ in,
 
 namespace {
 
-// Sparse product dispatch by S (see the kernels above).  Returns the number
-// of column-sum partials written to o.part (0: none, the small-S kernel only).
-template <class TV, class TX>
-int spmm(hipStream_t s, illum_ctx *ctx, const illum_sparse *F, int S, const TV *val, const TX *X,
-         const double *R, double *Y, int subx, const int *done, SweepOut o, const char *stage) {
+// Sparse product of one sweep, dispatched by S (see the kernels above).
+// Returns the number of column-sum partial rows written to part (0: none,
+// the rows kernel; the caller then runs colnorm_partial_kernel).
+template <class TX>
+int sweep_product(hipStream_t s, illum_ctx *ctx, const illum_sparse *F, int S, const TX *X,
+                  const double *R, double *Y, const int *done, double *part, MixedIO m,
+                  const char *stage) {
   StageTimer st(ctx, stage);
   const int64_t nf = F->n;
-  const unsigned blocks = (unsigned)((nf + 3) / 4);
   int nparts = 0;
   if (S >= 128) {
     // PER: the widest of 8, 4, 2, 1 dividing S whose gathered chunk
@@ -377,27 +669,32 @@ int spmm(hipStream_t s, illum_ctx *ctx, const illum_sparse *F, int S, const TV *
     // ILLUM_SPMM_CHUNK_BYTES overrides the 128 MB budget.
     const char *env = getenv("ILLUM_SPMM_CHUNK_BYTES");
     const double budget = env ? atof(env) : 128e6;
+    // Also 64 PER <= S, so every lane of a chunk has work.
     int per = 8;
-    while (per > 1 && (S % per != 0 || (double)nf * 64 * per * sizeof(TX) > budget)) per >>= 1;
-    const int nrb8 = (int)(((nf + 3) / 4 + 7) / 8 * 8);
+    while (per > 1 && (S % per != 0 || 64 * per > S ||
+                       (double)nf * 64 * per * sizeof(TX) > budget))
+      per >>= 1;
+    const int64_t ngrp = (nf + 4 * SWEEP_RPW - 1) / (4 * SWEEP_RPW);
+    const int ngrp8 = (int)((ngrp + 7) / 8 * 8);
     const int nch = (S + 64 * per - 1) / (64 * per);
-    const unsigned grid = (unsigned)((int64_t)nrb8 * nch);
-#define ILLUM_SPMM_WIDE(P)                                                                   \
-  spmm_wide_kernel<TV, TX, P><<<grid, 256, 0, s>>>(nf, S, nrb8, F->d_rowptr, F->d_colidx, val, X, \
-                                                   R, Y, subx, done, o)
-    if (per == 8) ILLUM_SPMM_WIDE(8);
-    else if (per == 4) ILLUM_SPMM_WIDE(4);
-    else if (per == 2) ILLUM_SPMM_WIDE(2);
-    else ILLUM_SPMM_WIDE(1);
-#undef ILLUM_SPMM_WIDE
+    const unsigned grid = (unsigned)((int64_t)ngrp8 * nch);
+#define ILLUM_SWEEP_WIDE(P)                                                                  \
+  sweep_wide_kernel<TX, P><<<grid, 256, 0, s>>>(nf, S, ngrp8, F->d_rowptr, F->d_colidx,       \
+                                                F->d_rval, X, R, Y, done, part, m)
+    if (per == 8) ILLUM_SWEEP_WIDE(8);
+    else if (per == 4) ILLUM_SWEEP_WIDE(4);
+    else if (per == 2) ILLUM_SWEEP_WIDE(2);
+    else ILLUM_SWEEP_WIDE(1);
+#undef ILLUM_SWEEP_WIDE
+    nparts = ngrp8;
   } else if (S > SPMM_NNZ_MAXS) {
-    spmm_rows_kernel<TV, TX><<<blocks, 256, 0, s>>>(nf, S, F->d_rowptr, F->d_colidx, val, X, R, Y,
-                                                    subx, done, o);
+    sweep_rows_kernel<TX><<<(unsigned)((nf + 3) / 4), 256, 0, s>>>(
+        nf, S, F->d_rowptr, F->d_colidx, F->d_rval, X, R, Y, done, m);
   } else {
     const unsigned nb = (unsigned)((nf + SPMM_NNZ_ROWS - 1) / SPMM_NNZ_ROWS);
-    spmm_nnz_kernel<TV, TX><<<nb, 256, 0, s>>>(nf, S, F->d_rowptr, F->d_colidx, val, X, R, Y, subx,
-                                               done, o);
-    nparts = o.part ? (int)nb : 0;
+    sweep_nnz_kernel<TX><<<nb, 256, 0, s>>>(nf, S, F->d_rowptr, F->d_colidx, F->d_rval, X, R, Y,
+                                            done, part, m);
+    nparts = (int)nb;
   }
   ILLUM_LAUNCH_CHECK();
   return nparts;
@@ -415,6 +712,10 @@ bool overlap(const void *a, const void *b, size_t bytes) {
   return x < y + bytes && y < x + bytes;
 }
 
+// Corrections kept in a ring (S >= 128) and the sweeps between two folds.
+constexpr int RAD_RING = 8;
+constexpr int RAD_FOLD = RAD_RING - 1;
+
 }  // namespace
 
 void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double rel_tol,
@@ -430,19 +731,30 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
   hipStream_t s = stream_of(ctx);
   const bool in_dev = flags & ILLUM_IN_ON_DEVICE, out_dev = flags & ILLUM_OUT_ON_DEVICE;
   const bool mixed = method == ILLUM_RAD_MIXED;
+  const bool wide = S >= 128;
+  // Mixed, S >= 128: the corrections stay in a ring and B is their sum
+  // (folded every RAD_FOLD sweeps); otherwise each product adds its
+  // correction to B ("B mode", two slots).
+  const bool ring = mixed && wide;
+  const int R = ring ? RAD_RING : 2;
 
   // Scratch from the context's workspace (no allocation in a time loop):
-  // control block, residual history, column norms and partial sums.
+  // control block, residual history, column norms, scale exponents and
+  // partial sums.
   const int G = (int)std::min<int64_t>(1024, std::max<int64_t>(1, n / 16));
-  const int64_t gparts = std::max<int64_t>(G, (nf + SPMM_NNZ_ROWS - 1) / SPMM_NNZ_ROWS);
+  const int64_t ngrp8 = ((nf + 4 * SWEEP_RPW - 1) / (4 * SWEEP_RPW) + 7) / 8 * 8;
+  const int64_t gparts = std::max<int64_t>(
+      {(int64_t)G, (nf + SPMM_NNZ_ROWS - 1) / SPMM_NNZ_ROWS, wide ? ngrp8 : 0});
   auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t o_hist = 256, o_e2 = o_hist + align(8 * ((size_t)max_iter + 1)),
-               o_col = o_e2 + align(8 * (size_t)S), o_part = o_col + align(8 * (size_t)S),
+               o_col = o_e2 + align(8 * (size_t)S), o_ex = o_col + align(8 * (size_t)S),
+               o_part = o_ex + align(4 * ((size_t)max_iter + 2) * S),
                ctl_bytes = o_part + 8 * (size_t)gparts * S;
   char *ws = (char *)ctx_workspace(ctx, WS_RAD_CTL, ctl_bytes);
   SolveCtl *ctl = (SolveCtl *)ws;
   double *hist = (double *)(ws + o_hist), *e2 = (double *)(ws + o_e2),
          *col = (double *)(ws + o_col), *part = (double *)(ws + o_part);
+  int *ex = mixed ? (int *)(ws + o_ex) : nullptr;
   const int *done = &ctl->done;
 
   // Right-hand sides, row-major FP64.  With S = 1 both layouts coincide and a
@@ -454,61 +766,84 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
     ILLUM_HIP(hipMemcpyAsync(Ehost.p, E, 8 * n, hipMemcpyHostToDevice, s));
     Ein = Ehost.p;
   }
-  const double *Er = Ein;
-  if (S > 1) {
-    double *t = (double *)ctx_workspace(ctx, WS_RAD_E, 8 * n);
-    transpose(s, nf, S, Ein, t, true);
-    Er = t;
-  }
   double *X0 = (double *)ctx_workspace(ctx, WS_RAD_X0, 8 * n);
   double *X1 = (double *)ctx_workspace(ctx, WS_RAD_X1, 8 * n);
-  // The mixed method accumulates B in place in the caller's B when it can
-  // (device, S = 1, not overlapping E).
+  // B mode accumulates B in place in the caller's B when it can (device,
+  // S = 1, not overlapping E); ring mode folds into Bacc.
   double *Bacc = nullptr;
-  float *D0 = nullptr, *D1 = nullptr;
+  _Float16 *Dring = nullptr;
   if (mixed) {
-    Bacc = out_dev && S == 1 && !overlap(B, Ein, 8 * n)
+    Bacc = !ring && out_dev && S == 1 && !overlap(B, Ein, 8 * n)
                ? B : (double *)ctx_workspace(ctx, WS_RAD_X2, 8 * n);
-    D0 = (float *)ctx_workspace(ctx, WS_RAD_D32, 8 * n);
-    D1 = D0 + n;
+    Dring = (_Float16 *)ctx_workspace(ctx, WS_RAD_D32, 2 * (size_t)R * n);
   }
 
   solve_init_kernel<<<1, 1, 0, s>>>(ctl);
   ILLUM_LAUNCH_CHECK();
-  colnorm_partial_kernel<double><<<G, 256, 0, s>>>(nf, S, Er, nullptr, part, nullptr);
-  ILLUM_LAUNCH_CHECK();
-  const unsigned fb = (unsigned)((S + COLNORM_CW - 1) / COLNORM_CW);
-  colnorm_final_kernel<<<fb, 256, 0, s>>>(G, S, part, e2, nullptr, ctl, nullptr, 0.0, 0);
-  ILLUM_LAUNCH_CHECK();
-  if (mixed) {
-    mixed_init_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, Er, D0, Bacc);
+  // ||E_s||^2 from the column-major input (and, mixed, the exponents of d_0
+  // and d_1), then E row-major (with d_0 and B = 0 in the same pass).  With
+  // S = 1 both layouts coincide and a device E is read in place (it is never
+  // written).
+  {
+    const int P = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (nf + 4095) / 4096));
+    const int64_t rows_per = (nf + P - 1) / P;
+    colnorm_cm_partial_kernel<<<dim3((unsigned)S, (unsigned)P), 256, 0, s>>>(nf, S, rows_per,
+                                                                            Ein, part);
+    ILLUM_LAUNCH_CHECK();
+    colnorm_final_kernel<<<(unsigned)((S + COLNORM_CW - 1) / COLNORM_CW), 256, 0, s>>>(
+        P, S, COLNORM_CW, part, e2, nullptr, ctl, nullptr, 0.0, 0, ex);
     ILLUM_LAUNCH_CHECK();
   }
+  const double *Er = Ein;
+  if (S > 1) {
+    double *t = (double *)ctx_workspace(ctx, WS_RAD_E, 8 * n);
+    dim3 g((unsigned)((nf + 31) / 32), (unsigned)((S + 31) / 32));
+    load_rhs_kernel<<<g, 256, 0, s>>>(nf, S, Ein, t, ex, mixed ? Dring : nullptr,
+                                      mixed && !ring ? Bacc : nullptr);
+    ILLUM_LAUNCH_CHECK();
+    Er = t;
+  } else if (mixed) {
+    mixed_init_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, S, Er, ex, Dring, Bacc);
+    ILLUM_LAUNCH_CHECK();
+  }
+  // The per-sweep column sums: 4 columns per block over the wide product's
+  // many partial rows, else COLNORM_CW.
+  const int cw = wide ? 4 : COLNORM_CW;
+  const unsigned fb = (unsigned)((S + cw - 1) / cw);
 
   // Sweep k's buffers.  FP64 Jacobi, B_{k+1} = E + F B_k from B_0 = E:
   // cur_k = E, X1, X0, X1, ...; nxt_k = X1, X0, X1, ...  Mixed refinement,
-  // r_0 = E: r_k = E, X0, X1, X0, ...; r_{k+1} = X0, X1, X0, ...; d_k
-  // alternates D0, D1.
+  // r_0 = E: r_k = E, X0, X1, X0, ...; r_{k+1} = X0, X1, X0, ...; d_k in
+  // slot k % R, its exponents in ex[k].
   auto cur = [&](int k) -> const double * { return k == 0 ? Er : (k & 1 ? X1 : X0); };
   auto nxt = [&](int k) -> double * { return k & 1 ? X0 : X1; };
   auto rk = [&](int k) -> const double * { return k == 0 ? Er : (k & 1 ? X0 : X1); };
   auto rk1 = [&](int k) -> double * { return k & 1 ? X1 : X0; };
+  auto slot = [&](int j) { return Dring + (int64_t)(j % R) * n; };
+  int folded = 0;  // ring mode: delta_j for j < min(folded, nprod) are in Bacc
   auto sweep = [&](int k) {
+    if (ring && k > 0 && k % RAD_FOLD == 0) {
+      fold_kernel<<<(unsigned)std::min<int64_t>((n + 255) / 256, 8192), 256, 0, s>>>(
+          n, S, R, Dring, ex, ctl, k - RAD_FOLD, k, k == RAD_FOLD ? 1 : 0, Bacc);
+      ILLUM_LAUNCH_CHECK();
+      folded = k;
+    }
     int nparts;
     const double *Y;
-    const double *Xn = nullptr;  // subtracted in the residual norm (FP64: cur)
+    const double *Xn = nullptr;  // subtracted in a separate residual norm (FP64: cur)
     if (!mixed) {
       // nxt = E + F cur; the residual of cur is nxt - cur
-      nparts = spmm<double, double>(s, ctx, F, S, F->d_rval, cur(k), Er, nxt(k), 0, done,
-                                    SweepOut{nullptr, nullptr, part, 1}, "radiosity_fp64_sweep");
+      nparts = sweep_product<double>(s, ctx, F, S, cur(k), Er, nxt(k), done, part,
+                                     MixedIO{nullptr, nullptr, nullptr, nullptr},
+                                     "radiosity_fp64_sweep");
       Y = nxt(k);
       Xn = cur(k);
     } else {
-      // d = D_k: B += d, r <- (r - d) + F d (FP64 values and accumulation),
-      // D_{k+1} = fp32(r)
-      float *Dk = k & 1 ? D1 : D0, *Dk1 = k & 1 ? D0 : D1;
-      nparts = spmm<double, float>(s, ctx, F, S, F->d_rval, Dk, rk(k), rk1(k), 1, done,
-                                   SweepOut{Bacc, Dk1, part, 0}, "radiosity_mixed_sweep");
+      // r_{k+1} = (r_k - delta_k) + F delta_k, d_{k+1} = fp16(r_{k+1} 2^ex[k+1])
+      MixedIO m{ex + (int64_t)k * S, ex + (int64_t)(k + 1) * S, slot(k + 1),
+                ring ? nullptr : Bacc};
+      nparts = sweep_product<_Float16>(s, ctx, F, S, slot(k), rk(k), rk1(k), done, part, m,
+                                       "radiosity_mixed_sweep");
       Y = rk1(k);
     }
     if (!nparts) {
@@ -516,10 +851,11 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
       ILLUM_LAUNCH_CHECK();
       nparts = G;
     }
-    colnorm_final_kernel<<<fb, 256, 0, s>>>(nparts, S, part, col, e2, ctl, hist, rel_tol, 1);
+    colnorm_final_kernel<<<fb, 256, 0, s>>>(nparts, S, cw, part, col, e2, ctl, hist, rel_tol, 1,
+                                            ex);
     ILLUM_LAUNCH_CHECK();
     if (fb > 1) {
-      decide_kernel<<<1, 256, 0, s>>>(S, col, e2, ctl, hist, rel_tol);
+      decide_kernel<<<1, 256, 0, s>>>(S, col, e2, ctl, hist, rel_tol, ex);
       ILLUM_LAUNCH_CHECK();
     }
   };
@@ -531,9 +867,40 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
   SolveCtl h{};
   int launched = 0;
   int batch = F->last_nprod > 0 ? F->last_nprod : 4;
+  // The output (column-major) is queued with each batch, ahead of the
+  // readback; a batch that does not end the solve has its output rewritten
+  // by the next one.
+  DBuf<double> Bout;
+  double *dst = B;
+  if (!out_dev) { Bout.reset(n); dst = Bout.p; }
+  auto output = [&]() {
+    if (ring) {
+      // B = sum of delta_j, j < nprod
+      dim3 g((unsigned)((nf + 31) / 32), (unsigned)((S + 63) / 64));
+      ring_out_kernel<<<g, 256, 0, s>>>(nf, S, R, Dring, ex, folded, ctl,
+                                        folded > 0 ? Bacc : nullptr, dst);
+      ILLUM_LAUNCH_CHECK();
+    } else if (mixed) {
+      if (S == 1) {
+        if (Bacc != dst) ILLUM_HIP(hipMemcpyAsync(dst, Bacc, 8 * n, hipMemcpyDeviceToDevice, s));
+      } else {
+        transpose(s, nf, S, (const double *)Bacc, dst, false);
+      }
+    } else {
+      // converged at sweep k -> B_k = cur(k); otherwise the newest iterate
+      dim3 g((unsigned)((nf + 31) / 32), (unsigned)((S + 31) / 32));
+      fp64_out_kernel<<<g, 256, 0, s>>>(nf, S, Er, X0, X1, ctl, dst);
+      ILLUM_LAUNCH_CHECK();
+    }
+    if (!out_dev) ILLUM_HIP(hipMemcpyAsync(B, dst, 8 * n, hipMemcpyDeviceToHost, s));
+  };
+  // With S = 1 the sweeps read a device E in place: a B overlapping it is
+  // written once the solve has ended.
+  const bool early_out = !(S == 1 && overlap(dst, Ein, 8 * n));
   for (;;) {
     const int nb = std::min(batch, max_iter - launched);
     for (int j = 0; j < nb; ++j) sweep(launched++);
+    if (early_out) output();
     ILLUM_HIP(hipMemcpyAsync(hbuf.data(), ws, o_hist + 8 * (size_t)launched,
                              hipMemcpyDeviceToHost, s));
     sync(ctx);
@@ -541,24 +908,14 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
     if (h.done || launched >= max_iter) break;
     batch = std::min(2 * batch, 64);
   }
+  if (!early_out) {
+    output();
+    if (!(in_dev && out_dev && (flags & ILLUM_ASYNC))) sync(ctx);
+  }
   const int nprod = h.nprod;
   F->last_nprod = nprod;
   std::vector<double> hh(nprod);
   std::memcpy(hh.data(), hbuf.data() + o_hist, 8 * (size_t)nprod);
-  // FP64: converged at sweep k -> B_k = cur(k); otherwise the newest iterate.
-  const double *Bcur = mixed ? Bacc : cur(h.done ? nprod - 1 : nprod);
-
-  // Output to column-major.
-  DBuf<double> Bout;
-  double *dst = B;
-  if (!out_dev) { Bout.reset(n); dst = Bout.p; }
-  if (S == 1) {
-    if (Bcur != dst) ILLUM_HIP(hipMemcpyAsync(dst, Bcur, 8 * n, hipMemcpyDeviceToDevice, s));
-  } else {
-    transpose(s, nf, S, Bcur, dst, false);
-  }
-  if (!out_dev) ILLUM_HIP(hipMemcpyAsync(B, dst, 8 * n, hipMemcpyDeviceToHost, s));
-  if (!(in_dev && out_dev && (flags & ILLUM_ASYNC))) sync(ctx);
   if (resid) std::copy(hh.begin(), hh.end(), resid);
   if (iters) *iters = nprod;
   const double rel = nprod > 0 ? hh[nprod - 1] : 1.0;

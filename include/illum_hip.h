@@ -59,7 +59,8 @@ extern "C" {
 
 /* radiosity solve methods */
 #define ILLUM_RAD_JACOBI_FP64   1   /* FP64 Jacobi sweeps B <- E + F B                  */
-#define ILLUM_RAD_MIXED         2   /* FP32 correction sweeps, FP64 residual refinement */
+#define ILLUM_RAD_MIXED         2   /* FP16 corrections with a per-column power-of-two
+                                       scale, FP64 residual refinement               */
 
 /* thermal precision modes */
 #define ILLUM_THERM_FP64        1   /* FP64 Crank-Nicolson, FP64 Thomas                  */
