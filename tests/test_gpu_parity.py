@@ -2,8 +2,8 @@
 
 Bars: visibility (the union-of-closest-hits sets of compute_F, illum.cpp:42-95),
 horizons and ray hits bit-exact; form factors within 1e-12 relative of the
-oracle's FP64 values (the FP32-stored copy within 1e-7); radiosity at the
-stated relative residual; Crank-Nicolson FP64 within 1e-9 K of the oracle
+oracle's FP64 values; radiosity at the stated relative residual, recomputed
+in FP64 on the host; Crank-Nicolson FP64 within 1e-9 K of the oracle
 and the mixed mode within 1e-3 K.
 """
 import numpy as np
@@ -268,6 +268,37 @@ def test_radiosity_wide_products_chunking_is_exact(api, monkeypatch, method):
         for s in (0, S - 1):
             Bo, _, _ = O.gauss_seidel(nf, cp, ri, v, E[:, s], max_sweeps=200, rel_tol=1e-13)
             np.testing.assert_allclose(B[:, s], Bo, rtol=1e-8, atol=1e-8 * np.abs(Bo).max())
+
+
+@pytest.mark.parametrize("S", [5, 128])
+def test_radiosity_mixed_column_scales_and_folds(api, S):
+    # The mixed method rounds each residual column to FP16 after a power-of-two
+    # scale taken from that column's norm: columns 60 orders of magnitude apart
+    # and an all-zero column must all reach the tolerance, with the residual
+    # recomputed in FP64 on the host.  Albedo 0.9 at 1e-12 takes more than 7
+    # products, so at S = 128 the ring of corrections is folded into B.
+    V, F = _mesh(api, 4, 40, seed=6)
+    ctx = api.IllumContext(V=V, F=F)
+    Fm = ctx.compute_F(0.9)
+    nf = len(F)
+    rng = np.random.default_rng(S + 1)
+    E = rng.uniform(0, 500, (nf, S)) * 10.0 ** np.linspace(-30, 30, S)[None, :]
+    E[:, 1] = 0.0
+    tol = 1e-12
+    B, res, it = api.radiosity_solve(Fm, E, method="mixed", rel_tol=tol, max_iter=400)
+    B64, res64, it64 = api.radiosity_solve(Fm, E, method="fp64", rel_tol=tol, max_iter=400)
+    assert it > 7 and res[-1] <= tol
+    assert abs(it - it64) <= 1, (it, it64)
+    import scipy.sparse as sps
+    cp, ri, v = Fm.csc()
+    A = sps.csc_matrix((v, ri, cp), shape=(nf, nf))
+    r = E - (B - A @ B)
+    nE = np.linalg.norm(E, axis=0)
+    assert np.all(np.linalg.norm(r[:, nE > 0], axis=0) <= 1.2 * tol * nE[nE > 0])
+    assert np.all(B[:, 1] == 0.0)
+    scale = np.abs(B64).max(axis=0)
+    ok = nE > 0
+    assert np.all(np.abs(B - B64)[:, ok].max(axis=0) <= 1e-10 * scale[ok])
 
 
 def _cn_case(nf, nz, seed):
