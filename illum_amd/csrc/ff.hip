@@ -390,6 +390,68 @@ __global__ __launch_bounds__(256) void ff_fill_kernel(
   }
 }
 
+// CSR(F) from CSC(F) when every row is short.  CSR row i lists the sources j
+// of the CSC entries (i, j) in ascending j, with the same values (both fills
+// evaluate ff_value on the same receiver / source arguments).
+//
+// csr_scatter_kernel: one wave per CSC column j; each entry (i, j, v) takes
+// the next slot of row i through an atomic cursor, into the unsorted arrays
+// tj / tv laid out by rowptr.
+__global__ __launch_bounds__(256) void csr_scatter_kernel(
+    int64_t nf, const int64_t *__restrict__ colptr, const int32_t *__restrict__ rowidx,
+    const double *__restrict__ cval, const int64_t *__restrict__ rowptr,
+    int32_t *__restrict__ cursor, int32_t *__restrict__ tj, double *__restrict__ tv) {
+  const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= nf) return;
+  for (int64_t k = colptr[j] + (threadIdx.x & 63); k < colptr[j + 1]; k += 64) {
+    const int32_t i = rowidx[k];
+    const int64_t at = rowptr[i] + atomicAdd(&cursor[i], 1);
+    tj[at] = (int32_t)j;
+    tv[at] = cval[k];
+  }
+}
+
+// csr_sort_kernel: one wave per row of at most CSR_SORT_MAX entries, staged
+// in LDS; an entry's place is the number of the row's entries with a smaller
+// source index (all distinct), counted against LDS broadcasts.
+constexpr int CSR_SORT_MAX = 1024;
+
+__global__ __launch_bounds__(256) void csr_sort_kernel(int64_t nf,
+                                                       const int64_t *__restrict__ rowptr,
+                                                       const int32_t *__restrict__ tj,
+                                                       const double *__restrict__ tv,
+                                                       int32_t *__restrict__ colidx,
+                                                       double *__restrict__ rval) {
+  __shared__ int32_t sj[4][CSR_SORT_MAX];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * 4 + w;
+  const bool valid = i < nf;
+  const int64_t base = valid ? rowptr[i] : 0;
+  const int n = valid ? (int)(rowptr[i + 1] - base) : 0;  // <= CSR_SORT_MAX (host check)
+  for (int e = lane; e < n; e += 64) sj[w][e] = tj[base + e];
+  __syncthreads();
+  for (int e0 = 0; e0 < n; e0 += 64) {
+    const int e = e0 + lane;
+    const int32_t me = e < n ? sj[w][e] : 0;
+    int rank = 0;
+    for (int q = 0; q < n; ++q) rank += sj[w][q] < me ? 1 : 0;
+    if (e < n) {
+      colidx[base + rank] = me;
+      rval[base + rank] = tv[base + e];
+    }
+  }
+}
+
+// Largest of n counts, into *out (zeroed by the caller).
+__global__ __launch_bounds__(256) void max_count_kernel(int64_t n, const int32_t *__restrict__ c,
+                                                        int32_t *__restrict__ out) {
+  int m = 0;
+  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256)
+    m = max(m, c[k]);
+  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
 // Transposes a (rows_p x 32*nwords)-bit matrix stored as rows of nwords words
 // (rows_p == 32*nwords, both multiples of 512) in 512x512-bit tiles.  The
 // tile is staged through LDS for coalesced 64-byte row segments; each 32x32
@@ -594,19 +656,21 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
     d_alb = (double *)ctx_workspace(ctx, WS_FF_ALBEDO, 8 * nf);
     ILLUM_HIP(hipMemcpyAsync(d_alb, albedo_vec, 8 * nf, hipMemcpyHostToDevice, s));
   }
-  int *status = (int *)ctx_workspace(ctx, WS_FF_STATUS, 4);
-  // M and its transpose live in the context's workspace.  Rows >= nf of M
-  // (padding to whole 512-row tiles) must read as zero for the transpose.
+  // status word (NaN seen) and the largest CSR row length
+  int *status = (int *)ctx_workspace(ctx, WS_FF_STATUS, 8);
+  int32_t *maxrow = (int32_t *)(status + 1);
+  // M (and, for rows too long to sort, its transpose) live in the context's
+  // workspace.  Rows >= nf of M (padding to whole 512-row tiles) must read as
+  // zero for the transpose.
   const size_t mbytes = (size_t)rows_p * nwords * 4;
   uint32_t *M = (uint32_t *)ctx_workspace(ctx, WS_FF_MASK, mbytes);
-  uint32_t *MT = (uint32_t *)ctx_workspace(ctx, WS_FF_MASK_T, mbytes);
   // counts: CSC column lengths (rows of M), then CSR row lengths (rows of M^T)
   int32_t *counts = (int32_t *)ctx_workspace(ctx, WS_FF_COUNTS, 8 * rows_p);
   int32_t *rcounts = counts + rows_p;
   ILLUM_HIP(hipMemsetAsync(rcounts, 0, 4 * nf, s));
   if (rows_p > nf)
     ILLUM_HIP(hipMemsetAsync(M + (size_t)nf * nwords, 0, (size_t)(rows_p - nf) * nwords * 4, s));
-  ILLUM_HIP(hipMemsetAsync(status, 0, 4, s));
+  ILLUM_HIP(hipMemsetAsync(status, 0, 8, s));
   sp->d_nrays = palloc<unsigned long long>(ctx, 1);
   ILLUM_HIP(hipMemsetAsync(sp->d_nrays, 0, 8, s));
 
@@ -629,12 +693,15 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
     StageTimer st(ctx, "ff_csc");
     scan_counts(ctx, counts, nf, sp->d_colptr);
   }
+  max_count_kernel<<<(unsigned)std::min<int64_t>(256, (nf + 255) / 256), 256, 0, s>>>(nf, rcounts,
+                                                                                  maxrow);
+  ILLUM_LAUNCH_CHECK();
   int64_t nnz = 0;
-  int hstatus = 0;
+  int hstatus[2] = {0, 0};
   ILLUM_HIP(hipMemcpyAsync(&nnz, sp->d_colptr + nf, 8, hipMemcpyDeviceToHost, s));
-  ILLUM_HIP(hipMemcpyAsync(&hstatus, status, 4, hipMemcpyDeviceToHost, s));
+  ILLUM_HIP(hipMemcpyAsync(hstatus, status, 8, hipMemcpyDeviceToHost, s));
   sync(ctx);
-  if (hstatus) throw Error(ILLUM_ERR_NAN, "computed a NaN form factor");
+  if (hstatus[0]) throw Error(ILLUM_ERR_NAN, "computed a NaN form factor");
   if (nnz >= ((int64_t)1 << 40)) throw Error(ILLUM_ERR_CAPACITY, "nnz out of range");
   sp->nnz = nnz;
   sp->has_csc = true;
@@ -648,14 +715,31 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
                                            albedo, sp->d_rowidx, sp->d_cval, nullptr);
     ILLUM_LAUNCH_CHECK();
   }
-  // CSR via the bit transpose.
   sp->d_rowptr = palloc<int64_t>(ctx, nf + 1);
   sp->d_colidx = palloc<int32_t>(ctx, nnz + CSR_PAD);
   sp->d_rval = palloc<double>(ctx, nnz + CSR_PAD);
   ILLUM_HIP(hipMemsetAsync(sp->d_colidx + nnz, 0, 4 * CSR_PAD, s));
   ILLUM_HIP(hipMemsetAsync(sp->d_rval + nnz, 0, 8 * CSR_PAD, s));
-  {
+  if (hstatus[1] <= CSR_SORT_MAX && !getenv("ILLUM_FF_CSR_TRANSPOSE")) {
+    // CSR by a sparse transpose of the CSC entries (rows short enough to sort
+    // in LDS); the transposed bit matrix's workspace holds the unsorted rows.
     StageTimer st(ctx, "ff_csr");
+    scan_counts(ctx, rcounts, nf, sp->d_rowptr);
+    const size_t tjb = ((size_t)4 * nnz + 255) & ~(size_t)255;
+    char *tmp = (char *)ctx_workspace(ctx, WS_FF_MASK_T, tjb + 8 * (size_t)std::max<int64_t>(1, nnz));
+    int32_t *tj = (int32_t *)tmp;
+    double *tv = (double *)(tmp + tjb);
+    ILLUM_HIP(hipMemsetAsync(counts, 0, 4 * nf, s));  // the CSC counts are spent: cursors
+    const unsigned g4 = (unsigned)((nf + 3) / 4);
+    csr_scatter_kernel<<<g4, 256, 0, s>>>(nf, sp->d_colptr, sp->d_rowidx, sp->d_cval,
+                                          sp->d_rowptr, counts, tj, tv);
+    ILLUM_LAUNCH_CHECK();
+    csr_sort_kernel<<<g4, 256, 0, s>>>(nf, sp->d_rowptr, tj, tv, sp->d_colidx, sp->d_rval);
+    ILLUM_LAUNCH_CHECK();
+  } else {
+    // CSR via the bit transpose.
+    StageTimer st(ctx, "ff_csr");
+    uint32_t *MT = (uint32_t *)ctx_workspace(ctx, WS_FF_MASK_T, mbytes);
     dim3 g(nwords / 16, (unsigned)(rows_p / 512));
     uint8_t *tile_nz = (uint8_t *)ctx_workspace(ctx, WS_FF_TILES, (size_t)g.x * g.y);
     bit_transpose_kernel<<<g, 256, 0, s>>>(M, MT, nwords, tile_nz);

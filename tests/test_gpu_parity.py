@@ -168,6 +168,28 @@ def test_form_factors_albedo_vector(api):
     np.testing.assert_allclose(gv, v, rtol=1e-12, atol=0)
 
 
+def test_csr_sparse_transpose_matches_bit_transpose(api, monkeypatch):
+    # CSR(F) is built by scattering the CSC entries into rows and sorting each
+    # row in LDS (rows of up to 1,024 entries), or, forced here, through the
+    # transposed visibility bits; both must give the same arrays, and CSR must
+    # be CSC transposed (scipy), with per-facet albedo.
+    import scipy.sparse as sps
+    V, F = _mesh(api, 4, 40, seed=9)
+    ctx = api.IllumContext(V=V, F=F)
+    rho = np.random.default_rng(3).uniform(0.05, 0.9, len(F))
+    Fa = ctx.compute_F(rho)
+    monkeypatch.setenv("ILLUM_FF_CSR_TRANSPOSE", "1")
+    Fb = ctx.compute_F(rho)
+    ra, rb = Fa.csr(), Fb.csr()
+    for x, y in zip(ra, rb):
+        assert np.array_equal(x, y)
+    cp, ri, v = Fa.csc()
+    A = sps.csc_matrix((v, ri, cp), shape=(len(F), len(F))).tocsr()
+    A.sort_indices()
+    assert np.array_equal(A.indptr, ra[0]) and np.array_equal(A.indices, ra[1])
+    assert np.array_equal(A.data, ra[2])
+
+
 @pytest.mark.parametrize("walk", ["default", "packet"])
 @pytest.mark.parametrize("nphi,theta_eps", [(65, 1e-3), (17, 1e-2)])
 def test_horizons_bitwise(api, monkeypatch, nphi, theta_eps, walk):
