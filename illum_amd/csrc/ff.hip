@@ -35,6 +35,7 @@ constexpr int FF_STACK = 8;  // LDS traversal stack entries per lane (deeper: th
 // at t <= 1; a ray with no hit below the bound is traced again unbounded,
 // which keeps the result identical to an unbounded closest-hit query.
 constexpr float FF_TINIT = 1.0009765625f;
+constexpr int FF_STAGE = 256;  // staged receivers per source row (CSC fill list)
 
 // Form factor F(i, j) for receiver i and source j (illum.cpp:100-119), in FP64
 // from the FP32 centroids/normals.  Returns 0 for r < DBL_EPSILON.
@@ -64,7 +65,7 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
     int *__restrict__ status, int nrows, int jstride, unsigned long long *__restrict__ stats,
     const float4 *__restrict__ clus, int nclus, int scap, const int32_t *__restrict__ body_of,
     const int32_t *__restrict__ body_off, const int32_t *__restrict__ roots,
-    int32_t *__restrict__ col_counts) {
+    int32_t *__restrict__ col_counts, int32_t *__restrict__ stage) {
   extern __shared__ uint32_t smem[];
   uint32_t *bits = smem;
   int *queue = (int *)(smem + nwords);
@@ -73,7 +74,7 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
   uint16_t *clist = (uint16_t *)(queue + FF_QCAP);
   int *stack = (int *)(clist + ((nclus + 1) & ~1));
   __shared__ int s_qn, s_rays, s_nc;
-  __shared__ int s_red[FF_BLOCK / 64];
+  __shared__ int s_red[FF_BLOCK / 64], s_scan[FF_BLOCK / 64];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   // row = output row; source j = row * jstride (jstride 1 except when sampling)
   const int row = xcd_remap(blockIdx.x, nrows);
@@ -274,6 +275,38 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(
     for (int w = 0; w < FF_BLOCK / 64; ++w) s += s_red[w];
     counts[row] = s;
   }
+  // 4. A row of at most FF_STAGE kept receivers is also listed, in ascending
+  //    order, in its staging slot, so the CSC fill reads the short list
+  //    rather than the bit row.  Threads own contiguous words; a block scan
+  //    of their counts places each thread's receivers.
+  if (stage) {
+    int total = 0;
+    for (int w = 0; w < FF_BLOCK / 64; ++w) total += s_red[w];
+    if (total <= FF_STAGE) {  // uniform
+      const int per = (nwords + FF_BLOCK - 1) / FF_BLOCK;
+      const int w0 = min(nwords, tid * per), w1 = min(nwords, w0 + per);
+      int c = 0;
+      for (int w = w0; w < w1; ++w) c += __popc(bits[w]);
+      int incl = c;
+      for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+      }
+      if (lane == 63) s_scan[wid] = incl;
+      __syncthreads();
+      int off = incl - c;
+      for (int w = 0; w < wid; ++w) off += s_scan[w];
+      int32_t *out = stage + (size_t)row * FF_STAGE;
+      for (int w = w0; w < w1; ++w) {
+        uint32_t rest = bits[w];
+        while (rest) {
+          const int b = __ffs(rest) - 1;
+          rest &= rest - 1u;
+          out[off++] = 32 * w + b;
+        }
+      }
+    }
+  }
   if (stats && tid == 0) atomicAdd(stats, (unsigned long long)s_rays);
   if (COUNT && stats) {
     // pair tests, box tests, triangle tests: wave sums, one atomic each.
@@ -312,15 +345,17 @@ constexpr int FILL_MAXW = 16;
 
 // Fills one CSC column (csr == 0: row of M is source j, bits are receivers i)
 // or one CSR row (csr == 1: row of M^T is receiver i, bits are sources j).
+// Rows of at most min_count entries are left to ff_fill_list_kernel.
 __global__ __launch_bounds__(256) void ff_fill_kernel(
     int nrows, int nwords, const uint32_t *__restrict__ m, const int64_t *__restrict__ ptr,
     int csr, const float4 *__restrict__ pr4, const float4 *__restrict__ n4,
     const double *__restrict__ area, const double *__restrict__ albedo_vec, double albedo,
     int32_t *__restrict__ idx_out, double *__restrict__ val_out,
-    const uint8_t *__restrict__ tile_nz) {
+    const uint8_t *__restrict__ tile_nz, int64_t min_count) {
   __shared__ int s_wave[4];
   const int row = blockIdx.x;
   if (row >= nrows) return;
+  if (ptr[row + 1] - ptr[row] <= min_count) return;  // filled from its staged list
   const int tid = threadIdx.x;
   const int per = (nwords + 255) / 256;
   const int w0 = tid * per, w1 = min(nwords, w0 + per);
@@ -390,6 +425,29 @@ __global__ __launch_bounds__(256) void ff_fill_kernel(
   }
 }
 
+// CSC columns of at most FF_STAGE entries, from the receivers ff_visibility
+// listed in ascending order in their staging slots: one wave per column,
+// the same values as ff_fill_kernel's (ff_value on the same arguments).
+__global__ __launch_bounds__(256) void ff_fill_list_kernel(
+    int64_t nf, const int32_t *__restrict__ stage, const int64_t *__restrict__ colptr,
+    const float4 *__restrict__ pr4, const float4 *__restrict__ n4,
+    const double *__restrict__ area, const double *__restrict__ albedo_vec, double albedo,
+    int32_t *__restrict__ rowidx, double *__restrict__ cval) {
+  const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= nf) return;
+  const int64_t base = colptr[j];
+  const int64_t n = colptr[j + 1] - base;
+  if (n > FF_STAGE) return;  // ff_fill_kernel reads this column's bit row
+  const float4 pj = pr4[j], nj = n4[j];
+  const double Aj = area[j];
+  for (int e = threadIdx.x & 63; e < n; e += 64) {
+    const int i = stage[(size_t)j * FF_STAGE + e];
+    const double rho = albedo_vec ? albedo_vec[i] : albedo;
+    rowidx[base + e] = i;
+    cval[base + e] = ff_value(pr4[i], n4[i], pj, nj, Aj) * rho;
+  }
+}
+
 // CSR(F) from CSC(F) when every row is short.  CSR row i lists the sources j
 // of the CSC entries (i, j) in ascending j, with the same values (both fills
 // evaluate ff_value on the same receiver / source arguments).
@@ -442,14 +500,32 @@ __global__ __launch_bounds__(256) void csr_sort_kernel(int64_t nf,
   }
 }
 
-// Largest of n counts, into *out (zeroed by the caller).
-__global__ __launch_bounds__(256) void max_count_kernel(int64_t n, const int32_t *__restrict__ c,
+// Largest of n counts of a and of b, into out[0] and out[1] (zeroed by the
+// caller): one atomic per block and array.
+__global__ __launch_bounds__(256) void max_count_kernel(int64_t n, const int32_t *__restrict__ a,
+                                                        const int32_t *__restrict__ b,
                                                         int32_t *__restrict__ out) {
-  int m = 0;
-  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256)
-    m = max(m, c[k]);
-  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
-  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+  __shared__ int red[2][4];
+  int ma = 0, mb = 0;
+  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * 256) {
+    ma = max(ma, a[k]);
+    mb = max(mb, b[k]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    ma = max(ma, __shfl_xor(ma, o));
+    mb = max(mb, __shfl_xor(mb, o));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = ma;
+    red[1][threadIdx.x >> 6] = mb;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    const int m = max(max(red[threadIdx.x][0], red[threadIdx.x][1]),
+                      max(red[threadIdx.x][2], red[threadIdx.x][3]));
+    atomicMax(&out[threadIdx.x], m);
+  }
 }
 
 // Transposes a (rows_p x 32*nwords)-bit matrix stored as rows of nwords words
@@ -656,9 +732,11 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
     d_alb = (double *)ctx_workspace(ctx, WS_FF_ALBEDO, 8 * nf);
     ILLUM_HIP(hipMemcpyAsync(d_alb, albedo_vec, 8 * nf, hipMemcpyHostToDevice, s));
   }
-  // status word (NaN seen) and the largest CSR row length
-  int *status = (int *)ctx_workspace(ctx, WS_FF_STATUS, 8);
+  // status word (NaN seen), the largest CSR row and CSC column lengths
+  int *status = (int *)ctx_workspace(ctx, WS_FF_STATUS, 12);
   int32_t *maxrow = (int32_t *)(status + 1);
+  // each source's kept receivers, listed when at most FF_STAGE
+  int32_t *stage = (int32_t *)ctx_workspace(ctx, WS_FF_STAGE, (size_t)nf * FF_STAGE * 4);
   // M (and, for rows too long to sort, its transpose) live in the context's
   // workspace.  Rows >= nf of M (padding to whole 512-row tiles) must read as
   // zero for the transpose.
@@ -670,7 +748,7 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   ILLUM_HIP(hipMemsetAsync(rcounts, 0, 4 * nf, s));
   if (rows_p > nf)
     ILLUM_HIP(hipMemsetAsync(M + (size_t)nf * nwords, 0, (size_t)(rows_p - nf) * nwords * 4, s));
-  ILLUM_HIP(hipMemsetAsync(status, 0, 8, s));
+  ILLUM_HIP(hipMemsetAsync(status, 0, 12, s));
   sp->d_nrays = palloc<unsigned long long>(ctx, 1);
   ILLUM_HIP(hipMemsetAsync(sp->d_nrays, 0, 8, s));
 
@@ -684,7 +762,7 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
         (float)offset, mesh->d_area, d_alb, albedo, sp->d_vis, M, counts, status,
         (int)nf, 1, sp->d_nrays, cluster_cull() ? (const float4 *)mesh->d_clus : nullptr,
         (int)mesh->nclus, stack_slots,
-        mesh->d_body, mesh->d_boff, bvh->d_roots, rcounts);
+        mesh->d_body, mesh->d_boff, bvh->d_roots, rcounts, stage);
     ILLUM_LAUNCH_CHECK();
   }
   // CSC
@@ -693,13 +771,13 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
     StageTimer st(ctx, "ff_csc");
     scan_counts(ctx, counts, nf, sp->d_colptr);
   }
-  max_count_kernel<<<(unsigned)std::min<int64_t>(256, (nf + 255) / 256), 256, 0, s>>>(nf, rcounts,
-                                                                                  maxrow);
+  max_count_kernel<<<(unsigned)std::min<int64_t>(64, (nf + 255) / 256), 256, 0, s>>>(
+      nf, rcounts, counts, maxrow);
   ILLUM_LAUNCH_CHECK();
   int64_t nnz = 0;
-  int hstatus[2] = {0, 0};
+  int hstatus[3] = {0, 0, 0};  // NaN, longest CSR row, longest CSC column
   ILLUM_HIP(hipMemcpyAsync(&nnz, sp->d_colptr + nf, 8, hipMemcpyDeviceToHost, s));
-  ILLUM_HIP(hipMemcpyAsync(hstatus, status, 8, hipMemcpyDeviceToHost, s));
+  ILLUM_HIP(hipMemcpyAsync(hstatus, status, 12, hipMemcpyDeviceToHost, s));
   sync(ctx);
   if (hstatus[0]) throw Error(ILLUM_ERR_NAN, "computed a NaN form factor");
   if (nnz >= ((int64_t)1 << 40)) throw Error(ILLUM_ERR_CAPACITY, "nnz out of range");
@@ -708,12 +786,20 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   sp->d_rowidx = palloc<int32_t>(ctx, nnz);
   sp->d_cval = palloc<double>(ctx, nnz);
   {
+    // columns of at most FF_STAGE entries from their staged lists, longer
+    // ones from their bit rows
     StageTimer st(ctx, "ff_csc");
-    ff_fill_kernel<<<(int)nf, 256, 0, s>>>((int)nf, nwords, M, sp->d_colptr, 0,
-                                           (const float4 *)mesh->d_pr4,
-                                           (const float4 *)mesh->d_n4, mesh->d_area, d_alb,
-                                           albedo, sp->d_rowidx, sp->d_cval, nullptr);
+    ff_fill_list_kernel<<<(unsigned)((nf + 3) / 4), 256, 0, s>>>(
+        nf, stage, sp->d_colptr, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4,
+        mesh->d_area, d_alb, albedo, sp->d_rowidx, sp->d_cval);
     ILLUM_LAUNCH_CHECK();
+    if (hstatus[2] > FF_STAGE) {
+      ff_fill_kernel<<<(int)nf, 256, 0, s>>>((int)nf, nwords, M, sp->d_colptr, 0,
+                                             (const float4 *)mesh->d_pr4,
+                                             (const float4 *)mesh->d_n4, mesh->d_area, d_alb,
+                                             albedo, sp->d_rowidx, sp->d_cval, nullptr, FF_STAGE);
+      ILLUM_LAUNCH_CHECK();
+    }
   }
   sp->d_rowptr = palloc<int64_t>(ctx, nf + 1);
   sp->d_colidx = palloc<int32_t>(ctx, nnz + CSR_PAD);
@@ -748,7 +834,7 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
     ff_fill_kernel<<<(int)nf, 256, 0, s>>>((int)nf, nwords, MT, sp->d_rowptr, 1,
                                            (const float4 *)mesh->d_pr4,
                                            (const float4 *)mesh->d_n4, mesh->d_area, d_alb,
-                                           albedo, sp->d_colidx, sp->d_rval, tile_nz);
+                                           albedo, sp->d_colidx, sp->d_rval, tile_nz, -1);
     ILLUM_LAUNCH_CHECK();
   }
   sync(ctx);
@@ -789,7 +875,7 @@ double time_visibility_sample(illum_bvh *bvh, int max_sources, int layout_choice
         (const float4 *)bvh->d_tris,
         1e-5f, mesh->d_area, nullptr, 1.0, vis.p, M.p, counts.p, status.p, nrows, jstride, nullptr,
         cluster_cull() ? (const float4 *)mesh->d_clus : nullptr, (int)mesh->nclus, stack_slots,
-        mesh->d_body, mesh->d_boff, bvh->d_roots, nullptr);
+        mesh->d_body, mesh->d_boff, bvh->d_roots, nullptr, nullptr);
     ILLUM_LAUNCH_CHECK();
     ILLUM_HIP(hipEventRecord(e1, s));
     ILLUM_HIP(hipEventSynchronize(e1));
@@ -833,7 +919,7 @@ void form_factor_work(illum_bvh *bvh, double offset, int64_t *out4) {
         (const float4 *)bvh->d_tris,
       (float)offset, mesh->d_area, nullptr, 1.0, vis, M, counts, status.p, (int)nf, 1, stats.p,
       cluster_cull() ? (const float4 *)mesh->d_clus : nullptr, (int)mesh->nclus, stack_slots,
-        mesh->d_body, mesh->d_boff, bvh->d_roots, nullptr);
+        mesh->d_body, mesh->d_boff, bvh->d_roots, nullptr, nullptr);
   ILLUM_LAUNCH_CHECK();
   unsigned long long h[4];
   ILLUM_HIP(hipMemcpyAsync(h, stats.p, 32, hipMemcpyDeviceToHost, s));

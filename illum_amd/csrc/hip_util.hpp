@@ -92,6 +92,7 @@ enum WorkspaceSlot {
   WS_FF_TILES = 10, // nonzero flags of the bit transpose's 512 x 512 tiles
   WS_FF_ALBEDO = 11,
   WS_FF_STATUS = 12,
+  WS_FF_STAGE = 13, // each source's kept receivers (short rows), for the CSC fill
 };
 
 // Device blocks from the context's reuse pool (see illum_ctx::pool): a free
