@@ -190,6 +190,30 @@ def test_csr_sparse_transpose_matches_bit_transpose(api, monkeypatch):
     assert np.array_equal(A.data, ra[2])
 
 
+def test_long_rows_and_columns_inward_sphere(api, monkeypatch):
+    # Facets of a sphere wound inward see nearly every other facet: columns
+    # longer than the 256-entry staging slot (filled from their bit rows) and
+    # CSR rows longer than the one-wave sort (sorted a block per row).  CSC
+    # against the oracle, CSR against the bit-transpose path.
+    V, F = _mesh(api, 2, 0)
+    F = np.ascontiguousarray(F[:, ::-1])
+    ctx = api.IllumContext(V=V, F=F)
+    Fa = ctx.compute_F(0.3)
+    cp, ri, v = Fa.csc()
+    assert np.diff(cp).max() > 256
+    o = O.geometry(V, F)
+    bits, _ = O.visibility(o)
+    ocp, ori, ov = O.form_factors(o, bits, albedo=0.3)
+    assert np.array_equal(cp, ocp) and np.array_equal(ri, ori)
+    np.testing.assert_allclose(v, ov, rtol=1e-12, atol=0)
+    ra = Fa.csr()
+    assert np.diff(ra[0]).max() > 128
+    monkeypatch.setenv("ILLUM_FF_CSR_TRANSPOSE", "1")
+    rb = ctx.compute_F(0.3).csr()
+    for x, y in zip(ra, rb):
+        assert np.array_equal(x, y)
+
+
 @pytest.mark.parametrize("walk", ["default", "packet"])
 @pytest.mark.parametrize("nphi,theta_eps", [(65, 1e-3), (17, 1e-2)])
 def test_horizons_bitwise(api, monkeypatch, nphi, theta_eps, walk):
