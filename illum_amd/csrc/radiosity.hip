@@ -395,8 +395,9 @@ __device__ void decide(int S, const double *col, const double *e2, SolveCtl *ctl
 // Adds the G partials of columns [b cw, (b + 1) cw) (block b) in a fixed
 // order into col[].  record = 1: nothing once done is set, and a single block
 // also takes the stop decision (with several blocks decide_kernel follows).
-// record = 0 with ex (the mixed method's start, col = ||E_s||^2): ex[0][s]
-// and ex[1][s] receive the scale exponents of d_0 and d_1.
+// record = 0 (the solve's start, col = ||E_s||^2) also resets the control
+// block, and with ex (the mixed method) ex[0][s] and ex[1][s] receive the
+// scale exponents of d_0 and d_1.
 constexpr int COLNORM_CW = 32;
 
 __global__ __launch_bounds__(256) void colnorm_final_kernel(int G, int S, int cw,
@@ -407,6 +408,10 @@ __global__ __launch_bounds__(256) void colnorm_final_kernel(int G, int S, int cw
                                                             double *__restrict__ hist, double tol,
                                                             int record, int *__restrict__ ex) {
   if (record && ctl->done) return;
+  if (!record && blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl->done = 0;
+    ctl->nprod = 0;
+  }
   __shared__ double red[256];
   const int c0 = blockIdx.x * cw;
   const int cwb = std::min(cw, S - c0);
@@ -430,11 +435,6 @@ __global__ __launch_bounds__(256) void decide_kernel(int S, const double *__rest
   if (ctl->done) return;
   __shared__ double red[256];
   decide(S, col, e2, ctl, hist, tol, red, ex);
-}
-
-__global__ void solve_init_kernel(SolveCtl *ctl) {
-  ctl->done = 0;
-  ctl->nprod = 0;
 }
 
 // Mixed method start (row-major E): d_0 = fp16(E 2^ex[0][s]), B = 0 (B
@@ -778,10 +778,8 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
     Dring = (_Float16 *)ctx_workspace(ctx, WS_RAD_D32, 2 * (size_t)R * n);
   }
 
-  solve_init_kernel<<<1, 1, 0, s>>>(ctl);
-  ILLUM_LAUNCH_CHECK();
-  // ||E_s||^2 from the column-major input (and, mixed, the exponents of d_0
-  // and d_1), then E row-major (with d_0 and B = 0 in the same pass).  With
+  // ||E_s||^2 from the column-major input (resetting the control block and,
+  // mixed, writing the exponents of d_0 and d_1), then E row-major (with d_0 and B = 0 in the same pass).  With
   // S = 1 both layouts coincide and a device E is read in place (it is never
   // written).
   {
