@@ -347,6 +347,41 @@ def test_radiosity_mixed_column_scales_and_folds(api, S):
     assert np.all(np.abs(B - B64)[:, ok].max(axis=0) <= 1e-10 * scale[ok])
 
 
+@pytest.mark.parametrize("S", [1, 128])
+@pytest.mark.parametrize("method", ["fp64", "mixed"])
+def test_radiosity_not_converged_reports_and_writes_B(api, S, method):
+    # Stopped by max_iter above the tolerance: status ILLUM_ERR_NOT_CONVERGED,
+    # with B written.  Mixed: B is the sum of the corrections so far, whose
+    # FP64 residual is exactly the last one reported.  FP64: B is the newest
+    # iterate, whose residual is below the last one reported (that of the
+    # iterate before it).
+    import scipy.sparse as sps
+    C, L = api.C, api._lib
+    V, F = _mesh(api, 3, 20, seed=4)
+    ctx = api.IllumContext(V=V, F=F)
+    Fm = ctx.compute_F(0.9)
+    nf = len(F)
+    E = np.random.default_rng(S).uniform(0, 500, (nf, S))
+    Ecm = np.ascontiguousarray(E.T)
+    Bcm = np.zeros_like(Ecm)
+    resid = np.zeros(3)
+    iters = C.c_int()
+    m = {"fp64": L.ILLUM_RAD_JACOBI_FP64, "mixed": L.ILLUM_RAD_MIXED}[method]
+    with pytest.raises(L.IllumError) as ei:
+        L.call("illum_radiosity_solve", Fm.h, api._ptr(Ecm), S, m, 1e-15, 2, api._ptr(Bcm),
+               api._ptr(resid), C.byref(iters), 0)
+    assert ei.value.code == L.ILLUM_ERR_NOT_CONVERGED
+    assert iters.value == 2 and resid[1] > 1e-15 and resid[1] < resid[0]
+    cp, ri, v = Fm.csc()
+    A = sps.csc_matrix((v, ri, cp), shape=(nf, nf))
+    B = Bcm.T
+    rel = (np.linalg.norm(E - (B - A @ B), axis=0) / np.linalg.norm(E, axis=0)).max()
+    if method == "mixed":
+        assert abs(rel - resid[1]) <= 1e-9 * resid[1]
+    else:
+        assert rel < resid[1]
+
+
 def _cn_case(nf, nz, seed):
     rng = np.random.default_rng(seed)
     z = 0.5 * (np.exp(np.linspace(0, 3, nz)) - 1 + 0.01) / (np.exp(3) - 1 + 0.01)
