@@ -522,6 +522,66 @@ def test_async_time_loop_matches_synchronous(api, method):
     assert res[-1] <= 1e-8
 
 
+@pytest.mark.parametrize("method,tol_K", [("fp64", 1e-5), ("mixed", 1e-3)])
+def test_coupled_loop_5120_facets_256_steps_vs_oracle(api, method, tol_K):
+    """The reference's CPU-runnable configuration: 5,120 facets, 64 azimuth
+    intervals (nphi = 65), one sun, 256 coupled steps on 40 layers
+    (illum_cli.cpp:403-456: direct -> E -= F_surf -> solve -> CN step), on
+    device buffers, against the oracle's replay with Gauss-Seidel to 1e-13.
+    Horizons bit-exact; temperatures within tol_K of the oracle."""
+    from illum_amd import _lib
+    V, F = _mesh(api, 5, 60, seed=31)
+    ctx = api.IllumContext(V=V, F=F)
+    Fm = ctx.compute_F(0.12)
+    nphi = 65
+    hz = ctx.make_horizons(nphi=nphi)
+    g = O.geometry(V, F)
+    ho, _ = O.horizons(g, nphi=nphi)
+    assert np.array_equal(hz, ho)
+    nf = len(F)
+    nz, nsteps = 40, 256
+    z = 0.5 * (np.exp(np.linspace(0, 3, nz)) - 1 + 0.01) / (np.exp(3) - 1 + 0.01)
+    dt = 5.342 * 3600 / nsteps
+    ang = 2 * np.pi * np.arange(nsteps) / nsteps
+    sun = 2.27e8 * np.stack([np.cos(ang) * 0.94, np.sin(ang) * 0.94, np.full(nsteps, 0.34)], 1)
+    c = api.default_context()
+    hzd = api.DeviceArray.from_host(c, np.ascontiguousarray(np.asarray(hz).T))
+    disk_h = api.fib_spiral(100)
+    disk = api.DeviceArray.from_host(c, np.ascontiguousarray(disk_h.T))
+    m = {"fp64": _lib.ILLUM_RAD_JACOBI_FP64, "mixed": _lib.ILLUM_RAD_MIXED}[method]
+    dev = _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_OUT_ON_DEVICE | _lib.ILLUM_ASYNC
+    th = api.ThermalModel(nf, z, 200.0, 1.19177e6, 200.0, emissivity=0.95, precision=method)
+    E = api.DeviceArray(c, (nf,))
+    B = api.DeviceArray(c, (nf,))
+    suns = [api.DeviceArray.from_host(c, sun[k]) for k in range(nsteps)]
+    resid = np.zeros(101)
+    it = api.C.c_int()
+    for k in range(nsteps):
+        _lib.call("illum_direct_radiosity", ctx.mesh.h, api._ptr(hzd), nphi, api._ptr(suns[k]), 1,
+                  api._ptr(disk), 100, api._ptr(E), dev)
+        th.get_radiosity(out=E, accumulate=True)
+        _lib.call("illum_radiosity_solve", Fm.h, api._ptr(E), 1, m, 1e-10, 100, api._ptr(B),
+                  api._ptr(resid), api.C.byref(it), dev)
+        _lib.call("illum_thermal_step", th.h, dt, api._ptr(B),
+                  _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_ASYNC)
+    c.synchronize()
+    Tg = th.T.copy()
+    th.close()
+    # oracle replay
+    bits, _ = O.visibility(g)
+    cp, ri, v = O.form_factors(g, bits, albedo=0.12)
+    T = np.full((nz + 1, nf), 200.0)
+    Qprev = np.zeros(nf)
+    Fs = np.zeros(nf)
+    for k in range(nsteps):
+        Eo = -Fs + O.direct(g, ho, sun[k], disk_h)
+        Bo, _, _ = O.gauss_seidel(nf, cp, ri, v, Eo, max_sweeps=200, rel_tol=1e-13)
+        T, Qprev, Fs, _ = O.thermal_run([0, nz], z, np.full(nz, 200.0), np.full(nz, 1.19177e6),
+                                        T, Qprev, Bo[None, :], dt, 0.95)
+    assert np.isfinite(Tg).all()
+    assert np.abs(Tg - T).max() <= tol_K
+
+
 def test_conductionQ_dropin_vs_reference_fortran_golden(api):
     import os
     gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "cn_golden.npz"))
