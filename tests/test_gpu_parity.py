@@ -382,6 +382,33 @@ def test_radiosity_not_converged_reports_and_writes_B(api, S, method):
         assert rel < resid[1]
 
 
+@pytest.mark.parametrize("S", [128, 200])
+@pytest.mark.parametrize("method", ["fp64", "mixed"])
+def test_radiosity_on_a_given_csr_matrix_vs_dense_solve(api, S, method):
+    # F from illum_sparse_create_csr (not a form-factor assembly): n = 1,001
+    # rows, so the wide product's last group of rows is partial; rows of
+    # random nonnegative values summing to 0.5.  B against a dense solve.
+    rng = np.random.default_rng(S)
+    n = 1001
+    rowptr, colidx, val = [0], [], []
+    for i in range(n):
+        k = int(rng.integers(0, 40))
+        cols = np.sort(rng.choice(np.delete(np.arange(n), i), size=k, replace=False))
+        w = rng.uniform(0, 1, k)
+        colidx.extend(cols.tolist())
+        val.extend((0.5 * w / max(w.sum(), 1e-300)).tolist())
+        rowptr.append(len(colidx))
+    Fm = api.SparseMatrix.from_csr(api.default_context(), n, rowptr, colidx, val)
+    E = rng.uniform(0, 500, (n, S))
+    B, res, it = api.radiosity_solve(Fm, E, method=method, rel_tol=1e-12, max_iter=400)
+    A = np.zeros((n, n))
+    for i in range(n):
+        A[i, colidx[rowptr[i]:rowptr[i + 1]]] = val[rowptr[i]:rowptr[i + 1]]
+    Bd = np.linalg.solve(np.eye(n) - A, E)
+    assert res[-1] <= 1e-12
+    np.testing.assert_allclose(B, Bd, rtol=1e-10, atol=1e-10 * np.abs(Bd).max())
+
+
 def _cn_case(nf, nz, seed):
     rng = np.random.default_rng(seed)
     z = 0.5 * (np.exp(np.linspace(0, 3, nz)) - 1 + 0.01) / (np.exp(3) - 1 + 0.01)
