@@ -308,7 +308,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the form-factor path "
-              "shards by source facet with no communication, see DESIGN.md", file=sys.stderr)
+              "would shard by source facet (a contiguous block of CSC columns per GPU, no "
+              "communication), which is not built here", file=sys.stderr)
         sys.exit(2)
 
     from illum_amd import api
