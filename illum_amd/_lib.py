@@ -24,6 +24,7 @@ ILLUM_IN_ON_DEVICE = 1
 ILLUM_OUT_ON_DEVICE = 2
 ILLUM_ACCUMULATE = 4
 ILLUM_ASYNC = 8
+ILLUM_FF_KEEP_VISIBILITY = 16
 ILLUM_RAD_JACOBI_FP64 = 1
 ILLUM_RAD_MIXED = 2
 ILLUM_THERM_FP64 = 1
@@ -63,12 +64,15 @@ SIGNATURES = {
     "illum_bvh_info": [_P, _P, _P, _P],
     "illum_bvh_closest_hit": [_P, _P, _P, _I64, _P, _P],
     "illum_form_factors": [_P, _D, _P, _D, _PP],
+    "illum_form_factors_ex": [_P, _D, _P, _D, _I, _PP],
     "illum_form_factors_work": [_P, _D, _P],
     "illum_sparse_create_csr": [_P, _I64, _I64, _P, _P, _P, _PP],
     "illum_sparse_destroy": [_P],
     "illum_sparse_info": [_P, _P, _P],
     "illum_sparse_copy_csc": [_P, _P, _P, _P],
     "illum_sparse_copy_csr": [_P, _P, _P, _P],
+    "illum_sparse_visibility_nnz": [_P, _P],
+    "illum_sparse_copy_visibility_lists": [_P, _P, _P],
     "illum_sparse_visibility_words": [_P, _P],
     "illum_sparse_copy_visibility": [_P, _P],
     "illum_sparse_rays": [_P, _P],

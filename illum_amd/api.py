@@ -40,6 +40,14 @@ def _is_dev(a):
     return a is not None and hasattr(a, "is_cuda") and a.is_cuda
 
 
+def _shape_error(msg):
+    return _lib.IllumError(_lib.ILLUM_ERR_SHAPE, msg)
+
+
+def _nbytes(a):
+    return a.nbytes if hasattr(a, "nbytes") else int(np.prod(a.shape)) * np.dtype(a.dtype).itemsize
+
+
 class Context:
     """One GPU + one HIP stream (illum_ctx).
 
@@ -281,6 +289,13 @@ class SparseMatrix:
         rowptr = np.ascontiguousarray(rowptr, np.int64)
         colidx = np.ascontiguousarray(colidx, np.int32)
         val = np.ascontiguousarray(val, np.float64)
+        # illum_sparse_create_csr reads rowptr[0..n] and rowptr[n] entries of
+        # colidx and val
+        if rowptr.shape != (n + 1,):
+            raise _shape_error("rowptr must hold n + 1 offsets")
+        nnz = int(rowptr[-1])
+        if colidx.shape != (nnz,) or val.shape != (nnz,):
+            raise _shape_error("colidx and val must hold rowptr[n] entries")
         h = C.c_void_p()
         call("illum_sparse_create_csr", ctx.h, n, int(rowptr[-1]), _ptr(rowptr), _ptr(colidx),
              _ptr(val), C.byref(h))
@@ -311,12 +326,23 @@ class SparseMatrix:
         call("illum_sparse_copy_csr", self.h, _ptr(rp), _ptr(ci), _ptr(v))
         return rp, ci, v
 
+    def visibility_lists(self):
+        """Raw visibility sets before the zero drop (assembled with
+        keep_visibility): (ptr, idx) with source j's hits idx[ptr[j]:ptr[j+1]]."""
+        nv = C.c_int64()
+        call("illum_sparse_visibility_nnz", self.h, C.byref(nv))
+        ptr = np.zeros(self.n + 1, np.int64)
+        idx = np.zeros(nv.value, np.int32)
+        call("illum_sparse_copy_visibility_lists", self.h, _ptr(ptr), _ptr(idx))
+        return ptr, idx
+
     def visibility(self):
-        """Raw visibility bit rows (n x words uint32) before the zero drop."""
-        w = C.c_int64()
-        call("illum_sparse_visibility_words", self.h, C.byref(w))
-        bits = np.zeros((self.n, w.value), np.uint32)
-        call("illum_sparse_copy_visibility", self.h, _ptr(bits))
+        """Raw visibility bit rows (n x ceil(n/32) uint32) before the zero drop."""
+        ptr, idx = self.visibility_lists()
+        words = (self.n + 31) // 32
+        bits = np.zeros((self.n, words), np.uint32)
+        rows = np.repeat(np.arange(self.n), np.diff(ptr))
+        np.bitwise_or.at(bits, (rows, idx >> 5), (np.uint32(1) << (idx & 31).astype(np.uint32)))
         return bits
 
     def rays(self):
@@ -364,19 +390,22 @@ class IllumContext:
         occ = Mesh(self.ctx, obj_path=path, vertex_divisor=vertex_divisor)
         self._horizon_ctx = (occ, BVH(occ, self.leaf_size))
 
-    def compute_F(self, albedo, offset=1e-5):
-        """albedo: scalar, per-facet array, or a path to an Armadillo vector."""
+    def compute_F(self, albedo, offset=1e-5, keep_visibility=False):
+        """albedo: scalar, per-facet array, or a path to an Armadillo vector.
+        keep_visibility: also keep the raw visibility sets (testing aid)."""
         if isinstance(albedo, str):
             from .armaio import load_mat
             albedo = load_mat(albedo).ravel()
         h = C.c_void_p()
+        flags = _lib.ILLUM_FF_KEEP_VISIBILITY if keep_visibility else 0
         if np.isscalar(albedo):
-            call("illum_form_factors", self.bvh.h, float(albedo), None, offset, C.byref(h))
+            call("illum_form_factors_ex", self.bvh.h, float(albedo), None, offset, flags,
+                 C.byref(h))
         else:
             a = np.ascontiguousarray(albedo, np.float64)
             if a.shape != (self.num_faces,):
-                raise ValueError("albedo vector must have one entry per facet")
-            call("illum_form_factors", self.bvh.h, 0.0, _ptr(a), offset, C.byref(h))
+                raise _shape_error("albedo vector must have one entry per facet")
+            call("illum_form_factors_ex", self.bvh.h, 0.0, _ptr(a), offset, flags, C.byref(h))
         return SparseMatrix(self.ctx, h)
 
     def render(self, data=None, mode="orthographic", width=512, height=512, az=0.0, el=0.0,
@@ -433,7 +462,13 @@ class IllumContext:
         sun = np.ascontiguousarray(sun.reshape(3, -1).T)  # S x 3 row-major == 3 x S col-major
         S = sun.shape[0]
         disk = np.asfortranarray(np.asarray(disk_xy, np.float64))
+        if disk.ndim != 2 or disk.shape[1] != 2:
+            raise _shape_error("disk_xy must be n x 2")
+        if self.horizons is None:
+            raise _lib.IllumError(_lib.ILLUM_ERR_ARG, "no horizons: call make_horizons first")
         hz = np.asfortranarray(self.horizons)
+        if hz.ndim != 2 or hz.shape[1] != self.num_faces or hz.shape[0] < 2:
+            raise _shape_error("horizons must be nphi x N")
         nphi = hz.shape[0]
         E = np.zeros((S, self.num_faces))
         call("illum_direct_radiosity", self.mesh.h, hz.ctypes.data_as(C.c_void_p), nphi,
@@ -442,18 +477,37 @@ class IllumContext:
 
 
 def radiosity_solve(F, E, method="mixed", rel_tol=1e-8, max_iter=200, B=None):
-    """Solve (I - F) B = E for E of shape (N,) or (N, S).  Returns (B, resid, iters)."""
+    """Solve (I - F) B = E for E of shape (N,) or (N, S).  Returns (B, resid, iters).
+
+    Device arrays (DeviceArray) hold the C ABI's column-major N x S block,
+    i.e. shape (N,) for one right-hand side or (S, N) row-major for S of
+    them; B must have E's shape and dtype float64."""
     m = {"fp64": _lib.ILLUM_RAD_JACOBI_FP64, "mixed": _lib.ILLUM_RAD_MIXED}[method]
     resid = np.zeros(max_iter + 1)
     iters = C.c_int()
     if _is_dev(E):
-        S = 1 if E.dim() == 1 else E.shape[1]
+        n = F.n
+        if E.dim() == 1:
+            S = 1
+            if E.shape != (n,):
+                raise _shape_error("device E must have shape (N,) or (S, N)")
+        elif E.dim() == 2 and E.shape[1] == n:
+            S = E.shape[0]
+        else:
+            raise _shape_error("device E must have shape (N,) or (S, N)")
+        if B is None or not _is_dev(B):
+            raise _lib.IllumError(_lib.ILLUM_ERR_ARG, "a device E needs a device B")
+        for a in (E, B):
+            if np.dtype(a.dtype) != np.float64 or _nbytes(a) != 8 * n * S:
+                raise _shape_error("E and B must each hold N x S doubles")
         call("illum_radiosity_solve", F.h, _ptr(E), S, m, rel_tol, max_iter, _ptr(B),
              _ptr(resid), C.byref(iters),
              _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_OUT_ON_DEVICE | _lib.ILLUM_ASYNC)
         out = B
     else:
         E2 = np.asarray(E, np.float64)
+        if E2.ndim not in (1, 2) or E2.shape[0] != F.n:
+            raise _shape_error("E must have shape (N,) or (N, S)")
         single = E2.ndim == 1
         Ecm = np.ascontiguousarray(E2.reshape(E2.shape[0], -1).T)  # S x N == N x S col-major
         S = Ecm.shape[0]
@@ -475,7 +529,6 @@ class ThermalModel:
 
     def __init__(self, nfaces, depths, thermal_inertia, rhoc, init_temp, emissivity=0.999,
                  precision="fp64", grid_of_facet=None, ti_scale=None, ctx=None):
-        self.ctx = ctx or default_context()
         grids = depths if (len(depths) and np.ndim(depths[0]) == 1) else [depths]
         zs, tis, rcs, off = [], [], [], [0]
         for g in grids:
@@ -507,6 +560,11 @@ class ThermalModel:
             if T0.shape != (self.nz_max + 1, nfaces):
                 raise _lib.IllumError(_lib.ILLUM_ERR_SHAPE,
                                       "initial temperature profile is an incompatible size")
+        if gof is not None and gof.shape != (nfaces,):
+            raise _shape_error("grid_of_facet must hold one grid per facet")
+        if tsc is not None and tsc.shape != (nfaces,):
+            raise _shape_error("ti_scale must hold one factor per facet")
+        self.ctx = ctx or default_context()
         h = C.c_void_p()
         call("illum_thermal_create", self.ctx.h, nfaces, len(grids), _ptr(self.goff),
              _ptr(self.z), _ptr(ti), _ptr(rc), _ptr(gof), _ptr(tsc), emissivity, T0s,
@@ -525,7 +583,16 @@ class ThermalModel:
         except Exception:
             pass
 
+    def _check_q(self, Q, nsteps=None):
+        want = (self.nf,) if nsteps is None else (nsteps, self.nf)
+        if _is_dev(Q):
+            if np.dtype(Q.dtype) != np.float64 or _nbytes(Q) != 8 * int(np.prod(want)):
+                raise _shape_error(f"Q must hold {want} doubles")
+        elif np.shape(Q) != want:
+            raise _shape_error(f"Q must have shape {want}")
+
     def step(self, dt, Q):
+        self._check_q(Q)
         if _is_dev(Q):
             # device Q: queued without waiting (DeviceArray reads are ordered after it)
             call("illum_thermal_step", self.h, dt, _ptr(Q),
@@ -537,6 +604,9 @@ class ThermalModel:
 
     def run(self, dt, Qseries):
         """nsteps consecutive steps; Qseries has shape (nsteps, N)."""
+        if len(np.shape(Qseries)) != 2:
+            raise _shape_error("Qseries must have shape (nsteps, N)")
+        self._check_q(Qseries, np.shape(Qseries)[0])
         if _is_dev(Qseries):
             n = Qseries.shape[0]
             call("illum_thermal_run", self.h, dt, _ptr(Qseries), n,

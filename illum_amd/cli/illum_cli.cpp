@@ -219,7 +219,7 @@ int main(int argc, char **argv) {
       timed("- building horizon map", [&] { ctx->make_horizons(nphi, theta_eps, offset); });
       if (!quiet)
         timed("- saving horizon map to " + out_dir + "/horizons", [&] {
-          illum_cli::save_dense(out_dir + "/horizons.bin", ctx->horizons, nphi, nf);
+          ctx->save_horizons(out_dir + "/horizons.bin");
         });
       return 0;
     }
@@ -229,11 +229,8 @@ int main(int argc, char **argv) {
     int hz_nphi = nphi;
     if (a.count("horizon_file")) {
       timed("- loading horizon map from " + a["horizon_file"], [&] {
-        int64_t r, c;
-        auto h = illum_cli::load_dense(a["horizon_file"], &r, &c);
-        if (c != nf) throw std::runtime_error("horizon map has the wrong number of columns");
-        hz_nphi = (int)r;
-        ctx->set_horizons(std::move(h), hz_nphi);
+        ctx->load_horizons(a["horizon_file"]);
+        hz_nphi = ctx->nphi();
       });
     } else {
       timed("- building horizon map", [&] { ctx->make_horizons(nphi, theta_eps, offset); });

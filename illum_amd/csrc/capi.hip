@@ -13,7 +13,7 @@ namespace illum {
 // Implemented in the kernel translation units.
 double time_visibility_sample(illum_bvh *bvh, int max_sources, int layout);
 illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_vec,
-                           double offset);
+                           double offset, int flags);
 void render(illum_bvh *bvh, int mode, int width, int height, double az, double el,
             const double *data, double lo, double hi, int32_t *facet, double *value,
             int flags);
@@ -186,7 +186,7 @@ void destroy_sparse(illum_sparse *s) {
   // the context's reuse pool takes the blocks (no hipFree, no device sync)
   for (void *p : {(void *)s->d_colptr, (void *)s->d_rowidx, (void *)s->d_cval,
                   (void *)s->d_rowptr, (void *)s->d_colidx, (void *)s->d_rval,
-                  (void *)s->d_vis, (void *)s->d_nrays})
+                  (void *)s->d_vptr, (void *)s->d_vidx, (void *)s->d_nrays})
     pool_free(s->ctx, p);
   delete s;
 }
@@ -490,7 +490,11 @@ int illum_mesh_from_obj(illum_ctx *ctx, const char *path, double vertex_divisor,
     std::vector<int32_t> F;
     read_obj(path, V, F, VN);
     const int64_t nv = (int64_t)(V.size() / 3);
-    const double *vn = VN.size() == V.size() ? VN.data() : nullptr;
+    // vertex normals are indexed by vertex index (obj_util.cpp:71-85): used
+    // when there is one for every vertex the faces reference
+    int64_t vmax = 0;
+    for (int32_t v : F) vmax = std::max<int64_t>(vmax, v);
+    const double *vn = (int64_t)(VN.size() / 3) > vmax ? VN.data() : nullptr;
     *out = upload_mesh(ctx, make_geometry(V.data(), nv, F.data(), (int64_t)(F.size() / 3), vn,
                                           vertex_divisor));
   });
@@ -655,11 +659,17 @@ int illum_bvh_closest_hit(illum_bvh *bvh, const float *orig, const float *dir, i
 
 int illum_form_factors(illum_bvh *bvh, double albedo, const double *albedo_vec, double offset,
                        illum_sparse **out) {
+  return illum_form_factors_ex(bvh, albedo, albedo_vec, offset, 0, out);
+}
+
+int illum_form_factors_ex(illum_bvh *bvh, double albedo, const double *albedo_vec,
+                          double offset, int flags, illum_sparse **out) {
   return guarded([&] {
     CHECK_ARG(bvh && out, "null argument");
     CHECK_ARG(std::isfinite(offset), "offset must be finite");
+    CHECK_ARG((flags & ~ILLUM_FF_KEEP_VISIBILITY) == 0, "unknown form-factor flags");
     set_device(bvh->mesh->ctx);
-    *out = form_factors(bvh, albedo, albedo_vec, offset);
+    *out = form_factors(bvh, albedo, albedo_vec, offset, flags);
   });
 }
 
@@ -755,8 +765,28 @@ int illum_sparse_copy_csr(const illum_sparse *sp, int64_t *rowptr, int32_t *coli
 int illum_sparse_visibility_words(const illum_sparse *sp, int64_t *words_per_row) {
   return guarded([&] {
     CHECK_ARG(sp && words_per_row, "null argument");
-    if (!sp->d_vis) throw Error(ILLUM_ERR_ARG, "matrix carries no visibility rows");
-    *words_per_row = sp->vis_words;
+    if (!sp->d_vptr) throw Error(ILLUM_ERR_ARG, "matrix carries no visibility sets");
+    *words_per_row = (sp->n + 31) / 32;
+  });
+}
+
+int illum_sparse_visibility_nnz(const illum_sparse *sp, int64_t *nnz) {
+  return guarded([&] {
+    CHECK_ARG(sp && nnz, "null argument");
+    if (!sp->d_vptr) throw Error(ILLUM_ERR_ARG, "matrix carries no visibility sets");
+    *nnz = sp->vnnz;
+  });
+}
+
+int illum_sparse_copy_visibility_lists(const illum_sparse *sp, int64_t *ptr, int32_t *idx) {
+  return guarded([&] {
+    CHECK_ARG(sp, "null matrix");
+    if (!sp->d_vptr) throw Error(ILLUM_ERR_ARG, "matrix carries no visibility sets");
+    hipStream_t s = stream_of(sp->ctx);
+    if (ptr) ILLUM_HIP(hipMemcpyAsync(ptr, sp->d_vptr, 8 * (sp->n + 1), hipMemcpyDeviceToHost, s));
+    if (idx && sp->vnnz)
+      ILLUM_HIP(hipMemcpyAsync(idx, sp->d_vidx, 4 * sp->vnnz, hipMemcpyDeviceToHost, s));
+    sync(sp->ctx);
   });
 }
 
@@ -773,8 +803,20 @@ int illum_sparse_rays(const illum_sparse *sp, int64_t *rays) {
 int illum_sparse_copy_visibility(const illum_sparse *sp, uint32_t *bits) {
   return guarded([&] {
     CHECK_ARG(sp && bits, "null argument");
-    if (!sp->d_vis) throw Error(ILLUM_ERR_ARG, "matrix carries no visibility rows");
-    ILLUM_HIP(hipMemcpy(bits, sp->d_vis, 4 * (size_t)sp->n * sp->vis_words, hipMemcpyDeviceToHost));
+    if (!sp->d_vptr) throw Error(ILLUM_ERR_ARG, "matrix carries no visibility sets");
+    // the lists expanded into bit rows on the host
+    std::vector<int64_t> ptr(sp->n + 1);
+    std::vector<int32_t> idx(std::max<int64_t>(1, sp->vnnz));
+    hipStream_t s = stream_of(sp->ctx);
+    ILLUM_HIP(hipMemcpyAsync(ptr.data(), sp->d_vptr, 8 * (sp->n + 1), hipMemcpyDeviceToHost, s));
+    if (sp->vnnz)
+      ILLUM_HIP(hipMemcpyAsync(idx.data(), sp->d_vidx, 4 * sp->vnnz, hipMemcpyDeviceToHost, s));
+    sync(sp->ctx);
+    const int64_t words = (sp->n + 31) / 32;
+    std::memset(bits, 0, 4 * (size_t)(sp->n * words));
+    for (int64_t j = 0; j < sp->n; ++j)
+      for (int64_t k = ptr[j]; k < ptr[j + 1]; ++k)
+        bits[j * words + (idx[k] >> 5)] |= 1u << (idx[k] & 31);
   });
 }
 

@@ -79,9 +79,9 @@ void destroy_thermal(illum_thermal *t);
 // allocate nothing; a context is used by one host thread at a time.
 void *ctx_workspace(illum_ctx *ctx, int slot, size_t bytes);
 enum WorkspaceSlot {
-  WS_FF_MASK = 0,
-  WS_FF_MASK_T = 1,
-  WS_FF_COUNTS = 2,
+  WS_FF_POOL = 0,   // kept receivers of every source (form-factor assembly)
+  WS_FF_MASK_T = 1, // unsorted CSR rows of the sparse transpose
+  WS_FF_COUNTS = 2, // per-source counts and pool offsets, CSR row lengths
   WS_RAD_CTL = 3,   // solver control, residual history, column norms, partial sums
   WS_RAD_E = 4,     // right-hand sides, row-major
   WS_RAD_X0 = 5,    // iterates / residuals (nf x S each)
@@ -89,10 +89,9 @@ enum WorkspaceSlot {
   WS_RAD_X2 = 7,
   WS_RAD_D32 = 8,   // FP16 corrections (ring)
   WS_SCAN_PART = 9, // scan_counts block totals
-  WS_FF_TILES = 10, // nonzero flags of the bit transpose's 512 x 512 tiles
+  WS_FF_VPOOL = 10, // raw visibility sets, when kept
   WS_FF_ALBEDO = 11,
   WS_FF_STATUS = 12,
-  WS_FF_STAGE = 13, // each source's kept receivers (short rows), for the CSC fill
 };
 
 // Device blocks from the context's reuse pool (see illum_ctx::pool): a free

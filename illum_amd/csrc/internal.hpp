@@ -165,8 +165,11 @@ struct illum_sparse {
   int64_t *d_rowptr = nullptr;
   int32_t *d_colidx = nullptr;
   double *d_rval = nullptr;
-  int64_t vis_words = 0;
-  uint32_t *d_vis = nullptr;   // raw visibility rows (union of first hits)
+  // raw visibility sets (union of first hits, before the zero drop), kept
+  // on request: source j's hits are d_vidx[d_vptr[j] .. d_vptr[j+1]), ascending
+  int64_t vnnz = 0;
+  int64_t *d_vptr = nullptr;
+  int32_t *d_vidx = nullptr;
   // visibility-kernel work: rays, pair tests, box tests, triangle tests
   unsigned long long *d_nrays = nullptr;
   // products the previous solve took: the next solve's first batch

@@ -160,13 +160,20 @@ void read_obj(const char *path, std::vector<double> &V, std::vector<int32_t> &F,
   V.clear(); F.clear(); VN.clear();
   std::string line;
   int64_t lineno = 0;
+  // The reference keeps only the first shape (obj_util.cpp:34, shape_index
+  // 0): the faces up to the first "g" or "o" line that follows a face.
+  // Vertices and normals form one pool for every shape, so they are read to
+  // the end of the file; later faces are skipped.
+  bool first_shape_done = false;
   while (std::getline(in, line)) {
     ++lineno;
     if (line.empty() || line[0] == '#') continue;
     std::istringstream ss(line);
     std::string tag;
     ss >> tag;
-    if (tag == "v") {
+    if (tag == "g" || tag == "o") {
+      if (!F.empty()) first_shape_done = true;
+    } else if (tag == "v") {
       double x, y, z;
       if (!(ss >> x >> y >> z)) throw Error(ILLUM_ERR_IO, "bad vertex at line " + std::to_string(lineno));
       V.push_back(x); V.push_back(y); V.push_back(z);
@@ -174,15 +181,17 @@ void read_obj(const char *path, std::vector<double> &V, std::vector<int32_t> &F,
       double x, y, z;
       if (!(ss >> x >> y >> z)) throw Error(ILLUM_ERR_IO, "bad normal at line " + std::to_string(lineno));
       VN.push_back(x); VN.push_back(y); VN.push_back(z);
-    } else if (tag == "f") {
+    } else if (tag == "f" && !first_shape_done) {
       std::vector<int32_t> idx;
       std::string tok;
       const int64_t nvert = (int64_t)(V.size() / 3);
       while (ss >> tok) {
         long k = std::strtol(tok.c_str(), nullptr, 10);  // vertex index before any '/'
         if (k == 0) throw Error(ILLUM_ERR_IO, "bad face index at line " + std::to_string(lineno));
+        // negative: relative to the vertices read so far; positive indices
+        // are checked once every vertex is read
         long v = k > 0 ? k - 1 : nvert + k;
-        if (v < 0 || v >= nvert) throw Error(ILLUM_ERR_IO, "face index out of range at line " + std::to_string(lineno));
+        if (v < 0) throw Error(ILLUM_ERR_IO, "face index out of range at line " + std::to_string(lineno));
         idx.push_back((int32_t)v);
       }
       if (idx.size() < 3) throw Error(ILLUM_ERR_IO, "face with < 3 vertices at line " + std::to_string(lineno));
@@ -192,6 +201,9 @@ void read_obj(const char *path, std::vector<double> &V, std::vector<int32_t> &F,
     }
   }
   if (F.empty()) throw Error(ILLUM_ERR_IO, std::string("no faces in OBJ file ") + path);
+  const int64_t nvert = (int64_t)(V.size() / 3);
+  for (int32_t v : F)
+    if (v >= nvert) throw Error(ILLUM_ERR_IO, std::string("face index out of range in ") + path);
 }
 
 // ---------------------------------------------------------------------------
@@ -210,18 +222,22 @@ HostGeometry make_geometry(const double *V, int64_t nv, const int32_t *F, int64_
   g.area.resize(nf);
   g.frame.resize(9 * nf);
   for (int64_t f = 0; f < nf; ++f) {
-    float v[3][3];
+    // u: the FP32 vertices as read; v: divided by the vertex divisor.  The
+    // face normal comes from u and everything else from v, as the reference
+    // computes n before building Tri(v0/100., v1/100., v2/100., n)
+    // (obj_util.cpp:70, :92).
+    float u[3][3], v[3][3];
     for (int k = 0; k < 3; ++k) {
       int32_t id = F[3 * f + k];
       if (id < 0 || id >= nv) throw Error(ILLUM_ERR_ARG, "face index out of range");
       for (int c = 0; c < 3; ++c) {
-        float x = (float)V[3 * id + c];
-        v[k][c] = vertex_divisor == 1.0 ? x : x / div;
+        u[k][c] = (float)V[3 * id + c];
+        v[k][c] = vertex_divisor == 1.0 ? u[k][c] : u[k][c] / div;
         g.tri[9 * f + 3 * k + c] = v[k][c];
       }
     }
-    float e1[3] = {v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]};
-    float e2[3] = {v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2]};
+    float e1[3] = {u[1][0] - u[0][0], u[1][1] - u[0][1], u[1][2] - u[0][2]};
+    float e2[3] = {u[2][0] - u[0][0], u[2][1] - u[0][1], u[2][2] - u[0][2]};
     float cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
                    e1[0] * e2[1] - e1[1] * e2[0]};
     float len = std::sqrt((cr[0] * cr[0] + cr[1] * cr[1]) + cr[2] * cr[2]);

@@ -895,14 +895,24 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
   // With S = 1 the sweeps read a device E in place: a B overlapping it is
   // written once the solve has ended.
   const bool early_out = !(S == 1 && overlap(dst, Ein, 8 * n));
+  // The control block is device-written: its product count sizes the host
+  // copies below, so it is checked against what was launched before use.
+  auto check_ctl = [&]() {
+    if (h.nprod < 0 || h.nprod > launched || launched > max_iter || (h.done != 0 && h.done != 1))
+      throw Error(ILLUM_ERR_HIP, "radiosity solve: inconsistent device control block (" +
+                                     std::to_string(h.nprod) + " products of " +
+                                     std::to_string(launched) + " launched)");
+  };
   for (;;) {
     const int nb = std::min(batch, max_iter - launched);
     for (int j = 0; j < nb; ++j) sweep(launched++);
     if (early_out) output();
+    // launched <= max_iter: the copy fits hbuf (o_hist + 8 (max_iter + 1) bytes)
     ILLUM_HIP(hipMemcpyAsync(hbuf.data(), ws, o_hist + 8 * (size_t)launched,
                              hipMemcpyDeviceToHost, s));
     sync(ctx);
     std::memcpy(&h, hbuf.data(), sizeof(SolveCtl));
+    check_ctl();
     if (h.done || launched >= max_iter) break;
     batch = std::min(2 * batch, 64);
   }
@@ -910,6 +920,8 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
     output();
     if (!(in_dev && out_dev && (flags & ILLUM_ASYNC))) sync(ctx);
   }
+  // 0 <= nprod <= launched <= max_iter: hh and the caller's resid (max_iter
+  // + 1 entries) hold it
   const int nprod = h.nprod;
   F->last_nprod = nprod;
   std::vector<double> hh(nprod);

@@ -56,6 +56,10 @@ extern "C" {
  * illum_thermal_get or synchronous step of that model.  Ignored where an
  * array is on the host. */
 #define ILLUM_ASYNC          8
+/* illum_form_factors_ex: also keep the raw visibility sets (the union of
+ * closest hits per source before the zero drop) for
+ * illum_sparse_copy_visibility_lists.  Testing aid; O(sets) device memory. */
+#define ILLUM_FF_KEEP_VISIBILITY 16
 
 /* radiosity solve methods */
 #define ILLUM_RAD_JACOBI_FP64   1   /* FP64 Jacobi sweeps B <- E + F B                  */
@@ -113,9 +117,13 @@ int illum_synth_ellipsoid(int subdiv, double a, double b, double c, int ncraters
 
 /* ---- meshes -------------------------------------------------------------
  * Vertices are rounded to FP32 and divided (in FP32) by vertex_divisor (100.0
- * reproduces the reference OBJ loader, obj_util.cpp:90-92; 1.0 = none).  When
- * VN (nv x 3 vertex normals) is given, each face normal is flipped to agree
- * with the sum of its vertex normals (obj_util.cpp:64-88). */
+ * reproduces the reference OBJ loader, obj_util.cpp:90-92; 1.0 = none); the
+ * face normal is computed from the undivided FP32 vertices (obj_util.cpp:70).
+ * When VN (vertex normals, indexed by vertex) is given, each face normal is
+ * flipped to agree with the sum of its vertex normals (obj_util.cpp:64-88).
+ * illum_mesh_from_obj loads the first shape of the file only (faces up to the
+ * first "g"/"o" line that follows a face, obj_util.cpp:34), fan-triangulates
+ * polygons and uses the "vn" normals when every face vertex has one. */
 int illum_mesh_create(illum_ctx *ctx, const double *V, int64_t nv, const int32_t *F,
                       int64_t nf, const double *VN, double vertex_divisor,
                       illum_mesh **out);
@@ -167,6 +175,12 @@ int illum_bvh_closest_hit(illum_bvh *bvh, const float *orig, const float *dir,
  * else the scalar albedo).  Stored both as CSC (reference layout) and CSR. */
 int illum_form_factors(illum_bvh *bvh, double albedo, const double *albedo_vec,
                        double offset, illum_sparse **out);
+/* illum_form_factors with flags (0 or ILLUM_FF_KEEP_VISIBILITY).  Device
+ * memory of an assembly is O(nnz): kept receivers are listed per source,
+ * never as N x N bits.  A body may hold up to ~1.2M facets (the per-source
+ * LDS bitset spans the source's body). */
+int illum_form_factors_ex(illum_bvh *bvh, double albedo, const double *albedo_vec,
+                          double offset, int flags, illum_sparse **out);
 /* Work of one visibility pass over all sources, counted by a separate
  * instrumented launch (the assembly kernel carries no counters): out4 =
  * {rays, facing tests of candidate pairs after the cluster pass, ray/box
@@ -181,9 +195,14 @@ int illum_sparse_copy_csc(const illum_sparse *sp, int64_t *colptr, int32_t *rowi
                           double *val);
 int illum_sparse_copy_csr(const illum_sparse *sp, int64_t *rowptr, int32_t *colidx,
                           double *val);
-/* Visibility set of the last illum_form_factors call before the zero drop:
- * the union of closest-hit facets per source, as nf rows of ceil(nf/32)
- * uint32 words (bit i of row j set <=> i in hits_j).  Testing aid. */
+/* Visibility sets of a matrix assembled with ILLUM_FF_KEEP_VISIBILITY: the
+ * union of closest-hit facets per source before the zero drop
+ * (illum.cpp:72-94).  As lists: hits_j = idx[ptr[j] .. ptr[j+1]) ascending
+ * (ptr: nf + 1, idx: illum_sparse_visibility_nnz entries); or as nf rows of
+ * ceil(nf/32) uint32 words, bit i of row j set <=> i in hits_j (small meshes).
+ * Testing aids. */
+int illum_sparse_visibility_nnz(const illum_sparse *sp, int64_t *nnz);
+int illum_sparse_copy_visibility_lists(const illum_sparse *sp, int64_t *ptr, int32_t *idx);
 int illum_sparse_visibility_words(const illum_sparse *sp, int64_t *words_per_row);
 int illum_sparse_copy_visibility(const illum_sparse *sp, uint32_t *bits);
 /* Number of visibility rays (facing-cull survivors) the assembly traced. */

@@ -19,6 +19,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -172,11 +173,44 @@ class illum_context {
     return E;
   }
 
-  // load_horizons (illum.hpp:27) for a map read by the caller: nphi x N column-major.
+  // A horizon map the caller already holds: nphi x N column-major.
   void set_horizons(std::vector<double> h, int nphi) {
     if ((int64_t)h.size() != (int64_t)nphi * num_faces) throw error(ILLUM_ERR_SHAPE, "horizon map size");
     horizons = std::move(h);
     nphi_ = nphi;
+  }
+
+  // save_horizons / load_horizons (illum.hpp:25-27): the nphi x N map in
+  // Armadillo's dense binary format, "ARMA_MAT_BIN_FN008\n<rows> <cols>\n"
+  // then the FP64 values column by column (what arma::mat::save writes).
+  void save_horizons(const std::string &path) const {
+    std::FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) throw error(ILLUM_ERR_IO, "cannot write " + path);
+    const std::string head = "ARMA_MAT_BIN_FN008\n" + std::to_string(nphi_) + " " +
+                             std::to_string(num_faces) + "\n";
+    const bool ok = std::fwrite(head.data(), 1, head.size(), f) == head.size() &&
+                    std::fwrite(horizons.data(), 8, horizons.size(), f) == horizons.size();
+    std::fclose(f);
+    if (!ok) throw error(ILLUM_ERR_IO, "short write to " + path);
+  }
+  void load_horizons(const std::string &path) {
+    std::FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) throw error(ILLUM_ERR_IO, "cannot read " + path);
+    char tag[20] = {0};
+    long long rows = 0, cols = 0;
+    const bool head_ok = std::fread(tag, 1, 19, f) == 19 &&
+                         std::string(tag, 19) == "ARMA_MAT_BIN_FN008\n" &&
+                         std::fscanf(f, "%lld %lld", &rows, &cols) == 2 && std::fgetc(f) == '\n';
+    if (!head_ok || rows < 2 || cols != num_faces) {
+      std::fclose(f);
+      throw error(head_ok ? ILLUM_ERR_SHAPE : ILLUM_ERR_IO,
+                  path + ": not an nphi x N FP64 Armadillo matrix");
+    }
+    std::vector<double> h((size_t)rows * (size_t)cols);
+    const bool ok = std::fread(h.data(), 8, h.size(), f) == h.size();
+    std::fclose(f);
+    if (!ok) throw error(ILLUM_ERR_IO, path + ": truncated");
+    set_horizons(std::move(h), (int)rows);
   }
 
   // The renderer's ray-cast (illum_renderer_cli.cpp:174-261): hit facet per

@@ -45,21 +45,23 @@ int oracle_geometry(const double *V, int64_t nv, const int32_t *F, int64_t nf,
                     float *rad, double *area, double *frame) {
   if (!V || !F || nf <= 0) return 1;
   for (int64_t f = 0; f < nf; ++f) {
-    float v[3][3];
+    // w: the FP32 vertices as loaded; v = w / divisor.  The normal is taken
+    // from w (obj_util.cpp:70), the triangle from v (obj_util.cpp:92).
+    float w[3][3], v[3][3];
     for (int k = 0; k < 3; ++k) {
       const int32_t id = F[3 * f + k];
       if (id < 0 || id >= nv) return 1;
       for (int c = 0; c < 3; ++c) {
-        const float x = (float)V[3 * id + c];
-        v[k][c] = divisor == 1.0 ? x : x / (float)divisor;
+        w[k][c] = (float)V[3 * id + c];
+        v[k][c] = divisor == 1.0 ? w[k][c] : w[k][c] / (float)divisor;
       }
     }
     if (tri9)
       for (int k = 0; k < 3; ++k)
         for (int c = 0; c < 3; ++c) tri9[9 * f + 3 * k + c] = v[k][c];
-    // n = normalize((v1 - v0) x (v2 - v0))
-    const float ax = v[1][0] - v[0][0], ay = v[1][1] - v[0][1], az = v[1][2] - v[0][2];
-    const float bx = v[2][0] - v[0][0], by = v[2][1] - v[0][1], bz = v[2][2] - v[0][2];
+    // n = normalize((w1 - w0) x (w2 - w0))
+    const float ax = w[1][0] - w[0][0], ay = w[1][1] - w[0][1], az = w[1][2] - w[0][2];
+    const float bx = w[2][0] - w[0][0], by = w[2][1] - w[0][1], bz = w[2][2] - w[0][2];
     const float cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
     const float len = std::sqrt((cx * cx + cy * cy) + cz * cz);
     float n0 = cx / len, n1 = cy / len, n2 = cz / len;

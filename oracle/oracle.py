@@ -74,6 +74,49 @@ def geometry(V, F, VN=None, divisor=1.0):
     return out
 
 
+def read_obj_first_shape(path):
+    """The OBJ input of obj_util::get_objects (obj_util.cpp:8-96) with
+    shape_index 0: returns (V, F, VN) where F holds the triangles of the
+    first shape only (a shape ends at a "g" or "o" line that follows faces,
+    as tinyobjloader splits them), polygons fan-triangulated, face vertices
+    taken before any "/" (normal indices in faces are not used: the reference
+    reads normals[3 * vertex_index], obj_util.cpp:71-85).  V and VN are the
+    whole file's pools; VN is None unless it has a normal for every vertex
+    the first shape uses."""
+    V, VN, shapes, cur = [], [], [], []
+    for line in open(path):
+        tok = line.split()
+        if not tok or tok[0].startswith("#"):
+            continue
+        if tok[0] in ("g", "o"):
+            if cur:
+                shapes.append(cur)
+                cur = []
+        elif tok[0] == "v":
+            V.append([float(x) for x in tok[1:4]])
+        elif tok[0] == "vn":
+            VN.append([float(x) for x in tok[1:4]])
+        elif tok[0] == "f":
+            idx = []
+            for t in tok[1:]:
+                k = int(t.split("/")[0])
+                idx.append(k - 1 if k > 0 else len(V) + k)
+            cur += [[idx[0], idx[t], idx[t + 1]] for t in range(1, len(idx) - 1)]
+    if cur:
+        shapes.append(cur)
+    V = np.asarray(V, np.float64)
+    F = np.asarray(shapes[0], np.int32)
+    VN = np.asarray(VN, np.float64) if len(VN) > F.max() else None
+    return V, F, VN
+
+
+def get_objects(path, divisor=100.0):
+    """Geometry of obj_util::get_objects + Tri (obj_util.cpp:43-93): the first
+    shape, face normals oriented by the vertex normals, vertices / 100."""
+    V, F, VN = read_obj_first_shape(path)
+    return geometry(V, F, VN=VN, divisor=divisor)
+
+
 def closest_hit(tri, orig, dirs, use_bvh=True, nthreads=None):
     tri = np.ascontiguousarray(tri, np.float32)
     orig = np.ascontiguousarray(orig, np.float32)

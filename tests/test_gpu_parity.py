@@ -70,7 +70,7 @@ def test_form_factors_vs_oracle(api, monkeypatch, subdiv, ncraters, albedo, walk
         monkeypatch.setenv("ILLUM_FF_TRAVERSAL", walk)
     V, F = _mesh(api, subdiv, ncraters, seed=subdiv)
     ctx = api.IllumContext(V=V, F=F, leaf_size=4)
-    Fm = ctx.compute_F(albedo)
+    Fm = ctx.compute_F(albedo, keep_visibility=True)
     o = O.geometry(V, F)
     bits, nrays = O.visibility(o, offset=1e-5)
     gvis = Fm.visibility()[:, : bits.shape[1]]
@@ -100,17 +100,18 @@ def test_autotuned_bvh_and_traversal_choice_do_not_change_results(api, monkeypat
     # leaf size, leaf size 1, the four-wide and two-wide near-first walks, the
     # wavefront packet walk and the stackless threaded walk must all agree.
     V, F = _mesh(api, 4, 40, seed=8)
-    ref = api.IllumContext(V=V, F=F, leaf_size=4).compute_F(0.1)
+    ref = api.IllumContext(V=V, F=F, leaf_size=4).compute_F(0.1, keep_visibility=True)
     tuned = api.IllumContext(V=V, F=F, leaf_size=0)
     trials = tuned.bvh.tuning()
     assert [(l, w) for l, w, _ in trials] == [(l, w) for l in (2, 3, 4, 6, 8) for w in (2, 4, 3, 5)]
     assert all(ms > 0 for _, _, ms in trials)
     assert tuned.bvh.info()["leaf_size"] == min(trials, key=lambda x: x[2])[0]
-    out = [tuned.compute_F(0.1), api.IllumContext(V=V, F=F, leaf_size=1).compute_F(0.1)]
+    kv = dict(keep_visibility=True)
+    out = [tuned.compute_F(0.1, **kv), api.IllumContext(V=V, F=F, leaf_size=1).compute_F(0.1, **kv)]
     for walk in ("threaded", "wide2", "wide4", "packet", "packet4"):
         monkeypatch.setenv("ILLUM_FF_TRAVERSAL", walk)
-        out.append(api.IllumContext(V=V, F=F, leaf_size=4).compute_F(0.1))
-        out.append(api.IllumContext(V=V, F=F, leaf_size=1).compute_F(0.1))
+        out.append(api.IllumContext(V=V, F=F, leaf_size=4).compute_F(0.1, **kv))
+        out.append(api.IllumContext(V=V, F=F, leaf_size=1).compute_F(0.1, **kv))
     r = ref.csc()
     for m in out:
         assert np.array_equal(m.visibility(), ref.visibility())
@@ -122,9 +123,9 @@ def test_cluster_cull_does_not_change_results(api, monkeypatch):
     # The cluster pass only skips receivers the per-pair test would reject:
     # same visibility and the same rays with it on and off.
     V, F = _mesh(api, 5, 60, seed=11)
-    on = api.IllumContext(V=V, F=F).compute_F(0.1)
+    on = api.IllumContext(V=V, F=F).compute_F(0.1, keep_visibility=True)
     monkeypatch.setenv("ILLUM_FF_CLUSTER_CULL", "0")
-    off = api.IllumContext(V=V, F=F).compute_F(0.1)
+    off = api.IllumContext(V=V, F=F).compute_F(0.1, keep_visibility=True)
     work_off = api.IllumContext(V=V, F=F).form_factor_work()
     monkeypatch.delenv("ILLUM_FF_CLUSTER_CULL")
     work_on = api.IllumContext(V=V, F=F).form_factor_work()
@@ -139,7 +140,7 @@ def test_cluster_cull_does_not_change_results(api, monkeypatch):
     # oracle must see the same order).
     V4, F4 = _mesh(api, 4, 40, seed=12)
     F4 = np.ascontiguousarray(F4[np.random.default_rng(3).permutation(len(F4))])
-    Fm = api.IllumContext(V=V4, F=F4).compute_F(0.1)
+    Fm = api.IllumContext(V=V4, F=F4).compute_F(0.1, keep_visibility=True)
     bits, nrays = O.visibility(O.geometry(V4, F4), offset=1e-5)
     assert np.array_equal(Fm.visibility()[:, : bits.shape[1]], bits)
     assert Fm.rays() == nrays
@@ -168,17 +169,18 @@ def test_form_factors_albedo_vector(api):
     np.testing.assert_allclose(gv, v, rtol=1e-12, atol=0)
 
 
-def test_csr_sparse_transpose_matches_bit_transpose(api, monkeypatch):
-    # CSR(F) is built by scattering the CSC entries into rows and sorting each
-    # row in LDS (rows of up to 1,024 entries), or, forced here, through the
-    # transposed visibility bits; both must give the same arrays, and CSR must
-    # be CSC transposed (scipy), with per-facet albedo.
+def test_csr_rank_sort_and_bitmap_paths_agree(api, monkeypatch):
+    # CSR(F) is built by scattering the CSC entries into rows, then ordering
+    # each row by rank in LDS (rows of up to 1,024 entries) or, forced here
+    # for every row, through an LDS bitmap over the body with the values
+    # evaluated again; both must give the same arrays, and CSR must be CSC
+    # transposed (scipy), with per-facet albedo.
     import scipy.sparse as sps
     V, F = _mesh(api, 4, 40, seed=9)
     ctx = api.IllumContext(V=V, F=F)
     rho = np.random.default_rng(3).uniform(0.05, 0.9, len(F))
     Fa = ctx.compute_F(rho)
-    monkeypatch.setenv("ILLUM_FF_CSR_TRANSPOSE", "1")
+    monkeypatch.setenv("ILLUM_FF_CSR_SORT_MAX", "0")
     Fb = ctx.compute_F(rho)
     ra, rb = Fa.csr(), Fb.csr()
     for x, y in zip(ra, rb):
@@ -192,23 +194,29 @@ def test_csr_sparse_transpose_matches_bit_transpose(api, monkeypatch):
 
 def test_long_rows_and_columns_inward_sphere(api, monkeypatch):
     # Facets of a sphere wound inward see nearly every other facet: columns
-    # longer than the 256-entry staging slot (filled from their bit rows) and
-    # CSR rows longer than the one-wave sort (sorted a block per row).  CSC
-    # against the oracle, CSR against the bit-transpose path.
-    V, F = _mesh(api, 2, 0)
+    # and CSR rows of ~1,279 entries, longer than the one-wave rank sort
+    # takes, so rows are ordered through the LDS bitmap.  CSC against the
+    # oracle, CSR against scipy's transpose and the all-bitmap path.
+    import scipy.sparse as sps
+    V, F = _mesh(api, 3, 0)
     F = np.ascontiguousarray(F[:, ::-1])
     ctx = api.IllumContext(V=V, F=F)
-    Fa = ctx.compute_F(0.3)
+    Fa = ctx.compute_F(0.3, keep_visibility=True)
     cp, ri, v = Fa.csc()
-    assert np.diff(cp).max() > 256
+    assert np.diff(cp).max() > 1024
     o = O.geometry(V, F)
     bits, _ = O.visibility(o)
+    assert np.array_equal(Fa.visibility()[:, : bits.shape[1]], bits)
     ocp, ori, ov = O.form_factors(o, bits, albedo=0.3)
     assert np.array_equal(cp, ocp) and np.array_equal(ri, ori)
     np.testing.assert_allclose(v, ov, rtol=1e-12, atol=0)
     ra = Fa.csr()
-    assert np.diff(ra[0]).max() > 128
-    monkeypatch.setenv("ILLUM_FF_CSR_TRANSPOSE", "1")
+    assert np.diff(ra[0]).max() > 1024
+    A = sps.csc_matrix((v, ri, cp), shape=(len(F), len(F))).tocsr()
+    A.sort_indices()
+    assert np.array_equal(A.indptr, ra[0]) and np.array_equal(A.indices, ra[1])
+    assert np.array_equal(A.data, ra[2])
+    monkeypatch.setenv("ILLUM_FF_CSR_SORT_MAX", "0")
     rb = ctx.compute_F(0.3).csr()
     for x, y in zip(ra, rb):
         assert np.array_equal(x, y)
@@ -641,7 +649,7 @@ def test_multibody_batch_vs_oracle_per_body(api, monkeypatch):
     V, F, off = api.concat_bodies(bodies)
     nf = len(F)
     ctx = api.IllumContext(V=V, F=F, bodies=off, leaf_size=0)  # autotuned forest
-    Fm = ctx.compute_F(0.1)
+    Fm = ctx.compute_F(0.1, keep_visibility=True)
     vis = _bits_to_sets(Fm.visibility(), nf)
     cp, ri, v = Fm.csc()
     A = sps.csc_matrix((v, ri, cp), shape=(nf, nf))

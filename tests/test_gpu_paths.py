@@ -1,0 +1,162 @@
+"""GPU tests of the input paths and error paths: the OBJ loader
+(obj_util.cpp:8-96), horizons against a separate occluder mesh
+(illum.cpp:333-349), the NaN and zero-pivot status codes (illum.cpp:121-123,
+tridag.for:10-12), device-resident right-hand-side blocks, and consecutive
+solves on one matrix.  Bars as in test_gpu_parity.py: geometry and horizons
+bit-exact against the oracle, radiosity at the requested residual."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+OBJ = os.path.join(ROOT, "tests", "golden", "two_shapes.obj")
+
+
+@pytest.fixture(scope="module")
+def api():
+    from illum_amd import api as A
+    A.default_context()
+    return A
+
+
+def _write_obj(path, V, F, scale=100.0):
+    with open(path, "w") as f:
+        for v in V * scale:
+            f.write("v %r %r %r\n" % tuple(float(x) for x in v))
+        for t in F + 1:
+            f.write("f %d %d %d\n" % tuple(t))
+
+
+def test_obj_loader_matches_oracle_get_objects(api):
+    ic = api.IllumContext(obj_path=OBJ)
+    assert ic.get_num_faces() == 10  # first shape only
+    g = ic.mesh.geometry()
+    o = O.get_objects(OBJ)
+    for k in ("centroid", "normal", "radius", "area", "frame"):
+        assert np.array_equal(g[k], o[k]), k
+    assert np.array_equal(ic.mesh.triangles(), o["tri"])
+
+
+def test_obj_synthetic_mesh_form_factors_and_horizons(api, tmp_path):
+    # a cratered ellipsoid written as OBJ in hundreds of units: the loader's
+    # /100 gives back the same scale (in FP32), and the whole path runs on it
+    V, F = api.synth_ellipsoid(3, ncraters=40, seed=17)
+    p = tmp_path / "body.obj"
+    _write_obj(p, V, F)
+    ic = api.IllumContext(obj_path=str(p))
+    o = O.get_objects(str(p))
+    assert np.array_equal(ic.mesh.geometry()["normal"], o["normal"])
+    Fm = ic.compute_F(0.1, keep_visibility=True)
+    bits, _ = O.visibility(o)
+    assert np.array_equal(Fm.visibility()[:, : bits.shape[1]], bits)
+    cp, ri, v = O.form_factors(o, bits, albedo=0.1)
+    gcp, gri, gv = Fm.csc()
+    assert np.array_equal(gcp, cp) and np.array_equal(gri, ri)
+    np.testing.assert_allclose(gv, v, rtol=1e-12, atol=0)
+    hz = ic.make_horizons(nphi=33)
+    ho, _ = O.horizons(o, nphi=33)
+    assert np.array_equal(np.asarray(hz), ho)
+
+
+def test_horizons_against_an_occluder_mesh(api, tmp_path):
+    # facets of one body, rays cast against another mesh: the same ellipsoid
+    # with other craters, scaled up by 1 % so it rises around the first
+    V, F = api.synth_ellipsoid(3, ncraters=40, seed=2)
+    Vo, Fo = api.synth_ellipsoid(4, ncraters=60, seed=5)
+    p = tmp_path / "occ.obj"
+    _write_obj(p, Vo * 1.01, Fo)
+    ic = api.IllumContext(V=V, F=F)
+    ic.set_horizon_obj_file(str(p))
+    hz = np.asarray(ic.make_horizons(nphi=33))
+    g = O.geometry(V, F)
+    occ = O.get_objects(str(p))
+    ho, _ = O.horizons(g, nphi=33, occ_tri=occ["tri"])
+    assert np.array_equal(hz, ho)
+    # the occluder matters: the facets' own-mesh map differs
+    own, _ = O.horizons(g, nphi=33)
+    assert not np.array_equal(ho, own)
+
+
+def test_nan_form_factor_is_reported(api):
+    from illum_amd import _lib
+    V, F = api.synth_ellipsoid(3, ncraters=40, seed=5)
+    ic = api.IllumContext(V=V, F=F)
+    g = O.geometry(V, F)
+    bits, _ = O.visibility(g)
+    cp, ri, v = O.form_factors(g, bits, albedo=1.0)
+    rho = np.full(len(F), 0.1)
+    rho[ri[0]] = np.nan  # a receiver some source sees
+    with pytest.raises(FloatingPointError):
+        O.form_factors(g, bits, albedo_vec=rho)
+    with pytest.raises(_lib.IllumError) as e:
+        ic.compute_F(rho)
+    assert e.value.code == _lib.ILLUM_ERR_NAN
+    # the context stays usable
+    assert ic.compute_F(0.1).nnz == len(v)
+
+
+def test_zero_pivot_is_reported(api):
+    from illum_amd import _lib
+    from tests.test_cpu_pins import zero_pivot_case
+    c = zero_pivot_case()
+    with pytest.raises(_lib.IllumError) as e:
+        api.conductionQ(**c)
+    assert e.value.code == _lib.ILLUM_ERR_ZERO_PIVOT
+    # one ulp away: the GPU step equals the oracle's
+    c2 = dict(c, emiss=float(np.nextafter(c["emiss"], 0)))
+    T, Ts, Fs = api.conductionQ(**c2)
+    To, Tso, Fso = O.conductionQ(**c2)
+    assert np.array_equal(T, To) and Ts == Tso and Fs == Fso
+    # the model form: same column as a two-layer facet
+    th = api.ThermalModel(1, c["z"], 1.0, 1.0, 1.0, emissivity=c["emiss"])
+    with pytest.raises(_lib.IllumError) as e:
+        th.step(1.0, np.array([20.0]))
+    assert e.value.code == _lib.ILLUM_ERR_ZERO_PIVOT
+
+
+@pytest.mark.parametrize("method", ["fp64", "mixed"])
+def test_radiosity_device_block_of_right_hand_sides(api, method):
+    V, F = api.synth_ellipsoid(3, ncraters=40, seed=6)
+    ic = api.IllumContext(V=V, F=F)
+    Fm = ic.compute_F(0.5)
+    n, S = len(F), 6
+    E = np.random.default_rng(2).uniform(0, 400, (n, S))
+    Bh, res_h, it_h = api.radiosity_solve(Fm, E, method=method, rel_tol=1e-10)
+    ctx = api.default_context()
+    # device layout: (S, N) row-major = the C ABI's N x S column-major
+    Ed = api.DeviceArray.from_host(ctx, np.ascontiguousarray(E.T))
+    Bd = api.DeviceArray(ctx, (S, n))
+    _, res_d, it_d = api.radiosity_solve(Fm, Ed, method=method, rel_tol=1e-10, B=Bd)
+    ctx.synchronize()
+    assert it_d == it_h and np.array_equal(res_d, res_h)
+    assert np.array_equal(Bd.to_host().T, Bh)
+    # shapes that do not hold N x S values are refused before the GPU call
+    from illum_amd import _lib
+    with pytest.raises(_lib.IllumError):
+        api.radiosity_solve(Fm, api.DeviceArray(ctx, (n, S)), method=method, B=Bd)
+    with pytest.raises(_lib.IllumError):
+        api.radiosity_solve(Fm, Ed, method=method, B=api.DeviceArray(ctx, (S, n - 1)))
+
+
+@pytest.mark.parametrize("method", ["fp64", "mixed"])
+def test_consecutive_solves_report_their_own_products(api, method):
+    # a converged solve followed by one stopped at max_iter = 1 on the same
+    # matrix and context: the second reports 1 product and 1 residual, not
+    # the first solve's count (the device control block is reset per solve)
+    from illum_amd import _lib
+    V, F = api.synth_ellipsoid(3, ncraters=40, seed=6)
+    ic = api.IllumContext(V=V, F=F)
+    Fm = ic.compute_F(0.5)
+    E = np.random.default_rng(3).uniform(0, 400, (len(F), 3))
+    B, res, it = api.radiosity_solve(Fm, E, method=method, rel_tol=1e-12, max_iter=200)
+    assert it > 2 and res[it - 1] <= 1e-12
+    with pytest.raises(_lib.IllumError) as e:
+        api.radiosity_solve(Fm, E, method=method, rel_tol=1e-12, max_iter=1)
+    assert e.value.code == _lib.ILLUM_ERR_NOT_CONVERGED
+    B2, res2, it2 = api.radiosity_solve(Fm, E, method=method, rel_tol=1e-12, max_iter=200)
+    assert it2 == it and np.array_equal(res2, res) and np.array_equal(B2, B)
