@@ -38,43 +38,72 @@ PEAK_FP32_TFS = 157.3
 # update 3, FP32 coefficients, residual, factorisation, reciprocal and sweeps
 # 16) on FP32 vector 157.3 TFLOP/s.
 CN_FLOPS = {"fp64": (19, 78.6), "mixed": (19, 157.3)}
-SUN_KM = 227390024.0          # illum_cli.cpp:337 (m -> km)
-ROT_PERIOD_S = 5.342 * 3600   # Vesta, the reference's body (runjob.sh:3-4)
+from illum_amd import configs  # noqa: E402  (seeded workloads of the configurations)
+
+SUN_KM = configs.SUN_KM
+ROT_PERIOD_S = configs.ROT_PERIOD_S
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(V, F, offset, albedo_vec, nsample, nthreads):
     """CPU oracle (restatement of compute_F, own BVH) on a bounded sample of
-    source facets, all host threads; returns pairs/s."""
+    source facets, all host threads; returns pairs/s and, per block, the
+    oracle's visibility rows and CSC columns (the parity check's reference)."""
     from oracle import oracle as O
     g = O.geometry(V, F)
     nf = len(F)
     blocks = 8
     per = max(1, nsample // blocks)
     t_total, pairs, rays = 0.0, 0, 0
+    ref = []
     for b in range(blocks):
         j0 = (b * nf) // blocks
         t0 = time.perf_counter()
         bits, nr = O.visibility(g, offset=offset, j0=j0, j1=j0 + per, nthreads=nthreads)
-        O.form_factors(g, bits, j0=j0, j1=j0 + per, albedo_vec=albedo_vec)
+        csc = O.form_factors(g, bits, j0=j0, j1=j0 + per, albedo_vec=albedo_vec)
         t_total += time.perf_counter() - t0
         pairs += per * (nf - 1)
         rays += nr
-    return pairs / t_total, dict(sources=per * blocks, seconds=t_total, rays=int(rays))
+        ref.append((j0, j0 + per, bits, csc))
+    return pairs / t_total, dict(sources=per * blocks, seconds=t_total, rays=int(rays)), ref, g
 
 
-def hemisphere_suns(S, seed):
-    rng = np.random.default_rng(seed)
-    z = rng.uniform(0, 1, S)
-    ph = rng.uniform(0, 2 * np.pi, S)
-    r = np.sqrt(1 - z * z)
-    return SUN_KM * np.stack([r * np.cos(ph), r * np.sin(ph), z])
+def check_form_factors(Fm_vis, ref, nf):
+    """GPU visibility sets and CSC columns against the oracle's blocks: rows
+    that differ (bit-exact bar) and the largest relative form-factor error."""
+    vptr, vidx = Fm_vis.visibility_lists()
+    gcp, gri, gv = Fm_vis.csc()
+    bad, worst, nrows = 0, 0.0, 0
+    for j0, j1, bits, (ocp, ori, ov) in ref:
+        rows = np.unpackbits(bits.view(np.uint8), bitorder="little", axis=1)[:, :nf]
+        for q, j in enumerate(range(j0, j1)):
+            bad += not np.array_equal(vidx[vptr[j]:vptr[j + 1]], np.flatnonzero(rows[q]))
+        nrows += j1 - j0
+        a, b = gcp[j0], gcp[j1]
+        if not (np.array_equal(gcp[j0:j1 + 1] - a, ocp) and np.array_equal(gri[a:b], ori)):
+            bad += 1
+            continue
+        if len(ov):
+            worst = max(worst, float(np.max(np.abs(gv[a:b] - ov) / np.abs(ov))))
+    return {"visibility_sets_checked": nrows, "visibility_sets_differing": int(bad),
+            "form_factor_max_rel_err": worst}
 
 
-def bench_horizons(api, illum, ctx, nphi=361, theta_eps=1e-3, offset=1e-5):
+def bench_horizons(api, illum, ctx, g, nphi=361, theta_eps=1e-3, offset=1e-5):
+    from oracle import oracle as O
     nf = illum.num_faces
     hz = api.DeviceArray(ctx, (nf, nphi))  # nphi x nf column-major
     illum.make_horizons(nphi, theta_eps, offset, out=hz)  # warm-up
@@ -87,15 +116,23 @@ def bench_horizons(api, illum, ctx, nphi=361, theta_eps=1e-3, offset=1e-5):
     ctx.set_profiling(False)
     kms = ctx.profile().get("horizons", (float("nan"), 1))[0]
     rays = nf * nphi * 11
+    # bit-exact check against the oracle on 8 blocks of 64 facets
+    hh = hz.to_host()
+    bad = 0
+    for b in range(8):
+        j0 = (b * nf) // 8
+        ho, _ = O.horizons(g, nphi=nphi, theta_eps=theta_eps, offset=offset, j0=j0, j1=j0 + 64)
+        bad += int(np.sum(np.any(hh[j0:j0 + 64] != ho.T, axis=1)))
     return hz, {"seconds": dt, "kernel_ms": kms, "rays_per_s": rays / (kms * 1e-3), "rays": rays,
-                "nphi": nphi, "theta_eps": theta_eps}
+                "nphi": nphi, "theta_eps": theta_eps, "facets_checked_vs_oracle": 512,
+                "facets_differing": bad}
 
 
 def bench_radiosity(api, illum, ctx, Fm, hz, S=512, tol=1e-8):
     from illum_amd import _lib
     nf = illum.num_faces
     nphi = hz.shape[1]
-    sun = api.DeviceArray.from_host(ctx, np.ascontiguousarray(hemisphere_suns(S, 1).T))
+    sun = api.DeviceArray.from_host(ctx, np.ascontiguousarray(configs.hemisphere_suns(S, 1).T))
     disk = api.DeviceArray.from_host(ctx, np.ascontiguousarray(api.fib_spiral(100).T))
     E = api.DeviceArray(ctx, (S, nf))  # nf x S column-major
     dev = _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_OUT_ON_DEVICE | _lib.ILLUM_ASYNC
@@ -141,23 +178,16 @@ def bench_radiosity(api, illum, ctx, Fm, hz, S=512, tol=1e-8):
         r = Eh - (Bm - A @ Bm)
         out[method]["host_check_rel_resid"] = float(np.max(np.linalg.norm(r, axis=0) /
                                                            np.linalg.norm(Eh, axis=0)))
+    # B against the oracle's Gauss-Seidel (FP64, to 1e-13) for 2 of the suns
+    from oracle import oracle as O
+    for method in ("fp64", "mixed"):
+        err = 0.0
+        for s in (0, 311):
+            Bo, _, _ = O.gauss_seidel(nf, cp, ri, v, Eh[:, s], max_sweeps=100, rel_tol=1e-13)
+            err = max(err, float(np.linalg.norm(Bh[method][s] - Bo) / np.linalg.norm(Bo)))
+        out[method]["rel_err_vs_oracle_gauss_seidel"] = err
     E.close()
     return out
-
-
-def insolation_chunks(normals, steps_per_rot, chunk):
-    """Q (W/m^2) for a sun on a 20-degree-inclined orbit about the body's z axis,
-    no shadowing: one rotation in steps_per_rot steps, returned in chunks."""
-    AU = 149.5978707e6
-    S0 = 1372 * (AU / SUN_KM) ** 2 * (1 - 0.1)
-    inc = np.deg2rad(20.0)
-    chunks = []
-    for c0 in range(0, steps_per_rot, chunk):
-        ang = 2 * np.pi * np.arange(c0 + 1, c0 + chunk + 1) / steps_per_rot
-        s = np.stack([np.cos(ang) * np.cos(inc), np.sin(ang) * np.cos(inc),
-                      np.full(chunk, np.sin(inc))])
-        chunks.append(np.ascontiguousarray(S0 * np.clip(s.T @ normals.T, 0, None)))
-    return chunks
 
 
 def bench_thermal(api, nsteps=4096, subdiv=7, seed=0, chunk=256):
@@ -165,20 +195,17 @@ def bench_thermal(api, nsteps=4096, subdiv=7, seed=0, chunk=256):
     chunk = max(4, min(chunk, nsteps))
     nsteps = max(chunk, nsteps - nsteps % chunk)
     ctx = api.default_context()
-    V, F = api.synth_ellipsoid(subdiv, ncraters=150, seed=seed)
+    V, F = configs.mesh("C4", seed=seed)
     nf = len(F)
     m = api.Mesh(ctx, V=V, F=F)
-    nrm = m.geometry()["normal"].astype(np.float64)
+    nrm = m.geometry()["normal"]
     m.close()
-    rng = np.random.default_rng(seed)
-    nzs = np.arange(32, 97)
-    grids = [0.5 * (np.exp(np.linspace(0, 3, n)) - 1 + 0.01) / (np.exp(3) - 1 + 0.01) for n in nzs]
-    gid = rng.integers(0, len(nzs), nf).astype(np.int32)
-    tis = np.exp(rng.normal(0, 0.5, nf))
+    grids, gid, tis = configs.c4_layers(nf, seed)
+    nzs = np.array([len(g) for g in grids])
     rotations = 4
-    spr = nsteps // rotations
+    spr = max(chunk, nsteps // rotations)
     dt = ROT_PERIOD_S / spr
-    qh = insolation_chunks(nrm, spr, chunk)
+    qh = [configs.insolation(nrm, c0, chunk, spr) for c0 in range(0, spr, chunk)]
     qd = [api.DeviceArray.from_host(ctx, q) for q in qh]
     out = {"facets": nf, "steps": nsteps, "dt_s": dt, "layers": "U{32..96}",
            "cell_steps": int(np.sum(nzs[gid])) * nsteps, "launches": nsteps // chunk}
@@ -320,7 +347,7 @@ def main():
     t0 = time.perf_counter()
     V, F = api.synth_ellipsoid(args.subdiv, ncraters=args.craters, seed=0)
     nf = len(F)
-    albedo = np.random.default_rng(0).uniform(0.05, 0.15, nf)
+    albedo = configs.facet_albedo(nf)
     ctx = api.default_context()
     illum = api.IllumContext(V=V, F=F, leaf_size=args.leaf_size, ctx=ctx)
     log(f"mesh {nf} facets, bvh {illum.bvh.info()}, leaf autotune {illum.bvh.tuning()}, "
@@ -360,35 +387,46 @@ def main():
     vis_ms, vis_n = prof.get("ff_visibility", (float("nan"), 1))
     vis_ms /= max(1, vis_n)
     rays = Fm.rays()
-    # Algorithmic HBM bytes of ff_visibility per launch: read the facet records
-    # (centroid+radius, normal: 32 B) and areas (8 B) once, write each
-    # source's visibility row and nonzero row (2 x W words of 4 B).  The
-    # 16 B-per-pair receiver stream of the facing cull (pairs x 16 B) is the
-    # same data re-read from L2/MALL for every source, not HBM traffic.
-    words = ((nf + 511) // 512) * 16
-    vis_bytes = nf * 40 + nf * words * 4 * 2
-    achieved = vis_bytes / (vis_ms * 1e-3) / 1e9
+    # VALU roofline of ff_visibility from algorithmic work, FP32 FLOP per
+    # launch: 8 per candidate pair (j, i != j), all N(N-1) of them (the facing
+    # cull of illum.cpp:56-68 as written, before any cluster pass), plus 12 per
+    # ray/box slab test and 40 per ray/triangle test counted per ray along its
+    # own near-first walk of the same tree (the per-lane two-wide walk's
+    # counters, in a separate instrumented launch) -- not the wavefront
+    # packet walk's union of nodes, which is how the kernel spends its time.
+    os.environ["ILLUM_FF_TRAVERSAL"] = "wide2"
+    st = illum.form_factor_work(offset)
+    del os.environ["ILLUM_FF_TRAVERSAL"]
+    st_packet = illum.form_factor_work(offset)  # the default walk, for the record
+    pairs = nf * (nf - 1)
+    flops = 8 * pairs + 12 * st["box_tests"] + 40 * st["tri_tests"]
+    tflops = flops / (vis_ms * 1e-3) / 1e12
+    # HBM, secondary: algorithmic bytes of the launch are the facet records read
+    # once (centroid+radius and normal 32 B, area 8 B: 40 B per facet), the
+    # per-source count and pool offset (12 B per facet) and, per kept entry,
+    # its 4 B pool index and 4 B CSR row-count update.
+    vis_bytes = nf * 52 + nnz * 8
+    achieved_gbs = vis_bytes / (vis_ms * 1e-3) / 1e9
     traffic = None
     pmc_path = os.path.join(ROOT, "profiles", f"ff_visibility_{nf}_pmc.json")
     if os.path.exists(pmc_path):
         pmc = json.load(open(pmc_path))
         traffic = (pmc["fetch_kib"] * pmc["fetch_scale"] + pmc["write_kib"] * pmc["write_scale"]) * 1024
-    # VALU work per launch from the kernel's own counters: 8 FLOP per facing
-    # test, 12 per ray/box slab test, 40 per ray/triangle test.
-    st = illum.form_factor_work(offset)  # separate instrumented launch
-    flops = 8 * st["pair_tests"] + 12 * st["box_tests"] + 40 * st["tri_tests"]
-    tflops = flops / (vis_ms * 1e-3) / 1e12
-    roofline = {"bound": "hbm", "kernel": "ff_visibility", "achieved": achieved,
-                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                "traffic": traffic, "avg_ms": vis_ms,
-                "note": "latency/VALU-bound ray traversal: HBM is not the limit",
-                "rays": rays, "rays_per_s": rays / (vis_ms * 1e-3),
-                "valu": {"work": st, "flop": flops, "achieved": tflops,
-                         "peak": PEAK_FP32_TFS, "unit": "TFLOP/s", "frac": tflops / PEAK_FP32_TFS}}
+    roofline = {"bound": "valu", "kernel": "ff_visibility", "achieved": tflops,
+                "peak": PEAK_FP32_TFS, "unit": "TFLOP/s", "frac": tflops / PEAK_FP32_TFS,
+                "traffic": traffic, "avg_ms": vis_ms, "flop": flops,
+                "work": {"pairs": pairs, "rays": rays, "box_tests_per_lane_walk": st["box_tests"],
+                         "tri_tests_per_lane_walk": st["tri_tests"],
+                         "packet_walk_counters": st_packet},
+                "rays_per_s": rays / (vis_ms * 1e-3),
+                "hbm": {"algorithmic_bytes": vis_bytes, "achieved": achieved_gbs,
+                        "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved_gbs / PEAK_HBM_GBS,
+                        "traffic": traffic}}
 
     secondary = {}
     if not args.quick:
-        hz, secondary["horizons"] = bench_horizons(api, illum, ctx)
+        from oracle import oracle as O
+        hz, secondary["horizons"] = bench_horizons(api, illum, ctx, O.geometry(V, F))
         log("horizons:", secondary["horizons"])
         secondary["radiosity"] = bench_radiosity(api, illum, ctx, Fm, hz)
         log("radiosity:", secondary["radiosity"])
@@ -399,15 +437,35 @@ def main():
         log("thermal:", secondary["thermal"])
 
     cpu = None
+    parity = {}
     if not args.no_cpu:
         nth = min(16, os.cpu_count() or 1)
         log(f"cpu baseline: {args.cpu_sample} sources on {nth} threads ...")
-        cv, info = cpu_baseline(V, F, offset, albedo, args.cpu_sample, nth)
+        cv, info, ref, _ = cpu_baseline(V, F, offset, albedo, args.cpu_sample, nth)
         cpu = {"value": cv, "unit": "pairs/s", "cores": nth, "kind": "port",
+               "cpu_model": cpu_model(), "nproc": os.cpu_count(),
                "sample": f"compute_F restated (oracle, own BVH) for {info['sources']} source facets "
                          f"in 8 evenly spaced contiguous blocks of the {nf}-facet mesh, "
                          f"{info['seconds']:.1f}s, {info['rays']} rays"}
         log(f"cpu {cv:.3e} pairs/s; gpu/cpu = {value / cv:.1f}x")
+        # the same sources' visibility sets and columns from the GPU, assembled
+        # again outside the timed steps with the raw sets kept
+        Fv = illum.compute_F(albedo, offset, keep_visibility=True)
+        parity["form_factors"] = check_form_factors(Fv, ref, nf)
+        Fv.close()
+        log("parity:", parity["form_factors"])
+    for leg in ("horizons", "radiosity", "thermal"):
+        if leg in secondary:
+            r = secondary[leg]
+            if leg == "horizons":
+                parity[leg] = {k: r[k] for k in ("facets_checked_vs_oracle", "facets_differing")}
+            elif leg == "radiosity":
+                parity[leg] = {m: {k: r[m][k] for k in ("final_rel_resid", "host_check_rel_resid",
+                                                         "rel_err_vs_oracle_gauss_seidel")}
+                               for m in ("fp64", "mixed")}
+            else:
+                parity[leg] = {k: r[k] for k in ("max_abs_T_diff_fp64_vs_oracle_K",
+                                                  "max_abs_Tsurf_diff_mixed_vs_oracle_K")}
 
     out = {
         "metric": "form-factor pairs/s",
@@ -429,6 +487,7 @@ def main():
                    "bvh": dict(illum.bvh.info(), walk=walk)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "parity": parity,
         "profile_ms": {k: v[0] / max(1, v[1]) for k, v in prof.items()},
         "secondary": secondary,
     }

@@ -515,8 +515,8 @@ def radiosity_solve(F, E, method="mixed", rel_tol=1e-8, max_iter=200, B=None):
         call("illum_radiosity_solve", F.h, _ptr(Ecm), S, m, rel_tol, max_iter, _ptr(Bcm),
              _ptr(resid), C.byref(iters), 0)
         out = Bcm[0] if single else Bcm.T
-    nres = min(len(resid), max(1, iters.value + 1))
-    return out, resid[:nres], iters.value
+    # one relative residual per product (the solve's stop decision after it)
+    return out, resid[:iters.value], iters.value
 
 
 class ThermalModel:

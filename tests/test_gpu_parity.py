@@ -558,16 +558,19 @@ def test_async_time_loop_matches_synchronous(api, method):
 
 
 @pytest.mark.parametrize("method,tol_K", [("fp64", 1e-5), ("mixed", 1e-3)])
-def test_coupled_loop_5120_facets_256_steps_vs_oracle(api, method, tol_K):
-    """The reference's CPU-runnable configuration: 5,120 facets, 64 azimuth
+def test_c1_coupled_loop_5120_facets_256_steps_vs_oracle(api, method, tol_K):
+    """The reference's CPU-runnable configuration C1 (illum_amd/configs.py):
+    5,120 facets (subdivision 4, 40 craters), albedo 0.1, 64 azimuth
     intervals (nphi = 65), one sun, 256 coupled steps on 40 layers
-    (illum_cli.cpp:403-456: direct -> E -= F_surf -> solve -> CN step), on
-    device buffers, against the oracle's replay with Gauss-Seidel to 1e-13.
-    Horizons bit-exact; temperatures within tol_K of the oracle."""
-    from illum_amd import _lib
-    V, F = _mesh(api, 5, 60, seed=31)
+    (illum_cli.cpp:403-456: direct -> E -= F_surf -> solve -> CN step) with
+    the solve to 1e-10, on device buffers, against the oracle's replay with
+    Gauss-Seidel to 1e-10.  Horizons bit-exact; temperatures within tol_K of
+    the oracle."""
+    from illum_amd import _lib, configs
+    V, F = configs.mesh("C1")
+    assert len(F) == 5120
     ctx = api.IllumContext(V=V, F=F)
-    Fm = ctx.compute_F(0.12)
+    Fm = ctx.compute_F(0.1)
     nphi = 65
     hz = ctx.make_horizons(nphi=nphi)
     g = O.geometry(V, F)
@@ -575,17 +578,17 @@ def test_coupled_loop_5120_facets_256_steps_vs_oracle(api, method, tol_K):
     assert np.array_equal(hz, ho)
     nf = len(F)
     nz, nsteps = 40, 256
-    z = 0.5 * (np.exp(np.linspace(0, 3, nz)) - 1 + 0.01) / (np.exp(3) - 1 + 0.01)
-    dt = 5.342 * 3600 / nsteps
-    ang = 2 * np.pi * np.arange(nsteps) / nsteps
-    sun = 2.27e8 * np.stack([np.cos(ang) * 0.94, np.sin(ang) * 0.94, np.full(nsteps, 0.34)], 1)
+    z = configs.layer_grid(nz)
+    dt = configs.ROT_PERIOD_S / nsteps
+    sun = np.stack([configs.orbit_sun(k, nsteps) for k in range(nsteps)])
     c = api.default_context()
     hzd = api.DeviceArray.from_host(c, np.ascontiguousarray(np.asarray(hz).T))
     disk_h = api.fib_spiral(100)
     disk = api.DeviceArray.from_host(c, np.ascontiguousarray(disk_h.T))
     m = {"fp64": _lib.ILLUM_RAD_JACOBI_FP64, "mixed": _lib.ILLUM_RAD_MIXED}[method]
     dev = _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_OUT_ON_DEVICE | _lib.ILLUM_ASYNC
-    th = api.ThermalModel(nf, z, 200.0, 1.19177e6, 200.0, emissivity=0.95, precision=method)
+    th = api.ThermalModel(nf, z, configs.TI, configs.RHOC, 200.0, emissivity=configs.EMISS,
+                          precision=method)
     E = api.DeviceArray(c, (nf,))
     B = api.DeviceArray(c, (nf,))
     suns = [api.DeviceArray.from_host(c, sun[k]) for k in range(nsteps)]
@@ -604,17 +607,20 @@ def test_coupled_loop_5120_facets_256_steps_vs_oracle(api, method, tol_K):
     th.close()
     # oracle replay
     bits, _ = O.visibility(g)
-    cp, ri, v = O.form_factors(g, bits, albedo=0.12)
+    cp, ri, v = O.form_factors(g, bits, albedo=0.1)
     T = np.full((nz + 1, nf), 200.0)
     Qprev = np.zeros(nf)
     Fs = np.zeros(nf)
     for k in range(nsteps):
         Eo = -Fs + O.direct(g, ho, sun[k], disk_h)
-        Bo, _, _ = O.gauss_seidel(nf, cp, ri, v, Eo, max_sweeps=200, rel_tol=1e-13)
-        T, Qprev, Fs, _ = O.thermal_run([0, nz], z, np.full(nz, 200.0), np.full(nz, 1.19177e6),
-                                        T, Qprev, Bo[None, :], dt, 0.95)
+        Bo, _, _ = O.gauss_seidel(nf, cp, ri, v, Eo, max_sweeps=200, rel_tol=1e-10)
+        T, Qprev, Fs, _ = O.thermal_run([0, nz], z, np.full(nz, configs.TI),
+                                        np.full(nz, configs.RHOC), T, Qprev, Bo[None, :], dt,
+                                        configs.EMISS)
     assert np.isfinite(Tg).all()
-    assert np.abs(Tg - T).max() <= tol_K
+    err = np.abs(Tg - T).max()
+    print(f"C1 {method}: max |T - oracle| {err:.2e} K over 256 coupled steps")
+    assert err <= tol_K
 
 
 def test_conductionQ_dropin_vs_reference_fortran_golden(api):
