@@ -81,6 +81,7 @@ SIGNATURES = {
     "illum_fib_spiral": [_P, _I],
     "illum_direct_radiosity": [_P, _P, _I, _P, _I, _P, _I, _P, _I],
     "illum_radiosity_solve": [_P, _P, _I, _I, _D, _I, _P, _P, _P, _I],
+    "illum_radiosity_solve_noalloc": [_P, _P, _I, _I, _D, _I, _P, _P, _P, _I],
     "illum_thermal_create": [_P, _I64, _I, _P, _P, _P, _P, _P, _P, _D, _D, _P, _I, _PP],
     "illum_thermal_destroy": [_P],
     "illum_thermal_info": [_P, _P, _P],

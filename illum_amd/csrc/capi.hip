@@ -25,7 +25,9 @@ void horizons(illum_bvh *bvh, illum_bvh *occl, int nphi, double theta_eps, doubl
 void direct_radiosity(illum_mesh *mesh, const double *hz, int nphi, const double *sun, int S,
                       const double *disk, int ndisk, double *E, int flags);
 void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double rel_tol,
-                     int max_iter, double *B, double *resid, int *iters, int flags);
+                     int max_iter, double *B, double *resid, int *iters, int flags,
+                     void *fixed_scratch);
+void *radiosity_noalloc_scratch(int device);
 illum_thermal *thermal_create(illum_ctx *ctx, int64_t nf, int ngrid, const int64_t *grid_off,
                               const double *z, const double *ti, const double *rhoc,
                               const int32_t *grid_of_facet, const double *ti_scale,
@@ -262,6 +264,7 @@ static illum_ctx *default_ctx() {
     auto c = std::unique_ptr<illum_ctx>(new illum_ctx());
     c->device = 0;
     ILLUM_HIP(hipSetDevice(0));
+    c->noalloc_scratch = radiosity_noalloc_scratch(0);
     hipStream_t s;
     ILLUM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     c->stream = s;
@@ -368,6 +371,9 @@ int illum_ctx_create(int device, illum_ctx **out) {
     auto c = std::unique_ptr<illum_ctx>(new illum_ctx());
     c->device = device;
     ILLUM_HIP(hipSetDevice(device));
+    // the no-allocation solve's scratch address on this device (resolved
+    // once per device, with it current)
+    c->noalloc_scratch = radiosity_noalloc_scratch(device);
     hipStream_t s;
     ILLUM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     c->stream = s;
@@ -861,7 +867,19 @@ int illum_radiosity_solve(illum_sparse *F, const double *E, int S, int method, d
   return guarded([&] {
     CHECK_ARG(F, "null matrix");
     set_device(F->ctx);
-    radiosity_solve(F, E, S, method, rel_tol, max_iter, B, resid, iters, flags);
+    radiosity_solve(F, E, S, method, rel_tol, max_iter, B, resid, iters, flags, nullptr);
+  });
+}
+
+int illum_radiosity_solve_noalloc(illum_sparse *F, const double *E, int S, int method,
+                                  double rel_tol, int max_iter, double *B, double *resid,
+                                  int *iters, int flags) {
+  return guarded([&] {
+    CHECK_ARG(F, "null matrix");
+    if (!F->ctx->noalloc_scratch) throw Error(ILLUM_ERR_HIP, "no-allocation scratch not resolved");
+    set_device(F->ctx);
+    radiosity_solve(F, E, S, method, rel_tol, max_iter, B, resid, iters, flags,
+                    F->ctx->noalloc_scratch);
   });
 }
 

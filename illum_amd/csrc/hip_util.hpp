@@ -54,8 +54,8 @@ struct StageTimer {
   illum_ctx *ctx;
   std::string name;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  StageTimer(illum_ctx *c, const char *n) : ctx(c), name(n) {
-    if (ctx->profiling) {
+  StageTimer(illum_ctx *c, const char *n, bool enable = true) : ctx(c), name(n) {
+    if (ctx->profiling && enable) {
       ILLUM_HIP(hipEventCreate(&e0));
       ILLUM_HIP(hipEventCreate(&e1));
       ILLUM_HIP(hipEventRecord(e0, stream_of(ctx)));

@@ -91,6 +91,9 @@ HostBVH build_bvh(const HostGeometry &g, int leaf_size);
 struct illum_ctx {
   int device = 0;
   void *stream = nullptr;              // hipStream_t
+  // Device address of the no-allocation solve's fixed scratch on this
+  // context's GPU (resolved at context creation).
+  void *noalloc_scratch = nullptr;
   // Auxiliary streams (default priority) and their join events, made on
   // first use, for independent launches forked from `stream` and joined
   // back into it within one call (the conduction depth buckets).

@@ -25,10 +25,59 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
 namespace illum {
+
+// Scratch of illum_radiosity_solve_noalloc: one fixed array per GPU (each
+// device that loads the library holds its own copy until the process ends).
+// Host code never names the array in a kernel argument or a copy: it uses the
+// device address that radiosity_noalloc_scratch resolved for that device.
+constexpr size_t RAD_NOALLOC_BYTES = (size_t)64 << 20;
+constexpr int RAD_NOALLOC_MAX_ITER = 256;
+static __device__ __align__(256) unsigned char scratch[RAD_NOALLOC_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[MAX_DEVICES];
+
+// Device address of `scratch` on `device`, which must be the current device
+// the first time it is asked for (illum_ctx_create calls this at set-up).
+void *radiosity_noalloc_scratch(int device) {
+  if (device < 0 || device >= MAX_DEVICES) throw Error(ILLUM_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = -1;
+    ILLUM_HIP(hipGetDevice(&cur));
+    if (cur != device) throw Error(ILLUM_ERR_ARG, "no-allocation scratch resolved off its device");
+    void *p = nullptr;
+    ILLUM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error(ILLUM_ERR_HIP, "hipGetSymbolAddress returned no address");
+    g_scratch_ptr[device] = p;
+  }
+  return g_scratch_ptr[device];
+}
+
+// Where a solve's device scratch comes from: the context's grow-only
+// workspace slots, or (fixed != nullptr) consecutive 256-byte aligned pieces
+// of a fixed array of `cap` bytes, which allocates nothing.
+struct RadScratch {
+  illum_ctx *ctx;
+  unsigned char *fixed;
+  size_t cap, used;
+  void *get(int slot, size_t bytes) {
+    if (!fixed) return ctx_workspace(ctx, slot, bytes);
+    const size_t off = (used + 255) & ~(size_t)255;
+    if (off > cap || bytes > cap - off)
+      throw Error(ILLUM_ERR_CAPACITY,
+                  "radiosity solve needs more than the " + std::to_string(cap >> 20) +
+                      " MiB no-allocation scratch (nf x S too large)");
+    used = off + bytes;
+    return fixed + off;
+  }
+};
 
 // Scale exponent of the mixed method's FP16 corrections for a column whose
 // residual has squared 2-norm nrm2: d = fp16(r 2^e), ||r||_2 2^e <= 2^13.
@@ -658,8 +707,8 @@ namespace {
 template <class TX>
 int sweep_product(hipStream_t s, illum_ctx *ctx, const illum_sparse *F, int S, const TX *X,
                   const double *R, double *Y, const int *done, double *part, MixedIO m,
-                  const char *stage) {
-  StageTimer st(ctx, stage);
+                  const char *stage, bool timed) {
+  StageTimer st(ctx, stage, timed);
   const int64_t nf = F->n;
   int nparts = 0;
   if (S >= 128) {
@@ -719,8 +768,18 @@ constexpr int RAD_FOLD = RAD_RING - 1;
 }  // namespace
 
 void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double rel_tol,
-                     int max_iter, double *B, double *resid, int *iters, int flags) {
+                     int max_iter, double *B, double *resid, int *iters, int flags,
+                     void *fixed_scratch) {
   illum_ctx *ctx = F->ctx;
+  // fixed_scratch: the no-allocation variant (device E and B, bounded
+  // max_iter, scratch carved from the fixed array, no profiling events)
+  const bool noalloc = fixed_scratch != nullptr;
+  if (noalloc && (!(flags & ILLUM_IN_ON_DEVICE) || !(flags & ILLUM_OUT_ON_DEVICE)))
+    throw Error(ILLUM_ERR_ARG, "the no-allocation solve takes E and B on the device");
+  if (noalloc && max_iter > RAD_NOALLOC_MAX_ITER)
+    throw Error(ILLUM_ERR_ARG, "the no-allocation solve takes max_iter <= " +
+                                   std::to_string(RAD_NOALLOC_MAX_ITER));
+  RadScratch scr{ctx, (unsigned char *)fixed_scratch, RAD_NOALLOC_BYTES, 0};
   if (!F->d_rowptr) throw Error(ILLUM_ERR_ARG, "matrix has no CSR form");
   if (S < 1 || max_iter < 1 || !(rel_tol >= 0)) throw Error(ILLUM_ERR_ARG, "bad solve parameters");
   if (method != ILLUM_RAD_JACOBI_FP64 && method != ILLUM_RAD_MIXED)
@@ -750,7 +809,7 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
                o_col = o_e2 + align(8 * (size_t)S), o_ex = o_col + align(8 * (size_t)S),
                o_part = o_ex + align(4 * ((size_t)max_iter + 2) * S),
                ctl_bytes = o_part + 8 * (size_t)gparts * S;
-  char *ws = (char *)ctx_workspace(ctx, WS_RAD_CTL, ctl_bytes);
+  char *ws = (char *)scr.get(WS_RAD_CTL, ctl_bytes);
   SolveCtl *ctl = (SolveCtl *)ws;
   double *hist = (double *)(ws + o_hist), *e2 = (double *)(ws + o_e2),
          *col = (double *)(ws + o_col), *part = (double *)(ws + o_part);
@@ -766,17 +825,20 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
     ILLUM_HIP(hipMemcpyAsync(Ehost.p, E, 8 * n, hipMemcpyHostToDevice, s));
     Ein = Ehost.p;
   }
-  double *X0 = (double *)ctx_workspace(ctx, WS_RAD_X0, 8 * n);
-  double *X1 = (double *)ctx_workspace(ctx, WS_RAD_X1, 8 * n);
+  double *X0 = (double *)scr.get(WS_RAD_X0, 8 * n);
+  double *X1 = (double *)scr.get(WS_RAD_X1, 8 * n);
   // B mode accumulates B in place in the caller's B when it can (device,
   // S = 1, not overlapping E); ring mode folds into Bacc.
   double *Bacc = nullptr;
   _Float16 *Dring = nullptr;
   if (mixed) {
     Bacc = !ring && out_dev && S == 1 && !overlap(B, Ein, 8 * n)
-               ? B : (double *)ctx_workspace(ctx, WS_RAD_X2, 8 * n);
-    Dring = (_Float16 *)ctx_workspace(ctx, WS_RAD_D32, 2 * (size_t)R * n);
+               ? B : (double *)scr.get(WS_RAD_X2, 8 * n);
+    Dring = (_Float16 *)scr.get(WS_RAD_D32, 2 * (size_t)R * n);
   }
+  // Right-hand sides, row-major, for S > 1 (taken before any launch, so a
+  // solve that does not fit the fixed scratch queues nothing).
+  double *Erow = S > 1 ? (double *)scr.get(WS_RAD_E, 8 * n) : nullptr;
 
   // ||E_s||^2 from the column-major input (resetting the control block and,
   // mixed, writing the exponents of d_0 and d_1), then E row-major (with d_0 and B = 0 in the same pass).  With
@@ -794,7 +856,7 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
   }
   const double *Er = Ein;
   if (S > 1) {
-    double *t = (double *)ctx_workspace(ctx, WS_RAD_E, 8 * n);
+    double *t = Erow;
     dim3 g((unsigned)((nf + 31) / 32), (unsigned)((S + 31) / 32));
     load_rhs_kernel<<<g, 256, 0, s>>>(nf, S, Ein, t, ex, mixed ? Dring : nullptr,
                                       mixed && !ring ? Bacc : nullptr);
@@ -833,7 +895,7 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
       // nxt = E + F cur; the residual of cur is nxt - cur
       nparts = sweep_product<double>(s, ctx, F, S, cur(k), Er, nxt(k), done, part,
                                      MixedIO{nullptr, nullptr, nullptr, nullptr},
-                                     "radiosity_fp64_sweep");
+                                     "radiosity_fp64_sweep", !noalloc);
       Y = nxt(k);
       Xn = cur(k);
     } else {
@@ -841,7 +903,7 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
       MixedIO m{ex + (int64_t)k * S, ex + (int64_t)(k + 1) * S, slot(k + 1),
                 ring ? nullptr : Bacc};
       nparts = sweep_product<_Float16>(s, ctx, F, S, slot(k), rk(k), rk1(k), done, part, m,
-                                       "radiosity_mixed_sweep");
+                                       "radiosity_mixed_sweep", !noalloc);
       Y = rk1(k);
     }
     if (!nparts) {
@@ -861,7 +923,12 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
   // Sweeps are launched in batches (the first as long as the previous solve
   // of this matrix took); the control block and the residual history are
   // read back, in one copy, after each.
-  std::vector<char> hbuf(o_hist + 8 * ((size_t)max_iter + 1));
+  // Control block and residual history read back to the host: on the stack
+  // for the no-allocation variant (max_iter bounded), else on the heap.
+  alignas(8) char hstack[256 + 8 * (RAD_NOALLOC_MAX_ITER + 1)];
+  std::vector<char> hheap;
+  if (!noalloc) hheap.resize(o_hist + 8 * ((size_t)max_iter + 1));
+  char *hbuf = noalloc ? hstack : hheap.data();
   SolveCtl h{};
   int launched = 0;
   int batch = F->last_nprod > 0 ? F->last_nprod : 4;
@@ -908,10 +975,10 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
     for (int j = 0; j < nb; ++j) sweep(launched++);
     if (early_out) output();
     // launched <= max_iter: the copy fits hbuf (o_hist + 8 (max_iter + 1) bytes)
-    ILLUM_HIP(hipMemcpyAsync(hbuf.data(), ws, o_hist + 8 * (size_t)launched,
+    ILLUM_HIP(hipMemcpyAsync(hbuf, ws, o_hist + 8 * (size_t)launched,
                              hipMemcpyDeviceToHost, s));
     sync(ctx);
-    std::memcpy(&h, hbuf.data(), sizeof(SolveCtl));
+    std::memcpy(&h, hbuf, sizeof(SolveCtl));
     check_ctl();
     if (h.done || launched >= max_iter) break;
     batch = std::min(2 * batch, 64);
@@ -924,11 +991,10 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
   // + 1 entries) hold it
   const int nprod = h.nprod;
   F->last_nprod = nprod;
-  std::vector<double> hh(nprod);
-  std::memcpy(hh.data(), hbuf.data() + o_hist, 8 * (size_t)nprod);
-  if (resid) std::copy(hh.begin(), hh.end(), resid);
+  if (resid && nprod > 0) std::memcpy(resid, hbuf + o_hist, 8 * (size_t)nprod);
   if (iters) *iters = nprod;
-  const double rel = nprod > 0 ? hh[nprod - 1] : 1.0;
+  double rel = 1.0;
+  if (nprod > 0) std::memcpy(&rel, hbuf + o_hist + 8 * (size_t)(nprod - 1), 8);
   if (!(rel <= rel_tol))
     throw Error(ILLUM_ERR_NOT_CONVERGED,
                 "radiosity solve did not reach the tolerance (relative residual " +

@@ -247,6 +247,22 @@ int illum_render(illum_bvh *bvh, int mode, int width, int height, double az_deg,
 int illum_radiosity_solve(illum_sparse *F, const double *E, int S, int method,
                           double rel_tol, int max_iter, double *B, double *resid,
                           int *iters, int flags);
+/* illum_radiosity_solve that allocates no device memory.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  E and B
+ * are device arrays (flags must hold ILLUM_IN_ON_DEVICE and
+ * ILLUM_OUT_ON_DEVICE) and max_iter <= 256.  Every scratch buffer of the solve
+ * (control block, residual history, iterates, FP16 corrections, partial sums)
+ * comes from one fixed 64 MiB array per GPU instead of the context's
+ * workspace; a solve that needs more (about nf x S > 1.3 million values for
+ * the mixed method, 2.4 million for FP64) returns ILLUM_ERR_CAPACITY before
+ * queuing anything.  All calls on one GPU share that array and it keeps the
+ * last call's contents (every solve initialises what it reads): calls that
+ * overlap on one GPU, from different contexts or threads, race.  Results are
+ * identical to illum_radiosity_solve's. */
+int illum_radiosity_solve_noalloc(illum_sparse *F, const double *E, int S, int method,
+                                  double rel_tol, int max_iter, double *B, double *resid,
+                                  int *iters, int flags);
 
 /* ---- thermal model (A7 + A8 + A9) --------------------------------------
  * ngrid layer grids: grid g has depths z[grid_off[g] .. grid_off[g+1]) (at
