@@ -25,6 +25,7 @@ ILLUM_OUT_ON_DEVICE = 2
 ILLUM_ACCUMULATE = 4
 ILLUM_ASYNC = 8
 ILLUM_FF_KEEP_VISIBILITY = 16
+ILLUM_SUN_PER_BODY = 32
 ILLUM_RAD_JACOBI_FP64 = 1
 ILLUM_RAD_MIXED = 2
 ILLUM_THERM_FP64 = 1

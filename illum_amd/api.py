@@ -121,6 +121,15 @@ class DeviceArray:
         call("illum_dev_copy", self.ctx.h, a.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes, 1)
         return a
 
+    def to_host_flat(self, start, count):
+        """Elements [start, start + count) of the flat (row-major) buffer."""
+        if start < 0 or count < 0 or (start + count) * self.dtype.itemsize > self.nbytes:
+            raise _shape_error("slice outside the device array")
+        a = np.empty(count, self.dtype)
+        src = C.c_void_p(self.ptr.value + start * self.dtype.itemsize)
+        call("illum_dev_copy", self.ctx.h, a.ctypes.data_as(C.c_void_p), src, a.nbytes, 1)
+        return a
+
     def data_ptr(self):
         return self.ptr.value
 
@@ -194,9 +203,11 @@ class Mesh:
         n = C.c_int64()
         call("illum_mesh_num_faces", h, C.byref(n))
         self.nf = n.value
+        self.nbody = 1
         if bodies is not None:
             off = np.ascontiguousarray(bodies, np.int64)
             call("illum_mesh_set_bodies", h, _ptr(off), len(off) - 1)
+            self.nbody = len(off) - 1
 
     def close(self):
         if self.h and self.ctx.h:
@@ -455,12 +466,22 @@ class IllumContext:
         from .armaio import load_mat
         self.horizons = load_mat(path)
 
-    def get_direct_radiosity(self, sun_position, disk_xy):
-        """sun_position: (3,) or (3, S) km; returns E of shape (N,) or (N, S)."""
+    def get_direct_radiosity(self, sun_position, disk_xy, per_body=False):
+        """sun_position: (3,) or (3, S) km; returns E of shape (N,) or (N, S).
+        per_body: a multi-body mesh's bodies each have their own S suns,
+        sun_position is (3, S * nbody) with body b's in columns b S .. b S + S - 1."""
         sun = np.asarray(sun_position, np.float64)
         single = sun.ndim == 1
         sun = np.ascontiguousarray(sun.reshape(3, -1).T)  # S x 3 row-major == 3 x S col-major
         S = sun.shape[0]
+        flags = 0
+        if per_body:
+            nb = self.mesh.nbody
+            if S % nb:
+                raise _shape_error("per-body suns must be S x nbody columns")
+            S //= nb
+            single = False
+            flags = _lib.ILLUM_SUN_PER_BODY
         disk = np.asfortranarray(np.asarray(disk_xy, np.float64))
         if disk.ndim != 2 or disk.shape[1] != 2:
             raise _shape_error("disk_xy must be n x 2")
@@ -472,7 +493,7 @@ class IllumContext:
         nphi = hz.shape[0]
         E = np.zeros((S, self.num_faces))
         call("illum_direct_radiosity", self.mesh.h, hz.ctypes.data_as(C.c_void_p), nphi,
-             _ptr(sun), S, disk.ctypes.data_as(C.c_void_p), disk.shape[0], _ptr(E), 0)
+             _ptr(sun), S, disk.ctypes.data_as(C.c_void_p), disk.shape[0], _ptr(E), flags)
         return E[0] if single else E.T
 
 

@@ -91,6 +91,16 @@ def insolation(normals, k0, nsteps, steps_per_rot, albedo=0.1):
     return np.ascontiguousarray(S0 * np.clip(s.T @ np.asarray(normals, np.float64).T, 0, None))
 
 
+def c5_suns(nbody, S=32, seed=0):
+    """3 x (S nbody) sun positions (km): body b's S suns (columns b S ..
+    b S + S - 1) are S evenly spaced steps of one rotation under a sun at a
+    per-body elevation U(0, 40) degrees above the equator."""
+    rng = np.random.default_rng(seed + 7)
+    inc = rng.uniform(0.0, 40.0, nbody)
+    return np.concatenate([np.stack([orbit_sun(k, S, inc[b]) for k in range(S)], 1)
+                           for b in range(nbody)], 1)
+
+
 def c5_bodies(nbody=64, seed=0, max_subdiv=7):
     """C5's uneven batch: [(V, F)] of nbody seeded ellipsoids, axis ratios
     U(0.6, 1.0) on 13 km, subdivision levels uniform on 3..max_subdiv (facet

@@ -20,7 +20,10 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <atomic>
 #include <cstring>
+#include <string>
+#include <thread>
 
 namespace illum {
 
@@ -337,12 +340,40 @@ HostBVH build_bvh(const HostGeometry &g, int leaf_size) {
     h.root_wide4 = {0};
     return h;
   }
-  // Forest: concatenate per-body trees, shifting node links and leaf ranges.
+  // Forest: the bodies' trees are built independently (in parallel, largest
+  // first), then concatenated in body order, shifting node links and leaf
+  // ranges.
+  std::vector<HostBVH> trees(nb);
+  {
+    std::vector<int> order(nb);
+    for (int b = 0; b < nb; ++b) order[b] = b;
+    std::sort(order.begin(), order.end(),
+              [&](int a, int c) { return off[a + 1] - off[a] > off[c + 1] - off[c]; });
+    std::atomic<int> next(0);
+    std::vector<std::string> errs(nb);
+    auto worker = [&]() {
+      for (int k; (k = next.fetch_add(1)) < nb;) {
+        const int b = order[k];
+        try {
+          trees[b] = build_one(g, off[b], off[b + 1], leaf_size, pad);
+        } catch (const std::exception &e) {
+          errs[b] = e.what();
+        }
+      }
+    };
+    const int nt = std::max(1, std::min<int>(nb, (int)std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto &t : pool) t.join();
+    for (int b = 0; b < nb; ++b)
+      if (!errs[b].empty()) throw Error(ILLUM_ERR_ARG, "BVH of body " + std::to_string(b) + ": " + errs[b]);
+  }
   HostBVH h;
   h.leaf_size = leaf_size;
   h.tris.resize(12 * (size_t)nf);
   for (int b = 0; b < nb; ++b) {
-    HostBVH t = build_one(g, off[b], off[b + 1], leaf_size, pad);
+    HostBVH t = std::move(trees[b]);
     const int32_t nbase = (int32_t)h.nnodes, ibase = (int32_t)h.ninner;
     const int32_t qbase = (int32_t)h.ninner4;
     const int32_t tbase = (int32_t)off[b];

@@ -283,7 +283,8 @@ __global__ __launch_bounds__(256) void direct_kernel(
     int64_t nf, int S, int nphi, const float4 *__restrict__ pr4,
     const double *__restrict__ frame, const double *__restrict__ hz,
     const double *__restrict__ sun, const double *__restrict__ disk, int ndisk,
-    double *__restrict__ E, int exact_only) {
+    double *__restrict__ E, int exact_only, const int32_t *__restrict__ body_of,
+    int per_body) {
   static_assert(L >= 1 && L <= 64 && (L & (L - 1)) == 0, "L divides the wavefront");
   const int64_t lane_id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t idx = lane_id / L;
@@ -297,7 +298,9 @@ __global__ __launch_bounds__(256) void direct_kernel(
   const double dphi = TWO_PI / (nphi - 1);
   const float4 pf = pr4[f];
   const double px = pf.x, py = pf.y, pz = pf.z;
-  const double sx = sun[3 * s], sy = sun[3 * s + 1], sz = sun[3 * s + 2];
+  // per_body: body b's S suns are columns b S .. b S + S - 1 of sun
+  const int64_t sc = per_body ? (int64_t)(body_of ? body_of[f] : 0) * S + s : s;
+  const double sx = sun[3 * sc], sy = sun[3 * sc + 1], sz = sun[3 * sc + 2];
   double Nx = sx - px, Ny = sy - py, Nz = sz - pz;
   const double nN = sqrt((Nx * Nx + Ny * Ny) + Nz * Nz);
   Nx = Nx / nN; Ny = Ny / nN; Nz = Nz / nN;
@@ -408,12 +411,14 @@ void direct_radiosity(illum_mesh *mesh, const double *hz, int nphi, const double
   if (nphi < 2 || S < 1 || ndisk < 1) throw Error(ILLUM_ERR_ARG, "bad direct radiosity sizes");
   if (!hz || !sun || !disk || !E) throw Error(ILLUM_ERR_ARG, "null pointer");
   const bool in_dev = flags & ILLUM_IN_ON_DEVICE, out_dev = flags & ILLUM_OUT_ON_DEVICE;
+  const bool per_body = flags & ILLUM_SUN_PER_BODY;
+  const int64_t nsun = per_body ? (int64_t)S * mesh->nbody : S;
   DBuf<double> d_h, d_sun, d_disk, d_E;
   const double *ph = hz, *psun = sun, *pdisk = disk;
   if (!in_dev) {
-    d_h.reset(nf * nphi); d_sun.reset(3 * S); d_disk.reset(2 * ndisk);
+    d_h.reset(nf * nphi); d_sun.reset(3 * nsun); d_disk.reset(2 * ndisk);
     ILLUM_HIP(hipMemcpyAsync(d_h.p, hz, 8 * nf * nphi, hipMemcpyHostToDevice, s));
-    ILLUM_HIP(hipMemcpyAsync(d_sun.p, sun, 8 * 3 * S, hipMemcpyHostToDevice, s));
+    ILLUM_HIP(hipMemcpyAsync(d_sun.p, sun, 8 * 3 * nsun, hipMemcpyHostToDevice, s));
     ILLUM_HIP(hipMemcpyAsync(d_disk.p, disk, 8 * 2 * ndisk, hipMemcpyHostToDevice, s));
     ph = d_h.p; psun = d_sun.p; pdisk = d_disk.p;
   }
@@ -428,7 +433,7 @@ void direct_radiosity(illum_mesh *mesh, const double *hz, int nphi, const double
               : L == 4 ? direct_kernel<4> : L == 2 ? direct_kernel<2> : direct_kernel<1>;
     kern<<<(unsigned)((lanes + 255) / 256), 256, 0, s>>>(
         nf, S, nphi, (const float4 *)mesh->d_pr4, mesh->d_frame, ph, psun, pdisk, ndisk, pE,
-        direct_exact_only() ? 1 : 0);
+        direct_exact_only() ? 1 : 0, mesh->d_body, per_body ? 1 : 0);
     ILLUM_LAUNCH_CHECK();
   }
   if (!out_dev) ILLUM_HIP(hipMemcpyAsync(E, pE, 8 * nf * S, hipMemcpyDeviceToHost, s));

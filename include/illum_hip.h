@@ -60,6 +60,10 @@ extern "C" {
  * closest hits per source before the zero drop) for
  * illum_sparse_copy_visibility_lists.  Testing aid; O(sets) device memory. */
 #define ILLUM_FF_KEEP_VISIBILITY 16
+/* illum_direct_radiosity: every body of a multi-body mesh has its own S suns,
+ * sun_km is 3 x (S nbody) column-major with body b's suns in columns
+ * b S .. b S + S - 1.  No reference counterpart (one body per process). */
+#define ILLUM_SUN_PER_BODY   32
 
 /* radiosity solve methods */
 #define ILLUM_RAD_JACOBI_FP64   1   /* FP64 Jacobi sweeps B <- E + F B                  */

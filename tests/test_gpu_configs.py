@@ -161,3 +161,72 @@ def test_c4_327680_facets_4096_steps(api, precision, tol):
         assert dT <= tol
     else:
         assert dTs <= tol
+
+
+@pytest.mark.parametrize("method", ["mixed"])
+def test_c5_64_bodies_one_launch_sequence(api, method):
+    """C5: 64 bodies (5.4 M facets, 1,280 to 327,680 each) through form
+    factors, horizons (nphi 361), direct insolation under each body's 32 suns,
+    the radiosity solve (S = 32) and 512 CN steps, one launch sequence
+    (illum_amd/batch.py).  Per body, against the oracle run on that body
+    alone, for two 1,280-facet bodies, a 5,120-facet body and a 20,480-facet
+    body (horizons and temperatures on a facet sample there): form-factor
+    columns within 1e-12, horizons bit-exact, E within 1e-12, B within 2e-8
+    of Gauss-Seidel, surface temperatures within 1e-3 K (FP64 conduction
+    from the oracle's own B)."""
+    from illum_amd import batch, configs
+    bodies = configs.c5_bodies()
+    nb, S = len(bodies), 32
+    suns = configs.c5_suns(nb, S)
+    r = batch.run_bodies(bodies, suns, S, method=method, thermal=method)
+    nf, off = r["nf"], r["offsets"]
+    assert nb == 64 and nf == 5434880
+    print("C5 stage seconds:", {k: round(v, 3) for k, v in r["seconds"].items()},
+          "products", r["solve"][2], "residual", r["solve"][1][-1])
+    assert r["solve"][1][-1] <= 1e-8
+    cp, ri, v = r["Fm"].csc()
+    disk = api.fib_spiral(100)
+    sizes = [len(F) for _, F in bodies]
+    picks = [sizes.index(1280), len(sizes) - 1 - sizes[::-1].index(1280), sizes.index(5120),
+             sizes.index(20480)]
+    errs = {}
+    for b in picks:
+        Vb, Fb = bodies[b]
+        lo, hi = int(off[b]), int(off[b + 1])
+        n = hi - lo
+        g = O.geometry(Vb, Fb)
+        bits, _ = O.visibility(g)
+        ocp, ori, ov = O.form_factors(g, bits, albedo=0.1)
+        a, z = cp[lo], cp[hi]
+        assert np.array_equal(cp[lo:hi + 1] - a, ocp), f"body {b} columns"
+        assert np.array_equal(ri[a:z] - lo, ori), f"body {b} rows"
+        ff_err = float(np.max(np.abs(v[a:z] - ov) / np.abs(ov))) if len(ov) else 0.0
+        samp = np.arange(n) if n <= 5120 else np.arange(0, n, n // 256)
+        hz = r["hz"].to_host_flat(lo * 361, n * 361).reshape(n, 361).T
+        if n <= 5120:
+            ho, _ = O.horizons(g, nphi=361)
+        else:
+            ho = np.concatenate([O.horizons(g, nphi=361, j0=j, j1=j + 1)[0] for j in samp], 1)
+        assert np.array_equal(hz[:, samp], ho), f"body {b} horizons"
+        sb = suns[:, b * S:(b + 1) * S]
+        E = np.stack([r["E"].to_host_flat(s * nf + lo, n) for s in range(S)], 1)
+        Eo = O.direct(g, hz, sb, disk)
+        e_err = float(np.max(np.abs(E - Eo) / np.maximum(np.abs(Eo), 1e-300)))
+        assert np.allclose(E, Eo, rtol=1e-12, atol=0), f"body {b} direct"
+        B = np.stack([r["B"].to_host_flat(s * nf + lo, n) for s in range(S)], 1)
+        Bo = np.stack([O.gauss_seidel(n, ocp, ori, ov, Eo[:, s], max_sweeps=200,
+                                      rel_tol=1e-13)[0] for s in range(S)], 1)
+        b_err = float(np.max(np.linalg.norm(B - Bo, axis=0) / np.linalg.norm(Bo, axis=0)))
+        assert b_err <= 2e-8, f"body {b} radiosity {b_err}"
+        # conduction of the sampled facets under the oracle's B
+        steps = r["steps"]
+        Q = np.stack([Bo[samp, (k * S) // steps] for k in range(steps)])
+        z = configs.layer_grid(40)
+        To, _, _, _ = O.thermal_run([0, 40], z, np.full(40, configs.TI), np.full(40, configs.RHOC),
+                                    np.full((41, len(samp)), 200.0), np.zeros(len(samp)), Q,
+                                    r["dt"], configs.EMISS)
+        Ts = r["th"].surface()[0][lo + samp]
+        t_err = float(np.max(np.abs(Ts - To[0])))
+        assert t_err <= 1e-3, f"body {b} surface temperature {t_err}"
+        errs[b] = dict(facets=n, ff=ff_err, E=e_err, B=b_err, Tsurf_K=t_err)
+    print("C5 per-body error vs oracle:", errs)
