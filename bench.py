@@ -263,44 +263,69 @@ def bench_thermal(api, nsteps=4096, subdiv=7, seed=0, chunk=256):
     return out
 
 
-def bench_coupled(api, illum, ctx, Fm, hz, nsteps=32):
-    """radiosity + thermal time loop (illum_cli.cpp:403-456): direct -> E -= F_surf ->
-    radiosity solve -> CN step, all on device buffers."""
+def bench_coupled(api, illum, ctx, Fm, hz, nsteps=256):
+    """The radiosity + thermal time loop (illum_cli.cpp:403-456) at one sun per
+    step: direct -> E -= F_surf -> radiosity solve to 1e-8 -> CN step (40
+    layers), device-resident, replayed from captured HIP graphs
+    (illum_timeloop_*), FP64 (Jacobi + FP64 CN) vs mixed (FP16 corrections
+    with FP64 residual + mixed CN).  The same steps made call by call through
+    the C ABI are timed beside it."""
     from illum_amd import _lib
     nf = illum.num_faces
     nphi = hz.shape[1]
-    z = 0.5 * (np.exp(np.linspace(0, 3, 40)) - 1 + 0.01) / (np.exp(3) - 1 + 0.01)
-    dt = ROT_PERIOD_S / 256
+    spr = 256
+    dt = ROT_PERIOD_S / spr
+    z = configs.layer_grid(40)
     disk = api.DeviceArray.from_host(ctx, np.ascontiguousarray(api.fib_spiral(100).T))
-    ang = 2 * np.pi * np.arange(nsteps) / 256
-    suns = [api.DeviceArray.from_host(ctx, SUN_KM * np.array([np.cos(a) * 0.94, np.sin(a) * 0.94, 0.34]))
-            for a in ang]
-    out = {"steps": nsteps, "facets": nf, "layers": 40}
+    sun = np.stack([configs.orbit_sun(k, spr) for k in range(nsteps)], 1)
+    sund = api.DeviceArray.from_host(ctx, np.ascontiguousarray(sun.T))
+    out = {"steps": nsteps, "facets": nf, "layers": 40, "suns_per_step": 1, "rel_tol": 1e-8}
     finals = {}
     dev = _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_OUT_ON_DEVICE | _lib.ILLUM_ASYNC
     for prec in ("fp64", "mixed"):
-        th = api.ThermalModel(nf, z, 200.0, 1.19177e6, 200.0, emissivity=0.95, precision=prec, ctx=ctx)
+        mk = lambda: api.ThermalModel(nf, z, configs.TI, configs.RHOC, 200.0,
+                                      emissivity=configs.EMISS, precision=prec, ctx=ctx)
+        th2 = mk()
+        loop = api.TimeLoop(Fm, illum, th2, hz, disk, method=prec, rel_tol=1e-8)
+        loop.run(sund, dt)  # first rotation: chooses the product budget, captures the graphs
+        ctx.synchronize()
+        t = time.perf_counter()
+        loop.run(sund, dt)
+        ctx.synchronize()
+        sec = time.perf_counter() - t
+        st = loop.stats()
+        finals[prec] = th2.surface()[0]
+        # call by call (direct, E -= F_surf, solve with host-checked stop, CN step)
+        thc = mk()
         E = api.DeviceArray(ctx, (nf,))
         B = api.DeviceArray(ctx, (nf,))
+        resid = np.zeros(101)
+        it = api.C.c_int()
+        m = {"fp64": _lib.ILLUM_RAD_JACOBI_FP64, "mixed": _lib.ILLUM_RAD_MIXED}[prec]
         ctx.synchronize()
         t = time.perf_counter()
         for k in range(nsteps):
-            _lib.call("illum_direct_radiosity", illum.mesh.h, api._ptr(hz), nphi, api._ptr(suns[k]),
-                      1, api._ptr(disk), 100, api._ptr(E), dev)
-            # rad = direct - F_surf (illum_cli.cpp:411-423), summed on the device
-            th.get_radiosity(out=E, accumulate=True)
-            api.radiosity_solve(Fm, E, method=prec, rel_tol=1e-8, max_iter=100, B=B)
-            th.step(dt, B)
+            p = api.C.c_void_p(sund.ptr.value + 24 * k)
+            _lib.call("illum_direct_radiosity", illum.mesh.h, api._ptr(hz), nphi, p, 1,
+                      api._ptr(disk), 100, api._ptr(E), dev)
+            thc.get_radiosity(out=E, accumulate=True)
+            _lib.call("illum_radiosity_solve", Fm.h, api._ptr(E), 1, m, 1e-8, 100, api._ptr(B),
+                      api._ptr(resid), api.C.byref(it), dev)
+            _lib.call("illum_thermal_step", thc.h, dt, api._ptr(B),
+                      _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_ASYNC)
         ctx.synchronize()
-        sec = time.perf_counter() - t
-        finals[prec] = th.surface()[0]
-        out[prec] = {"seconds": sec, "steps_per_s": nsteps / sec}
-        th.close()
-        E.close()
-        B.close()
-        log(f"coupled {prec}: {nsteps / sec:.1f} steps/s")
+        sec_calls = time.perf_counter() - t
+        out[prec] = {"seconds": sec, "steps_per_s": nsteps / sec, "products_budget": st["products"],
+                     "max_products": st["max_products"], "unconverged": st["unconverged"],
+                     "max_rel_resid": st["max_resid"],
+                     "call_by_call_steps_per_s": nsteps / sec_calls}
+        for x in (thc, E, B, loop, th2):
+            x.close()
+        log(f"coupled {prec}: {nsteps / sec:.1f} steps/s graph, {nsteps / sec_calls:.1f} call by call")
     out["mixed_speedup"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
     out["max_abs_Tsurf_diff_mixed_vs_fp64_K"] = float(np.max(np.abs(finals["mixed"] - finals["fp64"])))
+    disk.close()
+    sund.close()
     return out
 
 

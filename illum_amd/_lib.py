@@ -26,6 +26,7 @@ ILLUM_ACCUMULATE = 4
 ILLUM_ASYNC = 8
 ILLUM_FF_KEEP_VISIBILITY = 16
 ILLUM_SUN_PER_BODY = 32
+ILLUM_TL_EAGER = 64
 ILLUM_RAD_JACOBI_FP64 = 1
 ILLUM_RAD_MIXED = 2
 ILLUM_THERM_FP64 = 1
@@ -90,6 +91,11 @@ SIGNATURES = {
     "illum_thermal_run": [_P, _D, _P, _I, _I],
     "illum_thermal_get": [_P, _P, _P, _P, _I],
     "illum_thermal_get_radiosity": [_P, _P, _I],
+    "illum_timeloop_create": [_P, _P, _P, _P, _I, _P, _I, _I, _D, _I, _I, _PP],
+    "illum_timeloop_run": [_P, _P, _I, _D, _I],
+    "illum_timeloop_stats": [_P, _P, _P, _P, _P, _P],
+    "illum_timeloop_get": [_P, _P, _P, _I],
+    "illum_timeloop_destroy": [_P],
     "illum_conductionQ": [_I, _P, _D, _D, _D, _P, _P, _P, _D, _P, _D, _P],
 }
 _RET = {"illum_last_error": C.c_char_p}

@@ -667,6 +667,69 @@ class ThermalModel:
         return out
 
 
+class TimeLoop:
+    """The coupled radiosity + thermal time loop (illum_cli.cpp:403-456) kept
+    on the device and replayed from captured HIP graphs (illum_timeloop_*).
+
+    F: SparseMatrix, illum: IllumContext (its mesh), thermal: ThermalModel;
+    horizons: DeviceArray of shape (N, nphi) (the nphi x N map column-major);
+    disk: DeviceArray of shape (2, ndisk) (the ndisk x 2 spiral column-major);
+    products: the solve's product budget per step (0 = chosen from the first
+    step); eager: queue kernels instead of replaying graphs."""
+
+    def __init__(self, F, illum, thermal, horizons, disk, method="mixed", rel_tol=1e-8,
+                 products=0, eager=False):
+        m = {"fp64": _lib.ILLUM_RAD_JACOBI_FP64, "mixed": _lib.ILLUM_RAD_MIXED}[method]
+        if horizons.shape[0] != illum.num_faces or len(horizons.shape) != 2:
+            raise _shape_error("horizons must be a DeviceArray of shape (N, nphi)")
+        if len(disk.shape) != 2 or disk.shape[0] != 2:
+            raise _shape_error("disk must be a DeviceArray of shape (2, ndisk)")
+        self.ctx = illum.ctx
+        self.nf = illum.num_faces
+        self._keep = (F, illum, thermal, horizons, disk)
+        h = C.c_void_p()
+        call("illum_timeloop_create", F.h, illum.mesh.h, thermal.h, _ptr(horizons),
+             horizons.shape[1], _ptr(disk), disk.shape[1], m, rel_tol, products,
+             _lib.ILLUM_TL_EAGER if eager else 0, C.byref(h))
+        self.h = h
+
+    def run(self, suns, dt):
+        """suns: (3, nsteps) km on the host, or a DeviceArray of shape (nsteps, 3)."""
+        if _is_dev(suns):
+            n = suns.shape[0]
+            call("illum_timeloop_run", self.h, _ptr(suns), n, dt, _lib.ILLUM_IN_ON_DEVICE)
+        else:
+            sc = np.ascontiguousarray(np.asarray(suns, np.float64).reshape(3, -1).T)
+            call("illum_timeloop_run", self.h, _ptr(sc), sc.shape[0], dt, 0)
+
+    def stats(self):
+        st = C.c_int64()
+        p, mp, un = C.c_int(), C.c_int(), C.c_int()
+        r = C.c_double()
+        call("illum_timeloop_stats", self.h, C.byref(st), C.byref(p), C.byref(mp), C.byref(un),
+             C.byref(r))
+        return dict(steps=st.value, products=p.value, max_products=mp.value,
+                    unconverged=un.value, max_resid=r.value)
+
+    def get(self):
+        """(E, B) of the last step."""
+        E = np.zeros(self.nf)
+        B = np.zeros(self.nf)
+        call("illum_timeloop_get", self.h, _ptr(E), _ptr(B), 0)
+        return E, B
+
+    def close(self):
+        if self.h and self.ctx.h:
+            call("illum_timeloop_destroy", self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def conductionQ(nz, z, dt, Qn, Qnp1, T, ti, rhoc, emiss, Tsurf, Fgeotherm):
     """conductionQ (conduction.hpp:16-28) on the GPU; returns (T, Tsurf, Fsurf)."""
     Tc = np.array(T, np.float64)

@@ -26,7 +26,7 @@ void direct_radiosity(illum_mesh *mesh, const double *hz, int nphi, const double
                       const double *disk, int ndisk, double *E, int flags);
 void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double rel_tol,
                      int max_iter, double *B, double *resid, int *iters, int flags,
-                     void *fixed_scratch);
+                     void *fixed_scratch, size_t fixed_cap, SolveStats *queue_stats);
 void *radiosity_noalloc_scratch(int device);
 illum_thermal *thermal_create(illum_ctx *ctx, int64_t nf, int ngrid, const int64_t *grid_off,
                               const double *z, const double *ti, const double *rhoc,
@@ -44,19 +44,7 @@ void set_last_error(const std::string &msg) { g_last_error = msg; }
 
 template <class Fn>
 static int guarded(Fn &&fn) {
-  try {
-    fn();
-    return ILLUM_OK;
-  } catch (const Error &e) {
-    set_last_error(e.what());
-    return e.code;
-  } catch (const std::bad_alloc &) {
-    set_last_error("host allocation failed");
-    return ILLUM_ERR_OOM;
-  } catch (const std::exception &e) {
-    set_last_error(e.what());
-    return ILLUM_ERR_ARG;
-  }
+  return guarded_call(std::forward<Fn>(fn));
 }
 
 void collect_profile(illum_ctx *ctx) {
@@ -867,7 +855,8 @@ int illum_radiosity_solve(illum_sparse *F, const double *E, int S, int method, d
   return guarded([&] {
     CHECK_ARG(F, "null matrix");
     set_device(F->ctx);
-    radiosity_solve(F, E, S, method, rel_tol, max_iter, B, resid, iters, flags, nullptr);
+    radiosity_solve(F, E, S, method, rel_tol, max_iter, B, resid, iters, flags, nullptr, 0,
+                    nullptr);
   });
 }
 
@@ -879,7 +868,7 @@ int illum_radiosity_solve_noalloc(illum_sparse *F, const double *E, int S, int m
     if (!F->ctx->noalloc_scratch) throw Error(ILLUM_ERR_HIP, "no-allocation scratch not resolved");
     set_device(F->ctx);
     radiosity_solve(F, E, S, method, rel_tol, max_iter, B, resid, iters, flags,
-                    F->ctx->noalloc_scratch);
+                    F->ctx->noalloc_scratch, RAD_NOALLOC_SCRATCH_BYTES, nullptr);
   });
 }
 

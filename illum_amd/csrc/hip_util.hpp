@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 #include <memory>
+#include <new>
 #include <string>
 
 #include "internal.hpp"
@@ -18,6 +19,25 @@
 #define ILLUM_LAUNCH_CHECK() ILLUM_HIP(hipGetLastError())
 
 namespace illum {
+
+// Runs fn, translating exceptions into a status code (the message kept for
+// illum_last_error): the body of every C entry point.
+template <class Fn>
+int guarded_call(Fn &&fn) {
+  try {
+    fn();
+    return ILLUM_OK;
+  } catch (const Error &e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_last_error("host allocation failed");
+    return ILLUM_ERR_OOM;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return ILLUM_ERR_ARG;
+  }
+}
 
 inline hipStream_t stream_of(const illum_ctx *ctx) { return (hipStream_t)ctx->stream; }
 
@@ -92,6 +112,18 @@ enum WorkspaceSlot {
   WS_FF_VPOOL = 10, // raw visibility sets, when kept
   WS_FF_ALBEDO = 11,
   WS_FF_STATUS = 12,
+};
+
+// Size of the no-allocation solve's fixed per-GPU scratch array.
+constexpr size_t RAD_NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+
+// Running record of a time loop's queued radiosity solves (device).
+struct SolveStats {
+  double max_resid;  // largest final relative residual
+  int max_products;  // most products a solve took
+  int unconverged;   // solves that ended above the tolerance
+  int solves;
+  int pad;
 };
 
 // Device blocks from the context's reuse pool (see illum_ctx::pool): a free

@@ -284,7 +284,7 @@ __global__ __launch_bounds__(256) void direct_kernel(
     const double *__restrict__ frame, const double *__restrict__ hz,
     const double *__restrict__ sun, const double *__restrict__ disk, int ndisk,
     double *__restrict__ E, int exact_only, const int32_t *__restrict__ body_of,
-    int per_body) {
+    int per_body, const int *__restrict__ step, const double *__restrict__ flux) {
   static_assert(L >= 1 && L <= 64 && (L & (L - 1)) == 0, "L divides the wavefront");
   const int64_t lane_id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t idx = lane_id / L;
@@ -298,8 +298,10 @@ __global__ __launch_bounds__(256) void direct_kernel(
   const double dphi = TWO_PI / (nphi - 1);
   const float4 pf = pr4[f];
   const double px = pf.x, py = pf.y, pz = pf.z;
-  // per_body: body b's S suns are columns b S .. b S + S - 1 of sun
-  const int64_t sc = per_body ? (int64_t)(body_of ? body_of[f] : 0) * S + s : s;
+  // per_body: body b's S suns are columns b S .. b S + S - 1 of sun; with a
+  // device step counter, step k reads the k-th such block of suns
+  int64_t sc = per_body ? (int64_t)(body_of ? body_of[f] : 0) * S + s : s;
+  if (step) sc += (int64_t)(*step) * S;
   const double sx = sun[3 * sc], sy = sun[3 * sc + 1], sz = sun[3 * sc + 2];
   double Nx = sx - px, Ny = sy - py, Nz = sz - pz;
   const double nN = sqrt((Nx * Nx + Ny * Ny) + Nz * Nz);
@@ -400,7 +402,10 @@ __global__ __launch_bounds__(256) void direct_kernel(
   const double ns = sqrt((sx * sx + sy * sy) + sz * sz);
   const double q = ONE_AU_KM / ns;
   E0 = E0 * (1372 * (q * q));
-  E[(int64_t)s * nf + f] = E0;
+  // flux: the time loop's surface flux F_surf; its radiosity is
+  // rad = -F_surf + direct (illum_cli.cpp:411-423), which equals
+  // direct - F_surf exactly
+  E[(int64_t)s * nf + f] = flux ? E0 - flux[f] : E0;
 }
 
 void direct_radiosity(illum_mesh *mesh, const double *hz, int nphi, const double *sun, int S,
@@ -433,11 +438,29 @@ void direct_radiosity(illum_mesh *mesh, const double *hz, int nphi, const double
               : L == 4 ? direct_kernel<4> : L == 2 ? direct_kernel<2> : direct_kernel<1>;
     kern<<<(unsigned)((lanes + 255) / 256), 256, 0, s>>>(
         nf, S, nphi, (const float4 *)mesh->d_pr4, mesh->d_frame, ph, psun, pdisk, ndisk, pE,
-        direct_exact_only() ? 1 : 0, mesh->d_body, per_body ? 1 : 0);
+        direct_exact_only() ? 1 : 0, mesh->d_body, per_body ? 1 : 0, nullptr, nullptr);
     ILLUM_LAUNCH_CHECK();
   }
   if (!out_dev) ILLUM_HIP(hipMemcpyAsync(E, pE, 8 * nf * S, hipMemcpyDeviceToHost, s));
   if (!(in_dev && out_dev && (flags & ILLUM_ASYNC))) sync(ctx);
+}
+
+// Queues direct insolation for one step of a time loop on stream s, all
+// arrays on the device: sun column (*step) S .. + S - 1, E = direct - sub.
+// No host synchronisation (graph-capturable).
+void direct_enqueue(illum_mesh *mesh, const double *hz, int nphi, const double *sun, int S,
+                    const double *disk, int ndisk, double *E, const int *step, const double *sub,
+                    hipStream_t s) {
+  const int64_t nf = mesh->g.nf;
+  const int64_t pairs = nf * S;
+  const int L = direct_lanes(pairs);
+  const int64_t lanes = pairs * L;
+  auto kern = L == 16 ? direct_kernel<16> : L == 8 ? direct_kernel<8>
+            : L == 4 ? direct_kernel<4> : L == 2 ? direct_kernel<2> : direct_kernel<1>;
+  kern<<<(unsigned)((lanes + 255) / 256), 256, 0, s>>>(
+      nf, S, nphi, (const float4 *)mesh->d_pr4, mesh->d_frame, hz, sun, disk, ndisk, E,
+      direct_exact_only() ? 1 : 0, mesh->d_body, 0, step, sub);
+  ILLUM_LAUNCH_CHECK();
 }
 
 }  // namespace illum

@@ -35,9 +35,8 @@ namespace illum {
 // device that loads the library holds its own copy until the process ends).
 // Host code never names the array in a kernel argument or a copy: it uses the
 // device address that radiosity_noalloc_scratch resolved for that device.
-constexpr size_t RAD_NOALLOC_BYTES = (size_t)64 << 20;
 constexpr int RAD_NOALLOC_MAX_ITER = 256;
-static __device__ __align__(256) unsigned char scratch[RAD_NOALLOC_BYTES];
+static __device__ __align__(256) unsigned char scratch[RAD_NOALLOC_SCRATCH_BYTES];
 
 constexpr int MAX_DEVICES = 64;
 static std::mutex g_scratch_mu;
@@ -486,6 +485,19 @@ __global__ __launch_bounds__(256) void decide_kernel(int S, const double *__rest
   decide(S, col, e2, ctl, hist, tol, red, ex);
 }
 
+// Queue-only solves (a time loop's steps): after the fixed product budget,
+// one thread folds the solve's outcome into the loop's running record.
+__global__ void solve_stats_kernel(const SolveCtl *__restrict__ ctl,
+                                   const double *__restrict__ hist,
+                                   SolveStats *__restrict__ st) {
+  const int k = ctl->nprod;
+  const double rel = k > 0 ? hist[k - 1] : 1.0;
+  st->solves += 1;
+  st->max_products = max(st->max_products, k);
+  if (!ctl->done) st->unconverged += 1;
+  st->max_resid = max_nan(st->max_resid, rel);
+}
+
 // Mixed method start (row-major E): d_0 = fp16(E 2^ex[0][s]), B = 0 (B
 // nullable).  r_0 = E is read from E itself by the first product.
 __global__ void mixed_init_kernel(int64_t n, int S, const double *__restrict__ E,
@@ -767,19 +779,43 @@ constexpr int RAD_FOLD = RAD_RING - 1;
 
 }  // namespace
 
+// Scratch bytes of a solve (the pieces RadScratch hands out, each 256-byte
+// aligned), for callers that own the buffer (a time loop).
+size_t radiosity_scratch_bytes(int64_t nf, int S, int method, int max_iter) {
+  const int64_t n = nf * S;
+  const bool mixed = method == ILLUM_RAD_MIXED, wide = S >= 128, ring = mixed && wide;
+  const int R = ring ? RAD_RING : 2;
+  const int G = (int)std::min<int64_t>(1024, std::max<int64_t>(1, n / 16));
+  const int64_t ngrp8 = ((nf + 4 * SWEEP_RPW - 1) / (4 * SWEEP_RPW) + 7) / 8 * 8;
+  const int64_t gparts = std::max<int64_t>(
+      {(int64_t)G, (nf + SPMM_NNZ_ROWS - 1) / SPMM_NNZ_ROWS, wide ? ngrp8 : 0});
+  auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t ctl = 256 + align(8 * ((size_t)max_iter + 1)) + 2 * align(8 * (size_t)S) +
+                     align(4 * ((size_t)max_iter + 2) * S) + 8 * (size_t)gparts * S;
+  size_t b = align(ctl) + 2 * align(8 * (size_t)n);
+  if (mixed) b += align(8 * (size_t)n) + align(2 * (size_t)R * n);
+  if (S > 1) b += align(8 * (size_t)n);
+  return b;
+}
+
 void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double rel_tol,
                      int max_iter, double *B, double *resid, int *iters, int flags,
-                     void *fixed_scratch) {
+                     void *fixed_scratch, size_t fixed_cap, SolveStats *queue_stats) {
   illum_ctx *ctx = F->ctx;
   // fixed_scratch: the no-allocation variant (device E and B, bounded
   // max_iter, scratch carved from the fixed array, no profiling events)
   const bool noalloc = fixed_scratch != nullptr;
+  // queue_stats: queue-only (a time loop's step, graph-capturable): exactly
+  // max_iter products (those after convergence return at once), the output,
+  // and the outcome folded into *queue_stats on the device; no host wait.
+  const bool queue_only = queue_stats != nullptr;
   if (noalloc && (!(flags & ILLUM_IN_ON_DEVICE) || !(flags & ILLUM_OUT_ON_DEVICE)))
     throw Error(ILLUM_ERR_ARG, "the no-allocation solve takes E and B on the device");
-  if (noalloc && max_iter > RAD_NOALLOC_MAX_ITER)
+  if (noalloc && !queue_only && max_iter > RAD_NOALLOC_MAX_ITER)
     throw Error(ILLUM_ERR_ARG, "the no-allocation solve takes max_iter <= " +
                                    std::to_string(RAD_NOALLOC_MAX_ITER));
-  RadScratch scr{ctx, (unsigned char *)fixed_scratch, RAD_NOALLOC_BYTES, 0};
+  if (queue_only && !noalloc) throw Error(ILLUM_ERR_ARG, "a queued solve needs its own scratch");
+  RadScratch scr{ctx, (unsigned char *)fixed_scratch, fixed_cap, 0};
   if (!F->d_rowptr) throw Error(ILLUM_ERR_ARG, "matrix has no CSR form");
   if (S < 1 || max_iter < 1 || !(rel_tol >= 0)) throw Error(ILLUM_ERR_ARG, "bad solve parameters");
   if (method != ILLUM_RAD_JACOBI_FP64 && method != ILLUM_RAD_MIXED)
@@ -927,7 +963,7 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
   // for the no-allocation variant (max_iter bounded), else on the heap.
   alignas(8) char hstack[256 + 8 * (RAD_NOALLOC_MAX_ITER + 1)];
   std::vector<char> hheap;
-  if (!noalloc) hheap.resize(o_hist + 8 * ((size_t)max_iter + 1));
+  if (!noalloc || queue_only) hheap.resize(queue_only ? 0 : o_hist + 8 * ((size_t)max_iter + 1));
   char *hbuf = noalloc ? hstack : hheap.data();
   SolveCtl h{};
   int launched = 0;
@@ -959,6 +995,13 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
     }
     if (!out_dev) ILLUM_HIP(hipMemcpyAsync(B, dst, 8 * n, hipMemcpyDeviceToHost, s));
   };
+  if (queue_only) {
+    for (int k = 0; k < max_iter; ++k) sweep(k);
+    output();
+    solve_stats_kernel<<<1, 1, 0, s>>>(ctl, hist, queue_stats);
+    ILLUM_LAUNCH_CHECK();
+    return;
+  }
   // With S = 1 the sweeps read a device E in place: a B overlapping it is
   // written once the solve has ended.
   const bool early_out = !(S == 1 && overlap(dst, Ein, 8 * n));

@@ -722,7 +722,7 @@ static void ensure_tables(illum_thermal *th, double dt) {
 
 template <bool MIXED>
 static void launch_all(illum_thermal *th, ThermalImpl *im, hipStream_t s0, double Fgeo,
-                       int nsteps, const double *dQ, double *Tss) {
+                       int nsteps, const double *dQ, double *Tss, bool fork_ok = true) {
   const char *eng = getenv("ILLUM_CN_ENGINE");
   const bool force_stream = eng && std::string(eng) == "stream";
   // The depth buckets touch disjoint slots and facets.  At one or two
@@ -730,7 +730,7 @@ static void launch_all(illum_thermal *th, ThermalImpl *im, hipStream_t s0, doubl
   // so with several buckets they run concurrently on up to three forked
   // streams (largest first), each filling the others' tails.
   const int nl = (int)th->launches.size();
-  const int nfork = nl > 1 ? std::min(nl, 3) : 0;
+  const int nfork = nl > 1 && fork_ok ? std::min(nl, 3) : 0;
   std::vector<hipStream_t> fs;
   if (nfork) fs = fork_streams(th->ctx, nfork);
   std::vector<int> order(nl);
@@ -813,6 +813,28 @@ void thermal_run(illum_thermal *th, double dt, const double *Q, int nsteps, int 
   if ((flags & ILLUM_IN_ON_DEVICE) && (flags & ILLUM_ASYNC)) return;
   check_status(th);
 }
+
+// Prepares the coefficient tables of step dt (host work, may allocate), so
+// that thermal_enqueue can then be captured.
+void thermal_prepare(illum_thermal *th, double dt) {
+  if (!(dt > 0)) throw Error(ILLUM_ERR_ARG, "dt must be positive");
+  ensure_tables(th, dt);
+}
+
+// Queues nsteps steps with the device Q series dQ on the context stream, on
+// that stream alone (no forked streams, no status readback, no host work):
+// graph-capturable once thermal_prepare(th, dt) has run.  A zero pivot stays
+// in the model's status word for the next checked call.
+void thermal_enqueue(illum_thermal *th, double dt, const double *dQ, int nsteps) {
+  ThermalImpl *im = impl_of(th);
+  if (im->dt != dt || !im->d_hdr) throw Error(ILLUM_ERR_ARG, "thermal tables not prepared for dt");
+  hipStream_t s = stream_of(th->ctx);
+  if (th->precision == ILLUM_THERM_MIXED) launch_all<true>(th, im, s, 0.0, nsteps, dQ, nullptr, false);
+  else launch_all<false>(th, im, s, 0.0, nsteps, dQ, nullptr, false);
+}
+
+// The model's sticky status: ILLUM_ERR_ZERO_PIVOT after a zero pivot.
+void thermal_check(illum_thermal *th) { check_status(th); }
 
 void thermal_get(illum_thermal *th, double *T, double *Tsurf, double *Fsurf, int flags) {
   illum_ctx *ctx = th->ctx;

@@ -64,6 +64,9 @@ extern "C" {
  * sun_km is 3 x (S nbody) column-major with body b's suns in columns
  * b S .. b S + S - 1.  No reference counterpart (one body per process). */
 #define ILLUM_SUN_PER_BODY   32
+/* illum_timeloop_create: queue each step's kernels directly instead of
+ * replaying captured HIP graphs (same results; for comparison). */
+#define ILLUM_TL_EAGER       64
 
 /* radiosity solve methods */
 #define ILLUM_RAD_JACOBI_FP64   1   /* FP64 Jacobi sweeps B <- E + F B                  */
@@ -79,6 +82,7 @@ typedef struct illum_mesh    illum_mesh;
 typedef struct illum_bvh     illum_bvh;
 typedef struct illum_sparse  illum_sparse;
 typedef struct illum_thermal illum_thermal;
+typedef struct illum_timeloop illum_timeloop;
 
 const char *illum_last_error(void);
 int illum_version(void);
@@ -298,6 +302,43 @@ int illum_thermal_get(illum_thermal *th, double *T, double *Tsurf, double *Fsurf
  * doubles), or out += -Fsurf with ILLUM_ACCUMULATE (the rad = direct -
  * Fsurf sum of illum_cli.cpp:411-423 without leaving the device). */
 int illum_thermal_get_radiosity(illum_thermal *th, double *out, int flags);
+
+/* ---- coupled radiosity + thermal time loop (illum_cli.cpp:403-456) ------
+ * One step: E = direct insolation under the step's sun minus the thermal
+ * model's surface flux (rad = -F_surf + direct, illum_cli.cpp:411-423), the
+ * solve (I - F) B = E by `method` to rel_tol, then one Crank-Nicolson step
+ * of th with Q = B (illum_cli.cpp:446).  Everything stays on the device and a
+ * run waits on the host only at its end: each solve queues a fixed budget of
+ * `products` products (0: twice what a checked solve of the first step
+ * takes, at least 4), products after convergence return at once, and a
+ * solve still above rel_tol after its budget is counted.  Steps are captured
+ * once into HIP graphs (1 and 8 steps) and replayed; ILLUM_TL_EAGER queues
+ * the same kernels directly.  horizons (nphi x nf column-major) and disk_xy
+ * (ndisk x 2 column-major) are device arrays that must outlive the loop; F,
+ * mesh and th belong to one context and must outlive it too.  The
+ * reference's fixed Gauss-Seidel sweep count (gs_steps) is replaced by the
+ * residual stop. */
+int illum_timeloop_create(illum_sparse *F, illum_mesh *mesh, illum_thermal *th,
+                          const double *horizons, int nphi, const double *disk_xy, int ndisk,
+                          int method, double rel_tol, int products, int flags,
+                          illum_timeloop **out);
+/* nsteps steps with suns sun_km (3 x nsteps column-major, km; host, or
+ * device with ILLUM_IN_ON_DEVICE) and time step dt.  Returns
+ * ILLUM_ERR_NOT_CONVERGED when a solve ended above rel_tol (the steps still
+ * ran) and ILLUM_ERR_ZERO_PIVOT after a zero pivot.  With ILLUM_IN_ON_DEVICE
+ * | ILLUM_ASYNC it returns once the steps are queued (illum_timeloop_stats
+ * then waits and reports). */
+int illum_timeloop_run(illum_timeloop *tl, const double *sun_km, int nsteps, double dt,
+                       int flags);
+/* steps run so far; the product budget; over the last run: the most products
+ * a solve took, the solves that ended above rel_tol, the largest final
+ * relative residual.  Any pointer may be null. */
+int illum_timeloop_stats(illum_timeloop *tl, int64_t *steps, int *products, int *max_products,
+                         int *unconverged, double *max_resid);
+/* E and B of the last step (nf doubles each; host, or device with
+ * ILLUM_OUT_ON_DEVICE); either may be null. */
+int illum_timeloop_get(illum_timeloop *tl, double *E, double *B, int flags);
+int illum_timeloop_destroy(illum_timeloop *tl);
 
 /* Single-facet drop-in for conductionQ (conduction.hpp:16-28), run on the
  * GPU.  T (nz) and Tsurf, Fsurf are updated in place. */
