@@ -48,6 +48,7 @@ SIGNATURES = {
     "illum_ctx_set_profiling": [_P, _I],
     "illum_ctx_reset_profile": [_P],
     "illum_ctx_get_profile": [_P, _I, _P, _P, _P, _I, _P],
+    "illum_ctx_tuning": [_P, _P, _I, _P],
     "illum_dev_alloc": [_P, _I64, _PP],
     "illum_dev_free": [_P, _P],
     "illum_dev_copy": [_P, _P, _P, _I64, _I],

@@ -82,6 +82,18 @@ class Context:
     def reset_profile(self):
         call("illum_ctx_reset_profile", self.h)
 
+    def tuning(self):
+        """The autotuner's choices on this context: [(key, choice, best_ms, trials)]."""
+        n = C.c_int()
+        call("illum_ctx_tuning", self.h, None, 0, C.byref(n))
+        buf = C.create_string_buffer(n.value + 1)
+        call("illum_ctx_tuning", self.h, buf, n.value + 1, C.byref(n))
+        out = []
+        for line in buf.value.decode().splitlines():
+            f = line.split()
+            out.append((f[0], int(f[1]), float(f[2]), f[3:]))
+        return out
+
     def profile(self):
         """{stage: (total_ms, launches)} of hipEvent-timed stages."""
         cap = 64

@@ -425,6 +425,22 @@ int illum_ctx_get_profile(illum_ctx *ctx, int cap, double *total_ms, int64_t *la
   });
 }
 
+int illum_ctx_tuning(illum_ctx *ctx, char *out, int cap, int *len) {
+  return guarded([&] {
+    CHECK_ARG(ctx && len, "null argument");
+    std::string r;
+    for (auto &t : ctx->tuned)
+      r += t.key + " " + std::to_string(t.choice) + " " + std::to_string(t.best_ms) + " " +
+           t.trials + "\n";
+    *len = (int)r.size();
+    if (out && cap > 0) {
+      const int n = std::min<int>(cap - 1, (int)r.size());
+      std::memcpy(out, r.data(), n);
+      out[n] = 0;
+    }
+  });
+}
+
 int illum_dev_alloc(illum_ctx *ctx, int64_t bytes, void **out) {
   return guarded([&] {
     CHECK_ARG(ctx && out && bytes >= 0, "bad argument");

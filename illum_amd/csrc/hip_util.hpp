@@ -2,9 +2,12 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdlib>
 #include <memory>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "internal.hpp"
 
@@ -143,6 +146,51 @@ T *palloc(illum_ctx *ctx, size_t n) {
 // them since.  A call that forks joins before it returns.
 std::vector<hipStream_t> fork_streams(illum_ctx *ctx, int n);
 void join_streams(illum_ctx *ctx, int n);
+
+// Autotuner: returns the cached choice for `key`, or times run(c) for each
+// candidate c on the context stream (hipEvents, best of two launches) and
+// keeps the fastest.  run(c) must give the same results for every c.  While
+// the stream is being captured into a graph nothing is timed: `fallback` is
+// returned and nothing cached.  ILLUM_NO_AUTOTUNE=1 returns `fallback`.
+// may_time = false (callers that must not create events or allocate, such
+// as the no-allocation solve) only looks the choice up.
+template <class Run>
+int autotune(illum_ctx *ctx, const std::string &key, const std::vector<int> &cands,
+             int fallback, Run &&run, bool may_time = true) {
+  auto it = ctx->tune_cache.find(key);
+  if (it != ctx->tune_cache.end()) return it->second;
+  const char *off = getenv("ILLUM_NO_AUTOTUNE");
+  if ((off && off[0] == '1') || !may_time) return fallback;
+  hipStream_t s = stream_of(ctx);
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  ILLUM_HIP(hipStreamIsCapturing(s, &cs));
+  if (cs != hipStreamCaptureStatusNone) return fallback;
+  hipEvent_t e0, e1;
+  ILLUM_HIP(hipEventCreate(&e0));
+  ILLUM_HIP(hipEventCreate(&e1));
+  int best = fallback;
+  float best_ms = 1e30f;
+  std::string trials;
+  for (int c : cands) {
+    float ms = 1e30f;
+    for (int rep = 0; rep < 2; ++rep) {  // the first launch also loads the code
+      ILLUM_HIP(hipEventRecord(e0, s));
+      run(c);
+      ILLUM_HIP(hipEventRecord(e1, s));
+      ILLUM_HIP(hipEventSynchronize(e1));
+      float t = 0;
+      ILLUM_HIP(hipEventElapsedTime(&t, e0, e1));
+      ms = std::min(ms, t);
+    }
+    trials += (trials.empty() ? "" : " ") + std::to_string(c) + ":" + std::to_string(ms);
+    if (ms < best_ms) { best_ms = ms; best = c; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ctx->tune_cache[key] = best;
+  ctx->tuned.push_back({key, best, (double)best_ms, trials});
+  return best;
+}
 
 // Drains pending stage events into the profile (after the stream is synced).
 void collect_profile(illum_ctx *ctx);

@@ -408,6 +408,33 @@ __global__ __launch_bounds__(256) void direct_kernel(
   E[(int64_t)s * nf + f] = flux ? E0 - flux[f] : E0;
 }
 
+// One direct_kernel launch over nf x S pairs with L lanes per pair.  L does
+// not change E (the lanes only share the integer count), so it is picked by
+// the autotuner per pair-count scale (ILLUM_DIRECT_LANES=1..16 overrides).
+static void direct_launch(illum_mesh *mesh, int nphi, const double *hz, const double *sun, int S,
+                          const double *disk, int ndisk, double *E, bool per_body,
+                          const int *step, const double *flux, hipStream_t s) {
+  const int64_t nf = mesh->g.nf;
+  const int64_t pairs = nf * S;
+  auto go = [&](int L) {
+    const int64_t lanes = pairs * L;
+    auto kern = L == 16 ? direct_kernel<16> : L == 8 ? direct_kernel<8>
+              : L == 4 ? direct_kernel<4> : L == 2 ? direct_kernel<2> : direct_kernel<1>;
+    kern<<<(unsigned)((lanes + 255) / 256), 256, 0, s>>>(
+        nf, S, nphi, (const float4 *)mesh->d_pr4, mesh->d_frame, hz, sun, disk, ndisk, E,
+        direct_exact_only() ? 1 : 0, mesh->d_body, per_body ? 1 : 0, step, flux);
+    ILLUM_LAUNCH_CHECK();
+  };
+  int L = direct_lanes(pairs);
+  if (!getenv("ILLUM_DIRECT_LANES")) {
+    int b = 0;
+    while (b < 62 && ((int64_t)1 << (b + 1)) <= pairs) ++b;
+    L = autotune(mesh->ctx, "direct_lanes[pairs~2^" + std::to_string(b) + "]", {1, 2, 4, 8, 16},
+                 L, go);
+  }
+  go(L);
+}
+
 void direct_radiosity(illum_mesh *mesh, const double *hz, int nphi, const double *sun, int S,
                       const double *disk, int ndisk, double *E, int flags) {
   illum_ctx *ctx = mesh->ctx;
@@ -431,36 +458,20 @@ void direct_radiosity(illum_mesh *mesh, const double *hz, int nphi, const double
   if (!out_dev) { d_E.reset(nf * S); pE = d_E.p; }
   {
     StageTimer st(ctx, "direct");
-    const int64_t pairs = nf * S;
-    const int L = direct_lanes(pairs);
-    const int64_t lanes = pairs * L;
-    auto kern = L == 16 ? direct_kernel<16> : L == 8 ? direct_kernel<8>
-              : L == 4 ? direct_kernel<4> : L == 2 ? direct_kernel<2> : direct_kernel<1>;
-    kern<<<(unsigned)((lanes + 255) / 256), 256, 0, s>>>(
-        nf, S, nphi, (const float4 *)mesh->d_pr4, mesh->d_frame, ph, psun, pdisk, ndisk, pE,
-        direct_exact_only() ? 1 : 0, mesh->d_body, per_body ? 1 : 0, nullptr, nullptr);
-    ILLUM_LAUNCH_CHECK();
+    direct_launch(mesh, nphi, ph, psun, S, pdisk, ndisk, pE, per_body, nullptr, nullptr, s);
   }
   if (!out_dev) ILLUM_HIP(hipMemcpyAsync(E, pE, 8 * nf * S, hipMemcpyDeviceToHost, s));
   if (!(in_dev && out_dev && (flags & ILLUM_ASYNC))) sync(ctx);
 }
 
 // Queues direct insolation for one step of a time loop on stream s, all
-// arrays on the device: sun column (*step) S .. + S - 1, E = direct - sub.
-// No host synchronisation (graph-capturable).
+// arrays on the device: sun column (*step) S .. + S - 1, E = direct - flux.
+// No host synchronisation (graph-capturable; the lanes per pair must have
+// been tuned by an earlier call outside the capture).
 void direct_enqueue(illum_mesh *mesh, const double *hz, int nphi, const double *sun, int S,
-                    const double *disk, int ndisk, double *E, const int *step, const double *sub,
+                    const double *disk, int ndisk, double *E, const int *step, const double *flux,
                     hipStream_t s) {
-  const int64_t nf = mesh->g.nf;
-  const int64_t pairs = nf * S;
-  const int L = direct_lanes(pairs);
-  const int64_t lanes = pairs * L;
-  auto kern = L == 16 ? direct_kernel<16> : L == 8 ? direct_kernel<8>
-            : L == 4 ? direct_kernel<4> : L == 2 ? direct_kernel<2> : direct_kernel<1>;
-  kern<<<(unsigned)((lanes + 255) / 256), 256, 0, s>>>(
-      nf, S, nphi, (const float4 *)mesh->d_pr4, mesh->d_frame, hz, sun, disk, ndisk, E,
-      direct_exact_only() ? 1 : 0, mesh->d_body, 0, step, sub);
-  ILLUM_LAUNCH_CHECK();
+  direct_launch(mesh, nphi, hz, sun, S, disk, ndisk, E, false, step, flux, s);
 }
 
 }  // namespace illum

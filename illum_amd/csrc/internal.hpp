@@ -111,6 +111,12 @@ struct illum_ctx {
   // matrix's blocks stay here and the next assembly takes them (best fit),
   // so a loop of assemblies allocates and frees nothing.  Blocks are reused
   // in stream order on this context's stream.
+  // Autotuner record: choices of launch shapes that do not change results,
+  // timed on the live inputs the first time a shape is seen (key -> choice),
+  // and every trial for reporting.
+  struct Tuned { std::string key; int choice; double best_ms; std::string trials; };
+  std::vector<Tuned> tuned;
+  std::map<std::string, int> tune_cache;
   std::multimap<size_t, void *> pool;   // capacity -> free block
   std::map<void *, size_t> pool_cap;    // every block from pool_alloc -> capacity
   size_t pool_bytes = 0;                // bytes held free in `pool`

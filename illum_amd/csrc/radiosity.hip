@@ -724,29 +724,43 @@ int sweep_product(hipStream_t s, illum_ctx *ctx, const illum_sparse *F, int S, c
   const int64_t nf = F->n;
   int nparts = 0;
   if (S >= 128) {
-    // PER: the widest of 8, 4, 2, 1 dividing S whose gathered chunk
-    // (nf x 64 PER values of X) stays within 128 MB, so it is served from the
-    // 256 MB memory-side cache rather than HBM.
-    // ILLUM_SPMM_CHUNK_BYTES overrides the 128 MB budget.
+    // PER: right-hand sides per lane.  The products of every PER are
+    // identical (FMA accumulation in CSR order) and idempotent at S >= 128
+    // (FP64: Y = R + F X; mixed ring mode leaves B alone), so the autotuner
+    // times the candidates (64 PER <= S, S % PER == 0) on the solve's first
+    // product.  Without it: the widest whose gathered chunk (nf x 64 PER
+    // values of X) stays within 128 MB (ILLUM_SPMM_CHUNK_BYTES), so it is
+    // served from the 256 MB memory-side cache rather than HBM.
     const char *env = getenv("ILLUM_SPMM_CHUNK_BYTES");
     const double budget = env ? atof(env) : 128e6;
-    // Also 64 PER <= S, so every lane of a chunk has work.
     int per = 8;
     while (per > 1 && (S % per != 0 || 64 * per > S ||
                        (double)nf * 64 * per * sizeof(TX) > budget))
       per >>= 1;
     const int64_t ngrp = (nf + 4 * SWEEP_RPW - 1) / (4 * SWEEP_RPW);
     const int ngrp8 = (int)((ngrp + 7) / 8 * 8);
-    const int nch = (S + 64 * per - 1) / (64 * per);
-    const unsigned grid = (unsigned)((int64_t)ngrp8 * nch);
-#define ILLUM_SWEEP_WIDE(P)                                                                  \
-  sweep_wide_kernel<TX, P><<<grid, 256, 0, s>>>(nf, S, ngrp8, F->d_rowptr, F->d_colidx,       \
+    auto go = [&](int P) {
+      const int nch = (S + 64 * P - 1) / (64 * P);
+      const unsigned grid = (unsigned)((int64_t)ngrp8 * nch);
+#define ILLUM_SWEEP_WIDE(Q)                                                                  \
+  sweep_wide_kernel<TX, Q><<<grid, 256, 0, s>>>(nf, S, ngrp8, F->d_rowptr, F->d_colidx,       \
                                                 F->d_rval, X, R, Y, done, part, m)
-    if (per == 8) ILLUM_SWEEP_WIDE(8);
-    else if (per == 4) ILLUM_SWEEP_WIDE(4);
-    else if (per == 2) ILLUM_SWEEP_WIDE(2);
-    else ILLUM_SWEEP_WIDE(1);
+      if (P == 8) ILLUM_SWEEP_WIDE(8);
+      else if (P == 4) ILLUM_SWEEP_WIDE(4);
+      else if (P == 2) ILLUM_SWEEP_WIDE(2);
+      else ILLUM_SWEEP_WIDE(1);
 #undef ILLUM_SWEEP_WIDE
+      ILLUM_LAUNCH_CHECK();
+    };
+    if (!env) {
+      std::vector<int> cands;
+      for (int P : {8, 4, 2, 1})
+        if (S % P == 0 && 64 * P <= S) cands.push_back(P);
+      per = autotune(ctx, std::string("spmm_per[") + (is_mixed<TX>() ? "fp16" : "fp64") +
+                              ",nf=" + std::to_string(nf) + ",S=" + std::to_string(S) + "]",
+                     cands, per, go, timed);
+    }
+    go(per);
     nparts = ngrp8;
   } else if (S > SPMM_NNZ_MAXS) {
     sweep_rows_kernel<TX><<<(unsigned)((nf + 3) / 4), 256, 0, s>>>(

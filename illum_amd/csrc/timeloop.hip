@@ -16,6 +16,7 @@
 #include "hip_util.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 namespace illum {
@@ -125,8 +126,9 @@ hipGraphExec_t capture(illum_timeloop *tl, int nsteps, double dt) {
   return e;
 }
 
-// The solve's product budget when none was given: twice what a checked
-// solve of the first step's right-hand side takes, at least 4.
+// The solve's product budget when none was given: two more than a checked
+// solve of the first step's right-hand side takes, at least 4 (a later
+// solve that needs more is counted as unconverged and reported).
 void choose_budget(illum_timeloop *tl) {
   hipStream_t s = stream_of(tl->ctx);
   set_int_kernel<<<1, 1, 0, s>>>(tl->d_step, 0);
@@ -141,7 +143,7 @@ void choose_budget(illum_timeloop *tl) {
     if (e.code != ILLUM_ERR_NOT_CONVERGED) throw;
     iters = 200;
   }
-  tl->budget = std::max(4, 2 * iters);
+  tl->budget = std::max(4, iters + 2);
 }
 
 }  // namespace
@@ -179,7 +181,10 @@ int illum_timeloop_create(illum_sparse *F, illum_mesh *mesh, illum_thermal *th,
     tl->method = method;
     tl->rel_tol = rel_tol;
     tl->budget = products;
-    tl->eager = flags & ILLUM_TL_EAGER;
+    // ILLUM_TL_EAGER=1 in the environment forces the eager form (the same
+    // kernels, queued directly), for tools that do not follow graph launches
+    const char *ev = getenv("ILLUM_TL_EAGER");
+    tl->eager = (flags & ILLUM_TL_EAGER) || (ev && ev[0] == '1');
     const int64_t nf = mesh->g.nf;
     tl->d_E = dalloc<double>(nf);
     tl->d_B = dalloc<double>(nf);
@@ -220,6 +225,11 @@ int illum_timeloop_run(illum_timeloop *tl, const double *sun_km, int nsteps, dou
     }
     thermal_prepare(tl->th, dt);
     set_int_kernel<<<1, 1, 0, s>>>(tl->d_step, 0);
+    ILLUM_LAUNCH_CHECK();
+    // one eager direct pass (its result is recomputed by step 0): tunes the
+    // kernel's lanes per pair before any capture
+    direct_enqueue(tl->mesh, tl->hz, tl->nphi, tl->d_sun, 1, tl->disk, tl->ndisk, tl->d_E,
+                   tl->d_step, tl->th->d_F, s);
     ILLUM_LAUNCH_CHECK();
     stats_reset_kernel<<<1, 1, 0, s>>>(tl->d_stats);
     ILLUM_LAUNCH_CHECK();

@@ -104,6 +104,17 @@ int illum_ctx_reset_profile(illum_ctx *ctx);
 int illum_ctx_get_profile(illum_ctx *ctx, int cap, double *total_ms, int64_t *launches,
                           char *names, int names_cap, int *n);
 
+/* The autotuner's record on this context: one line per tuned launch shape,
+ * "key choice best_ms c1:ms1 c2:ms2 ...".  Launch shapes that do not change
+ * results (the direct kernel's lanes per facet-sun pair, the wide SpMM's
+ * right-hand sides per lane, the conduction buckets' stream layout) are
+ * timed on the live inputs the first time a shape is seen and the fastest is
+ * kept for the context's life; ILLUM_NO_AUTOTUNE=1 keeps the built-in
+ * heuristics.  The BVH leaf size and walk are tuned by illum_bvh_build.
+ * Writes at most cap bytes (NUL-terminated) and the full length to *len.
+ * No reference counterpart. */
+int illum_ctx_tuning(illum_ctx *ctx, char *out, int cap, int *len);
+
 /* ---- device memory on the context's GPU (for the *_ON_DEVICE paths) -----
  * No counterpart in the reference (it has no device); lets C/C++ callers keep
  * E, B, horizons and Q series resident in HBM between calls.  kind: 0 host to
@@ -309,7 +320,7 @@ int illum_thermal_get_radiosity(illum_thermal *th, double *out, int flags);
  * solve (I - F) B = E by `method` to rel_tol, then one Crank-Nicolson step
  * of th with Q = B (illum_cli.cpp:446).  Everything stays on the device and a
  * run waits on the host only at its end: each solve queues a fixed budget of
- * `products` products (0: twice what a checked solve of the first step
+ * `products` products (0: two more than a checked solve of the first step
  * takes, at least 4), products after convergence return at once, and a
  * solve still above rel_tol after its budget is counted.  Steps are captured
  * once into HIP graphs (1 and 8 steps) and replayed; ILLUM_TL_EAGER queues

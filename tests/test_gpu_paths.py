@@ -160,3 +160,32 @@ def test_consecutive_solves_report_their_own_products(api, method):
     assert e.value.code == _lib.ILLUM_ERR_NOT_CONVERGED
     B2, res2, it2 = api.radiosity_solve(Fm, E, method=method, rel_tol=1e-12, max_iter=200)
     assert it2 == it and np.array_equal(res2, res) and np.array_equal(B2, B)
+
+
+def test_autotuned_launch_shapes_do_not_change_results(api, monkeypatch):
+    # The autotuner picks the direct kernel's lanes per pair and the wide
+    # SpMM's right-hand sides per lane by timing; every choice must give the
+    # same E and B as each forced choice.
+    from illum_amd import configs
+    V, F = api.synth_ellipsoid(4, ncraters=40, seed=23)
+    ic = api.IllumContext(V=V, F=F)
+    Fm = ic.compute_F(0.3)
+    ic.make_horizons(nphi=33)
+    disk = api.fib_spiral(100)
+    sun = configs.hemisphere_suns(136, 4)
+    E = ic.get_direct_radiosity(sun, disk)
+    for L in (1, 2, 4, 8, 16):
+        monkeypatch.setenv("ILLUM_DIRECT_LANES", str(L))
+        assert np.array_equal(ic.get_direct_radiosity(sun, disk), E), L
+    monkeypatch.delenv("ILLUM_DIRECT_LANES")
+    for method in ("fp64", "mixed"):
+        B, res, it = api.radiosity_solve(Fm, E, method=method, rel_tol=1e-10)
+        # a chunk budget of 1 byte forces one right-hand side per lane
+        monkeypatch.setenv("ILLUM_SPMM_CHUNK_BYTES", "1")
+        B1, res1, it1 = api.radiosity_solve(Fm, E, method=method, rel_tol=1e-10)
+        monkeypatch.delenv("ILLUM_SPMM_CHUNK_BYTES")
+        assert it == it1 and np.array_equal(res, res1) and np.array_equal(B, B1)
+    keys = [t[0] for t in api.default_context().tuning()]
+    assert any(k.startswith("direct_lanes[") for k in keys)
+    assert any(k.startswith("spmm_per[fp64") for k in keys)
+    assert any(k.startswith("spmm_per[fp16") for k in keys)
