@@ -202,18 +202,130 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ)
 //   z_j = (rho_j + alpha_j z_{j+1}) / beta_j,
 //   d_1 = z_1,  d_j = z_j + (gamma_j / beta_j) d_{j-1},
 // so only row 1 (whose diagonal follows the surface temperature) differs
-// between steps.  The pivots of rows >= 2 are therefore computed once per
-// launch, pre-scaled by the facet's conductivity factor s2 (P'_j = s2 /
-// beta_j), which folds s2 out of every step: with the grid's unscaled
-// alpha, gamma (wavefront-uniform scalars)
-//   z_j = (alpha_j P'_j) z_{j+1} + (alpha_j e_{j+1} - gamma_j e_j) P'_j,
-//   e_j = T_j - T_{j-1},
-//   d_j = z_j + gamma_j P'_j d_{j-1}.
-// The column T (FP64), z and P' (FP32) stay in registers.  rho is carried
-// halved (the factor 2 is applied in the FP64 update); the FP32 sweeps use
-// fused multiply-adds.
-template <int NZ>
+// between steps.  Everything of rows >= 2 is therefore computed once per
+// launch, per lane (the facet's conductivity factor s2 scales alpha and
+// gamma): with P'_j = s2 / beta_j and the grid's unscaled alpha, gamma,
+//   Q_j = alpha_j P'_j,  R_j = gamma_j P'_j,  e_j = T_j - T_{j-1},
+//   z_j = Q_j (z_{j+1} + e_{j+1}) - R_j e_j,
+//   d_j = R_j d_{j-1} + z_j.
+// A step then costs per layer one FP64 difference and one FP64 update, one
+// add, one multiply and two fused multiply-adds in FP32, with no coefficient
+// loads.  The column T (FP64), z, Q and R (FP32) stay in registers.  rho is
+// carried halved (the factor 2 is applied in the FP64 update).  GEO adds the
+// geothermal term on the bottom row (the thermal model's flux is 0,
+// thermal.hpp:79, so the usual instantiation leaves it out).
+template <int NZ, bool GEO>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ), cn_waves(NZ)))) void cn_mixed_kernel(
+    const int32_t *__restrict__ fidx, const int32_t *__restrict__ wgrid, int64_t block0,
+    int64_t nf, int64_t nslot, const double *__restrict__ s2f, const double *__restrict__ hdr,
+    const double *__restrict__ coef, const float4 *__restrict__ coefp,
+    const int64_t *__restrict__ goff, const int64_t *__restrict__ goffp, double emiss,
+    double Fgeo, int nsteps, const double *__restrict__ Q, double *__restrict__ Qprev,
+    double *__restrict__ T, double *__restrict__ Fs, double *__restrict__ Tsurf_series) {
+  const int64_t blk = block0 + blockIdx.x;
+  const int64_t slot = blk * 64 + threadIdx.x;
+  const int f = fidx[slot];
+  const bool act = f >= 0;
+  const int g = wgrid[blk];
+  const GridHdr H = *(const GridHdr *)(hdr + 6 * g);
+  const int nz = (int)H.nz;  // uniform across the wavefront, <= NZ
+  const double *al = coef + 2 * goff[g];
+  // per layer j (entry j - 1): alpha, gamma, alpha + gamma, 1 on the bottom row;
+  // zeros past the grid's nz up to NZ (the bucket's depth)
+  const float4 *cq = coefp + goffp[g];
+  const double s2 = (act && s2f) ? s2f[f] : 1.0;
+  const float s2s = (float)s2;
+  const double k1 = s2 * H.k1, kfirst = s2 * H.kfirst;
+  const float geo_half = (float)(0.5 * ((H.geo * Fgeo) / H.dzlast));
+  const double a1 = s2 * al[0], g1 = s2 * al[1];
+  const float a1s = (float)a1;
+
+  double Tl[NZ + 1];  // Tl[0] surface, Tl[1..nz] layers
+#pragma unroll
+  for (int l = 0; l <= NZ; ++l) Tl[l] = (act && l <= nz) ? T[(int64_t)l * nslot + slot] : 0.0;
+  double Qn = act ? Qprev[f] : 0.0;
+  double Fsurf = act ? Fs[f] : 0.0;
+
+  // Time-invariant factors of rows nz..2 (zero for rows past nz, so those
+  // rows give z_j = d_j = 0 and T_j stays 0; the bottom row's alpha is 0, so
+  // the row below it does not reach it).
+  float Qc[NZ + 1], Rc[NZ + 1];
+  float Gb = 0.0f;  // GEO: the bottom row's geothermal term in z's scaling
+  {
+    float Mn = 0.0f;  // gamma_{j+1} P'_{j+1}
+#pragma unroll
+    for (int j = NZ; j >= 2; --j) {
+      if (j <= nz) {
+        const float4 c = cq[j - 1];
+        const float bj = __builtin_fmaf(s2s, c.z, 1.0f);
+        const float beta = __builtin_fmaf(-s2s * c.x, Mn, bj);
+        const float Pp = s2s * __builtin_amdgcn_rcpf(beta);
+        Qc[j] = c.x * Pp;
+        Rc[j] = c.y * Pp;
+        if (GEO && c.w != 0.0f) Gb = (geo_half / s2s) * Pp;
+        Mn = Rc[j];
+      } else {
+        Qc[j] = 0.0f;
+        Rc[j] = 0.0f;
+      }
+    }
+  }
+  const float M2 = Rc[2];  // gamma_2 P'_2: enters row 1's pivot every step
+
+  for (int step = 0; step < nsteps; ++step) {
+    const double Qnp1 = act ? Q[(int64_t)step * nf + f] : 0.0;
+    double ann, annp1, bn;
+    surface_bc(Tl[0], Qn, Qnp1, k1, emiss, &ann, &annp1, &bn);
+    float z[NZ + 1];
+    float znext = 0.0f, enext = 0.0f;  // z and T difference of row j+1
+#pragma unroll
+    for (int j = NZ; j >= 2; --j) {
+      const float ej = (float)(Tl[j] - Tl[j - 1]);
+      float zj = __builtin_fmaf(Qc[j], znext + enext, -Rc[j] * ej);
+      if (GEO && j == nz) zj += Gb;
+      z[j] = zj;
+      znext = zj;
+      enext = ej;
+      if ((j & 7) == 0) __builtin_amdgcn_sched_barrier(0);
+    }
+    {
+      const double b1 = ((1.0 + a1) + g1) - g1 * bn;
+      const double surf = (0.5 * g1) * ((ann + annp1) + (2.0 * (bn - 1.0)) * Tl[1]);
+      const float rho1 = __builtin_fmaf(a1s, enext, (float)surf);
+      z[1] = __builtin_fmaf(a1s, znext, rho1) *
+             __builtin_amdgcn_rcpf(__builtin_fmaf(-a1s, M2, (float)b1));
+    }
+    float d = z[1];
+    Tl[1] = __builtin_fma(2.0, (double)d, Tl[1]);
+#pragma unroll
+    for (int j = 2; j <= NZ; ++j) {
+      d = __builtin_fmaf(Rc[j], d, z[j]);
+      Tl[j] = __builtin_fma(2.0, (double)d, Tl[j]);
+      if ((j & 7) == 0) __builtin_amdgcn_sched_barrier(0);
+    }
+    const double T1 = Tl[1];
+    const double Tsurf = 0.5 * ((annp1 + bn * T1) + T1);
+    Fsurf = ((-kfirst) * (T1 - Tsurf)) / H.z1;
+    Tl[0] = Tsurf;
+    if (Tsurf_series && act) Tsurf_series[(int64_t)step * nf + f] = Tsurf;
+    Qn = Qnp1;
+  }
+  if (act) {
+#pragma unroll
+    for (int l = 0; l <= NZ; ++l)
+      if (l <= nz) T[(int64_t)l * nslot + slot] = Tl[l];
+    Fs[f] = Fsurf;
+    Qprev[f] = Qn;
+  }
+}
+
+// The same step with the per-lane pivots P'_j kept and the grid's alpha,
+// gamma reloaded every step by scalar loads (one register array fewer than
+// cn_mixed_kernel): z_j = (alpha_j P'_j) z_{j+1} + (alpha_j e_{j+1} -
+// gamma_j e_j) P'_j, d_j = z_j + gamma_j P'_j d_{j-1}.  Kept for comparison
+// (ILLUM_CN_MIXED_FORM=1).
+template <int NZ>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ), cn_waves(NZ)))) void cn_mixed_p_kernel(
     const int32_t *__restrict__ fidx, const int32_t *__restrict__ wgrid, int64_t block0,
     int64_t nf, int64_t nslot, const double *__restrict__ s2f, const double *__restrict__ hdr,
     const double *__restrict__ coef, const float4 *__restrict__ coefp,
@@ -720,6 +832,17 @@ static void ensure_tables(illum_thermal *th, double dt) {
   im->dt = dt;
 }
 
+// Form of the mixed register kernel: 0 = cn_mixed_kernel (Q, R per lane, no
+// coefficient loads in the step loop), 1 = cn_mixed_p_kernel (pivots per
+// lane, coefficients reloaded).  Form 0 is the faster in every bucket,
+// including 80 and 96 layers where its arrays spill a few registers (327,680
+// facets x 4,096 steps, 32-96 layers: 57 ms against 79 ms).
+// ILLUM_CN_MIXED_FORM=1 selects form 1, for comparison.
+static int cn_mixed_form(int) {
+  const char *e = getenv("ILLUM_CN_MIXED_FORM");
+  return e && atoi(e) == 1 ? 1 : 0;
+}
+
 template <bool MIXED>
 static void launch_all(illum_thermal *th, ThermalImpl *im, hipStream_t s0, double Fgeo,
                        int nsteps, const double *dQ, double *Tss, bool fork_ok = true) {
@@ -750,7 +873,9 @@ static void launch_all(illum_thermal *th, ThermalImpl *im, hipStream_t s0, doubl
 #define ILLUM_CN_REG(NZ, GMREG)                                                              \
   case NZ:                                                                                   \
     if (MIXED)                                                                               \
-      cn_mixed_kernel<NZ><<<nb, 64, 0, s>>>(                                                 \
+      (cn_mixed_form(NZ) == 1 ? cn_mixed_p_kernel<NZ>                                         \
+       : Fgeo != 0.0          ? cn_mixed_kernel<NZ, true>                                     \
+                              : cn_mixed_kernel<NZ, false>)<<<nb, 64, 0, s>>>(                \
           th->d_fidx, th->d_wgrid, L.block0, th->nf, th->nslot, th->d_s2, im->d_hdr,         \
           im->d_coef, (const float4 *)im->d_coefp, th->d_goff, im->d_goffp, th->emiss, Fgeo, \
           nsteps, dQ, th->d_Qprev, th->d_T, th->d_F, Tss);                                   \
