@@ -928,10 +928,47 @@ void thermal_run(illum_thermal *th, double dt, const double *Q, int nsteps, int 
     ILLUM_HIP(hipMemcpyAsync(th->d_Qdev, Q, 8 * need, hipMemcpyHostToDevice, s));
     dQ = th->d_Qdev;
   }
+  const bool mixed = th->precision == ILLUM_THERM_MIXED;
+  auto launch = [&](bool fork) {
+    if (mixed) launch_all<true>(th, im, s, Fgeo, nsteps, dQ, Tsurf_series, fork);
+    else launch_all<false>(th, im, s, Fgeo, nsteps, dQ, Tsurf_series, fork);
+  };
+  // With several depth buckets, whether they run concurrently on forked
+  // streams or one after another does not change the results (they touch
+  // disjoint slots); the autotuner times both once per model shape, on a
+  // snapshot of the state restored before each trial and after tuning.
+  bool fork = true;
+  if (th->launches.size() > 1) {
+    std::string key = std::string("cn_streams[") + (mixed ? "mixed" : "fp64") +
+                      ",nslot=" + std::to_string(th->nslot) + ",buckets=" +
+                      std::to_string(th->launches.size()) + ",steps=" + std::to_string(nsteps) + "]";
+    const bool cached = ctx->tune_cache.count(key) != 0;
+    const char *off = getenv("ILLUM_NO_AUTOTUNE");
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    ILLUM_HIP(hipStreamIsCapturing(s, &cs));
+    if (cached || (off && off[0] == '1') || cs != hipStreamCaptureStatusNone) {
+      fork = autotune(ctx, key, {1, 0}, 1, [&](int) {}, false) == 1;
+    } else {
+      const size_t tb = 8 * (size_t)(th->nzmax + 1) * th->nslot;
+      DBuf<double> snap(tb / 8 + 2 * th->nf);
+      auto copy = [&](bool save) {
+        double *dT = snap.p, *dq = snap.p + tb / 8, *dF = dq + th->nf;
+        auto cp = [&](void *a, void *b, size_t n) {
+          ILLUM_HIP(hipMemcpyAsync(save ? a : b, save ? b : a, n, hipMemcpyDeviceToDevice, s));
+        };
+        cp(dT, th->d_T, tb);
+        cp(dq, th->d_Qprev, 8 * th->nf);
+        cp(dF, th->d_F, 8 * th->nf);
+      };
+      copy(true);
+      fork = autotune(ctx, key, {1, 0}, 1, [&](int c) { copy(false); launch(c == 1); }) == 1;
+      copy(false);
+      sync(ctx);
+    }
+  }
   {
-    StageTimer st(ctx, th->precision == ILLUM_THERM_MIXED ? "cn_mixed" : "cn_fp64");
-    if (th->precision == ILLUM_THERM_MIXED) launch_all<true>(th, im, s, Fgeo, nsteps, dQ, Tsurf_series);
-    else launch_all<false>(th, im, s, Fgeo, nsteps, dQ, Tsurf_series);
+    StageTimer st(ctx, mixed ? "cn_mixed" : "cn_fp64");
+    launch(fork);
   }
   // A zero pivot is sticky in d_status; an asynchronous step leaves its check
   // to the next synchronous step or thermal_get.

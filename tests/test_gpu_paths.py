@@ -189,3 +189,30 @@ def test_autotuned_launch_shapes_do_not_change_results(api, monkeypatch):
     assert any(k.startswith("direct_lanes[") for k in keys)
     assert any(k.startswith("spmm_per[fp64") for k in keys)
     assert any(k.startswith("spmm_per[fp16") for k in keys)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_autotuned_conduction_stream_layout_does_not_change_results(api, monkeypatch, precision):
+    # Several depth buckets: the tuner times forked against sequential bucket
+    # launches on a restored snapshot; the state after the run must equal a
+    # run without tuning, and later runs reuse the choice.
+    from illum_amd import configs
+    rng = np.random.default_rng(8)
+    nf = 2000
+    grids = [configs.layer_grid(n) for n in (30, 45, 60, 75, 90)]
+    gid = rng.integers(0, len(grids), nf).astype(np.int32)
+    tis = np.exp(rng.normal(0, 0.5, nf))
+    Q = rng.uniform(0, 600, (24, nf))
+    out = []
+    for tune in ("0", "1"):
+        monkeypatch.setenv("ILLUM_NO_AUTOTUNE", "1" if tune == "0" else "0")
+        th = api.ThermalModel(nf, grids, 200.0, 1.19177e6, 230.0, emissivity=0.95,
+                              precision=precision, grid_of_facet=gid, ti_scale=tis)
+        th.run(60.0, Q[:12])
+        th.run(60.0, Q[12:])
+        out.append((th.T, th.surface()))
+        th.close()
+    (Ta, (sa, fa)), (Tb, (sb, fb)) = out
+    assert np.array_equal(Ta, Tb, equal_nan=True)
+    assert np.array_equal(sa, sb) and np.array_equal(fa, fb)
+    assert any(k.startswith("cn_streams[" + precision) for k, *_ in api.default_context().tuning())
