@@ -97,6 +97,8 @@ SIGNATURES = {
     "illum_timeloop_stats": [_P, _P, _P, _P, _P, _P],
     "illum_timeloop_get": [_P, _P, _P, _I],
     "illum_timeloop_destroy": [_P],
+    "illum_knn": [_P, _P, _I64, _P, _I64, _I, _P, _P, _I],
+    "illum_mesh_transfer": [_P, _P, _I64, _P, _I64, _P, _I, _I, _I, _P, _P, _P],
     "illum_conductionQ": [_I, _P, _D, _D, _D, _P, _P, _P, _D, _P, _D, _P],
 }
 _RET = {"illum_last_error": C.c_char_p}

@@ -351,6 +351,24 @@ int illum_timeloop_stats(illum_timeloop *tl, int64_t *steps, int *products, int 
 int illum_timeloop_get(illum_timeloop *tl, double *E, double *B, int flags);
 int illum_timeloop_destroy(illum_timeloop *tl);
 
+/* ---- mesh-field transfer (src/py/interpolate_mesh_field.py,
+ * src/py/build_mesh_transfer_tables.py) ----------------------------------
+ * illum_knn: for each of nt target points the k (<= 32) nearest of ns source
+ * points (both n x 3 row-major FP64), nearest first: idx nt x k, dist
+ * (nullable) nt x k Euclidean distances; the reference queries a scipy
+ * cKDTree.  Exact brute force in FP64, ties to the smaller source index.
+ * Host or device arrays per flags. */
+int illum_knn(illum_ctx *ctx, const double *src, int64_t ns, const double *tgt, int64_t nt,
+              int k, int32_t *idx, double *dist, int flags);
+/* The interpolation of interpolate_mesh_field.py on host arrays: Xsrc is
+ * nlayers x ns row-major, Xtgt nlayers x nt.  mode 0 nearest, 1 weighted
+ * (the reference's weights D / sum(D) over the k nearest), 2 area_weighted
+ * (the same weights on X times the source areas, divided by the target's
+ * area; area_src and area_tgt required). */
+int illum_mesh_transfer(illum_ctx *ctx, const double *src, int64_t ns, const double *tgt,
+                        int64_t nt, const double *Xsrc, int nlayers, int mode, int k,
+                        const double *area_src, const double *area_tgt, double *Xtgt);
+
 /* Single-facet drop-in for conductionQ (conduction.hpp:16-28), run on the
  * GPU.  T (nz) and Tsurf, Fsurf are updated in place. */
 int illum_conductionQ(int nz, const double *z, double dt, double Qn, double Qnp1,

@@ -279,3 +279,40 @@ def ref_conductionQ(nz, z, dt, Qn, Qnp1, T, ti, rhoc, emiss, Tsurf, Fgeotherm):
                       _p(ti), _p(rhoc), C.byref(a[8]), C.byref(a[9]), C.byref(a[10]),
                       C.byref(a[11]))
     return T, a[9].value, a[11].value
+
+
+# --- mesh-field transfer (src/py/interpolate_mesh_field.py,
+# build_mesh_transfer_tables.py), with the scipy cKDTree the scripts query ---
+def transfer_field(P_src, X_src, P_tgt, mode="nearest", neighbors=5, A_src=None, A_tgt=None):
+    """X_tgt (nlayers x nt) by the loops of interpolate_mesh_field.py:62-103."""
+    import scipy.spatial
+    X_src = np.atleast_2d(X_src)
+    tree = scipy.spatial.cKDTree(P_src)
+    X_tgt = np.empty((X_src.shape[0], P_tgt.shape[0]))
+    for i, p in enumerate(P_tgt):
+        if mode == "nearest":
+            _, j = tree.query(p)
+            X_tgt[:, i] = X_src[:, j]
+            continue
+        D, I = tree.query(p, neighbors)
+        lam = D / np.sum(D)
+        if mode == "weighted":
+            X_tgt[:, i] = X_src[:, I] @ lam
+        else:
+            for layer in range(X_src.shape[0]):
+                X_tgt[layer, i] = lam @ (X_src[layer, I] * A_src[I]) / A_tgt[i]
+    return X_tgt
+
+
+def transfer_tables(P_list, num_neighbors=5):
+    """(downsample, neighbour) tables of build_mesh_transfer_tables.py:30-48."""
+    import scipy.spatial
+    down, neigh = {}, {}
+    kd_src = scipy.spatial.cKDTree(P_list[0])
+    for i in range(1, len(P_list)):
+        kd_tgt = scipy.spatial.cKDTree(P_list[i])
+        down[i] = np.array([kd_src.query(p)[1] for p in P_list[i]], dtype=np.uint32)
+        neigh[i] = np.array([kd_tgt.query(p, num_neighbors)[1] for p in P_list[i]],
+                            dtype=np.uint32)
+        kd_src = kd_tgt
+    return down, neigh

@@ -193,3 +193,19 @@ def test_binding_shape_checks_raise_before_any_gpu_call():
     ic.num_faces, ic.horizons = 5, np.zeros((9, 4))
     with pytest.raises(_lib.IllumError):
         ic.get_direct_radiosity(np.array([1e8, 0, 0]), O.fib_spiral(100))
+
+
+def test_transfer_files_use_the_scripts_c_order_convention(tmp_path):
+    # src/py/arma.py writes the header, "n" for a vector or "rows cols", then
+    # the values in C order; the transfer tools read and write the same bytes
+    from illum_amd import transfer
+    X = np.arange(6, dtype=np.float64).reshape(2, 3)
+    transfer.arma_tofile(X, tmp_path / "x.bin")
+    raw = open(tmp_path / "x.bin", "rb").read()
+    assert raw.startswith(b"ARMA_MAT_BIN_FN008\n2 3\n")
+    assert np.array_equal(np.frombuffer(raw[23:], np.float64), X.ravel())
+    assert np.array_equal(transfer.arma_fromfile(tmp_path / "x.bin"), X)
+    u = np.array([3, 1, 2], np.uint32)
+    transfer.arma_tofile(u, tmp_path / "u.bin")
+    assert open(tmp_path / "u.bin", "rb").read().startswith(b"ARMA_MAT_BIN_IU004\n3\n")
+    assert np.array_equal(transfer.arma_fromfile(tmp_path / "u.bin", np.uint32), u)
