@@ -69,6 +69,7 @@ SIGNATURES = {
     "illum_form_factors": [_P, _D, _P, _D, _PP],
     "illum_form_factors_ex": [_P, _D, _P, _D, _I, _PP],
     "illum_form_factors_work": [_P, _D, _P],
+    "illum_form_factors_work_ex": [_P, _D, _P, _I],
     "illum_sparse_create_csr": [_P, _I64, _I64, _P, _P, _P, _PP],
     "illum_sparse_destroy": [_P],
     "illum_sparse_info": [_P, _P, _P],

@@ -453,10 +453,14 @@ class IllumContext:
 
     def form_factor_work(self, offset=1e-5):
         """Work of one visibility pass (separate instrumented launch): rays, pair
-        (facing) tests after the cluster pass, box tests, triangle tests."""
-        out = np.zeros(4, np.int64)
-        call("illum_form_factors_work", self.bvh.h, offset, _ptr(out))
-        return dict(zip(("rays", "pair_tests", "box_tests", "tri_tests"), map(int, out)))
+        (facing) tests after the cluster pass, box and triangle tests per active
+        lane, and for the wavefront walks their walks, node visits and triangle
+        tests per wavefront and the walks with a fixed axis permutation."""
+        out = np.zeros(8, np.int64)
+        call("illum_form_factors_work_ex", self.bvh.h, offset, _ptr(out), 8)
+        return dict(zip(("rays", "pair_tests", "box_tests", "tri_tests", "wave_walks",
+                         "wave_node_visits", "wave_tri_tests", "wave_walks_fixed_perm"),
+                        map(int, out)))
 
     def make_horizons(self, nphi=361, theta_eps=1e-3, offset=1e-5, out=None):
         occ = self._horizon_ctx[1].h if self._horizon_ctx else None

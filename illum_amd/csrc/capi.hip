@@ -17,7 +17,7 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
 void render(illum_bvh *bvh, int mode, int width, int height, double az, double el,
             const double *data, double lo, double hi, int32_t *facet, double *value,
             int flags);
-void form_factor_work(illum_bvh *bvh, double offset, int64_t *out4);
+void form_factor_work(illum_bvh *bvh, double offset, int64_t *out, int n);
 void bvh_closest_hit(illum_bvh *bvh, const float *orig, const float *dir, int64_t nray,
                      int32_t *hit, float *t);
 void horizons(illum_bvh *bvh, illum_bvh *occl, int nphi, double theta_eps, double offset,
@@ -695,11 +695,15 @@ int illum_render(illum_bvh *bvh, int mode, int width, int height, double az_deg,
 }
 
 int illum_form_factors_work(illum_bvh *bvh, double offset, int64_t *out4) {
+  return illum_form_factors_work_ex(bvh, offset, out4, 4);
+}
+
+int illum_form_factors_work_ex(illum_bvh *bvh, double offset, int64_t *out, int n) {
   return guarded([&] {
-    CHECK_ARG(bvh && out4, "null argument");
+    CHECK_ARG(bvh && out && n >= 1 && n <= 8, "bad argument");
     CHECK_ARG(std::isfinite(offset), "offset must be finite");
     set_device(bvh->mesh->ctx);
-    form_factor_work(bvh, offset, out4);
+    form_factor_work(bvh, offset, out, n);
   });
 }
 

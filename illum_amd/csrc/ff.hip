@@ -33,8 +33,15 @@
 namespace illum {
 
 constexpr int FF_BLOCK = 256;
-constexpr int FF_PER = 4;                          // candidates per thread per chunk
-constexpr int FF_QCAP = FF_BLOCK * (FF_PER + 1);   // queue capacity (ints)
+constexpr int FF_MAXPER = 4;      // clusters per wavefront per fill round, at most
+constexpr int FF_MAXCHUNK = 1024;  // rays per walk round, at most
+// Ray queue entry: direction key << FF_IDX_BITS | (receiver - body start).
+constexpr int FF_IDX_BITS = 21;
+constexpr int FF_IDX_MASK = (1 << FF_IDX_BITS) - 1;
+constexpr int FF_NKEY = 96;  // 6 permutation classes x 4 x 4 direction cells
+// Workgroups per CU the kernel is compiled for (8 wavefronts per SIMD: at
+// most 64 VGPRs); the LDS per workgroup is sized for it where the body allows.
+constexpr int FF_WAVES_PER_SIMD = 8;
 constexpr int FF_STACK = 8;  // LDS traversal stack entries per lane (deeper: threaded walk)
 // First pass of each visibility ray prunes at this distance: the ray aims at a
 // centroid lying on its target triangle, so the closest hit is almost always
@@ -65,6 +72,10 @@ struct FFArgs {
   int nrows;    // sources of the launch: j = row * jstride
   int jstride;
   int scap;     // LDS traversal stack entries per lane
+  int qchunk;   // rays per walk round (a multiple of FF_BLOCK, <= FF_MAXCHUNK)
+  int fper;     // clusters per wavefront per fill round (<= FF_MAXPER)
+  int qcap;     // ray queue entries: qchunk + one fill round (fper * 256)
+  int qsort;    // 1: each round's rays are ordered by direction key first
   float offset;
   double albedo;
   const float4 *pr4, *n4, *nodes, *wide, *wide4, *tris, *clus;
@@ -138,18 +149,146 @@ __device__ void append_bits_sorted(const uint32_t *bits, int nw, int lo, int32_t
   __syncthreads();  // s_scan and s_base are reused by the next call
 }
 
+// Direction key of a ray for grouping rays into wavefronts: its permutation
+// class (ray_perm: the dominant axis and its sign) and a 4 x 4 cell of the
+// two other components over the dominant one, Morton-ordered.  Only the
+// grouping depends on it; each ray's result does not.
+__device__ __forceinline__ int ray_dir_key(float dx, float dy, float dz) {
+  const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+  int kz = 0;
+  float m = ax;
+  if (ay > m) { kz = 1; m = ay; }
+  if (az > m) kz = 2;
+  const float dk = kz == 0 ? dx : (kz == 1 ? dy : dz);
+  const float du = kz == 0 ? dy : (kz == 1 ? dz : dx);
+  const float dv = kz == 0 ? dz : (kz == 1 ? dx : dy);
+  const float im = __builtin_amdgcn_rcpf(fabsf(dk));
+  // cells of u, v in [-1, 1]; NaN (a zero direction) lands in cell 0
+  const int iu = (int)fminf(fmaxf((du * im + 1.0f) * 2.0f, 0.0f), 3.0f);
+  const int iv = (int)fminf(fmaxf((dv * im + 1.0f) * 2.0f, 0.0f), 3.0f);
+  const int cell = (iu & 1) | ((iv & 1) << 1) | ((iu & 2) << 1) | ((iv & 2) << 2);
+  return (2 * kz + (dk < 0.0f ? 1 : 0)) * 16 + cell;
+}
+
+// Block-wide exclusive scan of one int per thread (blockDim == 256).
+__device__ __forceinline__ int block_exscan256(int v, int *s_wave, int *total) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  int x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_wave[wid] = x;
+  __syncthreads();
+  int off = 0, tot = 0;
+  for (int w = 0; w < 4; ++w) {
+    const int sw = s_wave[w];
+    if (w < wid) off += sw;
+    tot += sw;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + x - v;
+}
+
+// Block-wide counting sort of n <= FF_MAXCHUNK queue entries by their
+// direction key (bits FF_IDX_BITS and up) from `in` to `out`; the order
+// within a key is arbitrary (LDS atomics).  `hist` holds FF_NKEY ints.
+// Called by the whole block; ends with a barrier.
+__device__ void sort_rays_by_key(const int32_t *in, int n, int32_t *out, int *hist,
+                                 int *s_wave) {
+  constexpr int PER = FF_MAXCHUNK / FF_BLOCK;
+  static_assert(FF_NKEY <= 2 * FF_BLOCK, "two histogram buckets per thread");
+  const int tid = threadIdx.x;
+  for (int k = tid; k < FF_NKEY; k += FF_BLOCK) hist[k] = 0;
+  __syncthreads();
+  int val[PER], key[PER], rk[PER];
+#pragma unroll
+  for (int m = 0; m < PER; ++m) {
+    const int e = tid + m * FF_BLOCK;
+    key[m] = -1;
+    val[m] = 0;
+    rk[m] = 0;
+    if (e < n) {
+      val[m] = in[e];
+      key[m] = (int)((uint32_t)val[m] >> FF_IDX_BITS);
+      rk[m] = atomicAdd(&hist[key[m]], 1);
+    }
+  }
+  __syncthreads();
+  const int b0 = 2 * tid, b1 = 2 * tid + 1;
+  const int h0 = b0 < FF_NKEY ? hist[b0] : 0, h1 = b1 < FF_NKEY ? hist[b1] : 0;
+  int total;
+  const int off = block_exscan256(h0 + h1, s_wave, &total);
+  if (b0 < FF_NKEY) hist[b0] = off;
+  if (b1 < FF_NKEY) hist[b1] = off + h0;
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < PER; ++m)
+    if (key[m] >= 0) out[hist[key[m]] + rk[m]] = val[m];
+  __syncthreads();
+}
+
+// Work counters of one thread (compiled away without COUNT): lane-level box
+// and triangle tests, and wavefront-level walks, node visits, triangle tests
+// and walks run with a fixed permutation class (equal on every lane).
+struct FFWork {
+  unsigned npair = 0, nbox = 0, ntri = 0, wwalk = 0, wnode = 0, wtri = 0, wspec = 0;
+};
+
+template <int LAYOUT, int PERM>
+__device__ __forceinline__ int packet_walk_p(const FFArgs &a, const DRay &r, bool act, int root2,
+                                             int root4, float tinit, float *t, FFWork &w) {
+  if (LAYOUT == 3)
+    return trace_closest_packet<PERM>(a.wide, a.tris, r, act, root2, tinit, t, w.nbox, w.ntri,
+                                      w.wnode, w.wtri);
+  return trace_closest_packet4<PERM>(a.wide4, a.tris, r, act, root4, tinit, t, w.nbox, w.ntri,
+                                     w.wnode, w.wtri);
+}
+
+// Wavefront walk (LAYOUT 3: two-wide tree, 5: four-wide).  With SPEC, a
+// wavefront whose active rays all share one permutation class (ray_perm)
+// runs the walk compiled for that class, whose triangle test selects its
+// axes at compile time; results are those of the general walk.  Called by
+// the whole wavefront with at least one active lane.
+template <int LAYOUT, bool SPEC>
+__device__ __forceinline__ int packet_walk(const FFArgs &a, const DRay &r, bool act, int root2,
+                                           int root4, float tinit, float *t, FFWork &w) {
+  ++w.wwalk;
+  if (SPEC) {
+    const uint64_t am = wave_ballot(act);
+    const int perm = ray_perm(r);
+    const int p0 = __builtin_amdgcn_readlane(perm, (int)__builtin_ctzll(am));
+    if (wave_ballot(act && perm != p0) == 0) {
+      ++w.wspec;
+      switch (p0) {
+        case 0: return packet_walk_p<LAYOUT, 0>(a, r, act, root2, root4, tinit, t, w);
+        case 1: return packet_walk_p<LAYOUT, 1>(a, r, act, root2, root4, tinit, t, w);
+        case 2: return packet_walk_p<LAYOUT, 2>(a, r, act, root2, root4, tinit, t, w);
+        case 3: return packet_walk_p<LAYOUT, 3>(a, r, act, root2, root4, tinit, t, w);
+        case 4: return packet_walk_p<LAYOUT, 4>(a, r, act, root2, root4, tinit, t, w);
+        default: return packet_walk_p<LAYOUT, 5>(a, r, act, root2, root4, tinit, t, w);
+      }
+    }
+  }
+  return packet_walk_p<LAYOUT, -1>(a, r, act, root2, root4, tinit, t, w);
+}
+
 // LAYOUT: 4 = near-first walk of the four-wide tree, 2 = of the two-wide
 // tree, 3 / 5 = wavefront packet walks over the two- / four-wide tree, 0 =
 // stackless threaded walk.  COUNT adds the work counters (pair, box and
 // triangle tests) to stats[1..3]; stats[0] always receives the rays.
 template <int LAYOUT, bool COUNT>
-__global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(FFArgs a) {
+__global__ __launch_bounds__(FF_BLOCK, FF_WAVES_PER_SIMD) void ff_visibility_kernel(FFArgs a) {
   extern __shared__ uint32_t smem[];
   uint32_t *bits = smem;
   int *queue = (int *)(smem + a.nwmax);
+  // with qsort: the sorted walk round and the key histogram
+  int *sorted = queue + a.qcap;
+  int *hist = sorted + (a.qsort ? a.qchunk : 0);
   // surviving receiver clusters as 16-bit offsets from the body's first
   // cluster, then scap x FF_BLOCK stack slots (LAYOUT 2 or 4)
-  uint16_t *clist = (uint16_t *)(queue + FF_QCAP);
+  uint16_t *clist = (uint16_t *)(hist + (a.qsort ? FF_NKEY : 0));
   int *stack = (int *)(clist + ((a.nclmax + 1) & ~1));
   __shared__ int s_qn, s_rays, s_nc;
   __shared__ int s_scan[FF_BLOCK / 64];
@@ -165,7 +304,7 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(FFArgs a) {
   const int nw = (hi - lo + 31) >> 5;
   for (int w = tid; w < nw; w += FF_BLOCK) bits[w] = 0u;
   if (tid == 0) { s_qn = 0; s_rays = 0; s_nc = 0; }
-  unsigned npair = 0, nbox = 0, ntri = 0;  // work counters of this thread
+  FFWork w;  // work counters of this thread
   const int root_thr = a.roots[4 * body], root_wide = a.roots[4 * body + 1];
   const int root_wide4 = a.roots[4 * body + 2];
   const int clo = lo >> 6, chi = (hi + 63) >> 6;  // clusters overlapping [lo, hi)
@@ -189,7 +328,7 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(FFArgs a) {
         keep = !(t <= -cc.w);
       }
     }
-    const uint64_t m = __ballot(keep);
+    const uint64_t m = wave_ballot(keep);
     if (m) {
       int slot = 0;
       if (lane == 0) slot = atomicAdd(&s_nc, __popcll(m));
@@ -200,23 +339,27 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(FFArgs a) {
   __syncthreads();
   const int ncl = s_nc;
 
-  // Per-pair facing test over the listed clusters, FF_PER * 4 clusters (one
-  // per wave and k) per round; survivors go to the LDS ray queue.
-  constexpr int CPR = FF_PER * (FF_BLOCK / 64);
-  for (int g0 = 0; g0 < ncl; g0 += CPR) {
+  // Per-pair facing test over the listed clusters, 4 fper clusters (one per
+  // wave and k) per round; survivors go to the LDS ray queue with their
+  // direction keys.
+  const int cpr = a.fper * (FF_BLOCK / 64);
+  for (int g0 = 0; g0 < ncl; g0 += cpr) {
 #pragma unroll
-    for (int k = 0; k < FF_PER; ++k) {
+    for (int k = 0; k < FF_MAXPER; ++k) {
+      if (k >= a.fper) break;
       const int ci = g0 + k * (FF_BLOCK / 64) + wid;
       const int i = ci < ncl ? (clo + (int)clist[ci]) * 64 + lane : hi;
       bool s = false;
+      int key = 0;
       if (i >= lo && i < hi && i != j) {
-        ++npair;
+        ++w.npair;
         const float4 pi = a.pr4[i];
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const float dot = (dx * nj.x + dy * nj.y) + dz * nj.z;
         s = !(dot <= -pi.w);
+        if (a.qsort) key = ray_dir_key(pi.x - qx, pi.y - qy, pi.z - qz);
       }
-      const uint64_t m = __ballot(s);
+      const uint64_t m = wave_ballot(s);
       if (m) {
         int slot = 0;
         if (lane == 0) {
@@ -224,71 +367,65 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(FFArgs a) {
           atomicAdd(&s_rays, __popcll(m));
         }
         slot = __shfl(slot, 0);
-        if (s) queue[slot + __popcll(m & lanemask_lt())] = i;
+        if (s) queue[slot + __popcll(m & lanemask_lt())] = (key << FF_IDX_BITS) | (i - lo);
       }
     }
     __syncthreads();
-    const bool last = g0 + CPR >= ncl;
+    const bool last = g0 + cpr >= ncl;
     int qrem = s_qn;
-    while (qrem >= FF_BLOCK || (last && qrem > 0)) {
-      const int take = qrem < FF_BLOCK ? qrem : FF_BLOCK;
-      if (LAYOUT == 3 || LAYOUT == 5) {
-        // wavefront walk: the whole wavefront takes part, idle lanes masked;
-        // a wavefront with no work left skips it
-        const bool act = tid < take;
-        if (__ballot(act)) {
-          const int i = act ? queue[qrem - take + tid] : j;
-          const float4 pi = a.pr4[i];
-          const DRay r = make_ray(qx, qy, qz, pi.x - qx, pi.y - qy, pi.z - qz);
-          float t;
-          int h = LAYOUT == 3
-                      ? trace_closest_packet(a.wide, a.tris, r, act, root_wide, FF_TINIT, &t, nbox,
-                                             ntri)
-                      : trace_closest_packet4(a.wide4, a.tris, r, act, root_wide4, FF_TINIT, &t,
-                                              nbox, ntri);
-          const bool again = act && h == -1;
-          if (__ballot(again)) {
-            const float inf = __builtin_huge_valf();
-            const int h2 = LAYOUT == 3
-                               ? trace_closest_packet(a.wide, a.tris, r, again, root_wide, inf, &t,
-                                                      nbox, ntri)
-                               : trace_closest_packet4(a.wide4, a.tris, r, again, root_wide4, inf,
-                                                       &t, nbox, ntri);
-            if (again) h = h2;
-          }
-          // h lies in [lo, hi): the walk covers only the body's tree
-          if (act && h >= 0 && h != j) atomicOr(&bits[(h - lo) >> 5], 1u << ((h - lo) & 31));
-        }
-      } else if (tid < take) {
-        const int i = queue[qrem - take + tid];
+    // Walk rounds of up to qchunk rays from the queue's end; each wavefront
+    // takes 64 consecutive entries at a time.
+    while (qrem >= a.qchunk || (last && qrem > 0)) {
+      const int n = qrem < a.qchunk ? qrem : a.qchunk;
+      const int *src = queue + (qrem - n);
+      if (a.qsort) {
+        sort_rays_by_key(src, n, sorted, hist, s_scan);
+        src = sorted;
+      }
+      for (int w0 = wid * 64; w0 < n; w0 += FF_BLOCK) {
+        const int e = w0 + lane;
+        const bool act = e < n;
+        const int i = act ? lo + (src[e] & FF_IDX_MASK) : j;
         const float4 pi = a.pr4[i];
         const DRay r = make_ray(qx, qy, qz, pi.x - qx, pi.y - qy, pi.z - qz);
         float t;
-        int h;
-        if (LAYOUT == 4) {
-          h = trace_closest_wide4(a.wide4, a.tris, r, root_wide4, FF_TINIT, stack + tid, FF_BLOCK,
-                                  a.scap, &t, nbox, ntri);
-          if (h == -1)
-            h = trace_closest_wide4(a.wide4, a.tris, r, root_wide4, __builtin_huge_valf(),
-                                    stack + tid, FF_BLOCK, a.scap, &t, nbox, ntri);
-          // a walk deeper than the LDS stack: the stackless walk answers the
-          // same unbounded query (identical result)
-          if (h == -2) h = trace_closest(a.nodes, a.tris, r, root_thr, &t, nbox, ntri);
-        } else if (LAYOUT == 2) {
-          h = trace_closest_wide(a.wide, a.tris, r, root_wide, FF_TINIT, stack + tid, FF_BLOCK,
-                                 a.scap, &t, nbox, ntri);
-          if (h == -1)
-            h = trace_closest_wide(a.wide, a.tris, r, root_wide, __builtin_huge_valf(),
-                                   stack + tid, FF_BLOCK, a.scap, &t, nbox, ntri);
-          if (h == -2) h = trace_closest(a.nodes, a.tris, r, root_thr, &t, nbox, ntri);
-        } else {
-          h = trace_closest(a.nodes, a.tris, r, root_thr, &t, nbox, ntri);
+        int h = -1;
+        if (LAYOUT == 3 || LAYOUT == 5) {
+          // wavefront walk: the whole wavefront takes part, idle lanes masked
+          h = packet_walk<LAYOUT, true>(a, r, act, root_wide, root_wide4, FF_TINIT, &t, w);
+          const bool again = act && h == -1;
+          if (wave_ballot(again)) {
+            const int h2 = packet_walk<LAYOUT, false>(a, r, again, root_wide, root_wide4,
+                                                      __builtin_huge_valf(), &t, w);
+            if (again) h = h2;
+          }
+        } else if (act) {
+          if (LAYOUT == 4) {
+            h = trace_closest_wide4(a.wide4, a.tris, r, root_wide4, FF_TINIT, stack + tid,
+                                    FF_BLOCK, a.scap, &t, w.nbox, w.ntri);
+            if (h == -1)
+              h = trace_closest_wide4(a.wide4, a.tris, r, root_wide4, __builtin_huge_valf(),
+                                      stack + tid, FF_BLOCK, a.scap, &t, w.nbox, w.ntri);
+            // a walk deeper than the LDS stack: the stackless walk answers
+            // the same unbounded query (identical result)
+            if (h == -2) h = trace_closest(a.nodes, a.tris, r, root_thr, &t, w.nbox, w.ntri);
+          } else if (LAYOUT == 2) {
+            h = trace_closest_wide(a.wide, a.tris, r, root_wide, FF_TINIT, stack + tid, FF_BLOCK,
+                                   a.scap, &t, w.nbox, w.ntri);
+            if (h == -1)
+              h = trace_closest_wide(a.wide, a.tris, r, root_wide, __builtin_huge_valf(),
+                                     stack + tid, FF_BLOCK, a.scap, &t, w.nbox, w.ntri);
+            if (h == -2) h = trace_closest(a.nodes, a.tris, r, root_thr, &t, w.nbox, w.ntri);
+          } else {
+            h = trace_closest(a.nodes, a.tris, r, root_thr, &t, w.nbox, w.ntri);
+          }
         }
-        if (h >= 0 && h != j) atomicOr(&bits[(h - lo) >> 5], 1u << ((h - lo) & 31));
+        // h lies in [lo, hi): the walk covers only the body's tree
+        if (act && h >= 0 && h != j) atomicOr(&bits[(h - lo) >> 5], 1u << ((h - lo) & 31));
       }
-      qrem -= take;
+      qrem -= n;
+      __syncthreads();  // the round's entries are read before the next round or fill
     }
-    __syncthreads();
     if (tid == 0) s_qn = qrem;
     __syncthreads();
   }
@@ -331,7 +468,7 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(FFArgs a) {
         const int b = __ffs(rest) - 1;
         rest &= rest - 1u;
         const int i = lo + 32 * w + b;
-        if (slot < FF_QCAP) queue[slot] = i;
+        if (slot < a.qcap) queue[slot] = i;
         else if (filter(i)) keep |= 1u << b;
         ++slot;
       }
@@ -339,7 +476,7 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(FFArgs a) {
     bits[w] = keep;
   }
   __syncthreads();
-  const int nlist = min(s_qn, FF_QCAP);
+  const int nlist = min(s_qn, a.qcap);
   for (int e = tid; e < nlist; e += FF_BLOCK) {
     const int i = queue[e];
     if (filter(i)) atomicOr(&bits[(i - lo) >> 5], 1u << ((i - lo) & 31));
@@ -352,36 +489,19 @@ __global__ __launch_bounds__(FF_BLOCK) void ff_visibility_kernel(FFArgs a) {
                        s_scan, &s_base);
   if (a.stats && tid == 0) atomicAdd(a.stats, (unsigned long long)s_rays);
   if (COUNT && a.stats) {
-    // pair tests, box tests, triangle tests: wave sums, one atomic each.
+    // pair tests, box tests, triangle tests: wave sums, one atomic each;
+    // the wavefront-level counters are equal on every lane (lane 0's).
     // Without COUNT the counters are dead and compile away.
-    unsigned long long w3[3] = {npair, nbox, ntri};
+    unsigned long long w3[3] = {w.npair, w.nbox, w.ntri};
     for (int q = 0; q < 3; ++q) {
       unsigned long long v = w3[q];
       for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
       if (lane == 0 && v) atomicAdd(a.stats + 1 + q, v);
     }
+    const unsigned long long w4[4] = {w.wwalk, w.wnode, w.wtri, w.wspec};
+    for (int q = 0; q < 4; ++q)
+      if (lane == 0 && w4[q]) atomicAdd(a.stats + 4 + q, w4[q]);
   }
-}
-
-// Block-wide exclusive scan of one int per thread (blockDim == 256).
-__device__ __forceinline__ int block_exscan256(int v, int *s_wave, int *total) {
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  int x = v;
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(x, o);
-    if (lane >= o) x += y;
-  }
-  if (lane == 63) s_wave[wid] = x;
-  __syncthreads();
-  int off = 0, tot = 0;
-  for (int w = 0; w < 4; ++w) {
-    const int sw = s_wave[w];
-    if (w < wid) off += sw;
-    tot += sw;
-  }
-  __syncthreads();
-  *total = tot;
-  return off + x - v;
 }
 
 // CSC(F) from the kept-receiver pool: one wave per column (= source j)
@@ -647,18 +767,67 @@ static BodyDims body_dims(const illum_mesh *mesh) {
   return d;
 }
 
-// Dynamic LDS of ff_visibility_kernel: bitset, ray queue, 16-bit cluster list
-// (rounded to whole words), stack.
-static size_t ff_lds_bytes(const BodyDims &d, int stack_slots) {
-  return (size_t)(d.nwmax + FF_QCAP) * 4 + (size_t)((d.nclmax + 1) & ~1) * 2 +
-         (size_t)stack_slots * FF_BLOCK * 4;
+// Ray queue of a launch: rays per walk round, clusters per wavefront per fill
+// round, capacity (a walk round plus a fill round), and whether each round is
+// sorted by direction.
+struct QShape {
+  int chunk = FF_BLOCK, fper = 1, cap = 2 * FF_BLOCK, sort = 0;
+};
+
+// Dynamic LDS of ff_visibility_kernel: bitset, ray queue, sorted round and
+// key histogram (when sorting), 16-bit cluster list (rounded to whole words),
+// stack.
+static size_t ff_lds_bytes(const BodyDims &d, int stack_slots, const QShape &q) {
+  return (size_t)(d.nwmax + q.cap) * 4 + (q.sort ? (size_t)(q.chunk + FF_NKEY) * 4 : 0) +
+         (size_t)((d.nclmax + 1) & ~1) * 2 + (size_t)stack_slots * FF_BLOCK * 4;
+}
+
+// Workgroups of one CU that a launch's LDS allows (160 KB per CU, in 512-byte
+// granules, with the kernel's static LDS).
+static int ff_groups_per_cu(size_t dyn) {
+  const size_t per = (dyn + 256 + 511) & ~(size_t)511;
+  return (int)std::min<size_t>(FF_WAVES_PER_SIMD, 160 * 1024 / per);
+}
+
+// The queue shape: among walk rounds of 1024 / 512 / 256 rays and fill
+// rounds of 4 / 2 / 1 clusters per wavefront, the one with the most
+// workgroups per CU, then the longest round, then the longest fill round.
+// Results do not depend on it.  ILLUM_FF_RAY_SORT=0 walks rays in queue
+// order; ILLUM_FF_RAY_CHUNK (256 / 512 / 1024) fixes the round size.
+static QShape ff_qshape(const BodyDims &d, int stack_slots) {
+  const char *e = getenv("ILLUM_FF_RAY_SORT");
+  const int sort = !(e && std::string(e) == "0");
+  int fixed = 0;
+  if (const char *c = getenv("ILLUM_FF_RAY_CHUNK")) {
+    const int v = atoi(c);
+    fixed = v >= 1024 ? 1024 : v >= 512 ? 512 : 256;
+  }
+  QShape best;
+  int best_groups = -1;
+  for (int chunk = FF_MAXCHUNK; chunk >= FF_BLOCK; chunk /= 2) {
+    if (fixed && chunk != fixed) continue;
+    for (int fper = FF_MAXPER; fper >= 1; fper /= 2) {
+      QShape q;
+      q.chunk = chunk;
+      q.fper = fper;
+      q.cap = chunk + fper * FF_BLOCK;
+      q.sort = sort;
+      const int g = ff_groups_per_cu(ff_lds_bytes(d, stack_slots, q));
+      if (g > best_groups) {
+        best = q;
+        best_groups = g;
+      }
+    }
+  }
+  return best;
 }
 
 // Checks the LDS a launch needs and raises the kernel's dynamic-LDS limit.
-static size_t ff_prepare(const illum_mesh *mesh, const void *kfn, int stack_slots) {
+static size_t ff_prepare(const illum_mesh *mesh, const void *kfn, int stack_slots,
+                         const QShape &q) {
   const BodyDims d = body_dims(mesh);
-  const size_t lds = ff_lds_bytes(d, stack_slots);
-  if (lds > 160 * 1024 || d.nclmax > 65536)
+  const size_t lds = ff_lds_bytes(d, stack_slots, q);
+  if (lds > 160 * 1024 || d.nclmax > 65536 || 32 * (int64_t)d.nwmax > FF_IDX_MASK + 1)
     throw Error(ILLUM_ERR_CAPACITY,
                 "a body has too many facets for the LDS visibility bitset (max ~1.2M per body)");
   if (lds > 64 * 1024)
@@ -667,7 +836,7 @@ static size_t ff_prepare(const illum_mesh *mesh, const void *kfn, int stack_slot
 }
 
 static FFArgs ff_args(const illum_bvh *bvh, int nrows, int jstride, int stack_slots,
-                      double offset, const double *albedo_vec, double albedo) {
+                      const QShape &q, double offset, const double *albedo_vec, double albedo) {
   const illum_mesh *mesh = bvh->mesh;
   const BodyDims d = body_dims(mesh);
   FFArgs a{};
@@ -677,6 +846,10 @@ static FFArgs ff_args(const illum_bvh *bvh, int nrows, int jstride, int stack_sl
   a.nrows = nrows;
   a.jstride = jstride;
   a.scap = stack_slots;
+  a.qchunk = q.chunk;
+  a.fper = q.fper;
+  a.qcap = q.cap;
+  a.qsort = q.sort;
   a.offset = (float)offset;
   a.albedo = albedo;
   a.pr4 = (const float4 *)mesh->d_pr4;
@@ -739,7 +912,8 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   // traversal stack: one slot per tree level (a push happens at most once per level)
   const int stack_slots = ff_stack_slots(bvh, layout);
   auto launch = ff_kernel<false>(layout);
-  const size_t lds = ff_prepare(mesh, (const void *)launch, stack_slots);
+  const QShape qs = ff_qshape(body_dims(mesh), stack_slots);
+  const size_t lds = ff_prepare(mesh, (const void *)launch, stack_slots, qs);
   const bool keep_vis = flags & ILLUM_FF_KEEP_VISIBILITY;
 
   std::unique_ptr<illum_sparse, void (*)(illum_sparse *)> sp(new illum_sparse(), destroy_sparse);
@@ -781,7 +955,7 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
     ILLUM_HIP(hipMemsetAsync(rcounts, 0, 4 * nf, s));
     ILLUM_HIP(hipMemsetAsync(ctl, 0, sizeof(Ctl), s));
     ILLUM_HIP(hipMemsetAsync(sp->d_nrays, 0, 8, s));
-    FFArgs a = ff_args(bvh, (int)nf, 1, stack_slots, offset, d_alb, albedo);
+    FFArgs a = ff_args(bvh, (int)nf, 1, stack_slots, qs, offset, d_alb, albedo);
     a.status = &ctl->status;
     a.stats = sp->d_nrays;
     a.clus = cluster_cull() ? (const float4 *)mesh->d_clus : nullptr;
@@ -897,7 +1071,8 @@ double time_visibility_sample(illum_bvh *bvh, int max_sources, int layout_choice
   const int layout = ff_resolve(bvh, layout_choice);
   const int stack_slots = ff_stack_slots(bvh, layout);
   auto launch = ff_kernel<false>(layout);
-  const size_t lds = ff_prepare(mesh, (const void *)launch, stack_slots);
+  const QShape qs = ff_qshape(body_dims(mesh), stack_slots);
+  const size_t lds = ff_prepare(mesh, (const void *)launch, stack_slots, qs);
   // kept lists of the sampled rows, 64 entries per row (a row that does not
   // fit is counted and not written)
   const int64_t cap = 64 * (int64_t)nrows;
@@ -908,7 +1083,7 @@ double time_visibility_sample(illum_bvh *bvh, int max_sources, int layout_choice
   hipEvent_t e0, e1;
   ILLUM_HIP(hipEventCreate(&e0));
   ILLUM_HIP(hipEventCreate(&e1));
-  FFArgs a = ff_args(bvh, nrows, jstride, stack_slots, 1e-5, nullptr, 1.0);
+  FFArgs a = ff_args(bvh, nrows, jstride, stack_slots, qs, 1e-5, nullptr, 1.0);
   a.status = status.p;
   a.clus = cluster_cull() ? (const float4 *)mesh->d_clus : nullptr;
   a.counts = counts.p;
@@ -934,10 +1109,12 @@ double time_visibility_sample(illum_bvh *bvh, int max_sources, int layout_choice
 }
 
 // Work of one full visibility pass (all sources), counted by the COUNT
-// instantiation: out4 = {rays, pair tests, box tests, triangle tests}.  The
-// assembly kernel itself carries no counters; this is a separate launch, and
-// writes no lists.
-void form_factor_work(illum_bvh *bvh, double offset, int64_t *out4) {
+// instantiation: out[0..n) of {rays, pair tests, box tests, triangle tests
+// (per active lane), wavefront walks, wavefront node visits, wavefront
+// triangle tests, walks with a fixed permutation class} (the last four: the
+// wavefront walks only).  The assembly kernel itself carries no counters;
+// this is a separate launch, and writes no lists.
+void form_factor_work(illum_bvh *bvh, double offset, int64_t *out, int n) {
   check_bvh_bodies(bvh);
   illum_mesh *mesh = bvh->mesh;
   illum_ctx *ctx = mesh->ctx;
@@ -946,20 +1123,21 @@ void form_factor_work(illum_bvh *bvh, double offset, int64_t *out4) {
   const int layout = ff_layout(bvh);
   const int stack_slots = ff_stack_slots(bvh, layout);
   auto launch = ff_kernel<true>(layout);
-  const size_t lds = ff_prepare(mesh, (const void *)launch, stack_slots);
+  const QShape qs = ff_qshape(body_dims(mesh), stack_slots);
+  const size_t lds = ff_prepare(mesh, (const void *)launch, stack_slots, qs);
   DBuf<int> status(1);
-  DBuf<unsigned long long> stats(4);
-  ILLUM_HIP(hipMemsetAsync(stats.p, 0, 32, s));
-  FFArgs a = ff_args(bvh, (int)nf, 1, stack_slots, offset, nullptr, 1.0);
+  DBuf<unsigned long long> stats(8);
+  ILLUM_HIP(hipMemsetAsync(stats.p, 0, 64, s));
+  FFArgs a = ff_args(bvh, (int)nf, 1, stack_slots, qs, offset, nullptr, 1.0);
   a.status = status.p;
   a.stats = stats.p;
   a.clus = cluster_cull() ? (const float4 *)mesh->d_clus : nullptr;
   launch<<<(int)nf, FF_BLOCK, lds, s>>>(a);
   ILLUM_LAUNCH_CHECK();
-  unsigned long long hs[4];
-  ILLUM_HIP(hipMemcpyAsync(hs, stats.p, 32, hipMemcpyDeviceToHost, s));
+  unsigned long long hs[8];
+  ILLUM_HIP(hipMemcpyAsync(hs, stats.p, 64, hipMemcpyDeviceToHost, s));
   sync(ctx);
-  for (int k = 0; k < 4; ++k) out4[k] = (int64_t)hs[k];
+  for (int k = 0; k < n && k < 8; ++k) out[k] = (int64_t)hs[k];
 }
 
 }  // namespace illum

@@ -32,6 +32,10 @@ struct DRay {
   int kx, ky, kz;     // axis permutation (kz = dominant axis)
 };
 
+// Wavefront ballot of a lane predicate, taken as a lane mask directly (no
+// round trip of the predicate through an int).
+__device__ __forceinline__ uint64_t wave_ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
 __device__ __forceinline__ float sel3(float x, float y, float z, int k) {
   return k == 0 ? x : (k == 1 ? y : z);
 }
@@ -65,29 +69,59 @@ __device__ __forceinline__ DRay make_ray(float ox, float oy, float oz, float dx,
   return r;
 }
 
-// Slab test against a node box. Returns the entry distance in *tn and
-// whether the box must be visited for a best distance tbest.
+// Box acceptance: tn <= tf && tf > 0 && tn <= tbest, written as the single
+// comparison max(tn, m) <= min(tf, tbest) with m the smallest positive float.
+// Equivalent whenever tbest > 0 (every caller's best distance is a hit
+// distance, a positive bound or +inf; a negative tbest rejects every box):
+// tf >= m is tf > 0, and FP32 denormals are kept (the kernels' FP mode), so
+// m and denormal tf compare as values.  One comparison gives a lane mask that
+// a ballot takes as is.
+__device__ __forceinline__ bool slab_accept(float tn, float tf, float tbest) {
+  return fmaxf(tn, __int_as_float(1)) <= fminf(tf, tbest);
+}
+
+// Slab test against a node box: whether the box must be visited for a best
+// distance tbest.
 __device__ __forceinline__ bool box_visit(const DRay &r, float4 lo, float4 hi, float tbest) {
   float t0x = __builtin_fmaf(lo.x, r.ix, r.nx), t1x = __builtin_fmaf(hi.x, r.ix, r.nx);
   float t0y = __builtin_fmaf(lo.y, r.iy, r.ny), t1y = __builtin_fmaf(hi.y, r.iy, r.ny);
   float t0z = __builtin_fmaf(lo.z, r.iz, r.nz), t1z = __builtin_fmaf(hi.z, r.iz, r.nz);
   float tn = fmaxf(fminf(t0x, t1x), fmaxf(fminf(t0y, t1y), fminf(t0z, t1z)));
   float tf = fminf(fmaxf(t0x, t1x), fminf(fmaxf(t0y, t1y), fmaxf(t0z, t1z)));
-  return tn <= tf && tf > 0.0f && tn <= tbest;
+  return slab_accept(tn, tf, tbest);
+}
+
+// Permutation class of a ray: f = 2 kz + (d_kz < 0), which fixes (kx, ky, kz)
+// as make_ray chooses them.  PermOf<f> gives the axes at compile time.
+template <int F>
+struct PermOf {
+  static constexpr int kz = F >> 1;
+  static constexpr int kx = (F & 1) ? (kz + 2) % 3 : (kz + 1) % 3;
+  static constexpr int ky = (F & 1) ? (kz + 1) % 3 : (kz + 2) % 3;
+};
+
+__device__ __forceinline__ int ray_perm(const DRay &r) {
+  return 2 * r.kz + (r.kx == (r.kz == 0 ? 2 : r.kz - 1) ? 1 : 0);
 }
 
 // Watertight two-sided test.  On a hit (t > 0) writes the reported distance.
+// PERM >= 0: the caller guarantees ray_perm(r) == PERM, so the axis
+// selections are resolved at compile time (same operations, same result).
+template <int PERM = -1>
 __device__ __forceinline__ bool tri_hit(const DRay &r, float4 a, float4 b, float4 c,
                                         float *tout) {
+  const int kx = PERM >= 0 ? PermOf<(PERM < 0 ? 0 : PERM)>::kx : r.kx;
+  const int ky = PERM >= 0 ? PermOf<(PERM < 0 ? 0 : PERM)>::ky : r.ky;
+  const int kz = PERM >= 0 ? PermOf<(PERM < 0 ? 0 : PERM)>::kz : r.kz;
   const float Ax0 = a.x - r.ox, Ay0 = a.y - r.oy, Az0 = a.z - r.oz;
   const float Bx0 = b.x - r.ox, By0 = b.y - r.oy, Bz0 = b.z - r.oz;
   const float Cx0 = c.x - r.ox, Cy0 = c.y - r.oy, Cz0 = c.z - r.oz;
-  const float Akx = sel3(Ax0, Ay0, Az0, r.kx), Aky = sel3(Ax0, Ay0, Az0, r.ky),
-              Akz = sel3(Ax0, Ay0, Az0, r.kz);
-  const float Bkx = sel3(Bx0, By0, Bz0, r.kx), Bky = sel3(Bx0, By0, Bz0, r.ky),
-              Bkz = sel3(Bx0, By0, Bz0, r.kz);
-  const float Ckx = sel3(Cx0, Cy0, Cz0, r.kx), Cky = sel3(Cx0, Cy0, Cz0, r.ky),
-              Ckz = sel3(Cx0, Cy0, Cz0, r.kz);
+  const float Akx = sel3(Ax0, Ay0, Az0, kx), Aky = sel3(Ax0, Ay0, Az0, ky),
+              Akz = sel3(Ax0, Ay0, Az0, kz);
+  const float Bkx = sel3(Bx0, By0, Bz0, kx), Bky = sel3(Bx0, By0, Bz0, ky),
+              Bkz = sel3(Bx0, By0, Bz0, kz);
+  const float Ckx = sel3(Cx0, Cy0, Cz0, kx), Cky = sel3(Cx0, Cy0, Cz0, ky),
+              Ckz = sel3(Cx0, Cy0, Cz0, kz);
   const float Ax = Akx - r.Sx * Akz, Ay = Aky - r.Sy * Akz;
   const float Bx = Bkx - r.Sx * Bkz, By = Bky - r.Sy * Bkz;
   const float Cx = Ckx - r.Sx * Ckz, Cy = Cky - r.Sy * Ckz;
@@ -163,7 +197,7 @@ __device__ __forceinline__ bool box_enter(const DRay &r, float4 lo, float4 hi, f
   float tn = fmaxf(fminf(t0x, t1x), fmaxf(fminf(t0y, t1y), fminf(t0z, t1z)));
   float tf = fminf(fmaxf(t0x, t1x), fminf(fmaxf(t0y, t1y), fmaxf(t0z, t1z)));
   *tn_out = tn;
-  return tn <= tf && tf > 0.0f && tn <= tbest;
+  return slab_accept(tn, tf, tbest);
 }
 
 // One step of the near-first closest-hit walk over the two-wide layout: one
@@ -258,30 +292,35 @@ __device__ __forceinline__ int trace_closest_wide(const float4 *__restrict__ wid
 // uses the lane's own best distance, and a node a lane does not need only
 // adds exact tests (its boxes lie inside the parent's, which the lane's ray
 // does not enter before its best distance).
+template <int PERM = -1>
 __device__ __forceinline__ int trace_closest_packet(const float4 *__restrict__ wide,
                                                     const float4 *__restrict__ tris,
                                                     const DRay &r, bool act, int root,
                                                     float tinit, float *tbest_out,
-                                                    unsigned &nbox, unsigned &ntri) {
+                                                    unsigned &nbox, unsigned &ntri,
+                                                    unsigned &wnode, unsigned &wtri) {
   const int lane = __lane_id();
-  float tbest = tinit;
+  // an inactive lane's best distance is -inf: every box test rejects it
+  float tbest = act ? tinit : -__builtin_huge_valf();
   int best = -1;
   int sp = 0;         // wavefront-uniform
   int cur = root;     // wavefront-uniform
   int stk = 0;        // lane k: stack entry k
   for (;;) {
     if (cur >= 0) {
+      // the whole 64-byte node in one go, both boxes tested on every lane
       const float4 a0 = wide[4 * cur], a1 = wide[4 * cur + 1];
       const float4 b0 = wide[4 * cur + 2], b1 = wide[4 * cur + 3];
       float tl = 0.0f, tr = 0.0f;
-      const bool hl = act && box_enter(r, a0, a1, tbest, &tl);
-      const bool hr = act && box_enter(r, b0, b1, tbest, &tr);
+      const bool hl = box_enter(r, a0, a1, tbest, &tl);
+      const bool hr = box_enter(r, b0, b1, tbest, &tr);
       if (act) nbox += 2;
-      const uint64_t ml = __ballot(hl), mr = __ballot(hr);
+      ++wnode;
+      const uint64_t ml = wave_ballot(hl), mr = wave_ballot(hr);
       const int cl = __float_as_int(a0.w), cr = __float_as_int(b0.w);
       if (ml && mr) {
         const uint64_t both = ml & mr;
-        const uint64_t lfirst = __ballot(hl && hr && tl <= tr);
+        const uint64_t lfirst = wave_ballot(tl <= tr) & both;
         const bool lnear = 2 * __popcll(lfirst) >= __popcll(both);
         if (lane == sp) stk = lnear ? cr : cl;
         ++sp;
@@ -294,12 +333,13 @@ __device__ __forceinline__ int trace_closest_packet(const float4 *__restrict__ w
       const int info = ~cur;
       const int first = info & 0xFFFFFF, cnt = info >> 24;
       if (act) ntri += cnt;
+      wtri += cnt;
       for (int k = 0; k < cnt; ++k) {
         const float4 a = tris[3 * (first + k)], b = tris[3 * (first + k) + 1],
                      c = tris[3 * (first + k) + 2];
         const int idx = __float_as_int(a.w);
         float t;
-        if (act && tri_hit(r, a, b, c, &t) && (t < tbest || (t == tbest && idx < best))) {
+        if (act && tri_hit<PERM>(r, a, b, c, &t) && (t < tbest || (t == tbest && idx < best))) {
           tbest = t;
           best = idx;
         }
@@ -332,7 +372,7 @@ __device__ __forceinline__ void box4_enter(const DRay &r, const float4 *__restri
     const float t0z = __builtin_fmaf(loz[q], r.iz, r.nz), t1z = __builtin_fmaf(hiz[q], r.iz, r.nz);
     const float tn = fmaxf(fminf(t0x, t1x), fmaxf(fminf(t0y, t1y), fminf(t0z, t1z)));
     const float tf = fminf(fmaxf(t0x, t1x), fminf(fmaxf(t0y, t1y), fmaxf(t0z, t1z)));
-    const bool ok = q < cnt && tn <= tf && tf > 0.0f && tn <= tbest;
+    const bool ok = q < cnt && slab_accept(tn, tf, tbest);
     key[q] = ok ? fminf(tn, 3.0e38f) : __builtin_huge_valf();
     ref[q] = __float_as_int(rfa[q]);
   }
@@ -353,13 +393,16 @@ __device__ __forceinline__ void cswap4(float *k, int *r, int a, int b) {
 // is representative), the nearest next and the others pushed farthest first.
 // A walk pushes at most three entries per level, so 3 * depth4 + 1 <= 64
 // entries suffice (the caller checks).
+template <int PERM = -1>
 __device__ __forceinline__ int trace_closest_packet4(const float4 *__restrict__ w4,
                                                      const float4 *__restrict__ tris,
                                                      const DRay &r, bool act, int root,
                                                      float tinit, float *tbest_out,
-                                                     unsigned &nbox, unsigned &ntri) {
+                                                     unsigned &nbox, unsigned &ntri,
+                                                     unsigned &wnode, unsigned &wtri) {
   const int lane = __lane_id();
-  float tbest = tinit;
+  // an inactive lane's best distance is -inf: every box test rejects it
+  float tbest = act ? tinit : -__builtin_huge_valf();
   int best = -1;
   int sp = 0;
   int cur = root;
@@ -370,11 +413,12 @@ __device__ __forceinline__ int trace_closest_packet4(const float4 *__restrict__ 
       int ref[4];
       box4_enter(r, w4 + 8 * cur, tbest, key, ref);
       if (act) nbox += 4;
+      ++wnode;
       float k[4];
       int n = 0;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const uint64_t m = __ballot(act && key[q] < __builtin_huge_valf());
+        const uint64_t m = wave_ballot(key[q] < __builtin_huge_valf());
         // representative entry distance: the first entering lane's
         k[q] = m ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(key[q]),
                                                               (int)__builtin_ctzll(m)))
@@ -397,12 +441,13 @@ __device__ __forceinline__ int trace_closest_packet4(const float4 *__restrict__ 
       const int info = ~cur;
       const int first = info & 0xFFFFFF, cnt = info >> 24;
       if (act) ntri += cnt;
+      wtri += cnt;
       for (int q = 0; q < cnt; ++q) {
         const float4 a = tris[3 * (first + q)], b = tris[3 * (first + q) + 1],
                      c = tris[3 * (first + q) + 2];
         const int idx = __float_as_int(a.w);
         float t;
-        if (act && tri_hit(r, a, b, c, &t) && (t < tbest || (t == tbest && idx < best))) {
+        if (act && tri_hit<PERM>(r, a, b, c, &t) && (t < tbest || (t == tbest && idx < best))) {
           tbest = t;
           best = idx;
         }
@@ -581,19 +626,20 @@ __device__ __forceinline__ bool trace_any_packet(const float4 *__restrict__ wide
   int stk = 0;
   for (;;) {
     const bool live = act && !hit;
-    if (!__ballot(live)) break;
+    if (!wave_ballot(live)) break;
     if (cur >= 0) {
       const float4 a0 = wide[4 * cur], a1 = wide[4 * cur + 1];
       const float4 b0 = wide[4 * cur + 2], b1 = wide[4 * cur + 3];
       float tl, tr;
-      const bool el = box_enter(r, a0, a1, inf, &tl);
-      const bool er = box_enter(r, b0, b1, inf, &tr);
-      const bool hl = live && el, hr = live && er;
-      const uint64_t ml = __ballot(hl), mr = __ballot(hr);
+      // a lane not searching tests against -inf: every box rejects it
+      const float tb = live ? inf : -inf;
+      const bool hl = box_enter(r, a0, a1, tb, &tl);
+      const bool hr = box_enter(r, b0, b1, tb, &tr);
+      const uint64_t ml = wave_ballot(hl), mr = wave_ballot(hr);
       const int cl = __float_as_int(a0.w), cr = __float_as_int(b0.w);
       if (ml && mr) {
         const uint64_t both = ml & mr;
-        const uint64_t lfirst = __ballot(hl && hr && tl <= tr);
+        const uint64_t lfirst = wave_ballot(tl <= tr) & both;
         const bool lnear = 2 * __popcll(lfirst) >= __popcll(both);
         if (lane == sp) stk = lnear ? cr : cl;
         ++sp;

@@ -206,6 +206,12 @@ int illum_form_factors_ex(illum_bvh *bvh, double albedo, const double *albedo_ve
  * slab tests, ray/triangle tests}.  Measurement aid, no reference
  * counterpart. */
 int illum_form_factors_work(illum_bvh *bvh, double offset, int64_t *out4);
+/* The same launch with out[0..n), 1 <= n <= 8: the four counters above, then
+ * the wavefront walks' own counts (zero for the per-lane walks): walks, node
+ * visits and triangle tests per wavefront, and walks run with the triangle
+ * test's axis permutation fixed for the whole wavefront.  Measurement aid, no
+ * reference counterpart. */
+int illum_form_factors_work_ex(illum_bvh *bvh, double offset, int64_t *out, int n);
 int illum_sparse_create_csr(illum_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rowptr,
                             const int32_t *colidx, const double *val, illum_sparse **out);
 int illum_sparse_destroy(illum_sparse *sp);
