@@ -16,6 +16,13 @@ Secondary measurements on the same JSON line (skip with --quick):
              FP64 vs mixed
   coupled    radiosity + thermal time step on the 81,920-facet mesh
              (direct -> radiosity solve -> CN step), FP64 vs mixed
+  c5         64 bodies of 1,280-327,680 facets (5.43 M) through form factors,
+             horizons, direct insolation under 32 suns per body, the mixed
+             solve and 512 mixed CN steps in one launch sequence; per-stage
+             seconds, and the smallest body checked against the oracle
+  transfer   k-NN mesh-field transfer from the 81,920-facet mesh to the
+             327,680-facet one (k = 5, FP64 brute force): pairs/s, checked
+             against scipy's cKDTree on a target sample
 
 Prints progress on stderr and ONE JSON line on stdout.
 """
@@ -263,6 +270,74 @@ def bench_thermal(api, nsteps=4096, subdiv=7, seed=0, chunk=256):
     return out
 
 
+def bench_c5(api, O):
+    """C5 in one launch sequence (illum_amd/batch.py), wall-clock per stage
+    with a device synchronisation around each.  The form-factor stage's rate
+    counts the candidate pairs of every body, sum N_b (N_b - 1)."""
+    from illum_amd import batch
+    bodies = configs.c5_bodies()
+    nb, S = len(bodies), 32
+    suns = configs.c5_suns(nb, S)
+    r = batch.run_bodies(bodies, suns, S, method="mixed", thermal="mixed")
+    sizes = [len(F) for _, F in bodies]
+    pairs = sum(n * (n - 1) for n in sizes)
+    t = r["seconds"]
+    out = {"bodies": nb, "facets": r["nf"], "facets_per_body": [min(sizes), max(sizes)],
+           "suns_per_body": S, "cn_steps": r["steps"], "nnz": r["Fm"].nnz,
+           "seconds": {k: round(v, 4) for k, v in t.items()},
+           "form_factor_pairs_per_s": pairs / t["form_factors"],
+           "radiosity_products": r["solve"][2], "radiosity_rel_resid": float(r["solve"][1][-1])}
+    # the smallest body against the oracle run on it alone: visibility sets,
+    # form factors and horizons
+    b = sizes.index(min(sizes))
+    Vb, Fb = bodies[b]
+    lo, hi = int(r["offsets"][b]), int(r["offsets"][b + 1])
+    g = O.geometry(Vb, Fb)
+    bits, _ = O.visibility(g)
+    ocp, ori, ov = O.form_factors(g, bits, albedo=0.1)
+    cp, ri, v = r["Fm"].csc()
+    a, z = cp[lo], cp[hi]
+    same = bool(np.array_equal(cp[lo:hi + 1] - a, ocp) and np.array_equal(ri[a:z] - lo, ori))
+    n = hi - lo
+    hzb = r["hz"].to_host_flat(lo * r["nphi"], n * r["nphi"]).reshape(n, r["nphi"]).T
+    ho, _ = O.horizons(g, nphi=r["nphi"])
+    out["parity_body"] = {"body": b, "facets": n, "form_factor_pattern_equal": same,
+                          "form_factor_max_rel_err": float(np.max(np.abs(v[a:z] - ov) / np.abs(ov)))
+                          if same and len(ov) else None,
+                          "horizons_equal": bool(np.array_equal(hzb, ho))}
+    for k in ("hz", "E", "B"):
+        r[k].close()
+    r["th"].close()
+    r["Fm"].close()
+    return out
+
+
+def bench_transfer(api):
+    """k-NN transfer tables between the 81,920- and 327,680-facet meshes
+    (build_mesh_transfer_tables.py's downsample query, k = 1, and its
+    neighbour query, k = 5), hipEvent-free wall-clock around one call each."""
+    from illum_amd import transfer
+    from scipy.spatial import cKDTree
+    Va, Fa = configs.mesh("C2")
+    Vb, Fb = configs.mesh("C4")
+    Pa, Pb = transfer.centroids(Va, Fa), transfer.centroids(Vb, Fb)
+    transfer.knn(Pa, Pb[:4096], 5)  # loads the code object
+    out = {"sources": len(Pa), "targets": len(Pb)}
+    for k in (1, 5):
+        t0 = time.perf_counter()
+        idx, dist = transfer.knn(Pa, Pb, k)
+        dt = time.perf_counter() - t0
+        samp = np.arange(0, len(Pb), 97)
+        dref, iref = cKDTree(Pa).query(Pb[samp], k)
+        iref = iref.reshape(len(samp), k)
+        dref = dref.reshape(len(samp), k)
+        out[f"k{k}"] = {"seconds": dt, "pairs_per_s": len(Pa) * len(Pb) / dt,
+                        "sample": len(samp),
+                        "index_mismatches": int(np.sum(idx[samp] != iref)),
+                        "max_abs_dist_err": float(np.max(np.abs(dist[samp] - dref)))}
+    return out
+
+
 def bench_coupled(api, illum, ctx, Fm, hz, nsteps=256):
     """The radiosity + thermal time loop (illum_cli.cpp:403-456) at one sun per
     step: direct -> E -= F_surf -> radiosity solve to 1e-8 -> CN step (40
@@ -341,6 +416,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--leaf-size", type=int, default=0, help="BVH leaf size (0 = autotune)")
     ap.add_argument("--quick", action="store_true", help="headline only")
+    ap.add_argument("--no-c5", action="store_true", help="skip the 64-body C5 leg")
     ap.add_argument("--thermal-steps", type=int, default=4096)
     ap.add_argument("--thermal-only", action="store_true",
                     help="only run the conduction leg and print its record")
@@ -468,6 +544,11 @@ def main():
         hz.close()
         secondary["thermal"] = bench_thermal(api, nsteps=args.thermal_steps)
         log("thermal:", secondary["thermal"])
+        if not args.no_c5:
+            secondary["c5"] = bench_c5(api, O)
+            log("c5:", secondary["c5"])
+        secondary["transfer"] = bench_transfer(api)
+        log("transfer:", secondary["transfer"])
 
     cpu = None
     parity = {}

@@ -69,8 +69,9 @@ struct FFArgs {
   int nf;
   int nwmax;    // LDS bitset words: the largest body's facets / 32
   int nclmax;   // LDS cluster-list entries: the largest body's clusters
-  int nrows;    // sources of the launch: j = row * jstride
+  int nrows;    // sources of the launch: j = row * jstride, or j = rows[row]
   int jstride;
+  const int32_t *rows;  // nullable: the launch's sources, listed
   int scap;     // LDS traversal stack entries per lane
   int qchunk;   // rays per walk round (a multiple of FF_BLOCK, <= FF_MAXCHUNK)
   int fper;     // clusters per wavefront per fill round (<= FF_MAXPER)
@@ -85,12 +86,14 @@ struct FFArgs {
   unsigned long long *stats;     // rays [, pair, box, triangle tests with COUNT]
   int32_t *row_counts;           // CSR row lengths (receiver i), nullable
   // Kept receivers of each source, ascending, in a pool: counts[row]
-  // entries from pool[offs[row]] (written when they fit below pool_cap).
+  // entries from pool[offs[row] - off_base] (written when they fit below
+  // pool_cap).
   int32_t *counts;
   int64_t *offs;
   int32_t *pool;
   unsigned long long *pool_n;
   int64_t pool_cap;
+  int64_t off_base;
   // The raw visibility sets (before the zero drop) in the same form;
   // vpool == nullptr: not kept.
   int32_t *vcounts;
@@ -104,11 +107,13 @@ struct FFArgs {
 // lo + 32 w + b) to a pool in ascending order: threads own contiguous words,
 // a block scan of their popcounts places each thread's entries, one atomic
 // reserves the block's segment.  *cnt_out / *off_out receive the count and
-// the segment's start; nothing is written past cap (the host then grows the
-// pool and runs the pass again).  Called by the whole block.
+// the segment's start plus off_base; nothing is written past cap (the host
+// then runs the sources that did not fit again, into a second pool).
+// Called by the whole block.
 __device__ void append_bits_sorted(const uint32_t *bits, int nw, int lo, int32_t *pool,
-                                   unsigned long long *pool_n, int64_t cap, int32_t *cnt_out,
-                                   int64_t *off_out, int *s_scan, long long *s_base) {
+                                   unsigned long long *pool_n, int64_t cap, int64_t off_base,
+                                   int32_t *cnt_out, int64_t *off_out, int *s_scan,
+                                   long long *s_base) {
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int per = (nw + FF_BLOCK - 1) / FF_BLOCK;
   const int w0 = min(nw, tid * per), w1 = min(nw, w0 + per);
@@ -131,7 +136,7 @@ __device__ void append_bits_sorted(const uint32_t *bits, int nw, int lo, int32_t
     const long long base = total ? (long long)atomicAdd(pool_n, (unsigned long long)total) : 0;
     *s_base = base;
     *cnt_out = total;
-    *off_out = base;
+    *off_out = base + off_base;
   }
   __syncthreads();
   const long long base = *s_base;
@@ -295,7 +300,8 @@ __global__ __launch_bounds__(FF_BLOCK, FF_WAVES_PER_SIMD) void ff_visibility_ker
   __shared__ long long s_base;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   // row = output row; source j = row * jstride (jstride 1 except when sampling)
-  const int row = xcd_remap(blockIdx.x, a.nrows);
+  const int r0 = xcd_remap(blockIdx.x, a.nrows);
+  const int row = a.rows ? a.rows[r0] : r0;
   const int j = row * a.jstride;
   // The source's body: receivers [lo, hi) (bit i - lo of the LDS row), its
   // trees' roots.
@@ -432,7 +438,7 @@ __global__ __launch_bounds__(FF_BLOCK, FF_WAVES_PER_SIMD) void ff_visibility_ker
 
   // The raw visibility set, on request.
   if (a.vpool)
-    append_bits_sorted(bits, nw, lo, a.vpool, a.vpool_n, a.vpool_cap, a.vcounts + row,
+    append_bits_sorted(bits, nw, lo, a.vpool, a.vpool_n, a.vpool_cap, 0, a.vcounts + row,
                        a.voffs + row, s_scan, &s_base);
 
   // Nonzero-form-factor filter.  A row's set bits cluster in a few words, so
@@ -485,8 +491,8 @@ __global__ __launch_bounds__(FF_BLOCK, FF_WAVES_PER_SIMD) void ff_visibility_ker
   __syncthreads();
   // The kept receivers, ascending.
   if (a.pool)
-    append_bits_sorted(bits, nw, lo, a.pool, a.pool_n, a.pool_cap, a.counts + row, a.offs + row,
-                       s_scan, &s_base);
+    append_bits_sorted(bits, nw, lo, a.pool, a.pool_n, a.pool_cap, a.off_base, a.counts + row,
+                       a.offs + row, s_scan, &s_base);
   if (a.stats && tid == 0) atomicAdd(a.stats, (unsigned long long)s_rays);
   if (COUNT && a.stats) {
     // pair tests, box tests, triangle tests: wave sums, one atomic each;
@@ -504,11 +510,12 @@ __global__ __launch_bounds__(FF_BLOCK, FF_WAVES_PER_SIMD) void ff_visibility_ker
   }
 }
 
-// CSC(F) from the kept-receiver pool: one wave per column (= source j)
+// CSC(F) from the kept-receiver pools: one wave per column (= source j)
 // copies its ascending segment to rowidx and evaluates the FP64 values.
 __global__ __launch_bounds__(256) void ff_fill_list_kernel(
-    int64_t nf, const int32_t *__restrict__ pool, const int64_t *__restrict__ offs,
-    const int64_t *__restrict__ colptr, const float4 *__restrict__ pr4,
+    int64_t nf, const int32_t *__restrict__ pool, int64_t cap, const int32_t *__restrict__ pool2,
+    const int64_t *__restrict__ offs, const int64_t *__restrict__ colptr,
+    const float4 *__restrict__ pr4,
     const float4 *__restrict__ n4, const double *__restrict__ area,
     const double *__restrict__ albedo_vec, double albedo, int32_t *__restrict__ rowidx,
     double *__restrict__ cval) {
@@ -516,7 +523,9 @@ __global__ __launch_bounds__(256) void ff_fill_list_kernel(
   if (j >= nf) return;
   const int64_t base = colptr[j];
   const int64_t n = colptr[j + 1] - base;
-  const int32_t *src = pool + offs[j];
+  // offsets at or past cap are the second pool's (sources run again)
+  const int64_t o = offs[j];
+  const int32_t *src = o < cap ? pool + o : pool2 + (o - cap);
   const float4 pj = pr4[j], nj = n4[j];
   const double Aj = area[j];
   for (int64_t e = threadIdx.x & 63; e < n; e += 64) {
@@ -538,6 +547,26 @@ __global__ __launch_bounds__(256) void list_copy_kernel(int64_t n, const int32_t
   const int64_t base = ptr[r], len = ptr[r + 1] - base;
   const int32_t *src = pool + offs[r];
   for (int64_t e = threadIdx.x & 63; e < len; e += 64) out[base + e] = src[e];
+}
+
+// Sources whose kept receivers did not fit a pool of cap entries (segment
+// end past cap): listed in rows[0..*nrows) in any order, their counts summed
+// into *need.
+__global__ __launch_bounds__(256) void ff_overflow_rows_kernel(
+    int64_t nf, const int32_t *__restrict__ counts, const int64_t *__restrict__ offs, int64_t cap,
+    int32_t *__restrict__ rows, int *__restrict__ nrows, unsigned long long *__restrict__ need) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool over = j < nf && counts[j] > 0 && offs[j] + counts[j] > cap;
+  const uint64_t m = wave_ballot(over);
+  if (!m) return;
+  const int lane = threadIdx.x & 63;
+  int slot = 0;
+  if (lane == (int)__builtin_ctzll(m)) slot = atomicAdd(nrows, __popcll(m));
+  slot = __shfl(slot, (int)__builtin_ctzll(m));
+  if (over) {
+    rows[slot + __popcll(m & lanemask_lt())] = (int32_t)j;
+    atomicAdd(need, (unsigned long long)counts[j]);
+  }
 }
 
 // CSR(F) from CSC(F).  CSR row i lists the sources j of the CSC entries
@@ -928,7 +957,10 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   }
   // Control words: NaN status, the longest CSR row and CSC column, the two
   // pool counters.
-  struct Ctl { int status, maxrow, maxcol, pad; unsigned long long pool_n, vpool_n; };
+  struct Ctl {
+    int status, maxrow, maxcol, nover;
+    unsigned long long pool_n, vpool_n, need, pool2_n;
+  };
   Ctl *ctl = (Ctl *)ctx_workspace(ctx, WS_FF_STATUS, sizeof(Ctl));
   // Per-source counts and pool offsets, CSR row lengths: 24 bytes per facet
   // (28 with the raw visibility sets).
@@ -940,28 +972,55 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   int32_t *vcounts = keep_vis ? (int32_t *)(cw + 24 * nf) : nullptr;
   sp->d_nrays = palloc<unsigned long long>(ctx, 1);
 
-  // Pools sized from the last assembly on this context, or 48 entries per
-  // facet; an overflow is found at the mid-point readback and the pass runs
-  // again with the exact size.
-  int64_t cap = pool_entries(ctx, WS_FF_POOL, 48 * nf + 4096);
-  int64_t vcap = keep_vis ? pool_entries(ctx, WS_FF_VPOOL, 64 * nf + 4096) : 0;
-  int64_t nnz = 0;
+  // Pool capacity: the last assembly's nonzeros on this tree plus 5 %, else
+  // an estimate from a counted sample of every 32nd source (nothing written)
+  // plus 25 %; at least what the context's workspace already holds.  Sources
+  // whose lists still do not fit run again into a second pool sized exactly.
+  auto base_args = [&](int nrows, int jstride) {
+    FFArgs a = ff_args(bvh, nrows, jstride, stack_slots, qs, offset, d_alb, albedo);
+    a.status = &ctl->status;
+    a.clus = cluster_cull() ? (const float4 *)mesh->d_clus : nullptr;
+    a.counts = counts;
+    a.offs = offs;
+    return a;
+  };
+  int64_t want = 0;
+  if (bvh->ff_nnz_last > 0) {
+    want = bvh->ff_nnz_last + bvh->ff_nnz_last / 20 + 4096;
+  } else {
+    const int js = nf >= 32 * 64 ? 32 : 1;
+    const int nrows = (int)((nf + js - 1) / js);
+    ILLUM_HIP(hipMemsetAsync(ctl, 0, sizeof(Ctl), s));
+    FFArgs a = base_args(nrows, js);
+    a.pool = counts;  // capacity 0: only the counts are written
+    a.pool_n = &ctl->pool_n;
+    a.pool_cap = 0;
+    launch<<<nrows, FF_BLOCK, lds, s>>>(a);
+    ILLUM_LAUNCH_CHECK();
+    Ctl h0{};
+    ILLUM_HIP(hipMemcpyAsync(&h0, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+    sync(ctx);
+    const double est = (double)h0.pool_n * (double)nf / (double)nrows;
+    want = (int64_t)(est * 1.25) + 4096;
+  }
+  int64_t cap = pool_entries(ctx, WS_FF_POOL, want);
+  // ILLUM_FF_POOL_ENTRIES fixes the first pool's capacity (to exercise the
+  // second pool; results are identical).
+  if (const char *e = getenv("ILLUM_FF_POOL_ENTRIES")) cap = std::max<int64_t>(0, atoll(e));
+  int64_t vcap = keep_vis ? pool_entries(ctx, WS_FF_VPOOL, 2 * want) : 0;
+  int64_t nnz = 0, cap2 = 0;
   Ctl h{};
   sp->d_colptr = palloc<int64_t>(ctx, nf + 1);
-  int32_t *pool = nullptr, *vpool = nullptr;
+  int32_t *pool = nullptr, *vpool = nullptr, *pool2 = nullptr;
   for (int pass = 0;; ++pass) {
-    pool = (int32_t *)ctx_workspace(ctx, WS_FF_POOL, 4 * (size_t)cap);
+    pool = (int32_t *)ctx_workspace(ctx, WS_FF_POOL, 4 * (size_t)std::max<int64_t>(1, cap));
     if (keep_vis) vpool = (int32_t *)ctx_workspace(ctx, WS_FF_VPOOL, 4 * (size_t)vcap);
     ILLUM_HIP(hipMemsetAsync(rcounts, 0, 4 * nf, s));
     ILLUM_HIP(hipMemsetAsync(ctl, 0, sizeof(Ctl), s));
     ILLUM_HIP(hipMemsetAsync(sp->d_nrays, 0, 8, s));
-    FFArgs a = ff_args(bvh, (int)nf, 1, stack_slots, qs, offset, d_alb, albedo);
-    a.status = &ctl->status;
+    FFArgs a = base_args((int)nf, 1);
     a.stats = sp->d_nrays;
-    a.clus = cluster_cull() ? (const float4 *)mesh->d_clus : nullptr;
     a.row_counts = rcounts;
-    a.counts = counts;
-    a.offs = offs;
     a.pool = pool;
     a.pool_n = &ctl->pool_n;
     a.pool_cap = cap;
@@ -982,18 +1041,51 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
     max_count_kernel<<<(unsigned)std::min<int64_t>(64, (nf + 255) / 256), 256, 0, s>>>(
         nf, rcounts, counts, &ctl->maxrow);
     ILLUM_LAUNCH_CHECK();
+    // the sources whose lists did not fit (none when the pool sufficed)
+    int32_t *orows = (int32_t *)ctx_workspace(ctx, WS_FF_ROWS, 4 * (size_t)nf);
+    ff_overflow_rows_kernel<<<(unsigned)((nf + 255) / 256), 256, 0, s>>>(
+        nf, counts, offs, cap, orows, &ctl->nover, &ctl->need);
+    ILLUM_LAUNCH_CHECK();
     ILLUM_HIP(hipMemcpyAsync(&nnz, sp->d_colptr + nf, 8, hipMemcpyDeviceToHost, s));
     ILLUM_HIP(hipMemcpyAsync(&h, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
     sync(ctx);
     if (h.status) throw Error(ILLUM_ERR_NAN, "computed a NaN form factor");
     if (nnz < 0 || (unsigned long long)nnz != h.pool_n)
       throw Error(ILLUM_ERR_HIP, "form factors: inconsistent pool count");
-    const bool grow = (int64_t)h.pool_n > cap || (keep_vis && (int64_t)h.vpool_n > vcap);
-    if (!grow) break;
-    if (pass > 0) throw Error(ILLUM_ERR_HIP, "form factors: pool overflow after resizing");
-    cap = std::max<int64_t>(cap, (int64_t)h.pool_n + (int64_t)h.pool_n / 8 + 4096);
-    if (keep_vis) vcap = std::max<int64_t>(vcap, (int64_t)h.vpool_n + (int64_t)h.vpool_n / 8 + 4096);
+    if (keep_vis && (int64_t)h.vpool_n > vcap) {
+      // the raw visibility sets did not fit: the whole pass again
+      if (pass > 0) throw Error(ILLUM_ERR_HIP, "form factors: pool overflow after resizing");
+      vcap = std::max<int64_t>(vcap, (int64_t)h.vpool_n + (int64_t)h.vpool_n / 8 + 4096);
+      cap = std::max<int64_t>(cap, (int64_t)h.pool_n + (int64_t)h.pool_n / 8 + 4096);
+      continue;
+    }
+    if (h.nover > 0) {
+      // Second pool: the overflowing sources again, their segments at
+      // offsets cap + (position in pool2); counts, CSR row lengths, NaN
+      // status and rays were all recorded by the first pass.
+      if (h.nover > nf || (int64_t)h.need > nnz)
+        throw Error(ILLUM_ERR_HIP, "form factors: inconsistent overflow count");
+      cap2 = (int64_t)h.need;
+      pool2 = (int32_t *)ctx_workspace(ctx, WS_FF_POOL2, 4 * (size_t)std::max<int64_t>(1, cap2));
+      FFArgs a2 = base_args(h.nover, 1);
+      a2.rows = orows;
+      a2.pool = pool2;
+      a2.pool_n = &ctl->pool2_n;
+      a2.pool_cap = cap2;
+      a2.off_base = cap;
+      {
+        StageTimer st(ctx, "ff_visibility");
+        launch<<<h.nover, FF_BLOCK, lds, s>>>(a2);
+        ILLUM_LAUNCH_CHECK();
+      }
+      ILLUM_HIP(hipMemcpyAsync(&h, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+      sync(ctx);
+      if ((int64_t)h.pool2_n != cap2)
+        throw Error(ILLUM_ERR_HIP, "form factors: second pool count differs");
+    }
+    break;
   }
+  bvh->ff_nnz_last = nnz;
   if (nnz >= ((int64_t)1 << 40)) throw Error(ILLUM_ERR_CAPACITY, "nnz out of range");
   sp->nnz = nnz;
   sp->has_csc = true;
@@ -1002,7 +1094,7 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   const unsigned g4 = (unsigned)((nf + 3) / 4);
   {
     StageTimer st(ctx, "ff_csc");
-    ff_fill_list_kernel<<<g4, 256, 0, s>>>(nf, pool, offs, sp->d_colptr,
+    ff_fill_list_kernel<<<g4, 256, 0, s>>>(nf, pool, cap, pool2, offs, sp->d_colptr,
                                            (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4,
                                            mesh->d_area, d_alb, albedo, sp->d_rowidx, sp->d_cval);
     ILLUM_LAUNCH_CHECK();

@@ -115,6 +115,8 @@ enum WorkspaceSlot {
   WS_FF_VPOOL = 10, // raw visibility sets, when kept
   WS_FF_ALBEDO = 11,
   WS_FF_STATUS = 12,
+  WS_FF_POOL2 = 13, // kept receivers of the sources that did not fit the pool
+  WS_FF_ROWS = 14,  // those sources
 };
 
 // Size of the no-allocation solve's fixed per-GPU scratch array.

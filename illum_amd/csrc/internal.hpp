@@ -158,6 +158,9 @@ struct illum_bvh {
   int tune_leaf[32] = {0};
   int tune_layout[32] = {0};
   double tune_ms[32] = {0};
+  // nonzeros of the last form-factor assembly on this tree (sizes the next
+  // one's pool), 0 before the first
+  int64_t ff_nnz_last = 0;
 };
 
 constexpr int64_t CSR_PAD = 64;

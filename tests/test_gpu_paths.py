@@ -216,3 +216,23 @@ def test_autotuned_conduction_stream_layout_does_not_change_results(api, monkeyp
     assert np.array_equal(Ta, Tb, equal_nan=True)
     assert np.array_equal(sa, sb) and np.array_equal(fa, fb)
     assert any(k.startswith("cn_streams[" + precision) for k, *_ in api.default_context().tuning())
+
+
+@pytest.mark.parametrize("entries", [0, 1500])
+def test_pool_overflow_reruns_sources_into_a_second_pool(api, monkeypatch, entries):
+    # A first pool too small for the kept lists: the sources that did not
+    # fit run again into a second pool, and the assembly is identical to one
+    # whose pool held everything (CSC, CSR and the raw visibility sets).
+    V, F = api.synth_ellipsoid(4, ncraters=40, seed=31)
+    ic = api.IllumContext(V=V, F=F)
+    ref = ic.compute_F(0.2, keep_visibility=True)
+    rcsc, rcsr, rvis = ref.csc(), ref.csr(), ref.visibility_lists()
+    assert ref.nnz > 2 * entries
+    monkeypatch.setenv("ILLUM_FF_POOL_ENTRIES", str(entries))
+    Fm = ic.compute_F(0.2, keep_visibility=True)
+    for a, b in zip(Fm.csc(), rcsc):
+        assert np.array_equal(a, b)
+    for a, b in zip(Fm.csr(), rcsr):
+        assert np.array_equal(a, b)
+    for a, b in zip(Fm.visibility_lists(), rvis):
+        assert np.array_equal(a, b)
