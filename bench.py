@@ -420,6 +420,8 @@ def main():
     ap.add_argument("--thermal-steps", type=int, default=4096)
     ap.add_argument("--thermal-only", action="store_true",
                     help="only run the conduction leg and print its record")
+    ap.add_argument("--radiosity-only", action="store_true",
+                    help="only the radiosity leg (81,920 facets, S = 512)")
     ap.add_argument("--coupled-only", action="store_true",
                     help="only run the coupled time-loop leg (81,920 facets) and print its record")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -456,11 +458,14 @@ def main():
     log(f"mesh {nf} facets, bvh {illum.bvh.info()}, leaf autotune {illum.bvh.tuning()}, "
         f"setup {time.perf_counter() - t0:.1f}s")
 
-    if args.coupled_only:
+    if args.coupled_only or args.radiosity_only:
         Fm = illum.compute_F(albedo, offset)
         hz = api.DeviceArray(ctx, (nf, 361))
         illum.make_horizons(361, 1e-3, offset, out=hz)
-        print(json.dumps({"coupled": bench_coupled(api, illum, ctx, Fm, hz)}), flush=True)
+        if args.radiosity_only:
+            print(json.dumps({"radiosity": bench_radiosity(api, illum, ctx, Fm, hz)}), flush=True)
+        else:
+            print(json.dumps({"coupled": bench_coupled(api, illum, ctx, Fm, hz)}), flush=True)
         return
     tune = illum.bvh.tuning()
     walk = ({2: "two-wide", 3: "wavefront packet (two-wide)", 4: "four-wide",

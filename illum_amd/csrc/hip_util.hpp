@@ -210,6 +210,19 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
   return x * per + (x < rem ? x : rem) + k;
 }
 
+// Maps workgroup ids so each XCD takes chunks of C consecutive ids in turn:
+// XCD x (ids b = x mod 8, dispatched round-robin) gets chunks x, x + 8, ...
+// of the id range.  Neighbouring work stays together within a chunk while
+// every XCD draws from the whole range, so uneven work (a facet region with
+// many rays) spreads over all XCDs.  Ids past the last whole round of 8
+// chunks map to themselves.  Bijective on [0, n); C = 1 is the identity.
+__device__ __forceinline__ int xcd_chunked(int b, int n, int C) {
+  const int full = (n / (8 * C)) * 8 * C;
+  if (b >= full) return b;
+  const int x = b & 7, k = b >> 3;
+  return ((k / C) * 8 + x) * C + k % C;
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
   const unsigned lane = __lane_id();
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));

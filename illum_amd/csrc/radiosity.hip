@@ -117,7 +117,12 @@ struct MixedIO {
   const int *en;  // exponents of the next correction d_{k+1}
   _Float16 *Dn;   // d_{k+1}
   double *B;      // B mode: B += delta_k; null: the corrections stay in a ring
+  int xc;         // row groups per chunk dealt to the XCDs in turn (0: one range per XCD)
 };
+
+__device__ __forceinline__ int64_t sweep_block(int b, int n, int xc) {
+  return xc > 0 ? xcd_chunked(b, n, xc) : xcd_remap(b, n);
+}
 
 template <class TX>
 constexpr bool is_mixed() { return std::is_same<TX, _Float16>::value; }
@@ -130,7 +135,7 @@ __global__ __launch_bounds__(256) void sweep_rows_kernel(
     const double *__restrict__ val, const TX *__restrict__ X, const double *__restrict__ R,
     double *__restrict__ Y, const int *__restrict__ done, MixedIO m) {
   if (*done) return;
-  const int64_t row = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  const int64_t row = sweep_block(blockIdx.x, gridDim.x, m.xc) * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (row >= nf) return;
   const int64_t k0 = rowptr[row], k1 = rowptr[row + 1];
@@ -198,7 +203,7 @@ __global__ __launch_bounds__(256) void sweep_wide_kernel(
   typedef double yvec __attribute__((ext_vector_type(PER)));
   __shared__ yvec red[3][64];
   const int ch = blockIdx.x / ngrp8, gb = blockIdx.x - ch * ngrp8;
-  const int64_t g = xcd_remap(gb, ngrp8);
+  const int64_t g = sweep_block(gb, ngrp8, m.xc);
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   const int s = (ch * 64 + lane) * PER;
@@ -284,7 +289,7 @@ __global__ __launch_bounds__(256) void sweep_nnz_kernel(
   if (*done) return;  // uniform: the whole grid returns or none of it
   const int rib = threadIdx.x / LPR;  // row in block
   const int sub = threadIdx.x % LPR;
-  const int64_t row = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * SPMM_NNZ_ROWS + rib;
+  const int64_t row = sweep_block(blockIdx.x, gridDim.x, m.xc) * SPMM_NNZ_ROWS + rib;
   const bool valid = row < nf;
   const int64_t k0 = valid ? rowptr[row] : 0, k1 = valid ? rowptr[row + 1] : 0;
   for (int s = 0; s < S; ++s) {
@@ -723,6 +728,13 @@ int sweep_product(hipStream_t s, illum_ctx *ctx, const illum_sparse *F, int S, c
   StageTimer st(ctx, stage, timed);
   const int64_t nf = F->n;
   int nparts = 0;
+  // Row groups go to the XCDs as one contiguous range each (xc = 0), or
+  // dealt in chunks of xc groups.  FP64 products gather 8-byte rows of X and
+  // favour the locality of ranges; FP16 corrections favour the balance of
+  // dealing.  Results do not depend on it: the wide kernel tunes it, the
+  // others keep ranges.  ILLUM_RAD_XCD_CHUNK fixes it.
+  const char *xenv = getenv("ILLUM_RAD_XCD_CHUNK");
+  m.xc = xenv ? std::max(0, atoi(xenv)) : 0;
   if (S >= 128) {
     // PER: right-hand sides per lane.  The products of every PER are
     // identical (FMA accumulation in CSR order) and idempotent at S >= 128
@@ -759,6 +771,11 @@ int sweep_product(hipStream_t s, illum_ctx *ctx, const illum_sparse *F, int S, c
       per = autotune(ctx, std::string("spmm_per[") + (is_mixed<TX>() ? "fp16" : "fp64") +
                               ",nf=" + std::to_string(nf) + ",S=" + std::to_string(S) + "]",
                      cands, per, go, timed);
+    }
+    if (!xenv) {
+      m.xc = autotune(ctx, std::string("spmm_xcd[") + (is_mixed<TX>() ? "fp16" : "fp64") +
+                               ",nf=" + std::to_string(nf) + ",S=" + std::to_string(S) + "]",
+                      {0, 1, 16}, 0, [&](int xc) { m.xc = xc; go(per); }, timed);
     }
     go(per);
     nparts = ngrp8;

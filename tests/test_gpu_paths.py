@@ -185,10 +185,17 @@ def test_autotuned_launch_shapes_do_not_change_results(api, monkeypatch):
         B1, res1, it1 = api.radiosity_solve(Fm, E, method=method, rel_tol=1e-10)
         monkeypatch.delenv("ILLUM_SPMM_CHUNK_BYTES")
         assert it == it1 and np.array_equal(res, res1) and np.array_equal(B, B1)
+        # every dealing of row groups to the XCDs
+        for xc in ("0", "1", "3", "16"):
+            monkeypatch.setenv("ILLUM_RAD_XCD_CHUNK", xc)
+            B1, res1, it1 = api.radiosity_solve(Fm, E, method=method, rel_tol=1e-10)
+            assert it == it1 and np.array_equal(res, res1) and np.array_equal(B, B1), xc
+        monkeypatch.delenv("ILLUM_RAD_XCD_CHUNK")
     keys = [t[0] for t in api.default_context().tuning()]
     assert any(k.startswith("direct_lanes[") for k in keys)
     assert any(k.startswith("spmm_per[fp64") for k in keys)
     assert any(k.startswith("spmm_per[fp16") for k in keys)
+    assert any(k.startswith("spmm_xcd[fp64") for k in keys)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
@@ -236,3 +243,22 @@ def test_pool_overflow_reruns_sources_into_a_second_pool(api, monkeypatch, entri
         assert np.array_equal(a, b)
     for a, b in zip(Fm.visibility_lists(), rvis):
         assert np.array_equal(a, b)
+
+
+def test_visibility_launch_orders_give_identical_assemblies(api, monkeypatch):
+    # Sources dealt to the XCDs in chunks (default 8), as one range per XCD,
+    # in plain order, or with rays walked unsorted: the assembly is the same.
+    V, F = api.synth_ellipsoid(4, ncraters=60, seed=41)
+    ic = api.IllumContext(V=V, F=F)
+    ref = ic.compute_F(0.2)
+    rcsc, rcsr = ref.csc(), ref.csr()
+    for var, val in (("ILLUM_FF_XCD_CHUNK", "0"), ("ILLUM_FF_XCD_CHUNK", "1"),
+                     ("ILLUM_FF_XCD_CHUNK", "5"), ("ILLUM_FF_RAY_SORT", "0"),
+                     ("ILLUM_FF_RAY_CHUNK", "256"), ("ILLUM_FF_RAY_CHUNK", "1024")):
+        monkeypatch.setenv(var, val)
+        Fm = ic.compute_F(0.2)
+        for a, b in zip(Fm.csc(), rcsc):
+            assert np.array_equal(a, b), (var, val)
+        for a, b in zip(Fm.csr(), rcsr):
+            assert np.array_equal(a, b), (var, val)
+        monkeypatch.delenv(var)
