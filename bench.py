@@ -420,6 +420,8 @@ def main():
     ap.add_argument("--thermal-steps", type=int, default=4096)
     ap.add_argument("--thermal-only", action="store_true",
                     help="only run the conduction leg and print its record")
+    ap.add_argument("--c5-only", action="store_true", help="only the 64-body C5 leg")
+    ap.add_argument("--transfer-only", action="store_true", help="only the k-NN transfer leg")
     ap.add_argument("--radiosity-only", action="store_true",
                     help="only the radiosity leg (81,920 facets, S = 512)")
     ap.add_argument("--coupled-only", action="store_true",
@@ -447,6 +449,13 @@ def main():
     from illum_amd import api
     if args.thermal_only:
         print(json.dumps({"thermal": bench_thermal(api, nsteps=args.thermal_steps)}), flush=True)
+        return
+    if args.c5_only:
+        from oracle import oracle as O
+        print(json.dumps({"c5": bench_c5(api, O)}), flush=True)
+        return
+    if args.transfer_only:
+        print(json.dumps({"transfer": bench_transfer(api)}), flush=True)
         return
     offset = 1e-5
     t0 = time.perf_counter()
