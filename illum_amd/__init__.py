@@ -5,5 +5,5 @@ include/illum_hip.h); this package is the host-side binding.
 """
 from ._lib import IllumError, load  # noqa: F401
 from .api import (  # noqa: F401
-    BVH, Context, DeviceArray, IllumContext, Mesh, SparseMatrix, ThermalModel, conductionQ,
-    default_context, fib_spiral, radiosity_solve, synth_ellipsoid)
+    BVH, Context, DeviceArray, IllumContext, Mesh, SparseMatrix, ThermalModel, TimeLoop,
+    conductionQ, default_context, fib_spiral, radiosity_solve, synth_ellipsoid)
