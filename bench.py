@@ -421,6 +421,8 @@ def main():
     ap.add_argument("--thermal-only", action="store_true",
                     help="only run the conduction leg and print its record")
     ap.add_argument("--c5-only", action="store_true", help="only the 64-body C5 leg")
+    ap.add_argument("--horizons-only", action="store_true",
+                    help="only the horizons leg (81,920 facets, nphi 361)")
     ap.add_argument("--transfer-only", action="store_true", help="only the k-NN transfer leg")
     ap.add_argument("--radiosity-only", action="store_true",
                     help="only the radiosity leg (81,920 facets, S = 512)")
@@ -467,6 +469,11 @@ def main():
     log(f"mesh {nf} facets, bvh {illum.bvh.info()}, leaf autotune {illum.bvh.tuning()}, "
         f"setup {time.perf_counter() - t0:.1f}s")
 
+    if args.horizons_only:
+        from oracle import oracle as O
+        hz, r = bench_horizons(api, illum, ctx, O.geometry(V, F))
+        print(json.dumps({"horizons": r}), flush=True)
+        return
     if args.coupled_only or args.radiosity_only:
         Fm = illum.compute_F(albedo, offset)
         hz = api.DeviceArray(ctx, (nf, 361))

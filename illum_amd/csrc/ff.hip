@@ -35,9 +35,6 @@ namespace illum {
 constexpr int FF_BLOCK = 256;
 constexpr int FF_MAXPER = 4;      // clusters per wavefront per fill round, at most
 constexpr int FF_MAXCHUNK = 1024;  // rays per walk round, at most
-// Ray queue entry: direction key << FF_IDX_BITS | (receiver - body start).
-constexpr int FF_IDX_BITS = 21;
-constexpr int FF_IDX_MASK = (1 << FF_IDX_BITS) - 1;
 constexpr int FF_NKEY = 96;  // 6 permutation classes x 4 x 4 direction cells
 // Workgroups per CU the kernel is compiled for (8 wavefronts per SIMD: at
 // most 64 VGPRs); the LDS per workgroup is sized for it where the body allows.
@@ -197,12 +194,14 @@ __device__ __forceinline__ int block_exscan256(int v, int *s_wave, int *total) {
   return off + x - v;
 }
 
-// Block-wide counting sort of n <= FF_MAXCHUNK queue entries by their
-// direction key (bits FF_IDX_BITS and up) from `in` to `out`; the order
+// Block-wide counting sort of n <= FF_MAXCHUNK queue entries (receivers
+// relative to the body start lo) by the direction key of the ray from
+// (qx, qy, qz) to the receiver's centroid, from `in` to `out`; the order
 // within a key is arbitrary (LDS atomics).  `hist` holds FF_NKEY ints.
 // Called by the whole block; ends with a barrier.
 __device__ void sort_rays_by_key(const int32_t *in, int n, int32_t *out, int *hist,
-                                 int *s_wave) {
+                                 int *s_wave, const float4 *__restrict__ pr4, int lo, float qx,
+                                 float qy, float qz) {
   constexpr int PER = FF_MAXCHUNK / FF_BLOCK;
   static_assert(FF_NKEY <= 2 * FF_BLOCK, "two histogram buckets per thread");
   const int tid = threadIdx.x;
@@ -217,7 +216,8 @@ __device__ void sort_rays_by_key(const int32_t *in, int n, int32_t *out, int *hi
     rk[m] = 0;
     if (e < n) {
       val[m] = in[e];
-      key[m] = (int)((uint32_t)val[m] >> FF_IDX_BITS);
+      const float4 p = pr4[lo + val[m]];
+      key[m] = ray_dir_key(p.x - qx, p.y - qy, p.z - qz);
       rk[m] = atomicAdd(&hist[key[m]], 1);
     }
   }
@@ -297,6 +297,8 @@ __global__ __launch_bounds__(FF_BLOCK, FF_WAVES_PER_SIMD) void ff_visibility_ker
   uint16_t *clist = (uint16_t *)(hist + (a.qsort ? FF_NKEY : 0));
   int *stack = (int *)(clist + ((a.nclmax + 1) & ~1));
   __shared__ int s_qn, s_rays, s_nc;
+  __shared__ float4 s_pn[2];  // source centroid, normal
+  __shared__ int4 s_dims;     // body start and end, first cluster, source
   __shared__ int s_scan[FF_BLOCK / 64];
   __shared__ long long s_base;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -319,6 +321,11 @@ __global__ __launch_bounds__(FF_BLOCK, FF_WAVES_PER_SIMD) void ff_visibility_ker
   const float4 pj = a.pr4[j], nj = a.n4[j];
   const float qx = pj.x + a.offset * nj.x, qy = pj.y + a.offset * nj.y,
               qz = pj.z + a.offset * nj.z;
+  if (tid == 0) {
+    s_pn[0] = pj;
+    s_pn[1] = nj;
+    s_dims = make_int4(lo, hi, clo, j);
+  }
   __syncthreads();
 
   // Cluster pass: a cluster of 64 consecutive receivers whose bound proves
@@ -340,42 +347,47 @@ __global__ __launch_bounds__(FF_BLOCK, FF_WAVES_PER_SIMD) void ff_visibility_ker
     if (m) {
       int slot = 0;
       if (lane == 0) slot = atomicAdd(&s_nc, __popcll(m));
-      slot = __shfl(slot, 0);
-      if (keep) clist[slot + __popcll(m & lanemask_lt())] = (uint16_t)(c - clo);
+      slot = __builtin_amdgcn_readfirstlane(slot);
+      if (keep) clist[slot + rank_in_mask(m)] = (uint16_t)(c - clo);
     }
   }
   __syncthreads();
-  const int ncl = s_nc;
+  const int ncl = __builtin_amdgcn_readfirstlane(s_nc);
 
   // Per-pair facing test over the listed clusters, 4 fper clusters (one per
-  // wave and k) per round; survivors go to the LDS ray queue with their
-  // direction keys.
+  // wave and k) per round; survivors go to the LDS ray queue.
   const int cpr = a.fper * (FF_BLOCK / 64);
   for (int g0 = 0; g0 < ncl; g0 += cpr) {
+    // The source's point and normal and the body's range, read from LDS for
+    // the round: kept in scalar registers they are spilled across the walks
+    // and reloaded for every candidate.
+    const float4 vp = s_pn[0], vn = s_pn[1];
+    const int4 vd = s_dims;
+    const float vpx = vp.x, vpy = vp.y, vpz = vp.z, vnx = vn.x, vny = vn.y, vnz = vn.z;
+    const int vlo = vd.x, vhi = vd.y, vclo = vd.z, vj = vd.w;
 #pragma unroll
     for (int k = 0; k < FF_MAXPER; ++k) {
       if (k >= a.fper) break;
       const int ci = g0 + k * (FF_BLOCK / 64) + wid;
-      const int i = ci < ncl ? (clo + (int)clist[ci]) * 64 + lane : hi;
-      bool s = false;
-      int key = 0;
-      if (i >= lo && i < hi && i != j) {
-        ++w.npair;
-        const float4 pi = a.pr4[i];
-        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float dot = (dx * nj.x + dy * nj.y) + dz * nj.z;
-        s = !(dot <= -pi.w);
-        if (a.qsort) key = ray_dir_key(pi.x - qx, pi.y - qy, pi.z - qz);
-      }
-      const uint64_t m = wave_ballot(s);
+      const int i = ci < ncl ? (vclo + (int)clist[ci]) * 64 + lane : vhi;
+      const bool valid = i >= vlo && i < vhi && i != vj;
+      if (valid) ++w.npair;
+      // Every lane evaluates the test (a lane without a candidate reads its
+      // own source and compares -inf, which fails), so the survivors' mask
+      // is one comparison.
+      const float4 pi = a.pr4[valid ? i : vj];
+      const float dx = pi.x - vpx, dy = pi.y - vpy, dz = pi.z - vpz;
+      const float dot = valid ? (dx * vnx + dy * vny) + dz * vnz : -__builtin_huge_valf();
+      const bool sv = !(dot <= -pi.w);
+      const uint64_t m = wave_ballot(sv);
       if (m) {
         int slot = 0;
         if (lane == 0) {
           slot = atomicAdd(&s_qn, __popcll(m));
           atomicAdd(&s_rays, __popcll(m));
         }
-        slot = __shfl(slot, 0);
-        if (s) queue[slot + __popcll(m & lanemask_lt())] = (key << FF_IDX_BITS) | (i - lo);
+        slot = __builtin_amdgcn_readfirstlane(slot);
+        if (sv) queue[slot + rank_in_mask(m)] = i - vlo;
       }
     }
     __syncthreads();
@@ -387,13 +399,13 @@ __global__ __launch_bounds__(FF_BLOCK, FF_WAVES_PER_SIMD) void ff_visibility_ker
       const int n = qrem < a.qchunk ? qrem : a.qchunk;
       const int *src = queue + (qrem - n);
       if (a.qsort) {
-        sort_rays_by_key(src, n, sorted, hist, s_scan);
+        sort_rays_by_key(src, n, sorted, hist, s_scan, a.pr4, lo, qx, qy, qz);
         src = sorted;
       }
       for (int w0 = wid * 64; w0 < n; w0 += FF_BLOCK) {
         const int e = w0 + lane;
         const bool act = e < n;
-        const int i = act ? lo + (src[e] & FF_IDX_MASK) : j;
+        const int i = act ? lo + src[e] : j;
         const float4 pi = a.pr4[i];
         const DRay r = make_ray(qx, qy, qz, pi.x - qx, pi.y - qy, pi.z - qz);
         float t;
@@ -566,7 +578,7 @@ __global__ __launch_bounds__(256) void ff_overflow_rows_kernel(
   if (lane == (int)__builtin_ctzll(m)) slot = atomicAdd(nrows, __popcll(m));
   slot = __shfl(slot, (int)__builtin_ctzll(m));
   if (over) {
-    rows[slot + __popcll(m & lanemask_lt())] = (int32_t)j;
+    rows[slot + rank_in_mask(m)] = (int32_t)j;
     atomicAdd(need, (unsigned long long)counts[j]);
   }
 }
@@ -858,7 +870,7 @@ static size_t ff_prepare(const illum_mesh *mesh, const void *kfn, int stack_slot
                          const QShape &q) {
   const BodyDims d = body_dims(mesh);
   const size_t lds = ff_lds_bytes(d, stack_slots, q);
-  if (lds > 160 * 1024 || d.nclmax > 65536 || 32 * (int64_t)d.nwmax > FF_IDX_MASK + 1)
+  if (lds > 160 * 1024 || d.nclmax > 65536)
     throw Error(ILLUM_ERR_CAPACITY,
                 "a body has too many facets for the LDS visibility bitset (max ~1.2M per body)");
   if (lds > 64 * 1024)

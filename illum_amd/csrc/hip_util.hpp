@@ -223,6 +223,13 @@ __device__ __forceinline__ int xcd_chunked(int b, int n, int C) {
   return ((k / C) * 8 + x) * C + k % C;
 }
 
+// Number of set bits of m in the lanes below this one (a lane's place among
+// the ballot's set lanes): two mbcnt instructions, no mask constant.
+__device__ __forceinline__ int rank_in_mask(uint64_t m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
   const unsigned lane = __lane_id();
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
