@@ -155,6 +155,46 @@ __device__ __forceinline__ bool tri_hit(const DRay &r, float4 a, float4 b, float
   return true;
 }
 
+// Whether tri_hit(r, a, b, c) reports a hit, for any-hit queries that need
+// no distance: the same edge functions and sign tests, then t = T / det > 0
+// decided without dividing.  t > 0 exactly when T and det have one sign and
+// the quotient does not round to zero, i.e. |T| > 2^-150 |det|; that is
+// tested as (|T| 2^75) 2^75 > |det|, whose two power-of-two scalings are
+// exact (an overflow to +inf still compares greater).  FP32 denormals are
+// kept (the kernels' FP mode).
+__device__ __forceinline__ bool tri_any(const DRay &r, float4 a, float4 b, float4 c) {
+  const float Ax0 = a.x - r.ox, Ay0 = a.y - r.oy, Az0 = a.z - r.oz;
+  const float Bx0 = b.x - r.ox, By0 = b.y - r.oy, Bz0 = b.z - r.oz;
+  const float Cx0 = c.x - r.ox, Cy0 = c.y - r.oy, Cz0 = c.z - r.oz;
+  const float Akx = sel3(Ax0, Ay0, Az0, r.kx), Aky = sel3(Ax0, Ay0, Az0, r.ky),
+              Akz = sel3(Ax0, Ay0, Az0, r.kz);
+  const float Bkx = sel3(Bx0, By0, Bz0, r.kx), Bky = sel3(Bx0, By0, Bz0, r.ky),
+              Bkz = sel3(Bx0, By0, Bz0, r.kz);
+  const float Ckx = sel3(Cx0, Cy0, Cz0, r.kx), Cky = sel3(Cx0, Cy0, Cz0, r.ky),
+              Ckz = sel3(Cx0, Cy0, Cz0, r.kz);
+  const float Ax = Akx - r.Sx * Akz, Ay = Aky - r.Sy * Akz;
+  const float Bx = Bkx - r.Sx * Bkz, By = Bky - r.Sy * Bkz;
+  const float Cx = Ckx - r.Sx * Ckz, Cy = Cky - r.Sy * Ckz;
+  float U = Cx * By - Cy * Bx;
+  float V = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
+    double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
+    double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
+    U = (float)(CxBy - CyBx);
+    V = (float)(AxCy - AyCx);
+    W = (float)(BxAy - ByAx);
+  }
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+  const float det = (U + V) + W;
+  if (det == 0.0f) return false;
+  const float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
+  const float T = (U * Az + V * Bz) + W * Cz;
+  if (det > 0.0f ? !(T > 0.0f) : !(T < 0.0f)) return false;
+  return (fabsf(T) * 0x1p75f) * 0x1p75f > fabsf(det);
+}
+
 // Closest hit over the threaded tree starting at node `root`; returns the
 // original facet index or -1.  nbox / ntri count the box and triangle tests.
 __device__ __forceinline__ int trace_closest(const float4 *__restrict__ nodes,
@@ -553,8 +593,8 @@ __device__ __forceinline__ int trace_any_wide4(const float4 *__restrict__ w4,
       const int first = info & 0xFFFFFF, cnt = info >> 24;
       for (int k = 0; k < cnt; ++k) {
         float t;
-        if (tri_hit(r, tris[3 * (first + k)], tris[3 * (first + k) + 1],
-                    tris[3 * (first + k) + 2], &t))
+        if (tri_any(r, tris[3 * (first + k)], tris[3 * (first + k) + 1],
+                    tris[3 * (first + k) + 2]))
           return 1;
       }
     }
@@ -594,8 +634,8 @@ __device__ __forceinline__ bool trace_any_wide(const float4 *__restrict__ wide,
       const int first = info & 0xFFFFFF, cnt = info >> 24;
       for (int k = 0; k < cnt; ++k) {
         float t;
-        if (tri_hit(r, tris[3 * (first + k)], tris[3 * (first + k) + 1],
-                    tris[3 * (first + k) + 2], &t))
+        if (tri_any(r, tris[3 * (first + k)], tris[3 * (first + k) + 1],
+                    tris[3 * (first + k) + 2]))
           return true;
       }
     }
@@ -653,8 +693,8 @@ __device__ __forceinline__ bool trace_any_packet(const float4 *__restrict__ wide
       const int first = info & 0xFFFFFF, cnt = info >> 24;
       for (int k = 0; k < cnt; ++k) {
         float t;
-        const bool h = tri_hit(r, tris[3 * (first + k)], tris[3 * (first + k) + 1],
-                               tris[3 * (first + k) + 2], &t);
+        const bool h = tri_any(r, tris[3 * (first + k)], tris[3 * (first + k) + 1],
+                               tris[3 * (first + k) + 2]);
         if (live && h) hit = true;
       }
     }
@@ -679,8 +719,8 @@ __device__ __forceinline__ bool trace_any(const float4 *__restrict__ nodes,
     const int first = info & 0xFFFFFF, cnt = info >> 24;
     for (int k = 0; k < cnt; ++k) {
       float t;
-      if (tri_hit(r, tris[3 * (first + k)], tris[3 * (first + k) + 1],
-                  tris[3 * (first + k) + 2], &t))
+      if (tri_any(r, tris[3 * (first + k)], tris[3 * (first + k) + 1],
+                  tris[3 * (first + k) + 2]))
         return true;
     }
     node = miss;
