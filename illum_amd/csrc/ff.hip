@@ -28,6 +28,7 @@
 #include <cmath>
 #include <algorithm>
 #include <cstdlib>
+#include <array>
 #include <string>
 
 namespace illum {
@@ -74,6 +75,14 @@ struct FFArgs {
   int fper;     // clusters per wavefront per fill round (<= FF_MAXPER)
   int qcap;     // ray queue entries: qchunk + one fill round (fper * 256)
   int qsort;    // 1: each round's rays are ordered by direction key first
+  int clcap;    // cluster-list entries (the clusters are passed in chunks of it)
+  // Paged bitset (PAGED launches): page table entries (pages of 1,024
+  // receivers of the largest body), pool pages; sources that touch more
+  // pages than the pool holds are listed in over_rows (count in *over_n)
+  // with nothing else written, for a launch with a larger pool.
+  int npgmax, npool;
+  int32_t *over_rows;
+  int *over_n;
   int xchunk;   // sources per chunk dealt to the XCDs in turn (0: one range per XCD)
   float offset;
   double albedo;
@@ -101,14 +110,15 @@ struct FFArgs {
   int64_t vpool_cap;
 };
 
-// Appends the set bits of an nw-word LDS row (bit b of word w = facet
-// lo + 32 w + b) to a pool in ascending order: threads own contiguous words,
-// a block scan of their popcounts places each thread's entries, one atomic
-// reserves the block's segment.  *cnt_out / *off_out receive the count and
-// the segment's start plus off_base; nothing is written past cap (the host
-// then runs the sources that did not fit again, into a second pool).
-// Called by the whole block.
-__device__ void append_bits_sorted(const uint32_t *bits, int nw, int lo, int32_t *pool,
+// Appends the set bits of an nw-word LDS row to a pool in ascending order:
+// word w is word_at(w), its bit b is facet base_at(w) + b, and the words'
+// facets ascend with w.  Threads own contiguous words, a block scan of their
+// popcounts places each thread's entries, one atomic reserves the block's
+// segment.  *cnt_out / *off_out receive the count and the segment's start
+// plus off_base; nothing is written past cap (the host then runs the sources
+// that did not fit again, into a second pool).  Called by the whole block.
+template <class WordAt, class BaseAt>
+__device__ void append_bits_sorted(WordAt word_at, BaseAt base_at, int nw, int32_t *pool,
                                    unsigned long long *pool_n, int64_t cap, int64_t off_base,
                                    int32_t *cnt_out, int64_t *off_out, int *s_scan,
                                    long long *s_base) {
@@ -116,7 +126,7 @@ __device__ void append_bits_sorted(const uint32_t *bits, int nw, int lo, int32_t
   const int per = (nw + FF_BLOCK - 1) / FF_BLOCK;
   const int w0 = min(nw, tid * per), w1 = min(nw, w0 + per);
   int c = 0;
-  for (int w = w0; w < w1; ++w) c += __popc(bits[w]);
+  for (int w = w0; w < w1; ++w) c += __popc(word_at(w));
   int incl = c;
   for (int o = 1; o < 64; o <<= 1) {
     const int y = __shfl_up(incl, o);
@@ -141,11 +151,12 @@ __device__ void append_bits_sorted(const uint32_t *bits, int nw, int lo, int32_t
   if (base + total <= cap) {
     int32_t *out = pool + base + off;
     for (int w = w0; w < w1; ++w) {
-      uint32_t rest = bits[w];
+      uint32_t rest = word_at(w);
+      const int fb = base_at(w);
       while (rest) {
         const int b = __ffs(rest) - 1;
         rest &= rest - 1u;
-        *out++ = lo + 32 * w + b;
+        *out++ = fb + b;
       }
     }
   }
@@ -284,19 +295,30 @@ __device__ __forceinline__ int packet_walk(const FFArgs &a, const DRay &r, bool 
 // tree, 3 / 5 = wavefront packet walks over the two- / four-wide tree, 0 =
 // stackless threaded walk.  COUNT adds the work counters (pair, box and
 // triangle tests) to stats[1..3]; stats[0] always receives the rays.
-template <int LAYOUT, bool COUNT>
+// PAGED keeps the visibility set in pages of 1,024 receivers allocated on
+// first hit instead of one bit per receiver of the body, so the LDS does not
+// grow with the body: for bodies whose bitset would limit the workgroups per
+// CU.
+template <int LAYOUT, bool COUNT, bool PAGED>
 __global__ __launch_bounds__(FF_BLOCK, FF_WAVES_PER_SIMD) void ff_visibility_kernel(FFArgs a) {
   extern __shared__ uint32_t smem[];
+  // LDS: the visibility set -- a bitset over the body (nwmax words), or
+  // (PAGED) the page table (npgmax entries, -1 = no page), each pool page's
+  // index and page number when listed, and the pool pages (32 words each) --
+  // then the ray queue, the sorted walk round and key histogram (qsort), the
+  // cluster list (16-bit offsets from its chunk's first cluster) and
+  // scap x FF_BLOCK stack slots (LAYOUT 2 or 4).
   uint32_t *bits = smem;
-  int *queue = (int *)(smem + a.nwmax);
-  // with qsort: the sorted walk round and the key histogram
+  int *ptab = (int *)smem;
+  int *pidx = ptab + a.npgmax;
+  int *ppage = pidx + a.npool;
+  uint32_t *pw = (uint32_t *)(ppage + a.npool);
+  int *queue = PAGED ? (int *)(pw + 32 * a.npool) : (int *)(smem + a.nwmax);
   int *sorted = queue + a.qcap;
   int *hist = sorted + (a.qsort ? a.qchunk : 0);
-  // surviving receiver clusters as 16-bit offsets from the body's first
-  // cluster, then scap x FF_BLOCK stack slots (LAYOUT 2 or 4)
   uint16_t *clist = (uint16_t *)(hist + (a.qsort ? FF_NKEY : 0));
-  int *stack = (int *)(clist + ((a.nclmax + 1) & ~1));
-  __shared__ int s_qn, s_rays, s_nc;
+  int *stack = (int *)(clist + ((a.clcap + 1) & ~1));
+  __shared__ int s_qn, s_rays, s_nc, s_np, s_over;
   __shared__ float4 s_pn[2];  // source centroid, normal
   __shared__ int4 s_dims;     // body start and end, first cluster, source
   __shared__ int s_scan[FF_BLOCK / 64];
@@ -307,13 +329,18 @@ __global__ __launch_bounds__(FF_BLOCK, FF_WAVES_PER_SIMD) void ff_visibility_ker
                               : xcd_remap(blockIdx.x, a.nrows);
   const int row = a.rows ? a.rows[r0] : r0;
   const int j = row * a.jstride;
-  // The source's body: receivers [lo, hi) (bit i - lo of the LDS row), its
-  // trees' roots.
+  // The source's body: receivers [lo, hi), its trees' roots.
   const int body = a.body_of ? a.body_of[j] : 0;
   const int lo = a.body_off[body], hi = a.body_off[body + 1];
   const int nw = (hi - lo + 31) >> 5;
-  for (int w = tid; w < nw; w += FF_BLOCK) bits[w] = 0u;
-  if (tid == 0) { s_qn = 0; s_rays = 0; s_nc = 0; }
+  const int npg = (hi - lo + 1023) >> 10;
+  if (PAGED) {
+    for (int p = tid; p < npg; p += FF_BLOCK) ptab[p] = -1;
+    for (int q = tid; q < 32 * a.npool; q += FF_BLOCK) pw[q] = 0u;
+  } else {
+    for (int w = tid; w < nw; w += FF_BLOCK) bits[w] = 0u;
+  }
+  if (tid == 0) { s_qn = 0; s_rays = 0; s_nc = 0; s_np = 0; s_over = 0; }
   FFWork w;  // work counters of this thread
   const int root_thr = a.roots[4 * body], root_wide = a.roots[4 * body + 1];
   const int root_wide4 = a.roots[4 * body + 2];
@@ -328,74 +355,48 @@ __global__ __launch_bounds__(FF_BLOCK, FF_WAVES_PER_SIMD) void ff_visibility_ker
   }
   __syncthreads();
 
-  // Cluster pass: a cluster of 64 consecutive receivers whose bound proves
-  // every member fails the facing test is dropped whole (the bound and its
-  // rounding margin are documented with cull_clusters in mesh.cpp); the
-  // others are listed for the per-pair test.  clus == nullptr keeps all.
-  for (int c0 = clo; c0 < chi; c0 += FF_BLOCK) {
-    const int c = c0 + tid;
-    bool keep = false;
-    if (c < chi) {
-      keep = true;
-      if (a.clus) {
-        const float4 cc = a.clus[c];
-        const float t = ((cc.x - pj.x) * nj.x + (cc.y - pj.y) * nj.y) + (cc.z - pj.z) * nj.z;
-        keep = !(t <= -cc.w);
-      }
-    }
-    const uint64_t m = wave_ballot(keep);
-    if (m) {
-      int slot = 0;
-      if (lane == 0) slot = atomicAdd(&s_nc, __popcll(m));
-      slot = __builtin_amdgcn_readfirstlane(slot);
-      if (keep) clist[slot + rank_in_mask(m)] = (uint16_t)(c - clo);
-    }
-  }
-  __syncthreads();
-  const int ncl = __builtin_amdgcn_readfirstlane(s_nc);
-
-  // Per-pair facing test over the listed clusters, 4 fper clusters (one per
-  // wave and k) per round; survivors go to the LDS ray queue.
-  const int cpr = a.fper * (FF_BLOCK / 64);
-  for (int g0 = 0; g0 < ncl; g0 += cpr) {
-    // The source's point and normal and the body's range, read from LDS for
-    // the round: kept in scalar registers they are spilled across the walks
-    // and reloaded for every candidate.
-    const float4 vp = s_pn[0], vn = s_pn[1];
-    const int4 vd = s_dims;
-    const float vpx = vp.x, vpy = vp.y, vpz = vp.z, vnx = vn.x, vny = vn.y, vnz = vn.z;
-    const int vlo = vd.x, vhi = vd.y, vclo = vd.z, vj = vd.w;
-#pragma unroll
-    for (int k = 0; k < FF_MAXPER; ++k) {
-      if (k >= a.fper) break;
-      const int ci = g0 + k * (FF_BLOCK / 64) + wid;
-      const int i = ci < ncl ? (vclo + (int)clist[ci]) * 64 + lane : vhi;
-      const bool valid = i >= vlo && i < vhi && i != vj;
-      if (valid) ++w.npair;
-      // Every lane evaluates the test (a lane without a candidate reads its
-      // own source and compares -inf, which fails), so the survivors' mask
-      // is one comparison.
-      const float4 pi = a.pr4[valid ? i : vj];
-      const float dx = pi.x - vpx, dy = pi.y - vpy, dz = pi.z - vpz;
-      const float dot = valid ? (dx * vnx + dy * vny) + dz * vnz : -__builtin_huge_valf();
-      const bool sv = !(dot <= -pi.w);
-      const uint64_t m = wave_ballot(sv);
-      if (m) {
-        int slot = 0;
-        if (lane == 0) {
-          slot = atomicAdd(&s_qn, __popcll(m));
-          atomicAdd(&s_rays, __popcll(m));
+  // Marks receiver lo + rel as hit on the lanes with `hit`; called by the
+  // whole wavefront.  PAGED: a page is taken from the pool on its first hit,
+  // by one lane for all lanes of the wavefront that hit it; an LDS
+  // compare-and-swap settles races between the block's 4 wavefronts, and the
+  // index drawn by a loser is not reused (at most 3 lost per page: a source
+  // hitting up to npool / 4 pages always fits).  A source needing more pages
+  // than the pool holds is flagged.  (Claiming the entry before drawing the
+  // index loses none but made the walk loop 25 % slower.)
+  auto set_hit = [&](bool hit, int rel) {
+    if (PAGED) {
+      const int p = rel >> 10;
+      int pg = hit ? ptab[p] : 0;
+      uint64_t m = wave_ballot(hit && pg < 0);
+      while (m) {
+        const int l = (int)__builtin_ctzll(m);
+        const int pl = __builtin_amdgcn_readlane(p, l);
+        int got = 0;
+        if (lane == l) {
+          const int mine = atomicAdd(&s_np, 1);
+          const int want = mine < a.npool ? mine : a.npool;
+          const int old = atomicCAS(&ptab[pl], -1, want);
+          got = old == -1 ? want : old;
         }
-        slot = __builtin_amdgcn_readfirstlane(slot);
-        if (sv) queue[slot + rank_in_mask(m)] = i - vlo;
+        got = __builtin_amdgcn_readlane(got, l);
+        const bool same = hit && pg < 0 && p == pl;
+        if (same) pg = got;
+        m &= ~wave_ballot(same);
       }
+      if (hit) {
+        if (pg >= a.npool) s_over = 1;
+        else atomicOr(&pw[32 * pg + ((rel >> 5) & 31)], 1u << (rel & 31));
+      }
+    } else if (hit) {
+      atomicOr(&bits[rel >> 5], 1u << (rel & 31));
     }
-    __syncthreads();
-    const bool last = g0 + cpr >= ncl;
+  };
+
+  // Walk rounds of up to qchunk rays from the queue's end (all of them with
+  // flush); each wavefront takes 64 consecutive entries at a time.
+  auto walk_rounds = [&](bool flush) {
     int qrem = s_qn;
-    // Walk rounds of up to qchunk rays from the queue's end; each wavefront
-    // takes 64 consecutive entries at a time.
-    while (qrem >= a.qchunk || (last && qrem > 0)) {
+    while (qrem >= a.qchunk || (flush && qrem > 0)) {
       const int n = qrem < a.qchunk ? qrem : a.qchunk;
       const int *src = queue + (qrem - n);
       if (a.qsort) {
@@ -441,72 +442,191 @@ __global__ __launch_bounds__(FF_BLOCK, FF_WAVES_PER_SIMD) void ff_visibility_ker
           }
         }
         // h lies in [lo, hi): the walk covers only the body's tree
-        if (act && h >= 0 && h != j) atomicOr(&bits[(h - lo) >> 5], 1u << ((h - lo) & 31));
+        set_hit(act && h >= 0 && h != j, h - lo);
       }
       qrem -= n;
       __syncthreads();  // the round's entries are read before the next round or fill
     }
     if (tid == 0) s_qn = qrem;
     __syncthreads();
-  }
-
-  // The raw visibility set, on request.
-  if (a.vpool)
-    append_bits_sorted(bits, nw, lo, a.vpool, a.vpool_n, a.vpool_cap, 0, a.vcounts + row,
-                       a.voffs + row, s_scan, &s_base);
-
-  // Nonzero-form-factor filter.  A row's set bits cluster in a few words, so
-  // their form factors are evaluated one per thread from a list (in the now
-  // idle ray queue) rather than by the words' owners.
-  //   1. Each thread reads and clears its words and lists the set bits; bits
-  //      beyond the queue's capacity are evaluated by the owner at once, into
-  //      its own (cleared) word.
-  //   2. The listed receivers are evaluated, setting their filter bits by LDS
-  //      atomics (no word is read as a visibility word any more).
-  // The filter bits and counts do not depend on who evaluates what.
-  const double Aj = a.area[j];
-  bool nan = false;
-  auto filter = [&](int i) -> bool {
-    const double rho = a.albedo_vec ? a.albedo_vec[i] : a.albedo;
-    const double v = ff_value(a.pr4[i], a.n4[i], pj, nj, Aj) * rho;
-    if (v != v) nan = true;
-    if (v == 0.0) return false;
-    // receiver i's row count of the transposed matrix (the CSR row
-    // lengths), so no pass over the transpose is needed for them
-    if (a.row_counts) atomicAdd(&a.row_counts[i], 1);
-    return true;
   };
-  if (tid == 0) s_qn = 0;
-  __syncthreads();
-  for (int w = tid; w < nw; w += FF_BLOCK) {
-    const uint32_t word = bits[w];
-    uint32_t keep = 0u;
-    if (word) {
-      int slot = atomicAdd(&s_qn, __popc(word));
-      uint32_t rest = word;
-      while (rest) {
-        const int b = __ffs(rest) - 1;
-        rest &= rest - 1u;
-        const int i = lo + 32 * w + b;
-        if (slot < a.qcap) queue[slot] = i;
-        else if (filter(i)) keep |= 1u << b;
-        ++slot;
+
+  // The receiver clusters in chunks of clcap (the whole body when its list
+  // fits).  Cluster pass: a cluster of 64 consecutive receivers whose bound
+  // proves every member fails the facing test is dropped whole (the bound
+  // and its rounding margin are documented with cull_clusters in mesh.cpp);
+  // the others are listed for the per-pair test.  clus == nullptr keeps all.
+  const int cpr = a.fper * (FF_BLOCK / 64);
+  for (int cc0 = clo; cc0 < chi; cc0 += a.clcap) {
+    const int cc1 = min(chi, cc0 + a.clcap);
+    for (int c0 = cc0; c0 < cc1; c0 += FF_BLOCK) {
+      const int c = c0 + tid;
+      bool keep = false;
+      if (c < cc1) {
+        keep = true;
+        if (a.clus) {
+          const float4 cc = a.clus[c];
+          const float t = ((cc.x - pj.x) * nj.x + (cc.y - pj.y) * nj.y) + (cc.z - pj.z) * nj.z;
+          keep = !(t <= -cc.w);
+        }
+      }
+      const uint64_t m = wave_ballot(keep);
+      if (m) {
+        int slot = 0;
+        if (lane == 0) slot = atomicAdd(&s_nc, __popcll(m));
+        slot = __builtin_amdgcn_readfirstlane(slot);
+        if (keep) clist[slot + rank_in_mask(m)] = (uint16_t)(c - cc0);
       }
     }
-    bits[w] = keep;
+    __syncthreads();
+    const int ncl = __builtin_amdgcn_readfirstlane(s_nc);
+
+    // Per-pair facing test over the listed clusters, 4 fper clusters (one
+    // per wave and k) per round; survivors go to the LDS ray queue.
+    for (int g0 = 0; g0 < ncl; g0 += cpr) {
+      // The source's point and normal and the body's range, read from LDS
+      // for the round: kept in scalar registers they are spilled across the
+      // walks and reloaded for every candidate.
+      const float4 vp = s_pn[0], vn = s_pn[1];
+      const int4 vd = s_dims;
+      const float vpx = vp.x, vpy = vp.y, vpz = vp.z, vnx = vn.x, vny = vn.y, vnz = vn.z;
+      const int vlo = vd.x, vhi = vd.y, vj = vd.w;
+#pragma unroll
+      for (int k = 0; k < FF_MAXPER; ++k) {
+        if (k >= a.fper) break;
+        const int ci = g0 + k * (FF_BLOCK / 64) + wid;
+        const int i = ci < ncl ? (cc0 + (int)clist[ci]) * 64 + lane : vhi;
+        const bool valid = i >= vlo && i < vhi && i != vj;
+        if (valid) ++w.npair;
+        // Every lane evaluates the test (a lane without a candidate reads
+        // its own source and compares -inf, which fails), so the survivors'
+        // mask is one comparison.
+        const float4 pi = a.pr4[valid ? i : vj];
+        const float dx = pi.x - vpx, dy = pi.y - vpy, dz = pi.z - vpz;
+        const float dot = valid ? (dx * vnx + dy * vny) + dz * vnz : -__builtin_huge_valf();
+        const bool sv = !(dot <= -pi.w);
+        const uint64_t m = wave_ballot(sv);
+        if (m) {
+          int slot = 0;
+          if (lane == 0) {
+            slot = atomicAdd(&s_qn, __popcll(m));
+            atomicAdd(&s_rays, __popcll(m));
+          }
+          slot = __builtin_amdgcn_readfirstlane(slot);
+          if (sv) queue[slot + rank_in_mask(m)] = i - vlo;
+        }
+      }
+      __syncthreads();
+      walk_rounds(false);
+    }
+    if (tid == 0) s_nc = 0;  // the next chunk's list
+    __syncthreads();
   }
-  __syncthreads();
-  const int nlist = min(s_qn, a.qcap);
-  for (int e = tid; e < nlist; e += FF_BLOCK) {
-    const int i = queue[e];
-    if (filter(i)) atomicOr(&bits[(i - lo) >> 5], 1u << ((i - lo) & 31));
+  walk_rounds(true);
+
+  // PAGED: a source that ran out of pool pages writes no lists (zero counts,
+  // where the launch has them); it is listed for a launch with a bitset over
+  // its body.  Its rays and work are counted here.
+  const bool over = PAGED && __builtin_amdgcn_readfirstlane(s_over);
+  if (over) {
+    if (tid == 0) {
+      if (a.counts) a.counts[row] = 0;
+      if (a.offs) a.offs[row] = 0;
+      if (a.vcounts) a.vcounts[row] = 0;
+      if (a.voffs) a.voffs[row] = 0;
+      if (a.over_rows) a.over_rows[atomicAdd(a.over_n, 1)] = row;
+    }
+  } else {
+    // PAGED: the pool pages in ascending page order (ppage, pidx), by a block
+    // scan over the page table; word w of the set is word w & 31 of listed
+    // page w >> 5.
+    int nvw = nw;
+    if (PAGED) {
+      const int per = (npg + FF_BLOCK - 1) / FF_BLOCK;
+      const int p0 = min(npg, tid * per), p1 = min(npg, p0 + per);
+      int c = 0;
+      for (int p = p0; p < p1; ++p) c += ptab[p] >= 0 ? 1 : 0;
+      int total;
+      int off = block_exscan256(c, s_scan, &total);
+      for (int p = p0; p < p1; ++p)
+        if (ptab[p] >= 0) {
+          ppage[off] = p;
+          pidx[off] = ptab[p];
+          ++off;
+        }
+      __syncthreads();
+      nvw = 32 * total;
+    }
+    auto word_ref = [&](int vw) -> uint32_t & {
+      return PAGED ? pw[32 * pidx[vw >> 5] + (vw & 31)] : bits[vw];
+    };
+    auto word_at = [&](int vw) -> uint32_t { return word_ref(vw); };
+    auto base_at = [&](int vw) -> int {
+      return PAGED ? lo + (ppage[vw >> 5] << 10) + ((vw & 31) << 5) : lo + 32 * vw;
+    };
+    auto facet_word = [&](int rel) -> uint32_t & {
+      return PAGED ? pw[32 * ptab[rel >> 10] + ((rel >> 5) & 31)] : bits[rel >> 5];
+    };
+
+    // The raw visibility set, on request.
+    if (a.vpool)
+      append_bits_sorted(word_at, base_at, nvw, a.vpool, a.vpool_n, a.vpool_cap, 0,
+                         a.vcounts + row, a.voffs + row, s_scan, &s_base);
+
+    // Nonzero-form-factor filter.  A row's set bits cluster in a few words, so
+    // their form factors are evaluated one per thread from a list (in the now
+    // idle ray queue) rather than by the words' owners.
+    //   1. Each thread reads and clears its words and lists the set bits; bits
+    //      beyond the queue's capacity are evaluated by the owner at once, into
+    //      its own (cleared) word.
+    //   2. The listed receivers are evaluated, setting their filter bits by LDS
+    //      atomics (no word is read as a visibility word any more).
+    // The filter bits and counts do not depend on who evaluates what.
+    const double Aj = a.area[j];
+    bool nan = false;
+    auto filter = [&](int i) -> bool {
+      const double rho = a.albedo_vec ? a.albedo_vec[i] : a.albedo;
+      const double v = ff_value(a.pr4[i], a.n4[i], pj, nj, Aj) * rho;
+      if (v != v) nan = true;
+      if (v == 0.0) return false;
+      // receiver i's row count of the transposed matrix (the CSR row
+      // lengths), so no pass over the transpose is needed for them
+      if (a.row_counts) atomicAdd(&a.row_counts[i], 1);
+      return true;
+    };
+    if (tid == 0) s_qn = 0;
+    __syncthreads();
+    for (int vw = tid; vw < nvw; vw += FF_BLOCK) {
+      const uint32_t word = word_ref(vw);
+      uint32_t keep = 0u;
+      if (word) {
+        int slot = atomicAdd(&s_qn, __popc(word));
+        uint32_t rest = word;
+        const int fb = base_at(vw);
+        while (rest) {
+          const int b = __ffs(rest) - 1;
+          rest &= rest - 1u;
+          const int i = fb + b;
+          if (slot < a.qcap) queue[slot] = i;
+          else if (filter(i)) keep |= 1u << b;
+          ++slot;
+        }
+      }
+      word_ref(vw) = keep;
+    }
+    __syncthreads();
+    const int nlist = min(s_qn, a.qcap);
+    for (int e = tid; e < nlist; e += FF_BLOCK) {
+      const int i = queue[e];
+      if (filter(i)) atomicOr(&facet_word(i - lo), 1u << ((i - lo) & 31));
+    }
+    if (nan) atomicOr(a.status, 1);
+    __syncthreads();
+    // The kept receivers, ascending.
+    if (a.pool)
+      append_bits_sorted(word_at, base_at, nvw, a.pool, a.pool_n, a.pool_cap, a.off_base,
+                         a.counts + row, a.offs + row, s_scan, &s_base);
   }
-  if (nan) atomicOr(a.status, 1);
-  __syncthreads();
-  // The kept receivers, ascending.
-  if (a.pool)
-    append_bits_sorted(bits, nw, lo, a.pool, a.pool_n, a.pool_cap, a.off_base, a.counts + row,
-                       a.offs + row, s_scan, &s_base);
   if (a.stats && tid == 0) atomicAdd(a.stats, (unsigned long long)s_rays);
   if (COUNT && a.stats) {
     // pair tests, box tests, triangle tests: wave sums, one atomic each;
@@ -664,52 +784,76 @@ __global__ __launch_bounds__(256) void max_count_kernel(int64_t n, const int32_t
 }
 
 // CSR rows longer than min_len: one block per row.  The row's unsorted
-// source indices set bits of an LDS bitmap over the row's body (F is
-// block-diagonal: sources and receiver share a body), which is listed in
-// order; each value is evaluated again with ff_value on the same receiver and
-// source as the CSC entry, so it is bit-identical.  Shorter rows return at
-// once (csr_sort_kernel orders them).
+// source indices set bits of an LDS bitmap over a window of the row's index
+// range [min, max] (at most 32 * wwords sources a window; the windows are
+// taken in order), which is listed in order; each value is evaluated again
+// with ff_value on the same receiver and source as the CSC entry, so it is
+// bit-identical.  Shorter rows return at once (csr_sort_kernel orders them).
 __global__ __launch_bounds__(256) void csr_long_kernel(
-    int64_t min_len, const int64_t *__restrict__ rowptr, const int32_t *__restrict__ tj,
-    const int32_t *__restrict__ body_of, const int32_t *__restrict__ body_off,
-    const float4 *__restrict__ pr4, const float4 *__restrict__ n4,
-    const double *__restrict__ area, const double *__restrict__ albedo_vec, double albedo,
-    int32_t *__restrict__ colidx, double *__restrict__ rval) {
+    int64_t min_len, int wwords, const int64_t *__restrict__ rowptr,
+    const int32_t *__restrict__ tj, const float4 *__restrict__ pr4,
+    const float4 *__restrict__ n4, const double *__restrict__ area,
+    const double *__restrict__ albedo_vec, double albedo, int32_t *__restrict__ colidx,
+    double *__restrict__ rval) {
   extern __shared__ uint32_t bm[];
   __shared__ int s_wave[4];
+  __shared__ int s_min[4], s_max[4];
   const int64_t i = blockIdx.x;
   const int64_t base = rowptr[i], len = rowptr[i + 1] - base;
   if (len <= min_len) return;  // uniform over the block
-  const int tid = threadIdx.x;
-  const int body = body_of ? body_of[i] : 0;
-  const int lo = body_off[body], hi = body_off[body + 1];
-  const int nw = (hi - lo + 31) >> 5;
-  for (int w = tid; w < nw; w += 256) bm[w] = 0u;
-  __syncthreads();
+  const int tid = threadIdx.x, lane = tid & 63;
+  // the row's index range
+  int mn = 0x7fffffff, mx = -1;
   for (int64_t e = tid; e < len; e += 256) {
-    const int q = tj[base + e] - lo;
-    atomicOr(&bm[q >> 5], 1u << (q & 31));
+    const int q = tj[base + e];
+    mn = min(mn, q);
+    mx = max(mx, q);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    mn = min(mn, __shfl_xor(mn, o));
+    mx = max(mx, __shfl_xor(mx, o));
+  }
+  if (lane == 0) {
+    s_min[tid >> 6] = mn;
+    s_max[tid >> 6] = mx;
   }
   __syncthreads();
-  const int per = (nw + 255) / 256;
-  const int w0 = min(nw, tid * per), w1 = min(nw, w0 + per);
-  int c = 0;
-  for (int w = w0; w < w1; ++w) c += __popc(bm[w]);
-  int total;
-  const int off = block_exscan256(c, s_wave, &total);
+  const int rmin = min(min(s_min[0], s_min[1]), min(s_min[2], s_min[3]));
+  const int rmax = max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
   const float4 pi = pr4[i], ni = n4[i];
   const double rho = albedo_vec ? albedo_vec[i] : albedo;
-  int64_t pos = base + off;
-  for (int w = w0; w < w1; ++w) {
-    uint32_t rest = bm[w];
-    while (rest) {
-      const int b = __ffs(rest) - 1;
-      rest &= rest - 1u;
-      const int j = lo + 32 * w + b;
-      colidx[pos] = j;
-      rval[pos] = ff_value(pi, ni, pr4[j], n4[j], area[j]) * rho;
-      ++pos;
+  int64_t pos = base;
+  const int64_t wbits = 32 * (int64_t)wwords;
+  for (int64_t w0 = rmin; w0 <= rmax; w0 += wbits) {
+    const int64_t span = rmax - w0 + 1 < wbits ? rmax - w0 + 1 : wbits;
+    const int nw = (int)((span + 31) >> 5);
+    for (int w = tid; w < nw; w += 256) bm[w] = 0u;
+    __syncthreads();
+    for (int64_t e = tid; e < len; e += 256) {
+      const int64_t q = tj[base + e] - w0;
+      if (q >= 0 && q < 32 * (int64_t)nw) atomicOr(&bm[q >> 5], 1u << (q & 31));
     }
+    __syncthreads();
+    const int per = (nw + 255) / 256;
+    const int v0 = min(nw, tid * per), v1 = min(nw, v0 + per);
+    int c = 0;
+    for (int w = v0; w < v1; ++w) c += __popc(bm[w]);
+    int total;
+    const int off = block_exscan256(c, s_wave, &total);
+    int64_t p = pos + off;
+    for (int w = v0; w < v1; ++w) {
+      uint32_t rest = bm[w];
+      while (rest) {
+        const int b = __ffs(rest) - 1;
+        rest &= rest - 1u;
+        const int j = (int)(w0 + 32 * w + b);
+        colidx[p] = j;
+        rval[p] = ff_value(pi, ni, pr4[j], n4[j], area[j]) * rho;
+        ++p;
+      }
+    }
+    pos += total;
+    __syncthreads();  // the bitmap is cleared for the next window
   }
 }
 
@@ -785,19 +929,24 @@ static int ff_stack_slots(const illum_bvh *bvh, int layout) {
   return 0;  // threaded walk; the wavefront walk keeps its stack in a VGPR
 }
 
+template <bool COUNT, bool PAGED>
+static decltype(&ff_visibility_kernel<4, COUNT, PAGED>) ff_kernel_t(int layout) {
+  return layout == 5   ? ff_visibility_kernel<5, COUNT, PAGED>
+         : layout == 4 ? ff_visibility_kernel<4, COUNT, PAGED>
+         : layout == 3 ? ff_visibility_kernel<3, COUNT, PAGED>
+         : layout == 2 ? ff_visibility_kernel<2, COUNT, PAGED>
+                       : ff_visibility_kernel<0, COUNT, PAGED>;
+}
 template <bool COUNT>
-static decltype(&ff_visibility_kernel<4, COUNT>) ff_kernel(int layout) {
-  return layout == 5   ? ff_visibility_kernel<5, COUNT>
-         : layout == 4 ? ff_visibility_kernel<4, COUNT>
-         : layout == 3 ? ff_visibility_kernel<3, COUNT>
-         : layout == 2 ? ff_visibility_kernel<2, COUNT>
-                       : ff_visibility_kernel<0, COUNT>;
+static decltype(&ff_visibility_kernel<4, COUNT, false>) ff_kernel(int layout, bool paged) {
+  return paged ? ff_kernel_t<COUNT, true>(layout) : ff_kernel_t<COUNT, false>(layout);
 }
 
-// Per-source LDS of a mesh's bodies: the largest body's bitset words and
-// cluster count (clusters overlapping its facet range).
+// Per-source LDS of a mesh's bodies: the largest body's bitset words, pages
+// of 1,024 receivers and cluster count (clusters overlapping its facet
+// range).
 struct BodyDims {
-  int nwmax = 0, nclmax = 0;
+  int nwmax = 0, npgmax = 0, nclmax = 0;
 };
 static BodyDims body_dims(const illum_mesh *mesh) {
   BodyDims d;
@@ -805,24 +954,43 @@ static BodyDims body_dims(const illum_mesh *mesh) {
   for (size_t b = 0; b + 1 < off.size(); ++b) {
     const int64_t lo = off[b], hi = off[b + 1];
     d.nwmax = std::max<int>(d.nwmax, (int)((hi - lo + 31) / 32));
+    d.npgmax = std::max<int>(d.npgmax, (int)((hi - lo + 1023) / 1024));
     d.nclmax = std::max<int>(d.nclmax, (int)((hi + 63) / 64 - lo / 64));
   }
   return d;
 }
 
-// Ray queue of a launch: rays per walk round, clusters per wavefront per fill
-// round, capacity (a walk round plus a fill round), and whether each round is
-// sorted by direction.
+// ILLUM_FF_PAGED: -1 unset (the shape chooser and the overflow rate
+// decide), 0 bitset only, 1 paged set whenever a launch can take one.
+static int ff_paged_env() {
+  const char *pg = getenv("ILLUM_FF_PAGED");
+  return pg ? (atoi(pg) ? 1 : 0) : -1;
+}
+
+// Pool pages of a paged launch: 32 or 16 (4 or 2 KB of visibility words).
+// A source touching more pages of receivers runs again with a bitset over
+// its body.  On icosphere-ordered meshes of 81,920 and 327,680 facets no
+// source needs more than 16 (8 leave a few hundred sources over).
+constexpr int FF_NPOOL_MAX = 32, FF_NPOOL_MIN = 16;
+
+// LDS shape of a launch: bitset or paged set (pool pages), ray queue (rays
+// per walk round, clusters per wavefront per fill round, capacity: a walk
+// round plus a fill round), whether each round is sorted by direction, and
+// the cluster-list entries.
 struct QShape {
   int chunk = FF_BLOCK, fper = 1, cap = 2 * FF_BLOCK, sort = 0;
+  int paged = 0, npool = 0, clcap = 0;
 };
 
-// Dynamic LDS of ff_visibility_kernel: bitset, ray queue, sorted round and
-// key histogram (when sorting), 16-bit cluster list (rounded to whole words),
+// Dynamic LDS of ff_visibility_kernel: visibility set (bitset, or page
+// table, two page lists and the pool), ray queue, sorted round and key
+// histogram (when sorting), 16-bit cluster list (rounded to whole words),
 // stack.
 static size_t ff_lds_bytes(const BodyDims &d, int stack_slots, const QShape &q) {
-  return (size_t)(d.nwmax + q.cap) * 4 + (q.sort ? (size_t)(q.chunk + FF_NKEY) * 4 : 0) +
-         (size_t)((d.nclmax + 1) & ~1) * 2 + (size_t)stack_slots * FF_BLOCK * 4;
+  const size_t set = q.paged ? (size_t)(d.npgmax + 2 * q.npool + 32 * q.npool) * 4
+                             : (size_t)d.nwmax * 4;
+  return set + (size_t)q.cap * 4 + (q.sort ? (size_t)(q.chunk + FF_NKEY) * 4 : 0) +
+         (size_t)((q.clcap + 1) & ~1) * 2 + (size_t)stack_slots * FF_BLOCK * 4;
 }
 
 // Workgroups of one CU that a launch's LDS allows (160 KB per CU, in 512-byte
@@ -832,12 +1000,20 @@ static int ff_groups_per_cu(size_t dyn) {
   return (int)std::min<size_t>(FF_WAVES_PER_SIMD, 160 * 1024 / per);
 }
 
-// The queue shape: among walk rounds of 1024 / 512 / 256 rays and fill
-// rounds of 4 / 2 / 1 clusters per wavefront, the one with the most
-// workgroups per CU, then the longest round, then the longest fill round.
-// Results do not depend on it.  ILLUM_FF_RAY_SORT=0 walks rays in queue
-// order; ILLUM_FF_RAY_CHUNK (256 / 512 / 1024) fixes the round size.
-static QShape ff_qshape(const BodyDims &d, int stack_slots) {
+// The LDS shape: among a bitset or a paged set (pool of 32 or 16 pages),
+// cluster lists of the whole body or chunks of 1,024, walk rounds of 1024 /
+// 512 / 256 rays and fill rounds of 4 / 2 / 1 clusters per wavefront, the
+// one with the most workgroups per CU, then the longest walk round (rays of
+// a longer round sort into more coherent wavefronts: 6.1 against 6.7 ms at
+// 81,920 facets), the longest fill round, a bitset, the larger pool and the
+// longest list.  Results do not depend on it.  ILLUM_FF_RAY_SORT=0 walks
+// rays in queue order; ILLUM_FF_RAY_CHUNK (256 / 512 / 1024) fixes the round
+// size; ILLUM_FF_PAGED=0 / 1 fixes the set and ILLUM_FF_NPOOL the pages of
+// its pool.
+//   full: the shape that runs sources out of pool pages again -- a bitset,
+// or for a body whose bitset exceeds the LDS the paged set with the largest
+// pool that fits (then the most workgroups, ...).
+static QShape ff_qshape(const BodyDims &d, int stack_slots, bool full = false) {
   const char *e = getenv("ILLUM_FF_RAY_SORT");
   const int sort = !(e && std::string(e) == "0");
   int fixed = 0;
@@ -845,23 +1021,55 @@ static QShape ff_qshape(const BodyDims &d, int stack_slots) {
     const int v = atoi(c);
     fixed = v >= 1024 ? 1024 : v >= 512 ? 512 : 256;
   }
+  const int paged_only = ff_paged_env();
+  int npool_fixed = 0;
+  if (!full)
+    if (const char *np = getenv("ILLUM_FF_NPOOL")) npool_fixed = std::max(1, atoi(np));
   QShape best;
-  int best_groups = -1;
-  for (int chunk = FF_MAXCHUNK; chunk >= FF_BLOCK; chunk /= 2) {
-    if (fixed && chunk != fixed) continue;
-    for (int fper = FF_MAXPER; fper >= 1; fper /= 2) {
-      QShape q;
-      q.chunk = chunk;
-      q.fper = fper;
-      q.cap = chunk + fper * FF_BLOCK;
-      q.sort = sort;
-      const int g = ff_groups_per_cu(ff_lds_bytes(d, stack_slots, q));
-      if (g > best_groups) {
-        best = q;
-        best_groups = g;
+  bool found = false;
+  std::array<int, 6> best_score{};
+  auto consider = [&](int paged, int np) {
+    for (int clcap : {d.nclmax, std::min(d.nclmax, 1024)}) {
+      if (clcap > 65536) continue;  // 16-bit list offsets
+      for (int chunk = FF_MAXCHUNK; chunk >= FF_BLOCK; chunk /= 2) {
+        if (fixed && chunk != fixed) continue;
+        for (int fper = FF_MAXPER; fper >= 1; fper /= 2) {
+          QShape q;
+          q.chunk = chunk;
+          q.fper = fper;
+          q.cap = chunk + fper * FF_BLOCK;
+          q.sort = sort;
+          q.paged = paged;
+          q.npool = np;
+          q.clcap = clcap;
+          const size_t lds = ff_lds_bytes(d, stack_slots, q);
+          if (lds > 160 * 1024) continue;
+          const int g = ff_groups_per_cu(lds);
+          const std::array<int, 6> score =
+              full ? std::array<int, 6>{np, g, chunk, fper, clcap, 0}
+                   : std::array<int, 6>{g, chunk, fper, !paged, np, clcap};
+          if (!found || score > best_score) {
+            best = q;
+            best_score = score;
+            found = true;
+          }
+        }
       }
     }
+  };
+  if (full || paged_only != 1) consider(0, 0);
+  if (full) {
+    // no bitset fits: the largest pool, from the body's page count down
+    if (!found && paged_only != 0)
+      for (int np = d.npgmax; np >= 1 && !found; np = np > FF_NPOOL_MAX ? np * 3 / 4 : 0)
+        consider(1, np);
+  } else if (paged_only != 0) {
+    if (npool_fixed) consider(1, std::min(npool_fixed, d.npgmax));
+    else
+      for (int np : {FF_NPOOL_MAX, FF_NPOOL_MIN}) consider(1, std::min(np, d.npgmax));
   }
+  if (!found)
+    throw Error(ILLUM_ERR_CAPACITY, "a body has too many facets for the LDS visibility set");
   return best;
 }
 
@@ -870,9 +1078,8 @@ static size_t ff_prepare(const illum_mesh *mesh, const void *kfn, int stack_slot
                          const QShape &q) {
   const BodyDims d = body_dims(mesh);
   const size_t lds = ff_lds_bytes(d, stack_slots, q);
-  if (lds > 160 * 1024 || d.nclmax > 65536)
-    throw Error(ILLUM_ERR_CAPACITY,
-                "a body has too many facets for the LDS visibility bitset (max ~1.2M per body)");
+  if (lds > 160 * 1024 || q.clcap > 65536)
+    throw Error(ILLUM_ERR_CAPACITY, "a body has too many facets for the LDS visibility set");
   if (lds > 64 * 1024)
     ILLUM_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   return lds;
@@ -900,6 +1107,9 @@ static FFArgs ff_args(const illum_bvh *bvh, int nrows, int jstride, int stack_sl
   if (const char *e = getenv("ILLUM_FF_XCD_CHUNK")) a.xchunk = std::max(0, atoi(e));
   a.qcap = q.cap;
   a.qsort = q.sort;
+  a.clcap = q.clcap;
+  a.npgmax = d.npgmax;
+  a.npool = q.npool;
   a.offset = (float)offset;
   a.albedo = albedo;
   a.pr4 = (const float4 *)mesh->d_pr4;
@@ -961,9 +1171,31 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   const int layout = ff_layout(bvh);
   // traversal stack: one slot per tree level (a push happens at most once per level)
   const int stack_slots = ff_stack_slots(bvh, layout);
-  auto launch = ff_kernel<false>(layout);
-  const QShape qs = ff_qshape(body_dims(mesh), stack_slots);
-  const size_t lds = ff_prepare(mesh, (const void *)launch, stack_slots, qs);
+  const BodyDims dims = body_dims(mesh);
+  // The shape that runs sources again (into the second pool, or after
+  // running out of pool pages): a bitset, or for a body whose bitset exceeds
+  // the LDS a paged set with the largest pool that fits.
+  const QShape qf = ff_qshape(dims, stack_slots, true);
+  auto launch_f = ff_kernel<false>(layout, qf.paged);
+  const size_t lds_f = ff_prepare(mesh, (const void *)launch_f, stack_slots, qf);
+  // The main shape.  Where a bitset fits, a paged set is dropped for this
+  // tree once more than 1/64 of a pass's sources ran out of pool pages
+  // (meshes whose facet order scatters a source's visible receivers over
+  // many pages), unless ILLUM_FF_PAGED=1.
+  const bool paged_forced = ff_paged_env() == 1;
+  QShape qs = ff_qshape(dims, stack_slots);
+  if (qs.paged && !qf.paged && bvh->ff_paged_off && !paged_forced) qs = qf;
+  auto launch = ff_kernel<false>(layout, qs.paged);
+  size_t lds = ff_prepare(mesh, (const void *)launch, stack_slots, qs);
+  auto paged_overflow = [&](int64_t nover, int64_t nrows) {
+    if (nover * 64 > nrows && !qf.paged && !paged_forced) bvh->ff_paged_off = true;
+  };
+  auto rerun_overflow = [&](int n) {
+    if (n > 0)
+      throw Error(ILLUM_ERR_CAPACITY,
+                  "a source's visible receivers span more pages of 1,024 facets than the "
+                  "LDS holds");
+  };
   const bool keep_vis = flags & ILLUM_FF_KEEP_VISIBILITY;
 
   std::unique_ptr<illum_sparse, void (*)(illum_sparse *)> sp(new illum_sparse(), destroy_sparse);
@@ -981,6 +1213,8 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   struct Ctl {
     int status, maxrow, maxcol, nover;
     unsigned long long pool_n, vpool_n, need, pool2_n;
+    // sources out of pool pages (paged launches): main pass, reruns
+    int nover_pg, nover_pg2;
   };
   Ctl *ctl = (Ctl *)ctx_workspace(ctx, WS_FF_STATUS, sizeof(Ctl));
   // Per-source counts and pool offsets, CSR row lengths: 24 bytes per facet
@@ -997,12 +1231,15 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
   // an estimate from a counted sample of every 32nd source (nothing written)
   // plus 25 %; at least what the context's workspace already holds.  Sources
   // whose lists still do not fit run again into a second pool sized exactly.
-  auto base_args = [&](int nrows, int jstride) {
-    FFArgs a = ff_args(bvh, nrows, jstride, stack_slots, qs, offset, d_alb, albedo);
+  int32_t *over_rows = (int32_t *)ctx_workspace(ctx, WS_FF_OVER, 4 * (size_t)nf);
+  auto base_args = [&](int nrows, int jstride, const QShape &q) {
+    FFArgs a = ff_args(bvh, nrows, jstride, stack_slots, q, offset, d_alb, albedo);
     a.status = &ctl->status;
     a.clus = cluster_cull() ? (const float4 *)mesh->d_clus : nullptr;
     a.counts = counts;
     a.offs = offs;
+    a.over_rows = over_rows;
+    a.over_n = &ctl->nover_pg;
     return a;
   };
   int64_t want = 0;
@@ -1012,7 +1249,7 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
     const int js = nf >= 32 * 64 ? 32 : 1;
     const int nrows = (int)((nf + js - 1) / js);
     ILLUM_HIP(hipMemsetAsync(ctl, 0, sizeof(Ctl), s));
-    FFArgs a = base_args(nrows, js);
+    FFArgs a = base_args(nrows, js, qs);
     a.pool = counts;  // capacity 0: only the counts are written
     a.pool_n = &ctl->pool_n;
     a.pool_cap = 0;
@@ -1023,6 +1260,18 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
     sync(ctx);
     const double est = (double)h0.pool_n * (double)nf / (double)nrows;
     want = (int64_t)(est * 1.25) + 4096;
+    if (qs.paged) {
+      // sources out of pool pages count nothing in the sample: with many of
+      // them the pass keeps a bitset, and the estimate is scaled up
+      paged_overflow(h0.nover_pg, nrows);
+      if (bvh->ff_paged_off && !qf.paged) {
+        qs = qf;
+        launch = launch_f;
+        lds = lds_f;
+      }
+      const int64_t ok = std::max<int64_t>(1, nrows - h0.nover_pg);
+      want = (int64_t)(est * 1.25 * (double)nrows / (double)ok) + 4096;
+    }
   }
   int64_t cap = pool_entries(ctx, WS_FF_POOL, want);
   // ILLUM_FF_POOL_ENTRIES fixes the first pool's capacity (to exercise the
@@ -1039,7 +1288,7 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
     ILLUM_HIP(hipMemsetAsync(rcounts, 0, 4 * nf, s));
     ILLUM_HIP(hipMemsetAsync(ctl, 0, sizeof(Ctl), s));
     ILLUM_HIP(hipMemsetAsync(sp->d_nrays, 0, 8, s));
-    FFArgs a = base_args((int)nf, 1);
+    FFArgs a = base_args((int)nf, 1, qs);
     a.stats = sp->d_nrays;
     a.row_counts = rcounts;
     a.pool = pool;
@@ -1054,6 +1303,35 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
       StageTimer st(ctx, "ff_visibility");
       launch<<<(int)nf, FF_BLOCK, lds, s>>>(a);
       ILLUM_LAUNCH_CHECK();
+    }
+    if (qs.paged) {
+      // Sources that ran out of pool pages wrote nothing: they run again
+      // with a set over their whole body (their rays are already counted).
+      int nov = 0;
+      ILLUM_HIP(hipMemcpyAsync(&nov, &ctl->nover_pg, 4, hipMemcpyDeviceToHost, s));
+      sync(ctx);
+      if (nov < 0 || nov > nf) throw Error(ILLUM_ERR_HIP, "form factors: bad page-overflow count");
+      paged_overflow(nov, nf);
+      if (nov > 0) {
+        // (a paged rerun that runs out of pages again lists the source in
+        // the pool-overflow rows, free until the pass ends, and fails)
+        FFArgs af = base_args(nov, 1, qf);
+        af.rows = over_rows;
+        af.over_rows = (int32_t *)ctx_workspace(ctx, WS_FF_ROWS, 4 * (size_t)nf);
+        af.over_n = &ctl->nover_pg2;
+        af.row_counts = rcounts;
+        af.pool = pool;
+        af.pool_n = &ctl->pool_n;
+        af.pool_cap = cap;
+        af.vcounts = vcounts;
+        af.voffs = voffs;
+        af.vpool = vpool;
+        af.vpool_n = &ctl->vpool_n;
+        af.vpool_cap = vcap;
+        StageTimer st(ctx, "ff_visibility_rerun");
+        launch_f<<<nov, FF_BLOCK, lds_f, s>>>(af);
+        ILLUM_LAUNCH_CHECK();
+      }
     }
     {
       StageTimer st(ctx, "ff_csc");
@@ -1070,6 +1348,7 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
     ILLUM_HIP(hipMemcpyAsync(&nnz, sp->d_colptr + nf, 8, hipMemcpyDeviceToHost, s));
     ILLUM_HIP(hipMemcpyAsync(&h, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
     sync(ctx);
+    rerun_overflow(h.nover_pg2);
     if (h.status) throw Error(ILLUM_ERR_NAN, "computed a NaN form factor");
     if (nnz < 0 || (unsigned long long)nnz != h.pool_n)
       throw Error(ILLUM_ERR_HIP, "form factors: inconsistent pool count");
@@ -1088,19 +1367,21 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
         throw Error(ILLUM_ERR_HIP, "form factors: inconsistent overflow count");
       cap2 = (int64_t)h.need;
       pool2 = (int32_t *)ctx_workspace(ctx, WS_FF_POOL2, 4 * (size_t)std::max<int64_t>(1, cap2));
-      FFArgs a2 = base_args(h.nover, 1);
+      FFArgs a2 = base_args(h.nover, 1, qf);
       a2.rows = orows;
+      a2.over_n = &ctl->nover_pg2;  // over_rows: free again
       a2.pool = pool2;
       a2.pool_n = &ctl->pool2_n;
       a2.pool_cap = cap2;
       a2.off_base = cap;
       {
-        StageTimer st(ctx, "ff_visibility");
-        launch<<<h.nover, FF_BLOCK, lds, s>>>(a2);
+        StageTimer st(ctx, "ff_visibility_rerun");
+        launch_f<<<h.nover, FF_BLOCK, lds_f, s>>>(a2);
         ILLUM_LAUNCH_CHECK();
       }
       ILLUM_HIP(hipMemcpyAsync(&h, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
       sync(ctx);
+      rerun_overflow(h.nover_pg2);
       if ((int64_t)h.pool2_n != cap2)
         throw Error(ILLUM_ERR_HIP, "form factors: second pool count differs");
     }
@@ -1157,12 +1438,15 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
                                        sp->d_rval);
     ILLUM_LAUNCH_CHECK();
     if (h.maxrow > sort_max) {
-      const size_t blds = 4 * (size_t)body_dims(mesh).nwmax;
-      if (blds > 64 * 1024)
-        ILLUM_HIP(hipFuncSetAttribute((const void *)csr_long_kernel,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)blds));
-      csr_long_kernel<<<(unsigned)nf, 256, blds, s>>>(
-          sort_max, sp->d_rowptr, tj, mesh->d_body, mesh->d_boff, (const float4 *)mesh->d_pr4,
+      // bitmap windows of up to 16,384 words (524,288 sources: one window
+      // for any row of a body that size or smaller); ILLUM_FF_CSR_WINDOW
+      // (words) narrows them, so that rows take several windows (results are
+      // identical)
+      int wwords = std::max(1, std::min(body_dims(mesh).nwmax, 16384));
+      if (const char *e = getenv("ILLUM_FF_CSR_WINDOW"))
+        wwords = std::max(1, std::min(wwords, atoi(e)));
+      csr_long_kernel<<<(unsigned)nf, 256, 4 * (size_t)wwords, s>>>(
+          sort_max, wwords, sp->d_rowptr, tj, (const float4 *)mesh->d_pr4,
           (const float4 *)mesh->d_n4, mesh->d_area, d_alb, albedo, sp->d_colidx, sp->d_rval);
       ILLUM_LAUNCH_CHECK();
     }
@@ -1183,8 +1467,8 @@ double time_visibility_sample(illum_bvh *bvh, int max_sources, int layout_choice
   const int nrows = (int)((nf + jstride - 1) / jstride);
   const int layout = ff_resolve(bvh, layout_choice);
   const int stack_slots = ff_stack_slots(bvh, layout);
-  auto launch = ff_kernel<false>(layout);
   const QShape qs = ff_qshape(body_dims(mesh), stack_slots);
+  auto launch = ff_kernel<false>(layout, qs.paged);
   const size_t lds = ff_prepare(mesh, (const void *)launch, stack_slots, qs);
   // kept lists of the sampled rows, 64 entries per row (a row that does not
   // fit is counted and not written)
@@ -1192,7 +1476,8 @@ double time_visibility_sample(illum_bvh *bvh, int max_sources, int layout_choice
   DBuf<int32_t> counts(nrows), pool(cap);
   DBuf<int64_t> offs(nrows);
   DBuf<unsigned long long> pool_n(1);
-  DBuf<int> status(1);
+  DBuf<int> status(1), over_n(1);
+  DBuf<int32_t> over_rows(nrows);
   hipEvent_t e0, e1;
   ILLUM_HIP(hipEventCreate(&e0));
   ILLUM_HIP(hipEventCreate(&e1));
@@ -1204,9 +1489,12 @@ double time_visibility_sample(illum_bvh *bvh, int max_sources, int layout_choice
   a.pool = pool.p;
   a.pool_n = pool_n.p;
   a.pool_cap = cap;
+  a.over_rows = over_rows.p;  // sources out of pool pages are timed, not rerun
+  a.over_n = over_n.p;
   float best = 1e30f;
   for (int rep = 0; rep < 2; ++rep) {  // first launch also loads the code object
     ILLUM_HIP(hipMemsetAsync(pool_n.p, 0, 8, s));
+    ILLUM_HIP(hipMemsetAsync(over_n.p, 0, 4, s));
     ILLUM_HIP(hipEventRecord(e0, s));
     launch<<<nrows, FF_BLOCK, lds, s>>>(a);
     ILLUM_LAUNCH_CHECK();
@@ -1235,16 +1523,20 @@ void form_factor_work(illum_bvh *bvh, double offset, int64_t *out, int n) {
   const int64_t nf = mesh->g.nf;
   const int layout = ff_layout(bvh);
   const int stack_slots = ff_stack_slots(bvh, layout);
-  auto launch = ff_kernel<true>(layout);
   const QShape qs = ff_qshape(body_dims(mesh), stack_slots);
+  auto launch = ff_kernel<true>(layout, qs.paged);
   const size_t lds = ff_prepare(mesh, (const void *)launch, stack_slots, qs);
-  DBuf<int> status(1);
+  DBuf<int> status(1), over_n(1);
+  DBuf<int32_t> over_rows(nf);
+  ILLUM_HIP(hipMemsetAsync(over_n.p, 0, 4, stream_of(ctx)));
   DBuf<unsigned long long> stats(8);
   ILLUM_HIP(hipMemsetAsync(stats.p, 0, 64, s));
   FFArgs a = ff_args(bvh, (int)nf, 1, stack_slots, qs, offset, nullptr, 1.0);
   a.status = status.p;
   a.stats = stats.p;
   a.clus = cluster_cull() ? (const float4 *)mesh->d_clus : nullptr;
+  a.over_rows = over_rows.p;  // sources out of pool pages are counted, not rerun
+  a.over_n = over_n.p;
   launch<<<(int)nf, FF_BLOCK, lds, s>>>(a);
   ILLUM_LAUNCH_CHECK();
   unsigned long long hs[8];

@@ -117,6 +117,7 @@ enum WorkspaceSlot {
   WS_FF_STATUS = 12,
   WS_FF_POOL2 = 13, // kept receivers of the sources that did not fit the pool
   WS_FF_ROWS = 14,  // those sources
+  WS_FF_OVER = 15,  // sources of a paged launch that ran out of pool pages
 };
 
 // Size of the no-allocation solve's fixed per-GPU scratch array.

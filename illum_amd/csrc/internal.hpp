@@ -161,6 +161,9 @@ struct illum_bvh {
   // nonzeros of the last form-factor assembly on this tree (sizes the next
   // one's pool), 0 before the first
   int64_t ff_nnz_last = 0;
+  // set once more than 1/64 of the sources of a paged visibility pass on
+  // this tree ran out of pool pages: later passes keep a bitset
+  bool ff_paged_off = false;
 };
 
 constexpr int64_t CSR_PAD = 64;

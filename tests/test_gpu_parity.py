@@ -196,7 +196,8 @@ def test_long_rows_and_columns_inward_sphere(api, monkeypatch):
     # Facets of a sphere wound inward see nearly every other facet: columns
     # and CSR rows of ~1,279 entries, longer than the one-wave rank sort
     # takes, so rows are ordered through the LDS bitmap.  CSC against the
-    # oracle, CSR against scipy's transpose and the all-bitmap path.
+    # oracle, CSR against scipy's transpose, the all-bitmap path and the
+    # all-bitmap path in narrow windows.
     import scipy.sparse as sps
     V, F = _mesh(api, 3, 0)
     F = np.ascontiguousarray(F[:, ::-1])
@@ -219,6 +220,11 @@ def test_long_rows_and_columns_inward_sphere(api, monkeypatch):
     monkeypatch.setenv("ILLUM_FF_CSR_SORT_MAX", "0")
     rb = ctx.compute_F(0.3).csr()
     for x, y in zip(ra, rb):
+        assert np.array_equal(x, y)
+    # bitmap windows of 3 words (96 sources): every row over many windows
+    monkeypatch.setenv("ILLUM_FF_CSR_WINDOW", "3")
+    rc = ctx.compute_F(0.3).csr()
+    for x, y in zip(ra, rc):
         assert np.array_equal(x, y)
 
 

@@ -262,3 +262,81 @@ def test_visibility_launch_orders_give_identical_assemblies(api, monkeypatch):
         for a, b in zip(Fm.csr(), rcsr):
             assert np.array_equal(a, b), (var, val)
         monkeypatch.delenv(var)
+
+
+@pytest.mark.parametrize("npool", ["32", "1"])
+def test_paged_visibility_set_gives_identical_assembly(api, monkeypatch, npool):
+    # The visibility set kept in pool pages of 1,024 receivers instead of a
+    # bitset over the body; with 1 pool page the sources whose visible
+    # receivers span 2 pages (908 of 20,480 here) run out of pages and run
+    # again with a bitset over their body.  CSC, CSR and the raw
+    # visibility sets must equal the bitset assembly's.
+    V, F = api.synth_ellipsoid(5, ncraters=80, seed=43)
+    ic = api.IllumContext(V=V, F=F)
+    monkeypatch.setenv("ILLUM_FF_PAGED", "0")
+    ref = ic.compute_F(0.2, keep_visibility=True)
+    rcsc, rcsr, rvis = ref.csc(), ref.csr(), ref.visibility_lists()
+    rays = ref.rays()
+    monkeypatch.setenv("ILLUM_FF_PAGED", "1")
+    monkeypatch.setenv("ILLUM_FF_NPOOL", npool)
+    ic.ctx.set_profiling(True)
+    ic.ctx.reset_profile()
+    Fm = ic.compute_F(0.2, keep_visibility=True)
+    prof = ic.ctx.profile()
+    ic.ctx.set_profiling(False)
+    if npool == "1":
+        assert prof.get("ff_visibility_rerun", (0, 0))[1] >= 1  # sources ran again
+    for a, b in zip(Fm.csc(), rcsc):
+        assert np.array_equal(a, b)
+    for a, b in zip(Fm.csr(), rcsr):
+        assert np.array_equal(a, b)
+    for a, b in zip(Fm.visibility_lists(), rvis):
+        assert np.array_equal(a, b)
+    assert Fm.rays() == rays
+
+
+def test_paged_work_count_with_sources_out_of_pool_pages(api, monkeypatch):
+    # The instrumented pass (no lists written) with a 1-page pool: sources
+    # out of pages store no counts and are counted, not run again.  Rays and
+    # pair tests do not depend on how the visibility set is kept.
+    V, F = api.synth_ellipsoid(5, ncraters=80, seed=43)
+    ic = api.IllumContext(V=V, F=F)
+    monkeypatch.setenv("ILLUM_FF_PAGED", "0")
+    ref = ic.form_factor_work()
+    monkeypatch.setenv("ILLUM_FF_PAGED", "1")
+    monkeypatch.setenv("ILLUM_FF_NPOOL", "1")
+    got = ic.form_factor_work()
+    assert got["rays"] == ref["rays"] > 0
+    assert got["pair_tests"] == ref["pair_tests"] > 0
+    assert got["box_tests"] > 0 and got["tri_tests"] > 0
+
+
+def test_body_whose_bitset_exceeds_the_lds_1310720_facets(api):
+    # One body of 1,310,720 facets (icosphere subdivision 8): a bitset over
+    # it (160 KB) leaves no LDS for the ray queue, so only the paged set can
+    # run it.  Visibility sets of 4 x 16 sources bit-exact against the
+    # oracle, their form-factor columns within 1e-12.
+    from oracle import oracle as O
+    V, F = api.synth_ellipsoid(8, ncraters=150, seed=3)
+    nf = len(F)
+    assert nf == 1310720
+    ic = api.IllumContext(V=V, F=F, leaf_size=4)
+    Fm = ic.compute_F(0.1, keep_visibility=True)
+    vptr, vidx = Fm.visibility_lists()
+    gcp, gri, gv = Fm.csc()
+    g = O.geometry(V, F)
+    worst = 0.0
+    for j0 in (0, nf // 3, nf // 2 + 77, nf - 16):
+        j1 = j0 + 16
+        bits, _ = O.visibility(g, offset=1e-5, j0=j0, j1=j1)
+        rows = np.unpackbits(bits.view(np.uint8), bitorder="little", axis=1)[:, :nf]
+        for q, j in enumerate(range(j0, j1)):
+            assert np.array_equal(vidx[vptr[j]:vptr[j + 1]], np.flatnonzero(rows[q])), j
+        ocp, ori, ov = O.form_factors(g, bits, j0=j0, j1=j1, albedo=0.1)
+        a, b = gcp[j0], gcp[j1]
+        assert np.array_equal(gcp[j0:j1 + 1] - a, ocp)
+        assert np.array_equal(gri[a:b], ori)
+        if len(ov):
+            worst = max(worst, float(np.max(np.abs(gv[a:b] - ov) / np.abs(ov))))
+    assert worst <= 1e-12
+    print(f"1,310,720 facets: nnz {Fm.nnz}, rays {Fm.rays()}, 64 sources bit-exact")
