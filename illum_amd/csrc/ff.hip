@@ -36,7 +36,11 @@ namespace illum {
 constexpr int FF_BLOCK = 256;
 constexpr int FF_MAXPER = 4;      // clusters per wavefront per fill round, at most
 constexpr int FF_MAXCHUNK = 1024;  // rays per walk round, at most
-constexpr int FF_NKEY = 96;  // 6 permutation classes x 4 x 4 direction cells
+#ifndef FF_KCELL
+#define FF_KCELL 8
+#endif
+// direction keys: 6 permutation classes x FF_KCELL^2 direction cells
+constexpr int FF_NKEY = 6 * FF_KCELL * FF_KCELL;
 // Workgroups per CU the kernel is compiled for (8 wavefronts per SIMD: at
 // most 64 VGPRs); the LDS per workgroup is sized for it where the body allows.
 constexpr int FF_WAVES_PER_SIMD = 8;
@@ -178,10 +182,14 @@ __device__ __forceinline__ int ray_dir_key(float dx, float dy, float dz) {
   const float dv = kz == 0 ? dz : (kz == 1 ? dx : dy);
   const float im = __builtin_amdgcn_rcpf(fabsf(dk));
   // cells of u, v in [-1, 1]; NaN (a zero direction) lands in cell 0
-  const int iu = (int)fminf(fmaxf((du * im + 1.0f) * 2.0f, 0.0f), 3.0f);
-  const int iv = (int)fminf(fmaxf((dv * im + 1.0f) * 2.0f, 0.0f), 3.0f);
-  const int cell = (iu & 1) | ((iv & 1) << 1) | ((iu & 2) << 1) | ((iv & 2) << 2);
-  return (2 * kz + (dk < 0.0f ? 1 : 0)) * 16 + cell;
+  constexpr float half = 0.5f * FF_KCELL;
+  const int iu = (int)fminf(fmaxf((du * im + 1.0f) * half, 0.0f), FF_KCELL - 1.0f);
+  const int iv = (int)fminf(fmaxf((dv * im + 1.0f) * half, 0.0f), FF_KCELL - 1.0f);
+  int cell = 0;
+#pragma unroll
+  for (int b = 0; (1 << b) < FF_KCELL; ++b)
+    cell |= (((iu >> b) & 1) << (2 * b)) | (((iv >> b) & 1) << (2 * b + 1));
+  return (2 * kz + (dk < 0.0f ? 1 : 0)) * (FF_KCELL * FF_KCELL) + cell;
 }
 
 // Block-wide exclusive scan of one int per thread (blockDim == 256).
@@ -1304,50 +1312,58 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
       launch<<<(int)nf, FF_BLOCK, lds, s>>>(a);
       ILLUM_LAUNCH_CHECK();
     }
-    if (qs.paged) {
-      // Sources that ran out of pool pages wrote nothing: they run again
-      // with a set over their whole body (their rays are already counted).
-      int nov = 0;
-      ILLUM_HIP(hipMemcpyAsync(&nov, &ctl->nover_pg, 4, hipMemcpyDeviceToHost, s));
+    // Counts scanned to column offsets, the longest row and column, the
+    // sources whose lists did not fit the pool (none when it sufficed), and
+    // the control words, in one host wait.
+    int32_t *orows = (int32_t *)ctx_workspace(ctx, WS_FF_ROWS, 4 * (size_t)nf);
+    auto scan_and_read = [&]() {
+      {
+        StageTimer st(ctx, "ff_csc");
+        scan_counts(ctx, counts, nf, sp->d_colptr);
+      }
+      max_count_kernel<<<(unsigned)std::min<int64_t>(64, (nf + 255) / 256), 256, 0, s>>>(
+          nf, rcounts, counts, &ctl->maxrow);
+      ILLUM_LAUNCH_CHECK();
+      ff_overflow_rows_kernel<<<(unsigned)((nf + 255) / 256), 256, 0, s>>>(
+          nf, counts, offs, cap, orows, &ctl->nover, &ctl->need);
+      ILLUM_LAUNCH_CHECK();
+      ILLUM_HIP(hipMemcpyAsync(&nnz, sp->d_colptr + nf, 8, hipMemcpyDeviceToHost, s));
+      ILLUM_HIP(hipMemcpyAsync(&h, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
       sync(ctx);
+    };
+    scan_and_read();
+    if (qs.paged && h.nover_pg != 0) {
+      // Sources that ran out of pool pages wrote no lists (zero counts):
+      // they run again with the rerun shape (their rays are already
+      // counted), and the counts are scanned again.  A paged rerun that runs
+      // out of pages again lists the source in the pool-overflow rows
+      // (overwritten below) and fails.
+      const int nov = h.nover_pg;
       if (nov < 0 || nov > nf) throw Error(ILLUM_ERR_HIP, "form factors: bad page-overflow count");
       paged_overflow(nov, nf);
-      if (nov > 0) {
-        // (a paged rerun that runs out of pages again lists the source in
-        // the pool-overflow rows, free until the pass ends, and fails)
-        FFArgs af = base_args(nov, 1, qf);
-        af.rows = over_rows;
-        af.over_rows = (int32_t *)ctx_workspace(ctx, WS_FF_ROWS, 4 * (size_t)nf);
-        af.over_n = &ctl->nover_pg2;
-        af.row_counts = rcounts;
-        af.pool = pool;
-        af.pool_n = &ctl->pool_n;
-        af.pool_cap = cap;
-        af.vcounts = vcounts;
-        af.voffs = voffs;
-        af.vpool = vpool;
-        af.vpool_n = &ctl->vpool_n;
-        af.vpool_cap = vcap;
+      FFArgs af = base_args(nov, 1, qf);
+      af.rows = over_rows;
+      af.over_rows = orows;
+      af.over_n = &ctl->nover_pg2;
+      af.row_counts = rcounts;
+      af.pool = pool;
+      af.pool_n = &ctl->pool_n;
+      af.pool_cap = cap;
+      af.vcounts = vcounts;
+      af.voffs = voffs;
+      af.vpool = vpool;
+      af.vpool_n = &ctl->vpool_n;
+      af.vpool_cap = vcap;
+      {
         StageTimer st(ctx, "ff_visibility_rerun");
         launch_f<<<nov, FF_BLOCK, lds_f, s>>>(af);
         ILLUM_LAUNCH_CHECK();
       }
+      // the pool-overflow count and need accumulate: cleared for the rescan
+      ILLUM_HIP(hipMemsetAsync(&ctl->nover, 0, sizeof(int), s));
+      ILLUM_HIP(hipMemsetAsync(&ctl->need, 0, sizeof(unsigned long long), s));
+      scan_and_read();
     }
-    {
-      StageTimer st(ctx, "ff_csc");
-      scan_counts(ctx, counts, nf, sp->d_colptr);
-    }
-    max_count_kernel<<<(unsigned)std::min<int64_t>(64, (nf + 255) / 256), 256, 0, s>>>(
-        nf, rcounts, counts, &ctl->maxrow);
-    ILLUM_LAUNCH_CHECK();
-    // the sources whose lists did not fit (none when the pool sufficed)
-    int32_t *orows = (int32_t *)ctx_workspace(ctx, WS_FF_ROWS, 4 * (size_t)nf);
-    ff_overflow_rows_kernel<<<(unsigned)((nf + 255) / 256), 256, 0, s>>>(
-        nf, counts, offs, cap, orows, &ctl->nover, &ctl->need);
-    ILLUM_LAUNCH_CHECK();
-    ILLUM_HIP(hipMemcpyAsync(&nnz, sp->d_colptr + nf, 8, hipMemcpyDeviceToHost, s));
-    ILLUM_HIP(hipMemcpyAsync(&h, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
-    sync(ctx);
     rerun_overflow(h.nover_pg2);
     if (h.status) throw Error(ILLUM_ERR_NAN, "computed a NaN form factor");
     if (nnz < 0 || (unsigned long long)nnz != h.pool_n)
