@@ -145,7 +145,17 @@ def bench_radiosity(api, illum, ctx, Fm, hz, S=512, tol=1e-8):
     dev = _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_OUT_ON_DEVICE | _lib.ILLUM_ASYNC
     _lib.call("illum_direct_radiosity", illum.mesh.h, api._ptr(hz), nphi, api._ptr(sun), S,
               api._ptr(disk), 100, api._ptr(E), dev)
-    out = {"S": S, "rel_tol": tol, "facets": nf, "nnz": Fm.nnz}
+    # direct insolation under the S suns, timed after the first call (which
+    # autotunes the launch shape)
+    ctx.synchronize()
+    ctx.reset_profile()
+    ctx.set_profiling(True)
+    _lib.call("illum_direct_radiosity", illum.mesh.h, api._ptr(hz), nphi, api._ptr(sun), S,
+              api._ptr(disk), 100, api._ptr(E), dev)
+    ctx.synchronize()
+    ctx.set_profiling(False)
+    direct_ms = ctx.profile().get("direct", (float("nan"), 1))[0]
+    out = {"S": S, "rel_tol": tol, "facets": nf, "nnz": Fm.nnz, "direct_ms": direct_ms}
     Bh = {}
     for method in ("fp64", "mixed"):
         B = api.DeviceArray(ctx, (S, nf))

@@ -258,11 +258,14 @@ static bool direct_exact_only() {
 }
 
 // Lanes per (facet, sun) pair: enough for about 2^20 lanes (4096 workgroups),
-// at most 16.  ILLUM_DIRECT_LANES=1|2|4|8|16 overrides.
+// at most 16 (0, -4, -8, -16: the wavefront-cooperative kernel with 1, 4, 8
+// or 16 lanes per pair).  ILLUM_DIRECT_LANES overrides.
 static int direct_lanes(int64_t pairs) {
   if (const char *e = getenv("ILLUM_DIRECT_LANES")) {
     const int v = atoi(e);
-    if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) return v;
+    if (v == -16 || v == -8 || v == -4 || v == 0 || v == 1 || v == 2 || v == 4 || v == 8 ||
+        v == 16)
+      return v;
   }
   int L = 1;
   while (L < 16 && pairs * L < ((int64_t)1 << 20)) L *= 2;
@@ -270,13 +273,156 @@ static int direct_lanes(int64_t pairs) {
 }
 
 // ---------------------------------------------------------------------------
-// Direct insolation.  L adjacent lanes per (facet, sun direction), each taking
-// every L-th disk sample, their counts summed by lane shuffles (an integer sum,
-// so E does not depend on L); L > 1 only when there are too few pairs to fill
-// the device (a single sun per time step).  E is nf x S column-major so
-// consecutive pairs write consecutive facets.
+// Direct insolation.  Two launch shapes with the same per-pair and
+// per-sample operations (so the same E):
+//   direct_kernel<L>: L adjacent lanes per (facet, sun direction), each
+//     taking every L-th disk sample, their counts summed by lane shuffles (an
+//     integer sum, so E does not depend on L).
+//   direct_wave_kernel<P>: P lanes per pair for the set-up and the whole-disk
+//     decision (P > 1 only adds wavefronts, to hide latency when the pairs
+//     are few); the pairs of a wavefront that the decision leaves open are
+//     then sampled one after the other by all 64 lanes.
+// E is nf x S column-major so consecutive pairs write consecutive facets.
 constexpr double ONE_AU_KM = 149.5978707e6;   // constants.hpp:7
 constexpr double SUN_RADIUS_KM = 695508;      // constants.hpp:8
+
+// One (facet, sun) pair: the facet's point and the sun disk's frame (N
+// toward the sun centre, T and B spanning the disk).
+struct DirectPair {
+  double px, py, pz, sx, sy, sz;
+  double Nx, Ny, Nz, Tx, Ty, Tz, Bx, By, Bz, nN;
+  int64_t f;
+  int s;
+};
+
+// Set-up of pair idx < nf S (facet idx mod nf, sun column idx / nf).
+__device__ __forceinline__ DirectPair direct_setup(int64_t idx, int64_t nf, int S,
+                                                   const float4 *__restrict__ pr4,
+                                                   const double *__restrict__ sun,
+                                                   const int32_t *__restrict__ body_of,
+                                                   int per_body, const int *__restrict__ step) {
+  DirectPair q;
+  q.f = idx % nf;
+  q.s = (int)(idx / nf);
+  const float4 pf = pr4[q.f];
+  q.px = pf.x; q.py = pf.y; q.pz = pf.z;
+  // per_body: body b's S suns are columns b S .. b S + S - 1 of sun; with a
+  // device step counter, step k reads the k-th such block of suns
+  int64_t sc = per_body ? (int64_t)(body_of ? body_of[q.f] : 0) * S + q.s : q.s;
+  if (step) sc += (int64_t)(*step) * S;
+  q.sx = sun[3 * sc]; q.sy = sun[3 * sc + 1]; q.sz = sun[3 * sc + 2];
+  double Nx = q.sx - q.px, Ny = q.sy - q.py, Nz = q.sz - q.pz;
+  q.nN = sqrt((Nx * Nx + Ny * Ny) + Nz * Nz);
+  Nx = Nx / q.nN; Ny = Ny / q.nN; Nz = Nz / q.nN;
+  q.Nx = Nx; q.Ny = Ny; q.Nz = Nz;
+  // deterministic tangent: project the least-aligned coordinate axis
+  const double ax = fabs(Nx), ay = fabs(Ny), az = fabs(Nz);
+  int e = 0;
+  double mn = ax;
+  if (ay < mn) { e = 1; mn = ay; }
+  if (az < mn) { e = 2; }
+  const double Ne = e == 0 ? Nx : (e == 1 ? Ny : Nz);
+  double Tx = (e == 0 ? 1.0 : 0.0) - Nx * Ne;
+  double Ty = (e == 1 ? 1.0 : 0.0) - Ny * Ne;
+  double Tz = (e == 2 ? 1.0 : 0.0) - Nz * Ne;
+  const double nT = sqrt((Tx * Tx + Ty * Ty) + Tz * Tz);
+  Tx = Tx / nT; Ty = Ty / nT; Tz = Tz / nT;
+  q.Tx = Tx; q.Ty = Ty; q.Tz = Tz;
+  q.Bx = Ty * Nz - Tz * Ny; q.By = Tz * Nx - Tx * Nz; q.Bz = Tx * Ny - Ty * Nx;
+  return q;
+}
+
+// Whole-disk decision.  Every disk sample lies on the plane through the sun
+// normal to N within R_sun of the sun, so its direction is within
+// atan(R_sun / |sun - p|) <= alpha = R_sun / |sun - p| of N (the larger
+// alpha only widens the bound): its zenith angle within alpha of N's, its
+// azimuth within alpha / min sin(theta) over [theta_c - alpha, theta_c +
+// alpha].  The thresholds h[i] + t dphi it can meet then lie in [min h,
+// max h + dphi] over the azimuth bins covered.  When the whole disk is below
+// (above) that band by a margin covering FP64 rounding, every (no) sample
+// counts, exactly as the per-sample loop would decide: returns ndisk - 1
+// (0).  Otherwise returns -1 and the samples decide.
+__device__ __forceinline__ int direct_decide(const DirectPair &q, const double *F,
+                                             const double *h, int nphi, int ndisk) {
+  const double TWO_PI = 2 * M_PI;
+  const double dphi = TWO_PI / (nphi - 1);
+  const double Nx = q.Nx, Ny = q.Ny, Nz = q.Nz;
+  const double alpha = SUN_RADIUS_KM / q.nN;
+  const double cz = (F[6] * Nx + F[7] * Ny) + F[8] * Nz;
+  const double thc = acos(fmin(1.0, fmax(-1.0, cz)));
+  const double smin = fmin(sin(thc - alpha), sin(thc + alpha));
+  if (!(thc - alpha > 1e-3 && thc + alpha < M_PI - 1e-3 && smin > 0.0)) return -1;
+  const double dph = alpha / smin * 1.0001 + 1e-9;
+  if (!(dph < 0.25)) return -1;
+  const double cx = (F[0] * Nx + F[1] * Ny) + F[2] * Nz;
+  const double cy = (F[3] * Nx + F[4] * Ny) + F[5] * Nz;
+  double phc = atan2(cy, cx);
+  if (phc < 0) phc = phc + TWO_PI;
+  const int nb = nphi - 1;  // bins of [0, 2 pi)
+  const int i0 = (int)floor((phc - dph) / dphi) - 1, i1 = (int)floor((phc + dph) / dphi) + 1;
+  double hlo = __builtin_huge_valf(), hhi = -__builtin_huge_valf();
+  bool nan = false;
+  for (int i = i0; i <= i1 + 1; ++i) {
+    // i1 + 1: the azimuth 2 pi itself maps to h[nphi - 1] when the range
+    // reaches the wrap
+    int ii = i;
+    if (i <= i1) {
+      while (ii < 0) ii += nb;
+      while (ii >= nb) ii -= nb;
+    } else {
+      ii = (i0 < 0 || i1 >= nb - 1) ? nb : -1;
+    }
+    if (ii < 0) continue;
+    const double hv = h[ii];
+    nan = nan || hv != hv;
+    hlo = fmin(hlo, hv);
+    hhi = fmax(hhi, hv);
+  }
+  const double eps = 1e-9;
+  if (!nan && thc + alpha + eps < hlo) return ndisk - 1;
+  if (!nan && thc - alpha - eps >= hhi + dphi) return 0;
+  return -1;
+}
+
+// Whether disk sample k of pair q is above the facet's horizon.
+__device__ __forceinline__ bool direct_sample(const DirectPair &q, int k, const double *F,
+                                              const double *h, int nphi,
+                                              const double *__restrict__ disk, int ndisk) {
+  const double TWO_PI = 2 * M_PI;
+  const double dphi = TWO_PI / (nphi - 1);
+  const double xk = disk[k], yk = disk[ndisk + k];
+  const double dx = q.sx + SUN_RADIUS_KM * (xk * q.Tx + yk * q.Bx);
+  const double dy = q.sy + SUN_RADIUS_KM * (xk * q.Ty + yk * q.By);
+  const double dz = q.sz + SUN_RADIUS_KM * (xk * q.Tz + yk * q.Bz);
+  double wx = dx - q.px, wy = dy - q.py, wz = dz - q.pz;
+  const double nw = sqrt((wx * wx + wy * wy) + wz * wz);
+  wx = wx / nw; wy = wy / nw; wz = wz / nw;
+  const double d0 = (F[0] * wx + F[1] * wy) + F[2] * wz;
+  const double d1 = (F[3] * wx + F[4] * wy) + F[5] * wz;
+  const double d2 = (F[6] * wx + F[7] * wy) + F[8] * wz;
+  const double theta = acos(d2);
+  double phi = atan2(d1, d0);
+  if (phi < 0) phi = phi + TWO_PI;
+  int pi_ = (int)floor(phi / dphi);
+  pi_ = pi_ < 0 ? 0 : (pi_ > nphi - 1 ? nphi - 1 : pi_);
+  const double t = (phi - pi_ * dphi) / dphi;
+  return theta < h[pi_] + t * dphi;
+}
+
+// E of pair q from its count of disk samples above the horizon.
+__device__ __forceinline__ void direct_store(const DirectPair &q, int count, const double *F,
+                                             int nphi, int64_t nf, double *__restrict__ E,
+                                             const double *__restrict__ flux) {
+  double E0 = (double)count / (nphi - 1);
+  E0 = E0 * fmax(0.0, (q.Nx * F[6] + q.Ny * F[7]) + q.Nz * F[8]);
+  const double ns = sqrt((q.sx * q.sx + q.sy * q.sy) + q.sz * q.sz);
+  const double r = ONE_AU_KM / ns;
+  E0 = E0 * (1372 * (r * r));
+  // flux: the time loop's surface flux F_surf; its radiosity is
+  // rad = -F_surf + direct (illum_cli.cpp:411-423), which equals
+  // direct - F_surf exactly
+  E[(int64_t)q.s * nf + q.f] = flux ? E0 - flux[q.f] : E0;
+}
 
 template <int L>
 __global__ __launch_bounds__(256) void direct_kernel(
@@ -292,134 +438,87 @@ __global__ __launch_bounds__(256) void direct_kernel(
   // a group of L lanes is all in range or all out of it, so the shuffles
   // below only read lanes of the caller's own (active) group
   if (idx >= nf * S) return;
-  const int64_t f = idx % nf;
-  const int s = (int)(idx / nf);
-  const double TWO_PI = 2 * M_PI;
-  const double dphi = TWO_PI / (nphi - 1);
-  const float4 pf = pr4[f];
-  const double px = pf.x, py = pf.y, pz = pf.z;
-  // per_body: body b's S suns are columns b S .. b S + S - 1 of sun; with a
-  // device step counter, step k reads the k-th such block of suns
-  int64_t sc = per_body ? (int64_t)(body_of ? body_of[f] : 0) * S + s : s;
-  if (step) sc += (int64_t)(*step) * S;
-  const double sx = sun[3 * sc], sy = sun[3 * sc + 1], sz = sun[3 * sc + 2];
-  double Nx = sx - px, Ny = sy - py, Nz = sz - pz;
-  const double nN = sqrt((Nx * Nx + Ny * Ny) + Nz * Nz);
-  Nx = Nx / nN; Ny = Ny / nN; Nz = Nz / nN;
-  // deterministic tangent: project the least-aligned coordinate axis
-  const double ax = fabs(Nx), ay = fabs(Ny), az = fabs(Nz);
-  int e = 0;
-  double mn = ax;
-  if (ay < mn) { e = 1; mn = ay; }
-  if (az < mn) { e = 2; }
-  const double Ne = e == 0 ? Nx : (e == 1 ? Ny : Nz);
-  double Tx = (e == 0 ? 1.0 : 0.0) - Nx * Ne;
-  double Ty = (e == 1 ? 1.0 : 0.0) - Ny * Ne;
-  double Tz = (e == 2 ? 1.0 : 0.0) - Nz * Ne;
-  const double nT = sqrt((Tx * Tx + Ty * Ty) + Tz * Tz);
-  Tx = Tx / nT; Ty = Ty / nT; Tz = Tz / nT;
-  const double Bx = Ty * Nz - Tz * Ny, By = Tz * Nx - Tx * Nz, Bz = Tx * Ny - Ty * Nx;
-  const double *F = frame + 9 * f;
-  const double *h = hz + f * nphi;
-  int count = 0;
-  // Whole-disk decision.  Every disk sample lies on the plane through the sun
-  // normal to N within R_sun of the sun, so its direction is within
-  // atan(R_sun / |sun - p|) <= alpha = R_sun / |sun - p| of N (the larger
-  // alpha only widens the bound): its zenith angle within alpha of
-  // N's, its azimuth within alpha / min sin(theta) over [theta_c - alpha,
-  // theta_c + alpha].  The thresholds h[i] + t dphi it can meet then lie in
-  // [min h, max h + dphi] over the azimuth bins covered.  When the whole disk
-  // is below (above) that band by a margin covering FP64 rounding, every
-  // (no) sample counts, exactly as the per-sample loop would decide; otherwise
-  // the loop runs.
-  bool decided = false;
-  if (!exact_only) {
-    const double alpha = SUN_RADIUS_KM / nN;
-    const double cz = (F[6] * Nx + F[7] * Ny) + F[8] * Nz;
-    const double thc = acos(fmin(1.0, fmax(-1.0, cz)));
-    const double smin = fmin(sin(thc - alpha), sin(thc + alpha));
-    if (thc - alpha > 1e-3 && thc + alpha < M_PI - 1e-3 && smin > 0.0) {
-      const double dph = alpha / smin * 1.0001 + 1e-9;
-      if (dph < 0.25) {
-        const double cx = (F[0] * Nx + F[1] * Ny) + F[2] * Nz;
-        const double cy = (F[3] * Nx + F[4] * Ny) + F[5] * Nz;
-        double phc = atan2(cy, cx);
-        if (phc < 0) phc = phc + TWO_PI;
-        const int nb = nphi - 1;  // bins of [0, 2 pi)
-        const int i0 = (int)floor((phc - dph) / dphi) - 1, i1 = (int)floor((phc + dph) / dphi) + 1;
-        double hlo = __builtin_huge_valf(), hhi = -__builtin_huge_valf();
-        bool nan = false;
-        for (int i = i0; i <= i1 + 1; ++i) {
-          // i1 + 1: the azimuth 2 pi itself maps to h[nphi - 1] when the range
-          // reaches the wrap
-          int ii = i;
-          if (i <= i1) {
-            while (ii < 0) ii += nb;
-            while (ii >= nb) ii -= nb;
-          } else {
-            ii = (i0 < 0 || i1 >= nb - 1) ? nb : -1;
-          }
-          if (ii < 0) continue;
-          const double hv = h[ii];
-          nan = nan || hv != hv;
-          hlo = fmin(hlo, hv);
-          hhi = fmax(hhi, hv);
-        }
-        const double eps = 1e-9;
-        if (!nan && thc + alpha + eps < hlo) {
-          count = sub == 0 ? ndisk - 1 : 0;
-          decided = true;
-        } else if (!nan && thc - alpha - eps >= hhi + dphi) {
-          count = 0;
-          decided = true;
-        }
-      }
-    }
-  }
-  for (int k = sub; k < ndisk - 1 && !decided; k += L) {
-    const double xk = disk[k], yk = disk[ndisk + k];
-    const double dx = sx + SUN_RADIUS_KM * (xk * Tx + yk * Bx);
-    const double dy = sy + SUN_RADIUS_KM * (xk * Ty + yk * By);
-    const double dz = sz + SUN_RADIUS_KM * (xk * Tz + yk * Bz);
-    double wx = dx - px, wy = dy - py, wz = dz - pz;
-    const double nw = sqrt((wx * wx + wy * wy) + wz * wz);
-    wx = wx / nw; wy = wy / nw; wz = wz / nw;
-    const double d0 = (F[0] * wx + F[1] * wy) + F[2] * wz;
-    const double d1 = (F[3] * wx + F[4] * wy) + F[5] * wz;
-    const double d2 = (F[6] * wx + F[7] * wy) + F[8] * wz;
-    const double theta = acos(d2);
-    double phi = atan2(d1, d0);
-    if (phi < 0) phi = phi + TWO_PI;
-    int pi_ = (int)floor(phi / dphi);
-    pi_ = pi_ < 0 ? 0 : (pi_ > nphi - 1 ? nphi - 1 : pi_);
-    const double t = (phi - pi_ * dphi) / dphi;
-    if (theta < h[pi_] + t * dphi) ++count;
-  }
+  const DirectPair q = direct_setup(idx, nf, S, pr4, sun, body_of, per_body, step);
+  const double *F = frame + 9 * q.f;
+  const double *h = hz + q.f * nphi;
+  const int dec = exact_only ? -1 : direct_decide(q, F, h, nphi, ndisk);
+  int count = dec >= 0 && sub == 0 ? dec : 0;
+  if (dec < 0)
+    for (int k = sub; k < ndisk - 1; k += L) count += direct_sample(q, k, F, h, nphi, disk, ndisk);
   for (int o = L / 2; o > 0; o >>= 1) count += __shfl_xor(count, o);
   if (sub != 0) return;
-  double E0 = (double)count / (nphi - 1);
-  E0 = E0 * fmax(0.0, (Nx * F[6] + Ny * F[7]) + Nz * F[8]);
-  const double ns = sqrt((sx * sx + sy * sy) + sz * sz);
-  const double q = ONE_AU_KM / ns;
-  E0 = E0 * (1372 * (q * q));
-  // flux: the time loop's surface flux F_surf; its radiosity is
-  // rad = -F_surf + direct (illum_cli.cpp:411-423), which equals
-  // direct - F_surf exactly
-  E[(int64_t)s * nf + f] = flux ? E0 - flux[f] : E0;
+  direct_store(q, count, F, nphi, nf, E, flux);
 }
 
-// One direct_kernel launch over nf x S pairs with L lanes per pair.  L does
-// not change E (the lanes only share the integer count), so it is picked by
-// the autotuner per pair-count scale (ILLUM_DIRECT_LANES=1..16 overrides).
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void direct_wave_kernel(
+    int64_t nf, int S, int nphi, const float4 *__restrict__ pr4,
+    const double *__restrict__ frame, const double *__restrict__ hz,
+    const double *__restrict__ sun, const double *__restrict__ disk, int ndisk,
+    double *__restrict__ E, int exact_only, const int32_t *__restrict__ body_of,
+    int per_body, const int *__restrict__ step, const double *__restrict__ flux) {
+  static_assert(P >= 1 && P <= 64 && (P & (P - 1)) == 0, "P divides the wavefront");
+  const int64_t lane_id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t idx = lane_id / P;
+  const bool owner = lane_id % P == 0;  // the lane that stores the pair's E
+  const int lane = threadIdx.x & 63;
+  const bool valid = idx < nf * S;
+  // a lane past the last pair sets up pair 0 and takes no further part
+  const DirectPair q = direct_setup(valid ? idx : 0, nf, S, pr4, sun, body_of, per_body, step);
+  const double *F = frame + 9 * q.f;
+  const double *h = hz + q.f * nphi;
+  int count = 0;
+  bool open = false;
+  if (valid) {
+    const int dec = exact_only ? -1 : direct_decide(q, F, h, nphi, ndisk);
+    open = dec < 0;
+    count = open ? 0 : dec;
+  }
+  // the open pairs, one after the other, each sampled by every lane
+  uint64_t m = __builtin_amdgcn_ballot_w64(open && owner);
+  while (m) {
+    const int l = (int)__builtin_ctzll(m);
+    m &= m - 1;
+    DirectPair o;
+    o.px = readlane_d(q.px, l); o.py = readlane_d(q.py, l); o.pz = readlane_d(q.pz, l);
+    o.sx = readlane_d(q.sx, l); o.sy = readlane_d(q.sy, l); o.sz = readlane_d(q.sz, l);
+    o.Tx = readlane_d(q.Tx, l); o.Ty = readlane_d(q.Ty, l); o.Tz = readlane_d(q.Tz, l);
+    o.Bx = readlane_d(q.Bx, l); o.By = readlane_d(q.By, l); o.Bz = readlane_d(q.Bz, l);
+    const int64_t of = __builtin_amdgcn_readlane((int)q.f, l);
+    const double *oF = frame + 9 * of;
+    const double *oh = hz + of * nphi;
+    int c = 0;
+    for (int k = lane; k < ndisk - 1; k += 64) c += direct_sample(o, k, oF, oh, nphi, disk, ndisk);
+    for (int w = 32; w > 0; w >>= 1) c += __shfl_xor(c, w);
+    if (lane == l) count = c;
+  }
+  if (valid && owner) direct_store(q, count, F, nphi, nf, E, flux);
+}
+
+// One direct insolation launch over nf x S pairs: L > 0 lanes per pair
+// (direct_kernel<L>), or direct_wave_kernel<P> for L = 0 (P = 1) and
+// L = -4, -8, -16 (P = -L).  The shape does not change E (the lanes only
+// share integer counts), so it is picked by the autotuner per pair-count
+// scale (ILLUM_DIRECT_LANES=-16|-8|-4|0|1|2|4|8|16 overrides).
 static void direct_launch(illum_mesh *mesh, int nphi, const double *hz, const double *sun, int S,
                           const double *disk, int ndisk, double *E, bool per_body,
                           const int *step, const double *flux, hipStream_t s) {
   const int64_t nf = mesh->g.nf;
   const int64_t pairs = nf * S;
   auto go = [&](int L) {
-    const int64_t lanes = pairs * L;
-    auto kern = L == 16 ? direct_kernel<16> : L == 8 ? direct_kernel<8>
-              : L == 4 ? direct_kernel<4> : L == 2 ? direct_kernel<2> : direct_kernel<1>;
+    const int64_t lanes = pairs * (L > 0 ? L : L == 0 ? 1 : -L);
+    auto kern = L == 0 ? direct_wave_kernel<1> : L == -4 ? direct_wave_kernel<4>
+              : L == -8 ? direct_wave_kernel<8> : L == -16 ? direct_wave_kernel<16>
+              : L == 16 ? direct_kernel<16>
+              : L == 8 ? direct_kernel<8> : L == 4 ? direct_kernel<4>
+              : L == 2 ? direct_kernel<2> : direct_kernel<1>;
     kern<<<(unsigned)((lanes + 255) / 256), 256, 0, s>>>(
         nf, S, nphi, (const float4 *)mesh->d_pr4, mesh->d_frame, hz, sun, disk, ndisk, E,
         direct_exact_only() ? 1 : 0, mesh->d_body, per_body ? 1 : 0, step, flux);
@@ -429,8 +528,8 @@ static void direct_launch(illum_mesh *mesh, int nphi, const double *hz, const do
   if (!getenv("ILLUM_DIRECT_LANES")) {
     int b = 0;
     while (b < 62 && ((int64_t)1 << (b + 1)) <= pairs) ++b;
-    L = autotune(mesh->ctx, "direct_lanes[pairs~2^" + std::to_string(b) + "]", {1, 2, 4, 8, 16},
-                 L, go);
+    L = autotune(mesh->ctx, "direct_lanes[pairs~2^" + std::to_string(b) + "]",
+                 {-16, -8, -4, 0, 1, 2, 4, 8, 16}, L, go);
   }
   go(L);
 }
