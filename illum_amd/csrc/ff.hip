@@ -1454,11 +1454,12 @@ illum_sparse *form_factors(illum_bvh *bvh, double albedo, const double *albedo_v
                                        sp->d_rval);
     ILLUM_LAUNCH_CHECK();
     if (h.maxrow > sort_max) {
-      // bitmap windows of up to 16,384 words (524,288 sources: one window
-      // for any row of a body that size or smaller); ILLUM_FF_CSR_WINDOW
-      // (words) narrows them, so that rows take several windows (results are
+      // bitmap windows of up to 15,872 words, 62 KB beside the kernel's
+      // static LDS within the default 64 KB (507,904 sources: one window for
+      // any row of a body that size or smaller); ILLUM_FF_CSR_WINDOW (words)
+      // narrows them, so that rows take several windows (results are
       // identical)
-      int wwords = std::max(1, std::min(body_dims(mesh).nwmax, 16384));
+      int wwords = std::max(1, std::min(body_dims(mesh).nwmax, 15872));
       if (const char *e = getenv("ILLUM_FF_CSR_WINDOW"))
         wwords = std::max(1, std::min(wwords, atoi(e)));
       csr_long_kernel<<<(unsigned)nf, 256, 4 * (size_t)wwords, s>>>(
