@@ -118,6 +118,14 @@ struct MixedIO {
   _Float16 *Dn;   // d_{k+1}
   double *B;      // B mode: B += delta_k; null: the corrections stay in a ring
   int xc;         // row groups per chunk dealt to the XCDs in turn (0: one range per XCD)
+  // Ring mode's FP32 remainders (the wide kernel): eps_k = r_k - delta_k is
+  // kept as q_k = fp32(eps_k 2^(ek + 11)) instead of r_k in FP64.  Qk null:
+  // eps_k is formed from R = r_k and d_k (r_k in FP64: the first product,
+  // r_0 = E, or every product without remainders).  Qn non-null receives
+  // q_{k+1} (Y is not written) and the column sums of eps_{k+1}^2 go to part
+  // after the residuals'; null: Y receives r_{k+1} in FP64.
+  const float *Qk;
+  float *Qn;
 };
 
 __device__ __forceinline__ int64_t sweep_block(int b, int n, int xc) {
@@ -160,19 +168,43 @@ __global__ __launch_bounds__(256) void sweep_rows_kernel(
 }
 
 // Wide form (S >= 128): lane l covers PER consecutive right-hand sides (one
-// vector load of PER values per nonzero).  A block is a group of
-// 4 SWEEP_RPW rows, interleaved over its four waves (wave w takes rows
-// 4 i + w), so the waves gather neighbouring rows together.  The grid is
-// column-chunk-major: chunk ch (columns [64 PER ch, 64 PER (ch + 1))) runs
-// over all groups before the next chunk starts, so the gathered block of X in
-// use is nf x 64 PER values, sized by the host to stay resident in the
-// memory-side cache; per chunk ngrp8 blocks (a multiple of 8, so each XCD's
-// blocks take a contiguous row range).  Requires S % PER == 0.
+// vector load of PER values per nonzero).  The grid is column-chunk-major:
+// chunk ch (columns [64 PER ch, 64 PER (ch + 1))) runs over all row blocks
+// before the next chunk starts.  Per chunk there are nblk row blocks (a
+// multiple of 8); block g owns rows [rblk[g], rblk[g + 1]), contiguous and
+// of about equal weight (nonzeros plus SWEEP_ROW_WEIGHT per row, see
+// row_blocks_kernel), interleaved over its four waves (wave w takes rows
+// rblk[g] + w + 4 i).  With one contiguous range of blocks per XCD, each
+// XCD's rows are a compact patch of the surface (facet order is spatially
+// coherent), so the rows of X they gather stay in that XCD's L2; weighting
+// by nonzeros keeps the XCDs' shares even where crater floors have rows of
+// hundreds of entries.  Requires S % PER == 0.
 // The block's column sums of squares of the residual elements go to
-// part[group][S]: each wave sums its rows in order, then waves 0..3 in order,
-// so the sums do not depend on PER either.
-constexpr int SWEEP_RPW = 4;
-constexpr int SWEEP_UNROLL = 8;  // gathers in flight per wave
+// part[g][S]: each wave sums its rows in order, then waves 0..3 in order,
+// so the sums do not depend on PER or on where a block runs.
+constexpr int SWEEP_RPW = 4;         // rows per wave of a row block, on average
+constexpr int SWEEP_ROW_WEIGHT = 8;  // a row's fixed cost, in nonzeros
+constexpr int SWEEP_UNROLL = 8;      // gathers in flight per wave
+
+// Row blocks of the wide product: rblk[k] = the largest m in [0, nf] with
+// rowptr[m] + SWEEP_ROW_WEIGHT m <= k W, W = total weight / nblk, for
+// k = 0..nblk (rblk[0] = 0, rblk[nblk] = nf).  Monotone in k.
+__global__ __launch_bounds__(256) void row_blocks_kernel(int64_t nf,
+                                                          const int64_t *__restrict__ rowptr,
+                                                          int nblk, int *__restrict__ rblk) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k > nblk) return;
+  if (k == nblk) { rblk[k] = (int)nf; return; }
+  const int64_t tot = rowptr[nf] + SWEEP_ROW_WEIGHT * nf;
+  // tot < 2^40 (nnz < 2^39, nf < 2^31) and k < 2^24: no overflow
+  const int64_t T = tot * k / nblk;
+  int64_t lo = 0, hi = nf;  // c(lo) <= T, find the largest such m
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (rowptr[mid] + SWEEP_ROW_WEIGHT * mid <= T) lo = mid; else hi = mid - 1;
+  }
+  rblk[k] = (int)lo;
+}
 
 // acc[p] += f x[p] with one rounding (a fused multiply-add per element).
 template <int PER, class AV, class XV>
@@ -181,93 +213,145 @@ __device__ __forceinline__ void fma_acc(AV &acc, double f, const XV &x) {
   for (int p = 0; p < PER; ++p) acc[p] = __builtin_fma(f, (double)x[p], acc[p]);
 }
 
+// Gathers of rows of X: `off` is the byte offset of the row (a wave-uniform
+// value read from a lane), `lo` the lane's byte offset within it.  BUF:
+// buffer loads with the row offset as the scalar offset (no per-lane address
+// arithmetic; the block must span < 2 GiB); else global loads.
+template <class XV, bool BUF>
+struct Gather {
+  const char *base;
+  __amdgpu_buffer_rsrc_t rs;
+  unsigned lo;
+  __device__ Gather(const void *X, unsigned xbytes, unsigned lane_off) : base((const char *)X), lo(lane_off) {
+    if constexpr (BUF) rs = __builtin_amdgcn_make_buffer_rsrc((void *)X, 0, (int)xbytes, 0x00020000);
+  }
+  __device__ __forceinline__ XV operator()(unsigned off) const {
+    if constexpr (BUF) {
+      if constexpr (sizeof(XV) == 4)
+        return __builtin_bit_cast(XV, __builtin_amdgcn_raw_buffer_load_b32(rs, lo, off, 0));
+      else if constexpr (sizeof(XV) == 8)
+        return __builtin_bit_cast(XV, __builtin_amdgcn_raw_buffer_load_b64(rs, lo, off, 0));
+      else
+        return __builtin_bit_cast(XV, __builtin_amdgcn_raw_buffer_load_b128(rs, lo, off, 0));
+    } else {
+      return *(const XV *)(base + (size_t)off + lo);
+    }
+  }
+};
+
 // A row's nonzeros are taken 64 at a time: the whole wave loads 64 column
-// indices (one per lane) and the gathers read them back with readlane, so
-// the SWEEP_UNROLL gathers of a group issue back to back with no scalar-load
-// round trip before their addresses; the values come by scalar loads.  Group
-// g + 1's gathers are issued before group g is accumulated (two register
-// sets), so a row costs about one memory round trip, not one per group; the
-// last group is padded with zero coefficients (the CSR arrays are padded, so
-// its indices and values are in bounds).  The row's own operands are loaded
-// before its gathers.  Each element is accumulated with fused multiply-adds
-// in CSR order.
-template <class TX, int PER>
+// indices (one per lane), turns them into byte offsets of rows of X, and the
+// gathers read the offsets back with readlane, so the SWEEP_UNROLL gathers
+// of a group issue back to back; the values come by scalar loads.  Each
+// element is accumulated with fused multiply-adds in CSR order.
+template <class TX, int PER, bool BUF>
 __global__ __launch_bounds__(256) void sweep_wide_kernel(
-    int64_t nf, int S, int ngrp8, const int64_t *__restrict__ rowptr,
+    int64_t nf, int S, int nblk, const int *__restrict__ rblk, const int64_t *__restrict__ rowptr,
     const int32_t *__restrict__ colidx, const double *__restrict__ val, const TX *__restrict__ X,
-    const double *__restrict__ R, double *__restrict__ Y, const int *__restrict__ done,
-    double *__restrict__ part, MixedIO m) {
+    unsigned xbytes, const double *__restrict__ R, double *__restrict__ Y,
+    const int *__restrict__ done, double *__restrict__ part, MixedIO m) {
   if (*done) return;  // uniform: the whole grid returns or none of it
   constexpr int U = SWEEP_UNROLL;
   typedef TX xvec __attribute__((ext_vector_type(PER)));
   typedef double yvec __attribute__((ext_vector_type(PER)));
-  __shared__ yvec red[3][64];
-  const int ch = blockIdx.x / ngrp8, gb = blockIdx.x - ch * ngrp8;
-  const int64_t g = sweep_block(gb, ngrp8, m.xc);
+  typedef float fvec __attribute__((ext_vector_type(PER)));
+  static_assert(sizeof(xvec) == 4 || sizeof(xvec) == 8 || sizeof(xvec) == 16 || !BUF,
+                "buffer gathers of 4, 8 or 16 bytes");
+  __shared__ yvec red[3][64], rede[3][64];
+  const int ch = blockIdx.x / nblk, gb = blockIdx.x - ch * nblk;
+  const int64_t g = sweep_block(gb, nblk, m.xc);
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   const int s = (ch * 64 + lane) * PER;
   const bool on = s < S;
-  yvec sq = (yvec)0.0;
-  for (int i = 0; i < SWEEP_RPW; ++i) {
-    const int64_t row = (g * SWEEP_RPW + i) * 4 + w;
-    if (row >= nf) break;
+  // Every lane runs the gathers (readlane needs the index lanes active); a
+  // lane past S reads column 0 of the same rows and discards the sums.
+  const Gather<xvec, BUF> gather(X, xbytes, (unsigned)(on ? s : 0) * (unsigned)sizeof(TX));
+  const unsigned rowbytes = (unsigned)S * (unsigned)sizeof(TX);
+  yvec sq = (yvec)0.0, se = (yvec)0.0;  // column sums of r_{k+1}^2 and eps_{k+1}^2
+  const int64_t r1 = rblk[g + 1];
+  for (int64_t row = rblk[g] + w; row < r1; row += 4) {
     const int64_t k0 = rowptr[row], k1 = rowptr[row + 1];
     const int64_t at = row * S + s;
     yvec acc = (yvec)0.0;
-    // Every lane runs the gathers (readlane needs the index lanes active);
-    // a lane past S reads column 0 of the same rows and discards the sums.
-    const int sx = on ? s : 0;
     for (int64_t kb = k0; kb < k1; kb += 64) {
       const int cnt = (int)std::min<int64_t>(64, k1 - kb);
-      const int cl = colidx[kb + lane];  // past the row: another valid index or the pad
+      // past the row: another valid index or the zero pad
+      const unsigned bo = (unsigned)colidx[kb + lane] * rowbytes;
       const double *vk = val + kb;  // wave-uniform: scalar loads
       int j = 0;
       for (; j + U <= cnt; j += U) {
         xvec x[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-          x[u] = *(const xvec *)(X + (int64_t)__builtin_amdgcn_readlane(cl, j + u) * S + sx);
+        for (int u = 0; u < U; ++u) x[u] = gather((unsigned)__builtin_amdgcn_readlane((int)bo, j + u));
 #pragma unroll
         for (int u = 0; u < U; ++u) fma_acc<PER>(acc, vk[j + u], x[u]);
       }
       for (; j < cnt; ++j)
-        fma_acc<PER>(acc, vk[j],
-                     *(const xvec *)(X + (int64_t)__builtin_amdgcn_readlane(cl, j) * S + sx));
+        fma_acc<PER>(acc, vk[j], gather((unsigned)__builtin_amdgcn_readlane((int)bo, j)));
     }
     if (!on) continue;
-    const yvec r = *(const yvec *)(R + at);
-    const yvec x = __builtin_convertvector(*(const xvec *)(X + at), yvec);
     if constexpr (is_mixed<TX>()) {
-      yvec ik, sn;
+      yvec ik, sn, isn;
 #pragma unroll
       for (int p = 0; p < PER; ++p) {
         ik[p] = ldexp(1.0, -m.ek[s + p]);
         sn[p] = ldexp(1.0, m.en[s + p]);
+        isn[p] = ldexp(1.0, -m.en[s + p]);
       }
-      const yvec xs = x * ik;
-      const yvec y = (r - xs) + acc * ik;
-      *(yvec *)(Y + at) = y;
+      // eps_k = r_k - delta_k, exact in FP64 (delta_k is r_k rounded)
+      yvec eps;
+      if (m.Qk) {
+        const fvec q = *(const fvec *)(m.Qk + at);
+#pragma unroll
+        for (int p = 0; p < PER; ++p) eps[p] = (double)q[p] * (ik[p] * 0x1p-11);
+      } else {
+        const yvec x = __builtin_convertvector(*(const xvec *)(X + at), yvec);
+        eps = *(const yvec *)(R + at) - x * ik;
+      }
+      const yvec y = eps + acc * ik;  // r_{k+1}
       sq = sq + y * y;
       xvec dn;
 #pragma unroll
       for (int p = 0; p < PER; ++p) dn[p] = to_half_scaled(y[p], sn[p]);
       *(xvec *)(m.Dn + at) = dn;
-      if (m.B) *(yvec *)(m.B + at) = *(const yvec *)(m.B + at) + xs;
+      if (m.Qn) {
+        // eps_{k+1} = r_{k+1} - delta_{k+1} (exact), stored rounded to FP32
+        yvec en;
+        fvec qn;
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+          en[p] = y[p] - (double)dn[p] * isn[p];
+          qn[p] = (float)(en[p] * (sn[p] * 0x1p11));
+        }
+        se = se + en * en;
+        *(fvec *)(m.Qn + at) = qn;
+      } else {
+        *(yvec *)(Y + at) = y;
+      }
     } else {
-      const yvec y = r + acc;
+      const yvec x = __builtin_convertvector(*(const xvec *)(X + at), yvec);
+      const yvec y = *(const yvec *)(R + at) + acc;
       *(yvec *)(Y + at) = y;
       const yvec d = y - x;
       sq = sq + d * d;
     }
   }
   if (w > 0) red[w - 1][lane] = sq;
+  if (m.Qn && w > 0) rede[w - 1][lane] = se;
   __syncthreads();
   if (w == 0 && on) {
     yvec t = sq + red[0][lane];
     t = t + red[1][lane];
     t = t + red[2][lane];
     *(yvec *)(part + g * S + s) = t;
+    if (m.Qn) {
+      // the remainders' column sums follow the residuals' (nblk rows later)
+      yvec te = se + rede[0][lane];
+      te = te + rede[1][lane];
+      te = te + rede[2][lane];
+      *(yvec *)(part + ((int64_t)nblk + g) * S + s) = te;
+    }
   }
 }
 
@@ -341,6 +425,7 @@ __global__ __launch_bounds__(256) void sweep_nnz_kernel(
 struct SolveCtl {
   int done;   // the tolerance was met; later sweeps of the solve do nothing
   int nprod;  // products taken
+  int floor;  // FP32 remainders: their rounding bound reached the tolerance (done is set too)
 };
 
 // Column sums over rows [r0, r1) of a row-major [rows][S] array, for the
@@ -415,21 +500,49 @@ __global__ __launch_bounds__(256) void colnorm_partial_kernel(int64_t nf, int S,
                        part + (int64_t)blockIdx.x * S);
 }
 
+// Bound on the FP32 rounding of a stored remainder q = fp32(eps 2^(e + 11)),
+// unscaled, in the 2-norm over nf elements: 2^-24 ||eps|| for normal FP32
+// values plus 2^-150 2^-(e + 11) per element for subnormal ones (taken twice
+// as large).  en = ||eps||_2, summed in FP64 before the rounding (itself
+// rounded by at most a few ulps, which the factor two covers).
+__device__ __forceinline__ double remainder_bound(double en, int e, double sqrt_nf) {
+  return 0x1p-24 * en * 1.0000001 + sqrt_nf * ldexp(1.0, -(160 + e));
+}
+
 // The stop decision: rel = max_s sqrt(col_s / e2_s) (sqrt(col_s) where e2_s =
 // 0) is stored as hist[nprod], the product counted, done set when rel <= tol.
 // ex (nullable, the mixed method): col is the residual r_{k+1} of product k =
 // nprod, and ex[k + 2][s] receives the scale exponent of d_{k+2}, which
 // product k + 1 writes (r_{k+2} is bounded by r_{k+1}).
+// tb (nullable, FP32 remainders): the carried r_{k+1} is the residual of
+// B_{k+1} up to the FP32 roundings t_i of the stored remainders q_1..q_k, so
+// ||E - (I - F) B_{k+1}|| <= ||r_{k+1}|| + tb_s with tb_s the sum of their
+// bounds; rel uses that sum, and tb_s then takes the bound of q_{k+1} (from
+// cole_s = ||eps_{k+1}||^2, scaled by ex[k + 1][s]), which product k wrote.
+// Once tb_s >= tol ||E_s|| no later product can meet the tolerance: floor
+// and done are set (the host solves again with FP64 residuals).
 // Called by one whole block after col[] is complete.
 __device__ void decide(int S, const double *col, const double *e2, SolveCtl *ctl, double *hist,
-                       double tol, double *red, int *ex) {
+                       double tol, double *red, int *ex, double *tb, const double *cole,
+                       double sqrt_nf) {
   const int k = ctl->nprod;
   double m = 0.0;
+  int fl = 0;
   for (int s = threadIdx.x; s < S; s += 256) {
     const double r2 = col[s];
-    m = max_nan(m, e2[s] > 0 ? sqrt(r2 / e2[s]) : sqrt(r2));
+    if (tb) {
+      const double en = sqrt(e2[s]);
+      const double cert = sqrt(r2) + tb[s];
+      m = max_nan(m, en > 0 ? cert / en : cert);
+      const double t = tb[s] + remainder_bound(sqrt(cole[s]), ex[(int64_t)(k + 1) * S + s], sqrt_nf);
+      tb[s] = t;
+      if (!(t < tol * (en > 0 ? en : 1.0))) fl = 1;
+    } else {
+      m = max_nan(m, e2[s] > 0 ? sqrt(r2 / e2[s]) : sqrt(r2));
+    }
     if (ex) ex[(int64_t)(k + 2) * S + s] = half_scale_exp(r2);
   }
+  if (fl) ctl->floor = 1;  // any thread; read after the barriers below
   __syncthreads();
   red[threadIdx.x] = m;
   __syncthreads();
@@ -441,16 +554,47 @@ __device__ void decide(int S, const double *col, const double *e2, SolveCtl *ctl
     const double rel = red[0];
     hist[k] = rel;
     ctl->nprod = k + 1;
-    if (rel <= tol) ctl->done = 1;
+    if (rel <= tol || ctl->floor) ctl->done = 1;
   }
 }
 
 // Adds the G partials of columns [b cw, (b + 1) cw) (block b) in a fixed
 // order into col[].  record = 1: nothing once done is set, and a single block
 // also takes the stop decision (with several blocks decide_kernel follows).
+// First stage of the wide product's column sums: its G partial rows (two
+// sets of G with FP32 remainders: blockIdx.z) reduced to WIDE_SUM_P rows per
+// set, out[set][p][s].  Block (c, p, set) sums columns [64 c, 64 c + 64) over
+// row chunk p: wave w takes rows w, w + 4, ... of the chunk in order (a
+// coalesced 512-byte read per row), then the waves are added in order 0..3.
+constexpr int WIDE_SUM_P = 64;
+
+__global__ __launch_bounds__(256) void wide_colsum_kernel(int G, int S,
+                                                          const double *__restrict__ part,
+                                                          double *__restrict__ out,
+                                                          const int *__restrict__ done) {
+  if (*done) return;
+  __shared__ double red[4][64];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int s = blockIdx.x * 64 + lane, p = blockIdx.y, set = blockIdx.z;
+  const int per = (G + WIDE_SUM_P - 1) / WIDE_SUM_P;
+  const int r0 = std::min(G, p * per), r1 = std::min(G, r0 + per);
+  const double *in = part + (int64_t)set * G * S;
+  double acc = 0.0;
+  if (s < S)
+    for (int r = r0 + w; r < r1; r += 4) acc += in[(int64_t)r * S + s];
+  red[w][lane] = acc;
+  __syncthreads();
+  if (w == 0 && s < S) {
+    double t = red[0][lane] + red[1][lane];
+    t = t + red[2][lane];
+    t = t + red[3][lane];
+    out[((int64_t)set * WIDE_SUM_P + p) * S + s] = t;
+  }
+}
+
 // record = 0 (the solve's start, col = ||E_s||^2) also resets the control
-// block, and with ex (the mixed method) ex[0][s] and ex[1][s] receive the
-// scale exponents of d_0 and d_1.
+// block (and tb, when given), and with ex (the mixed method) ex[0][s] and
+// ex[1][s] receive the scale exponents of d_0 and d_1.
 constexpr int COLNORM_CW = 32;
 
 __global__ __launch_bounds__(256) void colnorm_final_kernel(int G, int S, int cw,
@@ -459,24 +603,35 @@ __global__ __launch_bounds__(256) void colnorm_final_kernel(int G, int S, int cw
                                                             const double *__restrict__ e2,
                                                             SolveCtl *__restrict__ ctl,
                                                             double *__restrict__ hist, double tol,
-                                                            int record, int *__restrict__ ex) {
+                                                            int record, int *__restrict__ ex,
+                                                            double *__restrict__ tb,
+                                                            double *__restrict__ cole,
+                                                            double sqrt_nf) {
   if (record && ctl->done) return;
   if (!record && blockIdx.x == 0 && threadIdx.x == 0) {
     ctl->done = 0;
     ctl->nprod = 0;
+    ctl->floor = 0;
   }
   __shared__ double red[256];
   const int c0 = blockIdx.x * cw;
   const int cwb = std::min(cw, S - c0);
   block_colsum<false>(0, G, S, c0, cwb, part, (const double *)nullptr, red, col);
+  // cole (record, FP32 remainders): the remainders' sums, G rows after the residuals'
+  if (record && cole)
+    block_colsum<false>(0, G, S, c0, cwb, part + (int64_t)G * S, (const double *)nullptr, red,
+                        cole);
   if (record && gridDim.x == 1) {
     __syncthreads();
-    decide(S, col, e2, ctl, hist, tol, red, ex);
-  } else if (!record && ex && (int)threadIdx.x < cwb) {
+    decide(S, col, e2, ctl, hist, tol, red, ex, tb, cole, sqrt_nf);
+  } else if (!record && (int)threadIdx.x < cwb) {
     // thread t wrote col[c0 + t] itself
-    const int e = half_scale_exp(col[c0 + threadIdx.x]);
-    ex[c0 + threadIdx.x] = e;
-    ex[S + c0 + threadIdx.x] = e;
+    if (ex) {
+      const int e = half_scale_exp(col[c0 + threadIdx.x]);
+      ex[c0 + threadIdx.x] = e;
+      ex[S + c0 + threadIdx.x] = e;
+    }
+    if (tb) tb[c0 + threadIdx.x] = 0.0;
   }
 }
 
@@ -484,10 +639,12 @@ __global__ __launch_bounds__(256) void decide_kernel(int S, const double *__rest
                                                      const double *__restrict__ e2,
                                                      SolveCtl *__restrict__ ctl,
                                                      double *__restrict__ hist, double tol,
-                                                     int *__restrict__ ex) {
+                                                     int *__restrict__ ex, double *__restrict__ tb,
+                                                     const double *__restrict__ cole,
+                                                     double sqrt_nf) {
   if (ctl->done) return;
   __shared__ double red[256];
-  decide(S, col, e2, ctl, hist, tol, red, ex);
+  decide(S, col, e2, ctl, hist, tol, red, ex, tb, cole, sqrt_nf);
 }
 
 // Queue-only solves (a time loop's steps): after the fixed product budget,
@@ -721,18 +878,26 @@ namespace {
 // Sparse product of one sweep, dispatched by S (see the kernels above).
 // Returns the number of column-sum partial rows written to part (0: none,
 // the rows kernel; the caller then runs colnorm_partial_kernel).
+// Row blocks per column chunk of the wide product (a multiple of 8, about
+// 4 SWEEP_RPW rows each on average).
+int64_t wide_blocks(int64_t nf) {
+  return ((nf + 4 * SWEEP_RPW - 1) / (4 * SWEEP_RPW) + 7) / 8 * 8;
+}
+
+// rblk (the wide product's row blocks, wide_blocks(nf) + 1 entries) is
+// filled by row_blocks_kernel once per solve.
 template <class TX>
 int sweep_product(hipStream_t s, illum_ctx *ctx, const illum_sparse *F, int S, const TX *X,
                   const double *R, double *Y, const int *done, double *part, MixedIO m,
-                  const char *stage, bool timed) {
+                  const int *rblk, const char *stage, bool timed) {
   StageTimer st(ctx, stage, timed);
   const int64_t nf = F->n;
   int nparts = 0;
-  // Row groups go to the XCDs as one contiguous range each (xc = 0), or
-  // dealt in chunks of xc groups.  FP64 products gather 8-byte rows of X and
-  // favour the locality of ranges; FP16 corrections favour the balance of
-  // dealing.  Results do not depend on it: the wide kernel tunes it, the
-  // others keep ranges.  ILLUM_RAD_XCD_CHUNK fixes it.
+  // Row blocks go to the XCDs as one contiguous range each (xc = 0), or
+  // dealt in chunks of xc blocks.  The wide kernel's blocks are balanced by
+  // weight, so ranges keep each XCD's gathers local without leaving one XCD
+  // the crater floors' long rows.  Results do not depend on it: the wide
+  // kernel tunes it, the others keep ranges.  ILLUM_RAD_XCD_CHUNK fixes it.
   const char *xenv = getenv("ILLUM_RAD_XCD_CHUNK");
   m.xc = xenv ? std::max(0, atoi(xenv)) : 0;
   if (S >= 128) {
@@ -749,18 +914,32 @@ int sweep_product(hipStream_t s, illum_ctx *ctx, const illum_sparse *F, int S, c
     while (per > 1 && (S % per != 0 || 64 * per > S ||
                        (double)nf * 64 * per * sizeof(TX) > budget))
       per >>= 1;
-    const int64_t ngrp = (nf + 4 * SWEEP_RPW - 1) / (4 * SWEEP_RPW);
-    const int ngrp8 = (int)((ngrp + 7) / 8 * 8);
+    const int nblk = (int)wide_blocks(nf);
+    // Buffer gathers address X by 32-bit byte offsets: X must span < 2 GiB
+    // (81,920 facets x 512 right-hand sides in FP64: 335 MB).
+    const uint64_t xbytes = (uint64_t)nf * S * sizeof(TX);
+    const bool buf = xbytes <= 0x7fffffffull;
     auto go = [&](int P) {
       const int nch = (S + 64 * P - 1) / (64 * P);
-      const unsigned grid = (unsigned)((int64_t)ngrp8 * nch);
-#define ILLUM_SWEEP_WIDE(Q)                                                                  \
-  sweep_wide_kernel<TX, Q><<<grid, 256, 0, s>>>(nf, S, ngrp8, F->d_rowptr, F->d_colidx,       \
-                                                F->d_rval, X, R, Y, done, part, m)
-      if (P == 8) ILLUM_SWEEP_WIDE(8);
-      else if (P == 4) ILLUM_SWEEP_WIDE(4);
-      else if (P == 2) ILLUM_SWEEP_WIDE(2);
-      else ILLUM_SWEEP_WIDE(1);
+      const unsigned grid = (unsigned)((int64_t)nblk * nch);
+#define ILLUM_SWEEP_WIDE(Q, B)                                                               \
+  sweep_wide_kernel<TX, Q, B><<<grid, 256, 0, s>>>(nf, S, nblk, rblk, F->d_rowptr,            \
+                                                   F->d_colidx, F->d_rval, X,                 \
+                                                   (unsigned)xbytes, R, Y, done, part, m)
+#define ILLUM_SWEEP_WIDE_P(Q)                                                                \
+  do {                                                                                       \
+    constexpr int xb = (int)sizeof(TX) * Q;                                                  \
+    if constexpr (xb == 4 || xb == 8 || xb == 16) {                                          \
+      if (buf) ILLUM_SWEEP_WIDE(Q, true); else ILLUM_SWEEP_WIDE(Q, false);                   \
+    } else {                                                                                 \
+      ILLUM_SWEEP_WIDE(Q, false);                                                            \
+    }                                                                                        \
+  } while (0)
+      if (P == 8) ILLUM_SWEEP_WIDE_P(8);
+      else if (P == 4) ILLUM_SWEEP_WIDE_P(4);
+      else if (P == 2) ILLUM_SWEEP_WIDE_P(2);
+      else ILLUM_SWEEP_WIDE_P(1);
+#undef ILLUM_SWEEP_WIDE_P
 #undef ILLUM_SWEEP_WIDE
       ILLUM_LAUNCH_CHECK();
     };
@@ -778,7 +957,7 @@ int sweep_product(hipStream_t s, illum_ctx *ctx, const illum_sparse *F, int S, c
                       {0, 1, 16}, 0, [&](int xc) { m.xc = xc; go(per); }, timed);
     }
     go(per);
-    nparts = ngrp8;
+    nparts = nblk;
   } else if (S > SPMM_NNZ_MAXS) {
     sweep_rows_kernel<TX><<<(unsigned)((nf + 3) / 4), 256, 0, s>>>(
         nf, S, F->d_rowptr, F->d_colidx, F->d_rval, X, R, Y, done, m);
@@ -808,6 +987,46 @@ bool overlap(const void *a, const void *b, size_t bytes) {
 constexpr int RAD_RING = 8;
 constexpr int RAD_FOLD = RAD_RING - 1;
 
+// Partial-sum blocks of the colnorm_partial_kernel pass (products that do
+// not sum their own residual norms).
+int colnorm_blocks(int64_t n) { return (int)std::min<int64_t>(1024, std::max<int64_t>(1, n / 16)); }
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Byte offsets of a solve's control workspace: control block, residual
+// history, column norms of E and of the residual, scale exponents, remainder
+// bounds, the wide product's row blocks and the partial column sums.
+// (The wide product's partial sums take two sets of rows: the residuals'
+// and, with FP32 remainders, the remainders'.)
+struct CtlLayout {
+  size_t hist, e2, col, cole, ex, tb, rblk, part2, part, bytes;
+  CtlLayout(int64_t nf, int S, int max_iter) {
+    const bool wide = S >= 128;
+    const int64_t nblk = wide ? wide_blocks(nf) : 0;
+    const int64_t gparts = std::max<int64_t>(
+        {(int64_t)colnorm_blocks(nf * S), (nf + SPMM_NNZ_ROWS - 1) / SPMM_NNZ_ROWS, 2 * nblk});
+    hist = 256;
+    e2 = hist + align256(8 * ((size_t)max_iter + 1));
+    col = e2 + align256(8 * (size_t)S);
+    cole = col + align256(8 * (size_t)S);
+    ex = cole + align256(8 * (size_t)S);
+    tb = ex + align256(4 * ((size_t)max_iter + 2) * S);
+    rblk = tb + align256(8 * (size_t)S);
+    part2 = rblk + align256(4 * ((size_t)nblk + 1));
+    part = part2 + align256(wide ? 8 * 2 * (size_t)WIDE_SUM_P * S : 0);
+    bytes = part + 8 * (size_t)gparts * S;
+  }
+};
+
+// FP32 remainders (ring mode) need a tolerance their rounding cannot reach:
+// each stored remainder loses at most 2^-35 of the residual it came from, so
+// their sum stays near 2^-35 ||E|| / (1 - rho(F)).  Tighter tolerances, and a
+// solve whose rounding bound reaches the tolerance after all, use FP64
+// residuals.
+constexpr double RAD_REMAINDER_MIN_TOL = 1e-9;
+// radiosity_solve flag (not part of the C ABI): FP64 residuals in ring mode.
+constexpr int RAD_FP64_RESIDUALS = 1 << 30;
+
 }  // namespace
 
 // Scratch bytes of a solve (the pieces RadScratch hands out, each 256-byte
@@ -816,16 +1035,9 @@ size_t radiosity_scratch_bytes(int64_t nf, int S, int method, int max_iter) {
   const int64_t n = nf * S;
   const bool mixed = method == ILLUM_RAD_MIXED, wide = S >= 128, ring = mixed && wide;
   const int R = ring ? RAD_RING : 2;
-  const int G = (int)std::min<int64_t>(1024, std::max<int64_t>(1, n / 16));
-  const int64_t ngrp8 = ((nf + 4 * SWEEP_RPW - 1) / (4 * SWEEP_RPW) + 7) / 8 * 8;
-  const int64_t gparts = std::max<int64_t>(
-      {(int64_t)G, (nf + SPMM_NNZ_ROWS - 1) / SPMM_NNZ_ROWS, wide ? ngrp8 : 0});
-  auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  const size_t ctl = 256 + align(8 * ((size_t)max_iter + 1)) + 2 * align(8 * (size_t)S) +
-                     align(4 * ((size_t)max_iter + 2) * S) + 8 * (size_t)gparts * S;
-  size_t b = align(ctl) + 2 * align(8 * (size_t)n);
-  if (mixed) b += align(8 * (size_t)n) + align(2 * (size_t)R * n);
-  if (S > 1) b += align(8 * (size_t)n);
+  size_t b = align256(CtlLayout(nf, S, max_iter).bytes) + 2 * align256(8 * (size_t)n);
+  if (mixed) b += align256(8 * (size_t)n) + align256(2 * (size_t)R * n);
+  if (S > 1) b += align256(8 * (size_t)n);
   return b;
 }
 
@@ -867,20 +1079,26 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
   // Scratch from the context's workspace (no allocation in a time loop):
   // control block, residual history, column norms, scale exponents and
   // partial sums.
-  const int G = (int)std::min<int64_t>(1024, std::max<int64_t>(1, n / 16));
-  const int64_t ngrp8 = ((nf + 4 * SWEEP_RPW - 1) / (4 * SWEEP_RPW) + 7) / 8 * 8;
-  const int64_t gparts = std::max<int64_t>(
-      {(int64_t)G, (nf + SPMM_NNZ_ROWS - 1) / SPMM_NNZ_ROWS, wide ? ngrp8 : 0});
-  auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  const size_t o_hist = 256, o_e2 = o_hist + align(8 * ((size_t)max_iter + 1)),
-               o_col = o_e2 + align(8 * (size_t)S), o_ex = o_col + align(8 * (size_t)S),
-               o_part = o_ex + align(4 * ((size_t)max_iter + 2) * S),
-               ctl_bytes = o_part + 8 * (size_t)gparts * S;
-  char *ws = (char *)scr.get(WS_RAD_CTL, ctl_bytes);
+  const int G = colnorm_blocks(n);
+  const CtlLayout L(nf, S, max_iter);
+  const size_t o_hist = L.hist;
+  char *ws = (char *)scr.get(WS_RAD_CTL, L.bytes);
   SolveCtl *ctl = (SolveCtl *)ws;
-  double *hist = (double *)(ws + o_hist), *e2 = (double *)(ws + o_e2),
-         *col = (double *)(ws + o_col), *part = (double *)(ws + o_part);
-  int *ex = mixed ? (int *)(ws + o_ex) : nullptr;
+  double *hist = (double *)(ws + L.hist), *e2 = (double *)(ws + L.e2),
+         *col = (double *)(ws + L.col), *part = (double *)(ws + L.part),
+         *part2 = (double *)(ws + L.part2);
+  int *ex = mixed ? (int *)(ws + L.ex) : nullptr;
+  // Ring mode keeps the residual as FP32 remainders q_k (host-driven solves
+  // at a tolerance they can reach, ILLUM_RAD_REMAINDER=0/1 to force), with
+  // the bound of their roundings in tb and their norms in cole.
+  const char *renv = getenv("ILLUM_RAD_REMAINDER");
+  const bool remainders =
+      ring && !queue_only && !(flags & RAD_FP64_RESIDUALS) &&
+      (renv ? atoi(renv) != 0 : rel_tol >= RAD_REMAINDER_MIN_TOL);
+  double *tb = remainders ? (double *)(ws + L.tb) : nullptr;
+  double *cole = remainders ? (double *)(ws + L.cole) : nullptr;
+  int *rblk = wide ? (int *)(ws + L.rblk) : nullptr;
+  const double sqrt_nf = std::sqrt((double)nf);
   const int *done = &ctl->done;
 
   // Right-hand sides, row-major FP64.  With S = 1 both layouts coincide and a
@@ -918,7 +1136,12 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
                                                                             Ein, part);
     ILLUM_LAUNCH_CHECK();
     colnorm_final_kernel<<<(unsigned)((S + COLNORM_CW - 1) / COLNORM_CW), 256, 0, s>>>(
-        P, S, COLNORM_CW, part, e2, nullptr, ctl, nullptr, 0.0, 0, ex);
+        P, S, COLNORM_CW, part, e2, nullptr, ctl, nullptr, 0.0, 0, ex, tb, nullptr, sqrt_nf);
+    ILLUM_LAUNCH_CHECK();
+  }
+  if (wide) {
+    const int nblk = (int)wide_blocks(nf);
+    row_blocks_kernel<<<(unsigned)((nblk + 256) / 256), 256, 0, s>>>(nf, F->d_rowptr, nblk, rblk);
     ILLUM_LAUNCH_CHECK();
   }
   const double *Er = Ein;
@@ -933,19 +1156,23 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
     mixed_init_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, S, Er, ex, Dring, Bacc);
     ILLUM_LAUNCH_CHECK();
   }
-  // The per-sweep column sums: 4 columns per block over the wide product's
-  // many partial rows, else COLNORM_CW.
-  const int cw = wide ? 4 : COLNORM_CW;
+  // The per-sweep column sums, COLNORM_CW columns per block (the wide
+  // product's many partial rows first reduced to WIDE_SUM_P by
+  // wide_colsum_kernel).
+  const int cw = COLNORM_CW;
   const unsigned fb = (unsigned)((S + cw - 1) / cw);
 
   // Sweep k's buffers.  FP64 Jacobi, B_{k+1} = E + F B_k from B_0 = E:
   // cur_k = E, X1, X0, X1, ...; nxt_k = X1, X0, X1, ...  Mixed refinement,
-  // r_0 = E: r_k = E, X0, X1, X0, ...; r_{k+1} = X0, X1, X0, ...; d_k in
-  // slot k % R, its exponents in ex[k].
+  // r_0 = E: B mode r_k = E, X0, X1, X0, ...; r_{k+1} = X0, X1, X0, ...;
+  // ring mode keeps the FP32 remainder q_k in X0 (k even) or X1 (k odd)
+  // instead, q_0 unused (the first product forms eps_0 from E); d_k in slot
+  // k % R, its exponents in ex[k].
   auto cur = [&](int k) -> const double * { return k == 0 ? Er : (k & 1 ? X1 : X0); };
   auto nxt = [&](int k) -> double * { return k & 1 ? X0 : X1; };
   auto rk = [&](int k) -> const double * { return k == 0 ? Er : (k & 1 ? X0 : X1); };
   auto rk1 = [&](int k) -> double * { return k & 1 ? X1 : X0; };
+  auto qk = [&](int k) -> float * { return (float *)(k & 1 ? X1 : X0); };
   auto slot = [&](int j) { return Dring + (int64_t)(j % R) * n; };
   int folded = 0;  // ring mode: delta_j for j < min(folded, nprod) are in Bacc
   auto sweep = [&](int k) {
@@ -960,29 +1187,40 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
     const double *Xn = nullptr;  // subtracted in a separate residual norm (FP64: cur)
     if (!mixed) {
       // nxt = E + F cur; the residual of cur is nxt - cur
-      nparts = sweep_product<double>(s, ctx, F, S, cur(k), Er, nxt(k), done, part,
-                                     MixedIO{nullptr, nullptr, nullptr, nullptr},
-                                     "radiosity_fp64_sweep", !noalloc);
+      nparts = sweep_product<double>(s, ctx, F, S, cur(k), Er, nxt(k), done, part, MixedIO{},
+                                     rblk, "radiosity_fp64_sweep", !noalloc);
       Y = nxt(k);
       Xn = cur(k);
     } else {
       // r_{k+1} = (r_k - delta_k) + F delta_k, d_{k+1} = fp16(r_{k+1} 2^ex[k+1])
       MixedIO m{ex + (int64_t)k * S, ex + (int64_t)(k + 1) * S, slot(k + 1),
                 ring ? nullptr : Bacc};
-      nparts = sweep_product<_Float16>(s, ctx, F, S, slot(k), rk(k), rk1(k), done, part, m,
+      if (remainders) {
+        m.Qk = k == 0 ? nullptr : qk(k);
+        m.Qn = qk(k + 1);
+      }
+      nparts = sweep_product<_Float16>(s, ctx, F, S, slot(k), rk(k),
+                                       remainders ? nullptr : rk1(k), done, part, m, rblk,
                                        "radiosity_mixed_sweep", !noalloc);
       Y = rk1(k);
     }
+    const double *psum = part;
     if (!nparts) {
       colnorm_partial_kernel<double><<<G, 256, 0, s>>>(nf, S, Y, Xn, part, done);
       ILLUM_LAUNCH_CHECK();
       nparts = G;
+    } else if (wide) {
+      wide_colsum_kernel<<<dim3((unsigned)((S + 63) / 64), WIDE_SUM_P, remainders ? 2 : 1), 256,
+                           0, s>>>(nparts, S, part, part2, done);
+      ILLUM_LAUNCH_CHECK();
+      nparts = WIDE_SUM_P;
+      psum = part2;
     }
-    colnorm_final_kernel<<<fb, 256, 0, s>>>(nparts, S, cw, part, col, e2, ctl, hist, rel_tol, 1,
-                                            ex);
+    colnorm_final_kernel<<<fb, 256, 0, s>>>(nparts, S, cw, psum, col, e2, ctl, hist, rel_tol, 1,
+                                            ex, tb, cole, sqrt_nf);
     ILLUM_LAUNCH_CHECK();
     if (fb > 1) {
-      decide_kernel<<<1, 256, 0, s>>>(S, col, e2, ctl, hist, rel_tol, ex);
+      decide_kernel<<<1, 256, 0, s>>>(S, col, e2, ctl, hist, rel_tol, ex, tb, cole, sqrt_nf);
       ILLUM_LAUNCH_CHECK();
     }
   };
@@ -1039,7 +1277,8 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
   // The control block is device-written: its product count sizes the host
   // copies below, so it is checked against what was launched before use.
   auto check_ctl = [&]() {
-    if (h.nprod < 0 || h.nprod > launched || launched > max_iter || (h.done != 0 && h.done != 1))
+    if (h.nprod < 0 || h.nprod > launched || launched > max_iter || (h.done != 0 && h.done != 1) ||
+        (h.floor != 0 && (h.floor != 1 || !remainders)))
       throw Error(ILLUM_ERR_HIP, "radiosity solve: inconsistent device control block (" +
                                      std::to_string(h.nprod) + " products of " +
                                      std::to_string(launched) + " launched)");
@@ -1057,6 +1296,11 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
     if (h.done || launched >= max_iter) break;
     batch = std::min(2 * batch, 64);
   }
+  // The FP32 remainders' rounding bound reached the tolerance: solve again
+  // with FP64 residuals (whose output replaces this one's).
+  if (h.floor)
+    return radiosity_solve(F, E, S, method, rel_tol, max_iter, B, resid, iters,
+                           flags | RAD_FP64_RESIDUALS, fixed_scratch, fixed_cap, queue_stats);
   if (!early_out) {
     output();
     if (!(in_dev && out_dev && (flags & ILLUM_ASYNC))) sync(ctx);

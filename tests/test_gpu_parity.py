@@ -310,11 +310,13 @@ def test_radiosity_vs_gauss_seidel(api, S, method):
     assert res[-1] <= 1e-10
 
 
+@pytest.mark.parametrize("tol", [1e-10, 1e-8])
 @pytest.mark.parametrize("method", ["fp64", "mixed"])
-def test_radiosity_wide_products_chunking_is_exact(api, monkeypatch, method):
+def test_radiosity_wide_products_chunking_is_exact(api, monkeypatch, method, tol):
     # S >= 128 runs the column-chunked product: every element sums its row in
     # CSR order whatever the chunk width, so forcing narrower chunks (a small
-    # cache budget) changes nothing; odd S takes one value per lane.
+    # cache budget) changes nothing; odd S takes one value per lane.  At 1e-8
+    # the mixed method keeps FP32 remainders, at 1e-10 FP64 residuals.
     V, F = _mesh(api, 4, 40, seed=6)
     ctx = api.IllumContext(V=V, F=F)
     Fm = ctx.compute_F(0.5)
@@ -323,15 +325,76 @@ def test_radiosity_wide_products_chunking_is_exact(api, monkeypatch, method):
     for S in (128, 130, 131, 512):
         E = np.random.default_rng(S).uniform(0, 500, (nf, S))
         monkeypatch.delenv("ILLUM_SPMM_CHUNK_BYTES", raising=False)
-        B, res, it = api.radiosity_solve(Fm, E, method=method, rel_tol=1e-10, max_iter=400)
-        assert res[-1] <= 1e-10
+        B, res, it = api.radiosity_solve(Fm, E, method=method, rel_tol=tol, max_iter=400)
+        assert res[-1] <= tol
         for budget in (nf * 64 * 8 * 1.5, nf * 64 * 8 * 0.5):  # per-lane widths 1..4
             monkeypatch.setenv("ILLUM_SPMM_CHUNK_BYTES", str(budget))
-            Bc, resc, itc = api.radiosity_solve(Fm, E, method=method, rel_tol=1e-10, max_iter=400)
-            assert itc == it and np.array_equal(Bc, B), (S, budget)
+            Bc, resc, itc = api.radiosity_solve(Fm, E, method=method, rel_tol=tol, max_iter=400)
+            assert itc == it and np.array_equal(Bc, B) and np.array_equal(resc, res), (S, budget)
         for s in (0, S - 1):
             Bo, _, _ = O.gauss_seidel(nf, cp, ri, v, E[:, s], max_sweeps=200, rel_tol=1e-13)
-            np.testing.assert_allclose(B[:, s], Bo, rtol=1e-8, atol=1e-8 * np.abs(Bo).max())
+            np.testing.assert_allclose(B[:, s], Bo, rtol=10 * tol, atol=10 * tol * np.abs(Bo).max())
+
+
+def _true_rel_resid(Fm, E, B):
+    import scipy.sparse as sps
+    cp, ri, v = Fm.csc()
+    A = sps.csc_matrix((v, ri, cp), shape=(len(E), len(E)))
+    nE = np.linalg.norm(E, axis=0)
+    nr = np.linalg.norm(E - (B - A @ B), axis=0)
+    return nr, nE
+
+
+@pytest.mark.parametrize("tol", [1e-8, 1e-9])
+def test_radiosity_fp32_remainders_bound_the_true_residual(api, monkeypatch, tol):
+    # Ring mode (S >= 128) at tolerances down to 1e-9 keeps the residual as
+    # FP32 remainders of the FP16 corrections.  The reported residual adds a
+    # bound on the remainders' roundings, so per column it bounds the residual
+    # recomputed in FP64 on the host; the solve takes the products the
+    # FP64-residual solve takes (ILLUM_RAD_REMAINDER=0) and agrees with it.
+    V, F = _mesh(api, 4, 40, seed=6)
+    ctx = api.IllumContext(V=V, F=F)
+    Fm = ctx.compute_F(0.5)
+    nf, S = len(F), 256
+    E = np.random.default_rng(3).uniform(0, 500, (nf, S)) * 10.0 ** np.linspace(-20, 20, S)[None, :]
+    E[:, 3] = 0.0
+    monkeypatch.delenv("ILLUM_RAD_REMAINDER", raising=False)
+    B, res, it = api.radiosity_solve(Fm, E, method="mixed", rel_tol=tol, max_iter=400)
+    monkeypatch.setenv("ILLUM_RAD_REMAINDER", "0")
+    B0, res0, it0 = api.radiosity_solve(Fm, E, method="mixed", rel_tol=tol, max_iter=400)
+    assert res[-1] <= tol and res0[-1] <= tol
+    assert abs(it - it0) <= 1, (it, it0)
+    nr, nE = _true_rel_resid(Fm, E, B)
+    ok = nE > 0
+    assert np.all(nr[ok] <= res[-1] * nE[ok]), (nr[ok] / nE[ok]).max()
+    assert np.all(B[:, 3] == 0.0)
+    scale = np.abs(B0).max(axis=0)
+    assert np.all(np.abs(B - B0)[:, ok].max(axis=0) <= 4 * tol * scale[ok])
+
+
+def test_radiosity_fp32_remainder_floor_solves_again_with_fp64_residuals(api, monkeypatch):
+    # Forced FP32 remainders with albedo 0.9.  At 1e-12 their rounding bound
+    # (about 2^-35 of the first residual) stays below the tolerance: the solve
+    # converges with the bound included, which bounds the residual recomputed
+    # on the host.  At 1e-14 the bound passes the tolerance after the first
+    # product, so the solve is taken again with FP64 residuals and returns
+    # exactly what that mode returns (without the second solve it could not
+    # converge).
+    V, F = _mesh(api, 4, 40, seed=6)
+    ctx = api.IllumContext(V=V, F=F)
+    Fm = ctx.compute_F(0.9)
+    nf, S = len(F), 128
+    E = np.random.default_rng(5).uniform(0, 500, (nf, S))
+    monkeypatch.setenv("ILLUM_RAD_REMAINDER", "1")
+    B, res, it = api.radiosity_solve(Fm, E, method="mixed", rel_tol=1e-12, max_iter=400)
+    assert res[-1] <= 1e-12
+    nr, nE = _true_rel_resid(Fm, E, B)
+    assert np.all(nr <= res[-1] * nE), (nr / nE).max()
+    B1, res1, it1 = api.radiosity_solve(Fm, E, method="mixed", rel_tol=1e-14, max_iter=400)
+    monkeypatch.setenv("ILLUM_RAD_REMAINDER", "0")
+    B0, res0, it0 = api.radiosity_solve(Fm, E, method="mixed", rel_tol=1e-14, max_iter=400)
+    assert it1 == it0 and np.array_equal(B1, B0) and np.array_equal(res1, res0)
+    assert res1[-1] <= 1e-14
 
 
 @pytest.mark.parametrize("S", [5, 128])
