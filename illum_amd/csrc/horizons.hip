@@ -24,7 +24,15 @@ namespace illum {
 // stack of `scap` entries in dynamic LDS (a walk needing more falls back to
 // the threaded walk, same answer); LAYOUT 2: the two-wide layout with a
 // (depth + 1)-entry stack; LAYOUT 0: the stackless threaded walk.
-template <int LAYOUT>
+//
+// Hit cache (CACHE): a lane keeps the triangle its last hit came from and
+// tests it first; if the ray hits it, the answer (some triangle of the
+// searched trees is hit) is known without a walk.  Exact: the cached
+// triangle is one of the searched trees' triangles and the test is the
+// walk's own predicate.  The bisection's rays after a hit rise towards the
+// same ridge, and its last steps differ by under a degree, so most of them
+// hit the cached triangle again.
+template <int LAYOUT, bool CACHE>
 __global__ __launch_bounds__(256) void horizons_kernel(
     int64_t nf, int nphi, const float4 *__restrict__ pr4, const float4 *__restrict__ n4,
     const double *__restrict__ frame, const float4 *__restrict__ nodes,
@@ -47,6 +55,7 @@ __global__ __launch_bounds__(256) void horizons_kernel(
   const double F02 = F[6], F12 = F[7], F22 = F[8];  // column n
   const double cp = cphi[k], sp = sphi[k];
   int m = 1;
+  int last = -1;  // leaf-order index of the last hit triangle (CACHE)
   for (;;) {
     const double st = th[4 * m + 1], ct = th[4 * m + 2];
     const double vx = cp * st, vy = sp * st, vz = ct;
@@ -58,15 +67,18 @@ __global__ __launch_bounds__(256) void horizons_kernel(
     // (one body, or an occluder mesh).
     const int b0 = body_of ? body_of[j] : 0, b1 = body_of ? b0 + 1 : nroots;
     bool hit = false;
+    int *ht = CACHE ? &last : nullptr;
+    if (CACHE && last >= 0)
+      hit = tri_any(r, tris[3 * last], tris[3 * last + 1], tris[3 * last + 2]);
     for (int b = b0; b < b1 && !hit; ++b) {
       if (LAYOUT == 4) {
         const int a = trace_any_wide4(wide4, tris, r, roots[4 * b + 2], hstack + threadIdx.x,
-                                      256, scap);
-        hit = a == -2 ? trace_any(nodes, tris, r, roots[4 * b]) : a == 1;
+                                      256, scap, ht);
+        hit = a == -2 ? trace_any(nodes, tris, r, roots[4 * b], ht) : a == 1;
       } else if (LAYOUT == 2) {
-        hit = trace_any_wide(wide, tris, r, roots[4 * b + 1], hstack + threadIdx.x, 256);
+        hit = trace_any_wide(wide, tris, r, roots[4 * b + 1], hstack + threadIdx.x, 256, ht);
       } else {
-        hit = trace_any(nodes, tris, r, roots[4 * b]);
+        hit = trace_any(nodes, tris, r, roots[4 * b], ht);
       }
     }
     m = 2 * m + (hit ? 0 : 1);
@@ -234,8 +246,12 @@ void horizons(illum_bvh *bvh, illum_bvh *occl, int nphi, double theta_eps, doubl
           d_s.p, d_tab.p, tree_nodes, dst, target->d_roots, target->nbody,
           occl ? nullptr : mesh->d_body);
     } else {
-      auto kern = layout == 4 ? horizons_kernel<4> : layout == 2 ? horizons_kernel<2>
-                                                                 : horizons_kernel<0>;
+      // ILLUM_HZ_CACHE=0 turns the hit cache off (same horizons, for comparison)
+      const char *ce = getenv("ILLUM_HZ_CACHE");
+      const bool cache = !(ce && std::string(ce) == "0");
+      auto kern = layout == 4 ? (cache ? horizons_kernel<4, true> : horizons_kernel<4, false>)
+                  : layout == 2 ? (cache ? horizons_kernel<2, true> : horizons_kernel<2, false>)
+                                : (cache ? horizons_kernel<0, true> : horizons_kernel<0, false>);
       kern<<<(unsigned)((total + 255) / 256), 256, lds, s>>>(
           nf, nphi, (const float4 *)mesh->d_pr4, (const float4 *)mesh->d_n4, mesh->d_frame,
           (const float4 *)target->d_nodes, (const float4 *)target->d_wide,

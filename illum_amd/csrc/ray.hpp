@@ -563,10 +563,12 @@ __device__ __forceinline__ int trace_closest_wide4(const float4 *__restrict__ w4
 }
 
 // Any hit (t > 0) over the four-wide tree rooted at `root`: returns 1 on a
-// hit, 0 for none, -2 when a push would exceed `cap` stack entries.
+// hit, 0 for none, -2 when a push would exceed `cap` stack entries.  On a
+// hit, *hit_tri (nullable) receives the hit triangle's leaf-order index.
 __device__ __forceinline__ int trace_any_wide4(const float4 *__restrict__ w4,
                                                const float4 *__restrict__ tris, const DRay &r,
-                                               int root, int *stack, int stride, int cap) {
+                                               int root, int *stack, int stride, int cap,
+                                               int *hit_tri = nullptr) {
   int sp = 0, cur = root;
   for (;;) {
     if (cur >= 0) {
@@ -594,8 +596,10 @@ __device__ __forceinline__ int trace_any_wide4(const float4 *__restrict__ w4,
       for (int k = 0; k < cnt; ++k) {
         float t;
         if (tri_any(r, tris[3 * (first + k)], tris[3 * (first + k) + 1],
-                    tris[3 * (first + k) + 2]))
+                    tris[3 * (first + k) + 2])) {
+          if (hit_tri) *hit_tri = first + k;
           return 1;
+        }
       }
     }
     if (sp == 0) return 0;
@@ -605,10 +609,12 @@ __device__ __forceinline__ int trace_any_wide4(const float4 *__restrict__ w4,
 }
 
 // Any hit (t > 0) over the two-wide tree rooted at `root`, nearer child
-// first (stack as in trace_closest_wide; the tree depth must fit it).
+// first (stack as in trace_closest_wide; the tree depth must fit it).  On a
+// hit, *hit_tri (nullable) receives the hit triangle's leaf-order index.
 __device__ __forceinline__ bool trace_any_wide(const float4 *__restrict__ wide,
                                                const float4 *__restrict__ tris, const DRay &r,
-                                               int root, int *stack, int stride) {
+                                               int root, int *stack, int stride,
+                                               int *hit_tri = nullptr) {
   const float inf = __builtin_huge_valf();
   int sp = 0;
   int cur = root;
@@ -635,8 +641,10 @@ __device__ __forceinline__ bool trace_any_wide(const float4 *__restrict__ wide,
       for (int k = 0; k < cnt; ++k) {
         float t;
         if (tri_any(r, tris[3 * (first + k)], tris[3 * (first + k) + 1],
-                    tris[3 * (first + k) + 2]))
+                    tris[3 * (first + k) + 2])) {
+          if (hit_tri) *hit_tri = first + k;
           return true;
+        }
       }
     }
     if (sp == 0) break;
@@ -705,10 +713,11 @@ __device__ __forceinline__ bool trace_any_packet(const float4 *__restrict__ wide
   return hit;
 }
 
-// Any hit (t > 0) over the threaded tree starting at node `root`.
+// Any hit (t > 0) over the threaded tree starting at node `root`.  On a
+// hit, *hit_tri (nullable) receives the hit triangle's leaf-order index.
 __device__ __forceinline__ bool trace_any(const float4 *__restrict__ nodes,
                                           const float4 *__restrict__ tris, const DRay &r,
-                                          int root) {
+                                          int root, int *hit_tri = nullptr) {
   const float inf = __builtin_huge_valf();
   int node = root;
   while (node >= 0) {
@@ -720,8 +729,10 @@ __device__ __forceinline__ bool trace_any(const float4 *__restrict__ nodes,
     for (int k = 0; k < cnt; ++k) {
       float t;
       if (tri_any(r, tris[3 * (first + k)], tris[3 * (first + k) + 1],
-                  tris[3 * (first + k) + 2]))
+                  tris[3 * (first + k) + 2])) {
+        if (hit_tri) *hit_tri = first + k;
         return true;
+      }
     }
     node = miss;
   }

@@ -147,13 +147,17 @@ def test_cluster_cull_does_not_change_results(api, monkeypatch):
 
 
 def test_horizon_traversal_choice_does_not_change_results(api, monkeypatch):
+    # Every walk, with the per-lane hit cache on (default) and off, gives the
+    # same horizons.
     V, F = _mesh(api, 3, 40, seed=9)
     ctx = api.IllumContext(V=V, F=F)
     a = ctx.make_horizons(nphi=33).copy()
-    for walk in ("threaded", "wide4", "packet"):
-        monkeypatch.setenv("ILLUM_HZ_TRAVERSAL", walk)
-        b = ctx.make_horizons(nphi=33)
-        assert np.array_equal(a, b), walk
+    for cache in ("1", "0"):
+        monkeypatch.setenv("ILLUM_HZ_CACHE", cache)
+        for walk in ("default", "threaded", "wide4", "packet"):
+            monkeypatch.setenv("ILLUM_HZ_TRAVERSAL", walk)
+            b = ctx.make_horizons(nphi=33)
+            assert np.array_equal(a, b), (walk, cache)
 
 
 def test_form_factors_albedo_vector(api):
