@@ -27,6 +27,7 @@ ILLUM_ASYNC = 8
 ILLUM_FF_KEEP_VISIBILITY = 16
 ILLUM_SUN_PER_BODY = 32
 ILLUM_TL_EAGER = 64
+ILLUM_Q_HELD = 128
 ILLUM_RAD_JACOBI_FP64 = 1
 ILLUM_RAD_MIXED = 2
 ILLUM_THERM_FP64 = 1

@@ -654,6 +654,22 @@ class ThermalModel:
             call("illum_thermal_run", self.h, dt, _ptr(q), n, 0)
         self.t += dt * n
 
+    def run_held(self, dt, Q, nsteps):
+        """nsteps consecutive steps under one flux Q of shape (N,), held for
+        all of them (one launch sequence; the same steps as nsteps calls of
+        step(dt, Q))."""
+        self._check_q(Q)
+        if nsteps < 1:
+            raise _lib.IllumError(_lib.ILLUM_ERR_ARG, "nsteps must be >= 1")
+        flags = _lib.ILLUM_Q_HELD
+        if _is_dev(Q):
+            call("illum_thermal_run", self.h, dt, _ptr(Q), nsteps,
+                 flags | _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_ASYNC)
+        else:
+            q = np.ascontiguousarray(Q, np.float64)
+            call("illum_thermal_run", self.h, dt, _ptr(q), nsteps, flags)
+        self.t += dt * nsteps
+
     @property
     def T(self):
         T = np.zeros((self.nf, self.nz_max + 1))

@@ -60,10 +60,17 @@ def run_bodies(bodies, suns, S, nphi=361, albedo=0.1, method="mixed", thermal="m
     base = B.ptr.value
 
     def conduction():
-        for k in range(steps):
+        # step k is driven by B column (k S) // steps: the steps of one column
+        # run as one launch sequence with that column held
+        k = 0
+        while k < steps:
             s = (k * S) // steps
-            _lib.call("illum_thermal_step", th.h, dt, C.c_void_p(base + 8 * nf * s),
-                      _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_ASYNC)
+            k1 = k
+            while k1 < steps and (k1 * S) // steps == s:
+                k1 += 1
+            _lib.call("illum_thermal_run", th.h, dt, C.c_void_p(base + 8 * nf * s), k1 - k,
+                      _lib.ILLUM_IN_ON_DEVICE | _lib.ILLUM_ASYNC | _lib.ILLUM_Q_HELD)
+            k = k1
 
     stage("conduction", conduction)
     sun_d.close()

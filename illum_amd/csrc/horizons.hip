@@ -522,14 +522,20 @@ __global__ __launch_bounds__(256) void direct_wave_kernel(
 // (direct_kernel<L>), or direct_wave_kernel<P> for L = 0 (P = 1) and
 // L = -4, -8, -16 (P = -L).  The shape does not change E (the lanes only
 // share integer counts), so it is picked by the autotuner per pair-count
-// scale (ILLUM_DIRECT_LANES=-16|-8|-4|0|1|2|4|8|16 overrides).
+// scale (ILLUM_DIRECT_LANES=-16|-8|-4|0|1|2|4|8|16 overrides).  The trials
+// cover the first DIRECT_TUNE_PAIRS pairs only (a grid over a prefix of the
+// pairs writes exactly their E, which the launch after the trials writes
+// again): a batch of 64 bodies x 32 suns has 1.7e8 pairs, and timing nine
+// shapes twice over all of them cost several times the launch itself.
+constexpr int64_t DIRECT_TUNE_PAIRS = (int64_t)1 << 21;
+
 static void direct_launch(illum_mesh *mesh, int nphi, const double *hz, const double *sun, int S,
                           const double *disk, int ndisk, double *E, bool per_body,
                           const int *step, const double *flux, hipStream_t s) {
   const int64_t nf = mesh->g.nf;
   const int64_t pairs = nf * S;
-  auto go = [&](int L) {
-    const int64_t lanes = pairs * (L > 0 ? L : L == 0 ? 1 : -L);
+  auto go_n = [&](int L, int64_t npairs) {
+    const int64_t lanes = npairs * (L > 0 ? L : L == 0 ? 1 : -L);
     auto kern = L == 0 ? direct_wave_kernel<1> : L == -4 ? direct_wave_kernel<4>
               : L == -8 ? direct_wave_kernel<8> : L == -16 ? direct_wave_kernel<16>
               : L == 16 ? direct_kernel<16>
@@ -544,10 +550,11 @@ static void direct_launch(illum_mesh *mesh, int nphi, const double *hz, const do
   if (!getenv("ILLUM_DIRECT_LANES")) {
     int b = 0;
     while (b < 62 && ((int64_t)1 << (b + 1)) <= pairs) ++b;
+    const int64_t sample = std::min(pairs, DIRECT_TUNE_PAIRS);
     L = autotune(mesh->ctx, "direct_lanes[pairs~2^" + std::to_string(b) + "]",
-                 {-16, -8, -4, 0, 1, 2, 4, 8, 16}, L, go);
+                 {-16, -8, -4, 0, 1, 2, 4, 8, 16}, L, [&](int c) { go_n(c, sample); });
   }
-  go(L);
+  go_n(L, pairs);
 }
 
 void direct_radiosity(illum_mesh *mesh, const double *hz, int nphi, const double *sun, int S,

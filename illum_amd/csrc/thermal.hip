@@ -78,7 +78,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ)
     const int32_t *__restrict__ fidx, const int32_t *__restrict__ wgrid, int64_t block0,
     int64_t nf, int64_t nslot, const double *__restrict__ s2f, const double *__restrict__ hdr,
     const double *__restrict__ coef, const int64_t *__restrict__ goff, double emiss, double Fgeo,
-    int nsteps, const double *__restrict__ Q, double *__restrict__ Qprev,
+    int nsteps, int64_t qs, const double *__restrict__ Q, double *__restrict__ Qprev,
     double *__restrict__ T, double *__restrict__ Fs, double *__restrict__ G,
     double *__restrict__ Tsurf_series, int *__restrict__ status, int rows) {
   __shared__ double s_al[NZ], s_ga[NZ], s_geo[NZ];
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ)
   bool zero_pivot = false;
 
   for (int step = 0; step < nsteps; ++step) {
-    const double Qnp1 = act ? Q[(int64_t)step * nf + f] : 0.0;
+    const double Qnp1 = act ? Q[(int64_t)step * qs + f] : 0.0;
     double ann, annp1, bn;
     surface_bc(Tl[0], Qn, Qnp1, k1, emiss, &ann, &annp1, &bn);
     const double a1 = s2 * s_al[0], g1 = s2 * s_ga[0];
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ)
     int64_t nf, int64_t nslot, const double *__restrict__ s2f, const double *__restrict__ hdr,
     const double *__restrict__ coef, const float4 *__restrict__ coefp,
     const int64_t *__restrict__ goff, const int64_t *__restrict__ goffp, double emiss,
-    double Fgeo, int nsteps, const double *__restrict__ Q, double *__restrict__ Qprev,
+    double Fgeo, int nsteps, int64_t qs, const double *__restrict__ Q, double *__restrict__ Qprev,
     double *__restrict__ T, double *__restrict__ Fs, double *__restrict__ Tsurf_series) {
   const int64_t blk = block0 + blockIdx.x;
   const int64_t slot = blk * 64 + threadIdx.x;
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ)
   const float M2 = Rc[2];  // gamma_2 P'_2: enters row 1's pivot every step
 
   for (int step = 0; step < nsteps; ++step) {
-    const double Qnp1 = act ? Q[(int64_t)step * nf + f] : 0.0;
+    const double Qnp1 = act ? Q[(int64_t)step * qs + f] : 0.0;
     double ann, annp1, bn;
     surface_bc(Tl[0], Qn, Qnp1, k1, emiss, &ann, &annp1, &bn);
     float z[NZ + 1];
@@ -330,7 +330,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ)
     int64_t nf, int64_t nslot, const double *__restrict__ s2f, const double *__restrict__ hdr,
     const double *__restrict__ coef, const float4 *__restrict__ coefp,
     const int64_t *__restrict__ goff, const int64_t *__restrict__ goffp, double emiss,
-    double Fgeo, int nsteps, const double *__restrict__ Q, double *__restrict__ Qprev,
+    double Fgeo, int nsteps, int64_t qs, const double *__restrict__ Q, double *__restrict__ Qprev,
     double *__restrict__ T, double *__restrict__ Fs, double *__restrict__ Tsurf_series) {
   const int64_t blk = block0 + blockIdx.x;
   const int64_t slot = blk * 64 + threadIdx.x;
@@ -378,7 +378,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(cn_waves(NZ)
   const float geo_z = geo_half / s2s;  // bottom-row geothermal term in z's scaling
 
   for (int step = 0; step < nsteps; ++step) {
-    const double Qnp1 = act ? Q[(int64_t)step * nf + f] : 0.0;
+    const double Qnp1 = act ? Q[(int64_t)step * qs + f] : 0.0;
     double ann, annp1, bn;
     surface_bc(Tl[0], Qn, Qnp1, k1, emiss, &ann, &annp1, &bn);
     // The sweeps run over all NZ rows without branches: rows past nz have
@@ -440,7 +440,7 @@ __global__ __launch_bounds__(64) void cn_stream_kernel(
     const int32_t *__restrict__ fidx, const int32_t *__restrict__ wgrid, int64_t block0,
     int64_t nf, int64_t nslot, const double *__restrict__ s2f, const double *__restrict__ hdr,
     const double *__restrict__ coef, const int64_t *__restrict__ goff, double emiss, double Fgeo,
-    int nsteps, const double *__restrict__ Q, double *__restrict__ Qprev,
+    int nsteps, int64_t qs, const double *__restrict__ Q, double *__restrict__ Qprev,
     double *__restrict__ T, double *__restrict__ Fs, double *__restrict__ G,
     float *__restrict__ G32, double *__restrict__ Tsurf_series, int *__restrict__ status,
     int rows) {
@@ -463,7 +463,7 @@ __global__ __launch_bounds__(64) void cn_stream_kernel(
   double Qn = Qprev[f];
   double Fsurf = Fs[f];
   for (int step = 0; step < nsteps; ++step) {
-    const double Qnp1 = Q[(int64_t)step * nf + f];
+    const double Qnp1 = Q[(int64_t)step * qs + f];
     double ann, annp1, bn;
     surface_bc(Tat(0), Qn, Qnp1, k1, emiss, &ann, &annp1, &bn);
     const double a1 = s2 * al[0], g1 = s2 * al[1];
@@ -845,7 +845,8 @@ static int cn_mixed_form(int) {
 
 template <bool MIXED>
 static void launch_all(illum_thermal *th, ThermalImpl *im, hipStream_t s0, double Fgeo,
-                       int nsteps, const double *dQ, double *Tss, bool fork_ok = true) {
+                       int nsteps, int64_t qs, const double *dQ, double *Tss,
+                       bool fork_ok = true) {
   const char *eng = getenv("ILLUM_CN_ENGINE");
   const bool force_stream = eng && std::string(eng) == "stream";
   // The depth buckets touch disjoint slots and facets.  At one or two
@@ -867,7 +868,7 @@ static void launch_all(illum_thermal *th, ThermalImpl *im, hipStream_t s0, doubl
     const hipStream_t s = nfork ? fs[oi % nfork] : s0;
 #define ILLUM_CN_ARGS                                                                        \
   th->d_fidx, th->d_wgrid, L.block0, th->nf, th->nslot, th->d_s2, im->d_hdr, im->d_coef,     \
-      th->d_goff, th->emiss, Fgeo, nsteps, dQ, th->d_Qprev, th->d_T, th->d_F
+      th->d_goff, th->emiss, Fgeo, nsteps, qs, dQ, th->d_Qprev, th->d_T, th->d_F
     const unsigned nb = (unsigned)L.nblocks;
     switch (force_stream ? 0 : L.nz_bucket) {
 #define ILLUM_CN_REG(NZ, GMREG)                                                              \
@@ -878,7 +879,7 @@ static void launch_all(illum_thermal *th, ThermalImpl *im, hipStream_t s0, doubl
                               : cn_mixed_kernel<NZ, false>)<<<nb, 64, 0, s>>>(                \
           th->d_fidx, th->d_wgrid, L.block0, th->nf, th->nslot, th->d_s2, im->d_hdr,         \
           im->d_coef, (const float4 *)im->d_coefp, th->d_goff, im->d_goffp, th->emiss, Fgeo, \
-          nsteps, dQ, th->d_Qprev, th->d_T, th->d_F, Tss);                                   \
+          nsteps, qs, dQ, th->d_Qprev, th->d_T, th->d_F, Tss);                                   \
     else                                                                                     \
       cn_fp64_kernel<NZ, GMREG><<<nb, 64, 0, s>>>(ILLUM_CN_ARGS, th->d_work, Tss,            \
                                                   th->d_status, th->nzmax + 1);              \
@@ -917,9 +918,13 @@ void thermal_run(illum_thermal *th, double dt, const double *Q, int nsteps, int 
   ensure_tables(th, dt);
   ThermalImpl *im = impl_of(th);
   const int64_t nf = th->nf;
+  // ILLUM_Q_HELD: Q is one row of nf values, the flux of every step (a row
+  // stride of 0), the same steps as nsteps single steps with that Q
+  const bool held = flags & ILLUM_Q_HELD;
+  const int64_t qs = held ? 0 : nf;
   const double *dQ = Q;
   if (!(flags & ILLUM_IN_ON_DEVICE)) {
-    const int64_t need = nf * nsteps;
+    const int64_t need = held ? nf : nf * nsteps;
     if (th->qdev_cap < need) {
       dfree(th->d_Qdev);
       th->d_Qdev = dalloc<double>(need);
@@ -930,8 +935,8 @@ void thermal_run(illum_thermal *th, double dt, const double *Q, int nsteps, int 
   }
   const bool mixed = th->precision == ILLUM_THERM_MIXED;
   auto launch = [&](bool fork) {
-    if (mixed) launch_all<true>(th, im, s, Fgeo, nsteps, dQ, Tsurf_series, fork);
-    else launch_all<false>(th, im, s, Fgeo, nsteps, dQ, Tsurf_series, fork);
+    if (mixed) launch_all<true>(th, im, s, Fgeo, nsteps, qs, dQ, Tsurf_series, fork);
+    else launch_all<false>(th, im, s, Fgeo, nsteps, qs, dQ, Tsurf_series, fork);
   };
   // With several depth buckets, whether they run concurrently on forked
   // streams or one after another does not change the results (they touch
@@ -991,8 +996,8 @@ void thermal_enqueue(illum_thermal *th, double dt, const double *dQ, int nsteps)
   ThermalImpl *im = impl_of(th);
   if (im->dt != dt || !im->d_hdr) throw Error(ILLUM_ERR_ARG, "thermal tables not prepared for dt");
   hipStream_t s = stream_of(th->ctx);
-  if (th->precision == ILLUM_THERM_MIXED) launch_all<true>(th, im, s, 0.0, nsteps, dQ, nullptr, false);
-  else launch_all<false>(th, im, s, 0.0, nsteps, dQ, nullptr, false);
+  if (th->precision == ILLUM_THERM_MIXED) launch_all<true>(th, im, s, 0.0, nsteps, th->nf, dQ, nullptr, false);
+  else launch_all<false>(th, im, s, 0.0, nsteps, th->nf, dQ, nullptr, false);
 }
 
 // The model's sticky status: ILLUM_ERR_ZERO_PIVOT after a zero pivot.

@@ -67,6 +67,10 @@ extern "C" {
 /* illum_timeloop_create: queue each step's kernels directly instead of
  * replaying captured HIP graphs (same results; for comparison). */
 #define ILLUM_TL_EAGER       64
+/* illum_thermal_run: Qseries is one row of nf values, the flux of every one
+ * of the nsteps steps (the same steps as nsteps calls of illum_thermal_step
+ * with that Q, run in one launch sequence).  No reference counterpart. */
+#define ILLUM_Q_HELD         128
 
 /* radiosity solve methods */
 #define ILLUM_RAD_JACOBI_FP64   1   /* FP64 Jacobi sweeps B <- E + F B                  */
@@ -308,7 +312,7 @@ int illum_thermal_info(const illum_thermal *th, int64_t *nf, int *nzmax);
  * doubles (host or device), Qprev is the previous step's Q (zero initially). */
 int illum_thermal_step(illum_thermal *th, double dt, const double *Q, int flags);
 /* nsteps consecutive steps, Q series nsteps x nf (row k = Q at step k+1),
- * run as one persistent launch sequence. */
+ * run as one persistent launch sequence (ILLUM_Q_HELD: one row for all). */
 int illum_thermal_run(illum_thermal *th, double dt, const double *Qseries, int nsteps,
                       int flags);
 /* T: (nzmax+1) x nf column-major (ragged facets padded with NaN), Tsurf and

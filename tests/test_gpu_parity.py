@@ -562,6 +562,39 @@ def test_thermal_ragged_layers_and_ti_scale(api, precision, tol):
     assert np.max(np.abs(Fs - Fo)) <= tol * 200
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_thermal_run_with_held_flux_equals_single_steps(api, precision):
+    """ILLUM_Q_HELD: one flux row held for n steps in one launch sequence gives
+    the state n single steps with that flux give, bit for bit, over every
+    engine (ragged grids of 32..150 layers), host and device flux."""
+    rng = np.random.default_rng(11)
+    nf = 700
+    depths = (32, 57, 96, 120, 150)
+    grids = [0.5 * (np.exp(np.linspace(0, 3, nz)) - 1 + 0.01) / (np.exp(3) - 1 + 0.01)
+             for nz in depths]
+    gid = rng.integers(0, len(depths), nf).astype(np.int32)
+    s = np.exp(rng.normal(0, 0.5, nf))
+    kw = dict(emissivity=0.95, grid_of_facet=gid, ti_scale=s, precision=precision)
+    a = api.ThermalModel(nf, grids, 200.0, 1.19177e6, 250.0, **kw)
+    b = api.ThermalModel(nf, grids, 200.0, 1.19177e6, 250.0, **kw)
+    Qs = [rng.uniform(0, 600, nf) for _ in range(3)]
+    counts = (5, 1, 12)
+    for Q, n in zip(Qs, counts):
+        for _ in range(n):
+            a.step(60.0, Q)
+    for i, (Q, n) in enumerate(zip(Qs, counts)):
+        if i == 2:
+            Qd = api.DeviceArray.from_host(api.default_context(), Q)
+            b.run_held(60.0, Qd, n)
+        else:
+            b.run_held(60.0, Q, n)
+    Ta, Tb = a.T, b.T
+    mask = ~np.isnan(Ta)
+    assert np.array_equal(np.isnan(Tb), ~mask)
+    assert np.array_equal(Ta[mask], Tb[mask])
+    assert all(np.array_equal(x, y) for x, y in zip(a.surface(), b.surface()))
+
+
 def test_thermal_get_radiosity_accumulates(api):
     rng = np.random.default_rng(9)
     nf = 300
