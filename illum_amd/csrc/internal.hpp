@@ -6,6 +6,8 @@
 #include <string>
 #include <vector>
 #include <map>
+#include <memory>
+#include <utility>
 #include <stdexcept>
 
 #include "../../include/illum_hip.h"
@@ -21,15 +23,30 @@ struct Error : std::runtime_error {
 
 void set_last_error(const std::string &msg);
 
+// Allocator whose value-initialisation leaves elements uninitialised, so
+// vector::resize does not zero (and fault in) hundreds of MB on one thread
+// for arrays that are then written in full by the threads computing them.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+  template <class U> struct rebind { using other = NoInitAlloc<U>; };
+  NoInitAlloc() = default;
+  template <class U> NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+  template <class U> void construct(U *p) noexcept { ::new ((void *)p) U; }
+  template <class U, class... A> void construct(U *p, A &&...a) {
+    ::new ((void *)p) U(std::forward<A>(a)...);
+  }
+};
+template <class T> using HostVec = std::vector<T, NoInitAlloc<T>>;
+
 // ---- host-side geometry (built once per mesh, uploaded) ------------------
 struct HostGeometry {
   int64_t nf = 0;
-  std::vector<float> tri;      // nf x 9 : v0, v1, v2 (FP32, after divisor)
-  std::vector<float> cen;      // nf x 3 : centroid
-  std::vector<float> nrm;      // nf x 3 : unit normal (FP32)
-  std::vector<float> rad;      // nf     : bounding radius
-  std::vector<double> area;    // nf     : |(v1-v0)x(v2-v0)|/2 in FP64
-  std::vector<double> frame;   // nf x 9 : b, t, n columns (FP64)
+  HostVec<float> tri;      // nf x 9 : v0, v1, v2 (FP32, after divisor)
+  HostVec<float> cen;      // nf x 3 : centroid
+  HostVec<float> nrm;      // nf x 3 : unit normal (FP32)
+  HostVec<float> rad;      // nf     : bounding radius
+  HostVec<double> area;    // nf     : |(v1-v0)x(v2-v0)|/2 in FP64
+  HostVec<double> frame;   // nf x 9 : b, t, n columns (FP64)
   // Bodies: facets [body_off[b], body_off[b+1]) form body b, a scene of its
   // own (empty or {0, nf}: one body).
   std::vector<int64_t> body_off;

@@ -19,6 +19,7 @@
 #include <cstring>
 #include <fstream>
 #include <sstream>
+#include <thread>
 #include <unordered_map>
 
 namespace illum {
@@ -221,7 +222,12 @@ HostGeometry make_geometry(const double *V, int64_t nv, const int32_t *F, int64_
   g.rad.resize(nf);
   g.area.resize(nf);
   g.frame.resize(9 * nf);
-  for (int64_t f = 0; f < nf; ++f) {
+  for (int64_t k = 0; k < 3 * nf; ++k)
+    if (F[k] < 0 || F[k] >= nv) throw Error(ILLUM_ERR_ARG, "face index out of range");
+  // Every facet's records depend on its own vertices only: facet ranges are
+  // computed on all host threads (the same values as one thread).
+  auto facets = [&](int64_t f0, int64_t f1) {
+  for (int64_t f = f0; f < f1; ++f) {
     // u: the FP32 vertices as read; v: divided by the vertex divisor.  The
     // face normal comes from u and everything else from v, as the reference
     // computes n before building Tri(v0/100., v1/100., v2/100., n)
@@ -229,7 +235,6 @@ HostGeometry make_geometry(const double *V, int64_t nv, const int32_t *F, int64_
     float u[3][3], v[3][3];
     for (int k = 0; k < 3; ++k) {
       int32_t id = F[3 * f + k];
-      if (id < 0 || id >= nv) throw Error(ILLUM_ERR_ARG, "face index out of range");
       for (int c = 0; c < 3; ++c) {
         u[k][c] = (float)V[3 * id + c];
         v[k][c] = vertex_divisor == 1.0 ? u[k][c] : u[k][c] / div;
@@ -289,6 +294,19 @@ HostGeometry make_geometry(const double *V, int64_t nv, const int32_t *F, int64_
     fr[0] = bu[0]; fr[1] = bu[1]; fr[2] = bu[2];
     fr[3] = tu[0]; fr[4] = tu[1]; fr[5] = tu[2];
     fr[6] = nu[0]; fr[7] = nu[1]; fr[8] = nu[2];
+  }
+  };
+  const int64_t chunk = 1 << 16;
+  const int nt = (int)std::max<int64_t>(
+      1, std::min<int64_t>((nf + chunk - 1) / chunk, std::thread::hardware_concurrency()));
+  if (nt == 1) {
+    facets(0, nf);
+  } else {
+    std::vector<std::thread> pool;
+    const int64_t per = (nf + nt - 1) / nt;
+    for (int t = 0; t < nt; ++t)
+      pool.emplace_back(facets, std::min(nf, t * per), std::min(nf, (t + 1) * per));
+    for (auto &th : pool) th.join();
   }
   return g;
 }

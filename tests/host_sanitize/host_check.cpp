@@ -62,8 +62,8 @@ static int fails = 0;
     }                                                                   \
   } while (0)
 
-template <class T>
-static bool same(const std::vector<T> &a, const T *b, size_t n) {
+template <class V, class T>
+static bool same(const V &a, const T *b, size_t n) {
   return a.size() == n && std::memcmp(a.data(), b, n * sizeof(T)) == 0;
 }
 
@@ -101,6 +101,15 @@ int main(int argc, char **argv) {
     CHECK(b.nnodes > 0 && b.tris.size() == 12 * (size_t)nf);
     CHECK(b.nodes.size() == 8 * (size_t)b.nnodes && b.wide.size() == 16 * (size_t)b.ninner);
     CHECK(b.wide4.size() == 32 * (size_t)b.ninner4);
+  }
+  {  // a mesh large enough for make_geometry to split over threads
+    int64_t nv6 = 0, nf6 = 0;
+    synth_sizes(6, &nv6, &nf6);
+    std::vector<double> V6(3 * nv6);
+    std::vector<int32_t> F6(3 * nf6);
+    synth_ellipsoid(6, 13.0, 11.0, 9.0, 50, 0.05, 0.3, 0.2, 1.5, 7, V6.data(), F6.data());
+    const HostGeometry g6 = make_geometry(V6.data(), nv6, F6.data(), nf6, nullptr, 1.0);
+    check_geometry(g6, V6.data(), nv6, F6.data(), nullptr, 1.0);
   }
   {  // three bodies: one forest, one root per body
     HostGeometry gb = g;
