@@ -57,83 +57,126 @@ struct Builder {
   std::vector<Box> tbox;
   std::vector<float> tcen;   // centroid of each triangle's box
   std::vector<int> perm;
+  std::vector<int> scratch;  // the partition's right-hand side, reused
   std::vector<BuildNode> nodes;
   int max_depth = 0;
 
   Builder(const HostGeometry &g_, int ls) : g(g_), leaf_size(ls) {}
 
-  int build(int first, int count, int depth) {
+  static constexpr int NB = 32;  // SAH bins per axis
+
+  // Node over perm[first, first + count), whose triangles' boxes span b and
+  // whose box centroids span cb (the root computes both; a child takes them
+  // from its parent's bins and partition pass, the same values, since min
+  // and max are exact).
+  //
+  // Binned SAH over the centroid box, NB bins per axis: one pass bins all
+  // three axes (per bin: count and triangle box); the split with the least
+  // cost (left area x count + right area x count) is taken, ties to the
+  // lower axis and bin.  A split just past an empty bin costs the same as
+  // the split before it, which wins the tie, so only splits past non-empty
+  // bins are evaluated.  The range is then partitioned stably.
+  int build(int first, int count, int depth, const Box &b, const Box &cb) {
     max_depth = std::max(max_depth, depth);
     int id = (int)nodes.size();
     nodes.emplace_back();
-    Box b, cb;
-    for (int k = first; k < first + count; ++k) {
-      b.grow(tbox[perm[k]]);
-      cb.grow(&tcen[3 * perm[k]]);
-    }
     nodes[id].box = b;
     if (count <= leaf_size) {
       nodes[id].first = first;
       nodes[id].count = count;
       return id;
     }
-    // Binned SAH over the centroid box, 32 bins per axis.
-    const int NB = 32;
+    Box bins[3][NB];
+    int cnt[3][NB] = {};
+    double lo[3], scale[3];
+    bool ok[3];
+    for (int axis = 0; axis < 3; ++axis) {
+      lo[axis] = cb.lo[axis];
+      const double hi = cb.hi[axis];
+      ok[axis] = hi > lo[axis];
+      scale[axis] = ok[axis] ? NB / (hi - lo[axis]) : 0.0;
+    }
+    for (int k = first; k < first + count; ++k) {
+      const int t = perm[k];
+      const float *c = &tcen[3 * t];
+      for (int axis = 0; axis < 3; ++axis) {
+        if (!ok[axis]) continue;
+        int bi = (int)((c[axis] - lo[axis]) * scale[axis]);
+        bi = std::min(NB - 1, std::max(0, bi));
+        cnt[axis][bi]++;
+        bins[axis][bi].grow(tbox[t]);
+      }
+    }
     int best_axis = -1, best_split = -1;
     double best_cost = DBL_MAX;
     for (int axis = 0; axis < 3; ++axis) {
-      double lo = cb.lo[axis], hi = cb.hi[axis];
-      if (!(hi > lo)) continue;
-      Box bins[NB];
-      int cnt[NB] = {0};
-      double scale = NB / (hi - lo);
-      for (int k = first; k < first + count; ++k) {
-        int t = perm[k];
-        int bi = (int)((tcen[3 * t + axis] - lo) * scale);
-        bi = std::min(NB - 1, std::max(0, bi));
-        cnt[bi]++;
-        bins[bi].grow(tbox[t]);
-      }
+      if (!ok[axis]) continue;
       double rarea[NB];
       int rcnt[NB];
       Box acc;
       int ac = 0;
       for (int s = NB - 1; s > 0; --s) {
-        acc.grow(bins[s]);
-        ac += cnt[s];
-        rarea[s] = acc.area();
+        if (cnt[axis][s]) {
+          acc.grow(bins[axis][s]);
+          ac += cnt[axis][s];
+        }
         rcnt[s] = ac;
+        // needed only past a non-empty bin s - 1 (see above)
+        if (cnt[axis][s - 1]) rarea[s] = acc.area();
       }
       Box lacc;
       int lc = 0;
       for (int s = 1; s < NB; ++s) {
-        lacc.grow(bins[s - 1]);
-        lc += cnt[s - 1];
-        if (lc == 0 || rcnt[s] == 0) continue;
+        if (!cnt[axis][s - 1]) continue;
+        lacc.grow(bins[axis][s - 1]);
+        lc += cnt[axis][s - 1];
+        if (rcnt[s] == 0) continue;
         double cost = lacc.area() * lc + rarea[s] * rcnt[s];
         if (cost < best_cost) { best_cost = cost; best_axis = axis; best_split = s; }
       }
     }
-    int mid;
-    if (best_axis < 0) {
-      mid = first + count / 2;  // all centroids coincide: split by index
-    } else {
-      double lo = cb.lo[best_axis], hi = cb.hi[best_axis];
-      double scale = NB / (hi - lo);
-      auto pivot = std::stable_partition(perm.begin() + first, perm.begin() + first + count,
-                                         [&](int t) {
-                                           int bi = (int)((tcen[3 * t + best_axis] - lo) * scale);
-                                           bi = std::min(NB - 1, std::max(0, bi));
-                                           return bi < best_split;
-                                         });
-      mid = (int)(pivot - perm.begin());
-      if (mid == first || mid == first + count) mid = first + count / 2;
+    int mid = -1;
+    Box lb, lcb, rb, rcb;
+    if (best_axis >= 0) {
+      const int ax = best_axis;
+      // stable partition: left elements compacted in place, right ones
+      // through the scratch buffer; the children's centroid boxes on the way
+      int nl = 0, nr = 0;
+      for (int k = first; k < first + count; ++k) {
+        const int t = perm[k];
+        const float *c = &tcen[3 * t];
+        int bi = (int)((c[ax] - lo[ax]) * scale[ax]);
+        bi = std::min(NB - 1, std::max(0, bi));
+        if (bi < best_split) { perm[first + nl++] = t; lcb.grow(c); }
+        else { scratch[nr++] = t; rcb.grow(c); }
+      }
+      std::copy(scratch.begin(), scratch.begin() + nr, perm.begin() + first + nl);
+      mid = first + nl;
+      for (int s = 0; s < best_split; ++s) if (cnt[ax][s]) lb.grow(bins[ax][s]);
+      for (int s = best_split; s < NB; ++s) if (cnt[ax][s]) rb.grow(bins[ax][s]);
     }
-    int l = build(first, mid - first, depth + 1);
-    int r = build(mid, first + count - mid, depth + 1);
+    if (mid < 0 || mid == first || mid == first + count) {
+      // all centroids coincide (or one side is empty): split by index
+      mid = first + count / 2;
+      lb = rb = lcb = rcb = Box();
+      for (int k = first; k < mid; ++k) { lb.grow(tbox[perm[k]]); lcb.grow(&tcen[3 * perm[k]]); }
+      for (int k = mid; k < first + count; ++k) { rb.grow(tbox[perm[k]]); rcb.grow(&tcen[3 * perm[k]]); }
+    }
+    int l = build(first, mid - first, depth + 1, lb, lcb);
+    int r = build(mid, first + count - mid, depth + 1, rb, rcb);
     nodes[id].left = l;
     nodes[id].right = r;
     return id;
+  }
+
+  int build_root(int count) {
+    scratch.resize(count);
+    Box b, cb;
+    for (int k = 0; k < count; ++k) {
+      b.grow(tbox[perm[k]]);
+      cb.grow(&tcen[3 * perm[k]]);
+    }
+    return build(0, count, 0, b, cb);
   }
 };
 
@@ -159,7 +202,7 @@ static HostBVH build_one(const HostGeometry &g, int64_t f0, int64_t f1, int leaf
     B.perm[k] = (int)k;
   }
   B.nodes.reserve(2 * nf / std::max(1, leaf_size) + 16);
-  B.build(0, (int)nf, 0);
+  B.build_root((int)nf);
 
   // Flatten in DFS pre-order, computing miss links.
   HostBVH h;
@@ -249,29 +292,38 @@ static HostBVH build_one(const HostGeometry &g, int64_t f0, int64_t f1, int leaf
   // Node = 8 float4: lo.x, lo.y, lo.z, hi.x, hi.y, hi.z of children 0..3
   // (SoA), child refs (>= 0 node, < 0 ~leaf info), (child count, 0, 0, 0).
   {
-    std::vector<std::vector<int>> kids;
-    std::vector<int> n4_of(nn, -1), d4;
+    // up to four children, in order (fixed records: no allocation per node)
+    struct Kids {
+      int c[4];
+      int n;
+      size_t size() const { return (size_t)n; }
+      int operator[](size_t q) const { return c[q]; }
+    };
+    std::vector<Kids> kids;
+    kids.reserve(nn / 2 + 1);
+    std::vector<int> n4_of(nn, -1);
     auto expand = [&](int id) {
-      std::vector<int> ch = {B.nodes[id].left, B.nodes[id].right};
-      while (ch.size() < 4) {
+      Kids ch{{B.nodes[id].left, B.nodes[id].right, 0, 0}, 2};
+      while (ch.n < 4) {
         int best = -1;
         double ba = -1.0;
-        for (size_t q = 0; q < ch.size(); ++q)
-          if (B.nodes[ch[q]].left >= 0 && B.nodes[ch[q]].box.area() > ba) {
-            ba = B.nodes[ch[q]].box.area();
-            best = (int)q;
+        for (int q = 0; q < ch.n; ++q)
+          if (B.nodes[ch.c[q]].left >= 0 && B.nodes[ch.c[q]].box.area() > ba) {
+            ba = B.nodes[ch.c[q]].box.area();
+            best = q;
           }
         if (best < 0) break;
-        const int m = ch[best];
-        ch.erase(ch.begin() + best);
-        ch.push_back(B.nodes[m].left);
-        ch.push_back(B.nodes[m].right);
+        const int m = ch.c[best];
+        for (int q = best; q + 1 < ch.n; ++q) ch.c[q] = ch.c[q + 1];  // erase, keeping order
+        ch.n -= 1;
+        ch.c[ch.n++] = B.nodes[m].left;
+        ch.c[ch.n++] = B.nodes[m].right;
       }
       return ch;
     };
     int depth4 = 0;
     if (B.nodes[0].left < 0) {
-      kids.push_back({0});  // root is a leaf
+      kids.push_back(Kids{{0, 0, 0, 0}, 1});  // root is a leaf
     } else {
       std::vector<std::pair<int, int>> st = {{0, 0}};
       while (!st.empty()) {
@@ -280,9 +332,9 @@ static HostBVH build_one(const HostGeometry &g, int64_t f0, int64_t f1, int leaf
         depth4 = std::max(depth4, dep);
         n4_of[id] = (int)kids.size();
         kids.push_back(expand(id));
-        const auto &ch = kids.back();
-        for (int q = (int)ch.size() - 1; q >= 0; --q)
-          if (B.nodes[ch[q]].left >= 0) st.push_back({ch[q], dep + 1});
+        const Kids ch = kids.back();
+        for (int q = ch.n - 1; q >= 0; --q)
+          if (B.nodes[ch.c[q]].left >= 0) st.push_back({ch.c[q], dep + 1});
       }
     }
     h.ninner4 = (int64_t)kids.size();
@@ -290,7 +342,7 @@ static HostBVH build_one(const HostGeometry &g, int64_t f0, int64_t f1, int leaf
     h.wide4.assign(32 * kids.size(), 0.0f);
     for (size_t k = 0; k < kids.size(); ++k) {
       float *o = &h.wide4[32 * k];
-      const auto &ch = kids[k];
+      const Kids &ch = kids[k];
       for (size_t q = 0; q < ch.size(); ++q) {
         const BuildNode &cn = B.nodes[ch[q]];
         for (int a = 0; a < 3; ++a) {
