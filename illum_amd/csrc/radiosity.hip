@@ -196,8 +196,8 @@ __global__ __launch_bounds__(256) void row_blocks_kernel(int64_t nf,
   if (k > nblk) return;
   if (k == nblk) { rblk[k] = (int)nf; return; }
   const int64_t tot = rowptr[nf] + SWEEP_ROW_WEIGHT * nf;
-  // tot < 2^40 (nnz < 2^39, nf < 2^31) and k < 2^24: no overflow
-  const int64_t T = tot * k / nblk;
+  // floor(tot k / nblk) without forming tot k: (tot mod nblk) k < nblk^2 <= 2^56
+  const int64_t T = (tot / nblk) * k + (tot % nblk) * k / nblk;
   int64_t lo = 0, hi = nf;  // c(lo) <= T, find the largest such m
   while (lo < hi) {
     const int64_t mid = (lo + hi + 1) >> 1;
@@ -542,7 +542,7 @@ __device__ void decide(int S, const double *col, const double *e2, SolveCtl *ctl
     }
     if (ex) ex[(int64_t)(k + 2) * S + s] = half_scale_exp(r2);
   }
-  if (fl) ctl->floor = 1;  // any thread; read after the barriers below
+  fl = __syncthreads_or(fl);  // block-wide: some column's bound reached the tolerance
   __syncthreads();
   red[threadIdx.x] = m;
   __syncthreads();
@@ -554,7 +554,8 @@ __device__ void decide(int S, const double *col, const double *e2, SolveCtl *ctl
     const double rel = red[0];
     hist[k] = rel;
     ctl->nprod = k + 1;
-    if (rel <= tol || ctl->floor) ctl->done = 1;
+    if (fl) ctl->floor = 1;
+    if (rel <= tol || fl) ctl->done = 1;
   }
 }
 
