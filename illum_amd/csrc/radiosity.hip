@@ -135,40 +135,9 @@ __device__ __forceinline__ int64_t sweep_block(int b, int n, int xc) {
 template <class TX>
 constexpr bool is_mixed() { return std::is_same<TX, _Float16>::value; }
 
-// One wave per row, lane s covers one right-hand side per 64-wide chunk.
-// The residual norm follows in colnorm_partial_kernel.
-template <class TX>
-__global__ __launch_bounds__(256) void sweep_rows_kernel(
-    int64_t nf, int S, const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colidx,
-    const double *__restrict__ val, const TX *__restrict__ X, const double *__restrict__ R,
-    double *__restrict__ Y, const int *__restrict__ done, MixedIO m) {
-  if (*done) return;
-  const int64_t row = sweep_block(blockIdx.x, gridDim.x, m.xc) * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (row >= nf) return;
-  const int64_t k0 = rowptr[row], k1 = rowptr[row + 1];
-  for (int s0 = 0; s0 < S; s0 += 64) {
-    const int s = s0 + lane;
-    if (s >= S) break;
-    double acc = 0;
-    for (int64_t k = k0; k < k1; ++k)
-      acc = __builtin_fma(val[k], (double)X[(int64_t)colidx[k] * S + s], acc);
-    const int64_t at = row * S + s;
-    if constexpr (is_mixed<TX>()) {
-      const double ik = ldexp(1.0, -m.ek[s]);
-      const double x = (double)X[at] * ik;
-      const double y = (R[at] - x) + acc * ik;
-      Y[at] = y;
-      if (m.B) m.B[at] = m.B[at] + x;
-      m.Dn[at] = to_half_scaled(y, ldexp(1.0, m.en[s]));
-    } else {
-      Y[at] = R[at] + acc;
-    }
-  }
-}
-
-// Wide form (S >= 128): lane l covers PER consecutive right-hand sides (one
-// vector load of PER values per nonzero).  The grid is column-chunk-major:
+// Wide form (S >= SWEEP_WIDE_MIN_S): lane l covers PER consecutive
+// right-hand sides (one vector load of PER values per nonzero); with S < 64
+// the lanes past S run the gathers on column 0 and discard them.  The grid is column-chunk-major:
 // chunk ch (columns [64 PER ch, 64 PER (ch + 1))) runs over all row blocks
 // before the next chunk starts.  Per chunk there are nblk row blocks (a
 // multiple of 8); block g owns rows [rblk[g], rblk[g + 1]), contiguous and
@@ -182,6 +151,8 @@ __global__ __launch_bounds__(256) void sweep_rows_kernel(
 // The block's column sums of squares of the residual elements go to
 // part[g][S]: each wave sums its rows in order, then waves 0..3 in order,
 // so the sums do not depend on PER or on where a block runs.
+constexpr int SWEEP_WIDE_MIN_S = 32;  // right-hand sides from which products take the wide form
+constexpr int RAD_RING_MIN_S = 128;   // mixed: corrections in a ring from here, else B mode
 constexpr int SWEEP_RPW = 4;         // rows per wave of a row block, on average
 constexpr int SWEEP_ROW_WEIGHT = 8;  // a row's fixed cost, in nonzeros
 constexpr int SWEEP_UNROLL = 8;      // gathers in flight per wave
@@ -306,8 +277,9 @@ __global__ __launch_bounds__(256) void sweep_wide_kernel(
 #pragma unroll
         for (int p = 0; p < PER; ++p) eps[p] = (double)q[p] * (ik[p] * 0x1p-11);
       } else {
-        const yvec x = __builtin_convertvector(*(const xvec *)(X + at), yvec);
-        eps = *(const yvec *)(R + at) - x * ik;
+        const yvec xs = __builtin_convertvector(*(const xvec *)(X + at), yvec) * ik;  // delta_k
+        eps = *(const yvec *)(R + at) - xs;
+        if (m.B) *(yvec *)(m.B + at) = *(const yvec *)(m.B + at) + xs;  // B mode
       }
       const yvec y = eps + acc * ik;  // r_{k+1}
       sq = sq + y * y;
@@ -901,14 +873,16 @@ int sweep_product(hipStream_t s, illum_ctx *ctx, const illum_sparse *F, int S, c
   // kernel tunes it, the others keep ranges.  ILLUM_RAD_XCD_CHUNK fixes it.
   const char *xenv = getenv("ILLUM_RAD_XCD_CHUNK");
   m.xc = xenv ? std::max(0, atoi(xenv)) : 0;
-  if (S >= 128) {
+  if (S >= SWEEP_WIDE_MIN_S) {
     // PER: right-hand sides per lane.  The products of every PER are
-    // identical (FMA accumulation in CSR order) and idempotent at S >= 128
-    // (FP64: Y = R + F X; mixed ring mode leaves B alone), so the autotuner
-    // times the candidates (64 PER <= S, S % PER == 0) on the solve's first
-    // product.  Without it: the widest whose gathered chunk (nf x 64 PER
-    // values of X) stays within 128 MB (ILLUM_SPMM_CHUNK_BYTES), so it is
-    // served from the 256 MB memory-side cache rather than HBM.
+    // identical (FMA accumulation in CSR order), so the autotuner times the
+    // candidates (S % PER == 0, 64 PER <= S or PER = 1) on the solve's first
+    // product when that product is idempotent (FP64: Y = R + F X; mixed ring
+    // mode leaves B alone; mixed B mode adds to B, so it is not tuned).
+    // Without it: the widest whose gathered chunk (nf x 64 PER values of X)
+    // stays within 128 MB (ILLUM_SPMM_CHUNK_BYTES), so it is served from the
+    // 256 MB memory-side cache rather than HBM.
+    const bool idem = !(is_mixed<TX>() && m.B);
     const char *env = getenv("ILLUM_SPMM_CHUNK_BYTES");
     const double budget = env ? atof(env) : 128e6;
     int per = 8;
@@ -944,24 +918,21 @@ int sweep_product(hipStream_t s, illum_ctx *ctx, const illum_sparse *F, int S, c
 #undef ILLUM_SWEEP_WIDE
       ILLUM_LAUNCH_CHECK();
     };
-    if (!env) {
-      std::vector<int> cands;
-      for (int P : {8, 4, 2, 1})
-        if (S % P == 0 && 64 * P <= S) cands.push_back(P);
+    std::vector<int> cands;
+    for (int P : {8, 4, 2, 1})
+      if (S % P == 0 && (64 * P <= S || P == 1)) cands.push_back(P);
+    if (!env && idem && cands.size() > 1) {
       per = autotune(ctx, std::string("spmm_per[") + (is_mixed<TX>() ? "fp16" : "fp64") +
                               ",nf=" + std::to_string(nf) + ",S=" + std::to_string(S) + "]",
                      cands, per, go, timed);
     }
-    if (!xenv) {
+    if (!xenv && idem) {
       m.xc = autotune(ctx, std::string("spmm_xcd[") + (is_mixed<TX>() ? "fp16" : "fp64") +
                                ",nf=" + std::to_string(nf) + ",S=" + std::to_string(S) + "]",
                       {0, 1, 16}, 0, [&](int xc) { m.xc = xc; go(per); }, timed);
     }
     go(per);
     nparts = nblk;
-  } else if (S > SPMM_NNZ_MAXS) {
-    sweep_rows_kernel<TX><<<(unsigned)((nf + 3) / 4), 256, 0, s>>>(
-        nf, S, F->d_rowptr, F->d_colidx, F->d_rval, X, R, Y, done, m);
   } else {
     const unsigned nb = (unsigned)((nf + SPMM_NNZ_ROWS - 1) / SPMM_NNZ_ROWS);
     sweep_nnz_kernel<TX><<<nb, 256, 0, s>>>(nf, S, F->d_rowptr, F->d_colidx, F->d_rval, X, R, Y,
@@ -1002,7 +973,7 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 struct CtlLayout {
   size_t hist, e2, col, cole, ex, tb, rblk, part2, part, bytes;
   CtlLayout(int64_t nf, int S, int max_iter) {
-    const bool wide = S >= 128;
+    const bool wide = S >= SWEEP_WIDE_MIN_S;
     const int64_t nblk = wide ? wide_blocks(nf) : 0;
     const int64_t gparts = std::max<int64_t>(
         {(int64_t)colnorm_blocks(nf * S), (nf + SPMM_NNZ_ROWS - 1) / SPMM_NNZ_ROWS, 2 * nblk});
@@ -1034,7 +1005,7 @@ constexpr int RAD_FP64_RESIDUALS = 1 << 30;
 // aligned), for callers that own the buffer (a time loop).
 size_t radiosity_scratch_bytes(int64_t nf, int S, int method, int max_iter) {
   const int64_t n = nf * S;
-  const bool mixed = method == ILLUM_RAD_MIXED, wide = S >= 128, ring = mixed && wide;
+  const bool mixed = method == ILLUM_RAD_MIXED, ring = mixed && S >= RAD_RING_MIN_S;
   const int R = ring ? RAD_RING : 2;
   size_t b = align256(CtlLayout(nf, S, max_iter).bytes) + 2 * align256(8 * (size_t)n);
   if (mixed) b += align256(8 * (size_t)n) + align256(2 * (size_t)R * n);
@@ -1070,11 +1041,11 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
   hipStream_t s = stream_of(ctx);
   const bool in_dev = flags & ILLUM_IN_ON_DEVICE, out_dev = flags & ILLUM_OUT_ON_DEVICE;
   const bool mixed = method == ILLUM_RAD_MIXED;
-  const bool wide = S >= 128;
+  const bool wide = S >= SWEEP_WIDE_MIN_S;  // the wide product (row blocks, two-stage sums)
   // Mixed, S >= 128: the corrections stay in a ring and B is their sum
   // (folded every RAD_FOLD sweeps); otherwise each product adds its
   // correction to B ("B mode", two slots).
-  const bool ring = mixed && wide;
+  const bool ring = mixed && S >= RAD_RING_MIN_S;
   const int R = ring ? RAD_RING : 2;
 
   // Scratch from the context's workspace (no allocation in a time loop):

@@ -317,16 +317,19 @@ def test_radiosity_vs_gauss_seidel(api, S, method):
 @pytest.mark.parametrize("tol", [1e-10, 1e-8])
 @pytest.mark.parametrize("method", ["fp64", "mixed"])
 def test_radiosity_wide_products_chunking_is_exact(api, monkeypatch, method, tol):
-    # S >= 128 runs the column-chunked product: every element sums its row in
-    # CSR order whatever the chunk width, so forcing narrower chunks (a small
-    # cache budget) changes nothing; odd S takes one value per lane.  At 1e-8
-    # the mixed method keeps FP32 remainders, at 1e-10 FP64 residuals.
+    # S >= 32 runs the wide product over row blocks: every element sums its
+    # row in CSR order whatever the chunk width or the blocks' dealing to the
+    # XCDs, so forcing narrower chunks (a small cache budget) or another
+    # dealing changes nothing; odd S takes one value per lane, S < 64 leaves
+    # lanes idle, and mixed solves below S = 128 add each correction to B
+    # (B mode).  At 1e-8 the mixed method keeps FP32 remainders (S >= 128),
+    # at 1e-10 FP64 residuals.
     V, F = _mesh(api, 4, 40, seed=6)
     ctx = api.IllumContext(V=V, F=F)
     Fm = ctx.compute_F(0.5)
     nf = len(F)
     cp, ri, v = Fm.csc()
-    for S in (128, 130, 131, 512):
+    for S in (32, 45, 96, 128, 130, 131, 512):
         E = np.random.default_rng(S).uniform(0, 500, (nf, S))
         monkeypatch.delenv("ILLUM_SPMM_CHUNK_BYTES", raising=False)
         B, res, it = api.radiosity_solve(Fm, E, method=method, rel_tol=tol, max_iter=400)
@@ -335,6 +338,12 @@ def test_radiosity_wide_products_chunking_is_exact(api, monkeypatch, method, tol
             monkeypatch.setenv("ILLUM_SPMM_CHUNK_BYTES", str(budget))
             Bc, resc, itc = api.radiosity_solve(Fm, E, method=method, rel_tol=tol, max_iter=400)
             assert itc == it and np.array_equal(Bc, B) and np.array_equal(resc, res), (S, budget)
+        monkeypatch.delenv("ILLUM_SPMM_CHUNK_BYTES", raising=False)
+        for xc in ("1", "16"):
+            monkeypatch.setenv("ILLUM_RAD_XCD_CHUNK", xc)
+            Bc, resc, itc = api.radiosity_solve(Fm, E, method=method, rel_tol=tol, max_iter=400)
+            assert itc == it and np.array_equal(Bc, B) and np.array_equal(resc, res), (S, xc)
+        monkeypatch.delenv("ILLUM_RAD_XCD_CHUNK", raising=False)
         for s in (0, S - 1):
             Bo, _, _ = O.gauss_seidel(nf, cp, ri, v, E[:, s], max_sweeps=200, rel_tol=1e-13)
             np.testing.assert_allclose(B[:, s], Bo, rtol=10 * tol, atol=10 * tol * np.abs(Bo).max())
