@@ -146,11 +146,12 @@ def test_cluster_cull_does_not_change_results(api, monkeypatch):
     assert Fm.rays() == nrays
 
 
-def test_horizon_traversal_choice_does_not_change_results(api, monkeypatch):
+@pytest.mark.parametrize("leaf", [1, 3, 8])
+def test_horizon_traversal_choice_does_not_change_results(api, monkeypatch, leaf):
     # Every walk, with the per-lane hit cache on (default) and off, gives the
-    # same horizons.
+    # same horizons, at several leaf sizes.
     V, F = _mesh(api, 3, 40, seed=9)
-    ctx = api.IllumContext(V=V, F=F)
+    ctx = api.IllumContext(V=V, F=F, leaf_size=leaf)
     a = ctx.make_horizons(nphi=33).copy()
     for cache in ("1", "0"):
         monkeypatch.setenv("ILLUM_HZ_CACHE", cache)
