@@ -17,13 +17,15 @@ from . import _lib, api
 
 
 def run_bodies(bodies, suns, S, nphi=361, albedo=0.1, method="mixed", thermal="mixed",
-               steps=512, rel_tol=1e-8, layers=40, leaf_size=4, ctx=None, keep_visibility=False):
+               steps=512, rel_tol=1e-8, layers=40, leaf_size=3, ctx=None, keep_visibility=False):
     """bodies: [(V, F)]; suns: 3 x (S nbody) km, body b's in columns
     b S .. b S + S - 1.  The conduction run covers one rotation of `steps`
     steps; step k is driven by the scattered radiosity of sun k S // steps
     (its B column, read on the device).  Returns a dict with the context,
     matrix, device arrays hz (nphi x N), E and B (N x S), the thermal model,
-    the body offsets and per-stage seconds."""
+    the body offsets and per-stage seconds.  leaf_size 3: the horizon walks,
+    most of the batch's time, are faster on small leaves, and the batch is
+    too large to build a tree per candidate leaf size for the autotuner."""
     ctx = ctx or api.default_context()
     t = {}
 
