@@ -532,8 +532,11 @@ def main():
     # cull of illum.cpp:56-68 as written, before any cluster pass), plus 12 per
     # ray/box slab test and 40 per ray/triangle test counted per ray along its
     # own near-first walk of the same tree (the per-lane two-wide walk's
-    # counters, in a separate instrumented launch) -- not the wavefront
-    # packet walk's union of nodes, which is how the kernel spends its time.
+    # counters, in a separate instrumented launch; they equal exactly the
+    # oracle's walk of the exported tree, oracle.walk_work, as
+    # test_per_ray_walk_work_equals_the_oracle_walk_of_the_same_tree checks)
+    # -- not the wavefront packet walk's union of nodes, which is how the
+    # kernel spends its time.
     os.environ["ILLUM_FF_TRAVERSAL"] = "wide2"
     st = illum.form_factor_work(offset)
     del os.environ["ILLUM_FF_TRAVERSAL"]

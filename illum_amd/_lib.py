@@ -66,6 +66,7 @@ SIGNATURES = {
     "illum_bvh_destroy": [_P],
     "illum_bvh_tuning": [_P, _I, _P, _P, _P, _P],
     "illum_bvh_info": [_P, _P, _P, _P],
+    "illum_bvh_export": [_P, _P, _P, _P, _P, _P],
     "illum_bvh_closest_hit": [_P, _P, _P, _I64, _P, _P],
     "illum_form_factors": [_P, _D, _P, _D, _PP],
     "illum_form_factors_ex": [_P, _D, _P, _D, _I, _PP],

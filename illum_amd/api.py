@@ -273,6 +273,18 @@ class BVH:
         call("illum_bvh_info", self.h, C.byref(n), C.byref(d), C.byref(l))
         return dict(nodes=n.value, depth=d.value, leaf_size=l.value)
 
+    def export(self):
+        """The tree as the walks read it (illum_bvh_export): dict of nodes
+        (threaded, n x 8), wide (two-wide, n x 16), tris (n x 12) float32 and
+        roots (bodies x 3: threaded, two-wide, four-wide) int32."""
+        sz = np.zeros(4, np.int64)
+        call("illum_bvh_export", self.h, _ptr(sz), None, None, None, None)
+        out = dict(nodes=np.zeros((sz[0], 8), np.float32), wide=np.zeros((sz[1], 16), np.float32),
+                   tris=np.zeros((sz[2], 12), np.float32), roots=np.zeros((sz[3], 3), np.int32))
+        call("illum_bvh_export", self.h, _ptr(sz), _ptr(out["nodes"]), _ptr(out["wide"]),
+             _ptr(out["tris"]), _ptr(out["roots"]))
+        return out
+
     def tuning(self):
         """[(leaf_size, layout, ms)] of the autotuning trials (empty unless
         leaf_size was 0); layout 2 = two-wide, 3 = wavefront packet over the

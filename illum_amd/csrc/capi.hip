@@ -658,6 +658,28 @@ int illum_bvh_info(const illum_bvh *bvh, int64_t *nodes, int *depth, int *leaf_s
   });
 }
 
+int illum_bvh_export(const illum_bvh *bvh, int64_t *sizes, float *nodes, float *wide,
+                     float *tris, int32_t *roots) {
+  return guarded([&] {
+    CHECK_ARG(bvh && sizes, "null argument");
+    const illum::HostBVH &h = bvh->h;
+    const int64_t nb = (int64_t)h.root_thr.size();
+    sizes[0] = h.nnodes;
+    sizes[1] = h.ninner;
+    sizes[2] = (int64_t)h.tris.size() / 12;
+    sizes[3] = nb;
+    if (nodes) std::memcpy(nodes, h.nodes.data(), 4 * h.nodes.size());
+    if (wide) std::memcpy(wide, h.wide.data(), 4 * h.wide.size());
+    if (tris) std::memcpy(tris, h.tris.data(), 4 * h.tris.size());
+    if (roots)
+      for (int64_t b = 0; b < nb; ++b) {
+        roots[3 * b] = h.root_thr[b];
+        roots[3 * b + 1] = b < (int64_t)h.root_wide.size() ? h.root_wide[b] : -1;
+        roots[3 * b + 2] = b < (int64_t)h.root_wide4.size() ? h.root_wide4[b] : -1;
+      }
+  });
+}
+
 int illum_bvh_closest_hit(illum_bvh *bvh, const float *orig, const float *dir, int64_t nray,
                           int32_t *hit, float *t) {
   return guarded([&] {

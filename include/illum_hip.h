@@ -184,6 +184,16 @@ int illum_bvh_tuning(const illum_bvh *bvh, int cap, int *leaf_sizes, int *layout
                      int *n);
 int illum_bvh_destroy(illum_bvh *bvh);
 int illum_bvh_info(const illum_bvh *bvh, int64_t *nodes, int *depth, int *leaf_size);
+/* The tree as the walks read it, copied to host arrays: sizes[4] = {threaded
+ * nodes, two-wide inner nodes, triangles, bodies}, then (each array nullable;
+ * a call with all of them null reports the sizes) nodes (8 floats per
+ * threaded node: lo.xyz, miss link, hi.xyz, leaf info), wide (16 floats per
+ * two-wide node: per child lo.xyz, reference, hi.xyz, 0), tris (12 floats per
+ * triangle: v0.xyz, facet index bits, v1.xyz, 0, v2.xyz, 0) and roots (per
+ * body: threaded, two-wide and four-wide root).  Measurement aid, no
+ * reference counterpart: lets a CPU walk count the work of the same tree. */
+int illum_bvh_export(const illum_bvh *bvh, int64_t *sizes, float *nodes, float *wide,
+                     float *tris, int32_t *roots);
 /* Closest-hit query for arbitrary rays (testing/diagnostics): origins and
  * directions are nray x 3 floats; hit receives the original facet index or
  * -1, t the parametric distance (in units of |d|). */

@@ -25,6 +25,9 @@
 //                            tridag.for:17-27 (textbook form, Golub & Van Loan 4.3)
 //   oracle_thermal_run       thermal_model::step, thermal.hpp:65-84
 //   oracle_render            the renderer's per-pixel ray-cast, illum_renderer_cli.cpp:174-261
+//   oracle_walk_work         measurement aid, no reference counterpart: the box and
+//                            triangle tests of compute_F's rays (illum.cpp:56-73) along
+//                            the product's own exported tree, walked one ray at a time
 //
 // Compiled with -ffp-contract=off: every expression is evaluated as written.
 #include <algorithm>
@@ -33,6 +36,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <thread>
 #include <vector>
 
@@ -364,6 +368,154 @@ double ff_value(const float *pi, const float *ni, const float *pj, const float *
 }
 
 }  // namespace
+
+// ---- Work of the product's per-ray walk ----------------------------------
+// The form-factor rays of sources [j0, j1) (those of oracle_visibility) walked
+// one at a time over the product library's own tree, exported as its walks
+// read it (illum_bvh_export), counting box and triangle tests: the near-first
+// walk of the two-wide tree (visit the child entered first, push the other,
+// prune boxes by the best distance so far), first bounded at t <= tinit and,
+// if it found nothing, unbounded; a walk that would push more than cap
+// entries ends there and the stackless walk of the threaded tree answers the
+// unbounded query instead.  These counts price a ray's own walk (12 FLOP per
+// box test, 40 per triangle test) for the visibility kernel's roofline.
+// out = {rays, box tests, triangle tests}; bits (nullable) receives the
+// closest hits as in oracle_visibility.
+namespace {
+
+inline int as_int(float f) {
+  int i;
+  std::memcpy(&i, &f, 4);
+  return i;
+}
+
+// Slab test of the box lo[0..2], hi[0..2]: entry distance tn; accepted when
+// tn <= tf, tf > 0 and tn <= tbest (max(tn, m) <= min(tf, tbest), m the
+// smallest positive float).
+bool slab_enter(const Ray &r, const float *lo, const float *hi, float tbest, float *tn_out) {
+  float tn = -INFINITY, tf = INFINITY;
+  for (int c = 0; c < 3; ++c) {
+    const float t0 = std::fma(lo[c], r.inv[c], r.no[c]), t1 = std::fma(hi[c], r.inv[c], r.no[c]);
+    tn = std::fmax(tn, std::fmin(t0, t1));
+    tf = std::fmin(tf, std::fmax(t0, t1));
+  }
+  *tn_out = tn;
+  return std::fmax(tn, std::numeric_limits<float>::denorm_min()) <= std::fmin(tf, tbest);
+}
+
+struct ExportedTree {
+  const float *nodes, *wide, *tris;
+  int root_thr, root_wide, cap;
+};
+
+void leaf_tests(const ExportedTree &w, const Ray &r, int info, float &tbest, int &best,
+                int64_t &ntri) {
+  const int first = info & 0xFFFFFF, cnt = info >> 24;
+  ntri += cnt;
+  for (int k = 0; k < cnt; ++k) {
+    const float *t = w.tris + 12 * (int64_t)(first + k);
+    const int idx = as_int(t[3]);
+    float d;
+    if (hit_tri(r, t, t + 4, t + 8, &d) && (d < tbest || (d == tbest && idx < best))) {
+      tbest = d;
+      best = idx;
+    }
+  }
+}
+
+// Near-first walk of the two-wide tree: the facet hit, -1 for none, -2 when
+// the walk needed more than cap stack entries.
+int walk_two_wide(const ExportedTree &w, const Ray &r, float tinit, int64_t &nbox,
+                  int64_t &ntri) {
+  float tbest = tinit;
+  int best = -1, sp = 0, cur = w.root_wide;
+  int stack[64];
+  for (;;) {
+    if (cur >= 0) {
+      nbox += 2;
+      const float *nd = w.wide + 16 * (int64_t)cur;
+      float tl, tr;
+      const bool hl = slab_enter(r, nd, nd + 4, tbest, &tl);
+      const bool hr = slab_enter(r, nd + 8, nd + 12, tbest, &tr);
+      const int cl = as_int(nd[3]), cr = as_int(nd[11]);
+      if (hl && hr) {
+        if (sp == w.cap || sp == 64) return -2;
+        const bool lnear = tl <= tr;
+        stack[sp++] = lnear ? cr : cl;
+        cur = lnear ? cl : cr;
+        continue;
+      }
+      if (hl) { cur = cl; continue; }
+      if (hr) { cur = cr; continue; }
+    } else {
+      leaf_tests(w, r, ~cur, tbest, best, ntri);
+    }
+    if (sp == 0) break;
+    cur = stack[--sp];
+  }
+  return best;
+}
+
+// Stackless walk of the threaded tree (miss links), unbounded.
+int walk_threaded(const ExportedTree &w, const Ray &r, int64_t &nbox, int64_t &ntri) {
+  float tbest = INFINITY;
+  int best = -1, node = w.root_thr;
+  while (node >= 0) {
+    const float *nd = w.nodes + 8 * (int64_t)node;
+    const int miss = as_int(nd[3]), info = as_int(nd[7]);
+    ++nbox;
+    float tn;
+    if (!slab_enter(r, nd, nd + 4, tbest, &tn)) { node = miss; continue; }
+    if (info < 0) { node = node + 1; continue; }
+    leaf_tests(w, r, info, tbest, best, ntri);
+    node = miss;
+  }
+  return best;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_walk_work(const float *cen, const float *nrm, const float *rad, int64_t nf,
+                     double offset, int64_t j0, int64_t j1, const float *nodes, int root_thr,
+                     const float *wide, int root_wide, const float *tris, int cap, float tinit,
+                     int nthreads, int64_t *out, uint32_t *bits) {
+  const ExportedTree w{nodes, wide, tris, root_thr, root_wide, cap};
+  const int64_t words = (nf + 31) / 32;
+  std::atomic<int64_t> nrays(0), nbox(0), ntri(0);
+  const float off = (float)offset;
+  parallel_for(j1 - j0, nthreads, [&](int64_t q) {
+    const int64_t j = j0 + q;
+    uint32_t *row = bits ? bits + q * words : nullptr;
+    if (row) std::memset(row, 0, 4 * words);
+    const float *pj = cen + 3 * j, *nj = nrm + 3 * j;
+    const float o[3] = {pj[0] + off * nj[0], pj[1] + off * nj[1], pj[2] + off * nj[2]};
+    int64_t lr = 0, lb = 0, lt = 0;
+    for (int64_t i = 0; i < nf; ++i) {
+      if (i == j) continue;
+      const float *pi = cen + 3 * i;
+      const float dx = pi[0] - pj[0], dy = pi[1] - pj[1], dz = pi[2] - pj[2];
+      if ((dx * nj[0] + dy * nj[1]) + dz * nj[2] <= -rad[i]) continue;
+      const float d[3] = {pi[0] - o[0], pi[1] - o[1], pi[2] - o[2]};
+      const Ray r = setup(o, d);
+      ++lr;
+      int h = walk_two_wide(w, r, tinit, lb, lt);
+      if (h == -1) h = walk_two_wide(w, r, INFINITY, lb, lt);
+      if (h == -2) h = walk_threaded(w, r, lb, lt);
+      if (row && h >= 0 && h != j) row[h >> 5] |= 1u << (h & 31);
+    }
+    nrays += lr;
+    nbox += lb;
+    ntri += lt;
+  });
+  out[0] = nrays.load();
+  out[1] = nbox.load();
+  out[2] = ntri.load();
+  return 0;
+}
+
+}  // extern "C"
 
 extern "C" {
 

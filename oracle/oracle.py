@@ -23,6 +23,8 @@ _SIG = {
     "oracle_geometry": ([_P, _I64, _P, _I64, _P, _D, _P, _P, _P, _P, _P, _P], _I),
     "oracle_closest_hit": ([_P, _I64, _P, _P, _I64, _I, _I, _P, _P], _I),
     "oracle_visibility": ([_P, _P, _P, _P, _I64, _D, _I64, _I64, _I, _I, _P], _I64),
+    "oracle_walk_work": ([_P, _P, _P, _I64, _D, _I64, _I64, _P, _I, _P, _I, _P, _I, C.c_float,
+                          _I, _P, _P], _I),
     "oracle_form_factors": ([_P, _P, _P, _I64, _P, _I64, _I64, _D, _P, _I64, _P, _P, _P], _I64),
     "oracle_horizons": ([_P, _P, _P, _I64, _I64, _P, _I64, _I, _D, _D, _I, _I, _P], _I64),
     "oracle_fib_spiral": ([_P, _I], _I),
@@ -147,6 +149,30 @@ def visibility(g, offset=1e-5, j0=0, j1=None, use_bvh=True, nthreads=None):
                                  _p(g["radius"]), nf, offset, j0, j1, int(use_bvh),
                                  nthreads or nthreads_default(), _p(bits))
     return bits, nr
+
+
+def walk_work(g, tree, body=0, offset=1e-5, j0=0, j1=None, cap=8, tinit=1.0009765625,
+              nthreads=None, with_bits=False):
+    """Box and triangle tests of the form-factor rays of sources [j0, j1)
+    walked one ray at a time over the product's exported tree (the dict of
+    BVH.export()): {"rays", "box_tests", "tri_tests"} and, with with_bits,
+    the visibility rows.  cap: the per-lane walk's stack entries (deeper
+    walks are answered by the threaded walk); tinit: its first bound."""
+    nf = g["tri"].shape[0]
+    j1 = nf if j1 is None else j1
+    words = (nf + 31) // 32
+    bits = np.zeros((j1 - j0, words), np.uint32) if with_bits else None
+    out = np.zeros(3, np.int64)
+    nodes = np.ascontiguousarray(tree["nodes"], np.float32)
+    wide = np.ascontiguousarray(tree["wide"], np.float32)
+    tris = np.ascontiguousarray(tree["tris"], np.float32)
+    roots = tree["roots"]
+    lib().oracle_walk_work(_p(g["centroid"]), _p(g["normal"]), _p(g["radius"]), nf, offset,
+                           j0, j1, _p(nodes), int(roots[body, 0]), _p(wide),
+                           int(roots[body, 1]), _p(tris), cap, tinit,
+                           nthreads or nthreads_default(), _p(out), _p(bits))
+    res = {"rays": int(out[0]), "box_tests": int(out[1]), "tri_tests": int(out[2])}
+    return (res, bits) if with_bits else res
 
 
 def form_factors(g, bits, j0=0, j1=None, albedo=1.0, albedo_vec=None):

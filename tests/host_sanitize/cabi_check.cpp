@@ -54,6 +54,17 @@ int main(int argc, char **argv) {
   std::vector<int> ls(64), lay(64);
   std::vector<double> tms(64);
   OK(illum_bvh_tuning(bvh, 64, ls.data(), lay.data(), tms.data(), &nt));
+  int64_t tsz[4] = {0, 0, 0, 0};
+  OK(illum_bvh_export(bvh, tsz, nullptr, nullptr, nullptr, nullptr));
+  if (tsz[2] != nf || tsz[3] != 1) {
+    std::fprintf(stderr, "cabi_check: exported tree sizes %lld triangles, %lld bodies\n",
+                 (long long)tsz[2], (long long)tsz[3]);
+    ++fails;
+  }
+  std::vector<float> tnodes(8 * tsz[0]), twide(16 * tsz[1]), ttris(12 * tsz[2]);
+  std::vector<int32_t> troots(3 * tsz[3]);
+  OK(illum_bvh_export(bvh, tsz, tnodes.data(), twide.data(), ttris.data(), troots.data()));
+  STATUS(illum_bvh_export(bvh, nullptr, nullptr, nullptr, nullptr, nullptr), ILLUM_ERR_ARG);
   std::vector<double> alb(nf, 0.12);
   illum_sparse *Fm = nullptr;
   OK(illum_form_factors_ex(bvh, 0.0, alb.data(), 1e-5, ILLUM_FF_KEEP_VISIBILITY, &Fm));

@@ -209,3 +209,54 @@ def test_transfer_files_use_the_scripts_c_order_convention(tmp_path):
     transfer.arma_tofile(u, tmp_path / "u.bin")
     assert open(tmp_path / "u.bin", "rb").read().startswith(b"ARMA_MAT_BIN_IU004\n3\n")
     assert np.array_equal(transfer.arma_fromfile(tmp_path / "u.bin", np.uint32), u)
+
+
+def _two_leaf_tree(tri9):
+    """A tree in the exported layout over nf <= 254 triangles: one inner node
+    whose children are leaves of the first and second half; threaded nodes
+    root, left leaf, right leaf."""
+    nf = tri9.shape[0]
+    h = nf // 2
+    t = tri9.reshape(nf, 3, 3)
+    pad = np.float32(1e-4)
+
+    def box(a, b):
+        return t[a:b].reshape(-1, 3).min(0) - pad, t[a:b].reshape(-1, 3).max(0) + pad
+
+    def fbits(i):
+        return np.array([i], np.int32).view(np.float32)[0]
+
+    info = lambda first, cnt: (cnt << 24) | first
+    (l0, l1), (r0, r1), (b0, b1) = box(0, h), box(h, nf), box(0, nf)
+    wide = np.zeros((1, 16), np.float32)
+    wide[0, 0:3], wide[0, 3], wide[0, 4:7] = l0, fbits(~info(0, h)), l1
+    wide[0, 8:11], wide[0, 11], wide[0, 12:15] = r0, fbits(~info(h, nf - h)), r1
+    nodes = np.zeros((3, 8), np.float32)
+    nodes[0, 0:3], nodes[0, 3], nodes[0, 4:7], nodes[0, 7] = b0, fbits(-1), b1, fbits(-1)
+    nodes[1, 0:3], nodes[1, 3], nodes[1, 4:7], nodes[1, 7] = l0, fbits(2), l1, fbits(info(0, h))
+    nodes[2, 0:3], nodes[2, 3], nodes[2, 4:7], nodes[2, 7] = r0, fbits(-1), r1, fbits(info(h, nf - h))
+    tris = np.zeros((nf, 12), np.float32)
+    tris[:, 0:3], tris[:, 4:7], tris[:, 8:11] = t[:, 0], t[:, 1], t[:, 2]
+    tris[:, 3] = np.arange(nf, dtype=np.int32).view(np.float32)
+    return dict(nodes=nodes, wide=wide, tris=tris, roots=np.array([[0, 0, -1]], np.int32))
+
+
+def test_walk_work_on_a_two_leaf_tree():
+    # The per-ray walk that prices the visibility kernel's roofline finds the
+    # same closest hits as the brute-force visibility, casts the same rays,
+    # and visits the one inner node once per walk (2 box tests); with no stack
+    # entries every walk that enters both leaves is answered by the threaded
+    # walk instead, still with the same hits.
+    V, F = O_mesh(1, 6, seed=2)
+    g = O.geometry(V, F)
+    tree = _two_leaf_tree(g["tri"])
+    bits, nrays = O.visibility(g, use_bvh=False)
+    w, wbits = O.walk_work(g, tree, with_bits=True)
+    assert np.array_equal(wbits, bits) and w["rays"] == nrays > 0
+    # every ray's first walk is bounded at t <= 1.00098 and hits its target
+    # facet's plane at about t = 1 (closed mesh), so no second walk runs
+    assert w["box_tests"] == 2 * nrays
+    assert nrays <= w["tri_tests"] <= len(F) * nrays
+    w0, b0 = O.walk_work(g, tree, cap=0, with_bits=True)
+    assert np.array_equal(b0, bits) and w0["rays"] == nrays
+    assert w0["box_tests"] > w["box_tests"]

@@ -146,6 +146,30 @@ def test_cluster_cull_does_not_change_results(api, monkeypatch):
     assert Fm.rays() == nrays
 
 
+@pytest.mark.parametrize("subdiv,leaf", [(4, 1), (4, 3), (5, 6)])
+def test_per_ray_walk_work_equals_the_oracle_walk_of_the_same_tree(api, monkeypatch, subdiv,
+                                                                    leaf):
+    # The visibility kernel's roofline prices its work by each ray's own
+    # walk: box and triangle tests counted by the instrumented per-lane walk
+    # (a bounded walk, then an unbounded one, the threaded walk past 8 stack
+    # entries).  The oracle's walk of the same exported tree counts exactly
+    # the same and finds the oracle's visibility sets.
+    monkeypatch.setenv("ILLUM_FF_TRAVERSAL", "wide2")
+    V, F = _mesh(api, subdiv, 40, seed=15)
+    ic = api.IllumContext(V=V, F=F, leaf_size=leaf)
+    work = ic.form_factor_work(1e-5)
+    tree = ic.bvh.export()
+    assert tree["roots"].shape == (1, 3) and tree["tris"].shape == (len(F), 12)
+    cap = min(ic.bvh.info()["depth"] + 1, 8)
+    g = O.geometry(V, F)
+    ow, bits = O.walk_work(g, tree, cap=cap, with_bits=True)
+    assert ow["rays"] == work["rays"] > 0
+    assert ow["box_tests"] == work["box_tests"]
+    assert ow["tri_tests"] == work["tri_tests"]
+    vis, _ = O.visibility(g)
+    assert np.array_equal(bits, vis)
+
+
 @pytest.mark.parametrize("leaf", [1, 3, 8])
 def test_horizon_traversal_choice_does_not_change_results(api, monkeypatch, leaf):
     # Every walk, with the per-lane hit cache on (default) and off, gives the
