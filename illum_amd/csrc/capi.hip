@@ -165,7 +165,7 @@ static void set_device(illum_ctx *ctx) { ILLUM_HIP(hipSetDevice(ctx->device)); }
 
 static void destroy_mesh(illum_mesh *m) {
   dfree(m->d_pr4); dfree(m->d_n4); dfree(m->d_area); dfree(m->d_frame); dfree(m->d_clus);
-  dfree(m->d_boff); dfree(m->d_body);
+  dfree(m->d_boff); dfree(m->d_body); dfree(m->d_dq);
   delete m;
 }
 static void destroy_bvh(illum_bvh *b) {
@@ -240,6 +240,10 @@ static illum_mesh *upload_mesh(illum_ctx *ctx, HostGeometry &&g) {
   m->d_clus = dalloc<float>(clus.size());
   ILLUM_HIP(hipMemcpyAsync(m->d_clus, clus.data(), 4 * clus.size(), hipMemcpyHostToDevice, s));
   upload_bodies(m.get());
+  // the direct-insolation queue: 8 open pairs per facet, 4,096 to 2^20
+  m->dq_cap = std::min<int64_t>(std::max<int64_t>(8 * nf, 4096), (int64_t)1 << 20);
+  m->d_dq = dalloc<int32_t>(m->dq_cap + 2);
+  ILLUM_HIP(hipMemsetAsync(m->d_dq, 0, 8, s));
   sync(ctx);
   return m.release();
 }

@@ -275,12 +275,14 @@ static bool direct_exact_only() {
 
 // Lanes per (facet, sun) pair: enough for about 2^20 lanes (4096 workgroups),
 // at most 16 (0, -4, -8, -16: the wavefront-cooperative kernel with 1, 4, 8
-// or 16 lanes per pair).  ILLUM_DIRECT_LANES overrides.
+// or 16 lanes per pair; DIRECT_SPLIT: the two-pass shape with a queue of open
+// pairs).  ILLUM_DIRECT_LANES overrides.
+constexpr int DIRECT_SPLIT = 64;
 static int direct_lanes(int64_t pairs) {
   if (const char *e = getenv("ILLUM_DIRECT_LANES")) {
     const int v = atoi(e);
     if (v == -16 || v == -8 || v == -4 || v == 0 || v == 1 || v == 2 || v == 4 || v == 8 ||
-        v == 16)
+        v == 16 || v == DIRECT_SPLIT)
       return v;
   }
   int L = 1;
@@ -378,9 +380,9 @@ __device__ __forceinline__ int direct_decide(const DirectPair &q, const double *
   const int i0 = (int)floor((phc - dph) / dphi) - 1, i1 = (int)floor((phc + dph) / dphi) + 1;
   double hlo = __builtin_huge_valf(), hhi = -__builtin_huge_valf();
   bool nan = false;
-  for (int i = i0; i <= i1 + 1; ++i) {
-    // i1 + 1: the azimuth 2 pi itself maps to h[nphi - 1] when the range
-    // reaches the wrap
+  // bin of step i (i0 <= i <= i1 + 1), or -1; i1 + 1: the azimuth 2 pi
+  // itself maps to h[nphi - 1] when the range reaches the wrap
+  auto bin = [&](int i) {
     int ii = i;
     if (i <= i1) {
       while (ii < 0) ii += nb;
@@ -388,11 +390,25 @@ __device__ __forceinline__ int direct_decide(const DirectPair &q, const double *
     } else {
       ii = (i0 < 0 || i1 >= nb - 1) ? nb : -1;
     }
-    if (ii < 0) continue;
-    const double hv = h[ii];
-    nan = nan || hv != hv;
-    hlo = fmin(hlo, hv);
-    hhi = fmax(hhi, hv);
+    return ii;
+  };
+  // the horizon samples are misses into a large array: the first
+  // DIRECT_DECIDE_BATCH are loaded together, then min / max taken in order
+  constexpr int DIRECT_DECIDE_BATCH = 8;
+  for (int i = i0; i <= i1 + 1; i += DIRECT_DECIDE_BATCH) {
+    double hv[DIRECT_DECIDE_BATCH];
+#pragma unroll
+    for (int u = 0; u < DIRECT_DECIDE_BATCH; ++u) {
+      const int ii = i + u <= i1 + 1 ? bin(i + u) : -1;
+      hv[u] = ii >= 0 ? h[ii] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < DIRECT_DECIDE_BATCH; ++u) {
+      if (!(i + u <= i1 + 1 && bin(i + u) >= 0)) continue;
+      nan = nan || hv[u] != hv[u];
+      hlo = fmin(hlo, hv[u]);
+      hhi = fmax(hhi, hv[u]);
+    }
   }
   const double eps = 1e-9;
   if (!nan && thc + alpha + eps < hlo) return ndisk - 1;
@@ -518,11 +534,92 @@ __global__ __launch_bounds__(256) void direct_wave_kernel(
   if (valid && owner) direct_store(q, count, F, nphi, nf, E, flux);
 }
 
+// Two-pass shape.  Pass 1 (direct_split_kernel): one lane per pair does the
+// set-up and the whole-disk decision and stores the decided pairs; the open
+// ones go to the mesh's queue (one atomic per wavefront), or, past its
+// capacity, are sampled by their own lane.  Pass 2 (direct_queue_kernel): a
+// fixed grid in which each wavefront takes queued pairs in turn and samples
+// one with all 64 lanes.  The open pairs' samples so run 64 to a wavefront
+// wherever they are, where one lane group per pair (direct_kernel) leaves the
+// rest of the wavefront idle and direct_wave_kernel serialises a wavefront's
+// open pairs.  The last workgroup of pass 2 to finish zeroes the queue's
+// count and its own counter for the next launch, so no clear is queued (the
+// launches may be captured into a graph).  Counts are integers: E is the
+// same as with the other shapes.
+constexpr int DIRECT_QUEUE_BLOCKS = 1024;
+
+__global__ __launch_bounds__(256) void direct_split_kernel(
+    int64_t nf, int S, int nphi, const float4 *__restrict__ pr4,
+    const double *__restrict__ frame, const double *__restrict__ hz,
+    const double *__restrict__ sun, const double *__restrict__ disk, int ndisk,
+    double *__restrict__ E, int exact_only, const int32_t *__restrict__ body_of,
+    int per_body, const int *__restrict__ step, const double *__restrict__ flux,
+    int32_t *__restrict__ dq, int64_t dq_cap) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool valid = idx < nf * S;
+  const DirectPair q = direct_setup(valid ? idx : 0, nf, S, pr4, sun, body_of, per_body, step);
+  const double *F = frame + 9 * q.f;
+  const double *h = hz + q.f * nphi;
+  int dec = 0;
+  if (valid) dec = exact_only ? -1 : direct_decide(q, F, h, nphi, ndisk);
+  const bool open = valid && dec < 0;
+  const uint64_t m = __builtin_amdgcn_ballot_w64(open);
+  int base = 0;
+  if (m) {
+    if (lane == (int)__builtin_ctzll(m)) base = atomicAdd(&dq[0], (int)__popcll(m));
+    base = __builtin_amdgcn_readlane(base, (int)__builtin_ctzll(m));
+  }
+  if (!valid) return;
+  if (open) {
+    const int64_t slot = (int64_t)base + __popcll(m & ((1ull << lane) - 1));
+    if (slot < dq_cap) {
+      dq[2 + slot] = (int32_t)idx;
+      return;
+    }
+    int c = 0;  // past the queue's capacity: this lane samples its pair
+    for (int k = 0; k < ndisk - 1; ++k) c += direct_sample(q, k, F, h, nphi, disk, ndisk);
+    dec = c;
+  }
+  direct_store(q, dec, F, nphi, nf, E, flux);
+}
+
+__global__ __launch_bounds__(256) void direct_queue_kernel(
+    int64_t nf, int S, int nphi, const float4 *__restrict__ pr4,
+    const double *__restrict__ frame, const double *__restrict__ hz,
+    const double *__restrict__ sun, const double *__restrict__ disk, int ndisk,
+    double *__restrict__ E, const int32_t *__restrict__ body_of, int per_body,
+    const int *__restrict__ step, const double *__restrict__ flux, int32_t *__restrict__ dq,
+    int64_t dq_cap) {
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t n = std::min<int64_t>(__builtin_amdgcn_readfirstlane(dq[0]), dq_cap);
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  for (int64_t e = (int64_t)blockIdx.x * 4 + wid; e < n; e += nw) {
+    const int64_t idx = __builtin_amdgcn_readfirstlane(dq[2 + e]);
+    const DirectPair q = direct_setup(idx, nf, S, pr4, sun, body_of, per_body, step);
+    const double *F = frame + 9 * q.f;
+    const double *h = hz + q.f * nphi;
+    int c = 0;
+    for (int k = lane; k < ndisk - 1; k += 64) c += direct_sample(q, k, F, h, nphi, disk, ndisk);
+    for (int w = 32; w > 0; w >>= 1) c += __shfl_xor(c, w);
+    if (lane == 0) direct_store(q, c, F, nphi, nf, E, flux);
+  }
+  // every wavefront has read the count before its workgroup arrives here
+  __syncthreads();
+  if (threadIdx.x == 0 && atomicAdd(&dq[1], 1) == (int)gridDim.x - 1) {
+    atomicExch(&dq[0], 0);
+    atomicExch(&dq[1], 0);
+  }
+}
+
 // One direct insolation launch over nf x S pairs: L > 0 lanes per pair
 // (direct_kernel<L>), or direct_wave_kernel<P> for L = 0 (P = 1) and
-// L = -4, -8, -16 (P = -L).  The shape does not change E (the lanes only
-// share integer counts), so it is picked by the autotuner per pair-count
-// scale (ILLUM_DIRECT_LANES=-16|-8|-4|0|1|2|4|8|16 overrides).  The trials
+// L = -4, -8, -16 (P = -L), or the two-pass shape for L = DIRECT_SPLIT
+// (below 2^31 pairs: the queue holds 32-bit pair indices).  The shape does
+// not change E (the lanes only share integer counts), so it is picked by the
+// autotuner per pair-count scale (ILLUM_DIRECT_LANES=-16|-8|-4|0|1|2|4|8|16|64
+// overrides).  The trials
 // cover the first DIRECT_TUNE_PAIRS pairs only (a grid over a prefix of the
 // pairs writes exactly their E, which the launch after the trials writes
 // again): a batch of 64 bodies x 32 suns has 1.7e8 pairs, and timing nine
@@ -535,6 +632,18 @@ static void direct_launch(illum_mesh *mesh, int nphi, const double *hz, const do
   const int64_t nf = mesh->g.nf;
   const int64_t pairs = nf * S;
   auto go_n = [&](int L, int64_t npairs) {
+    if (L == DIRECT_SPLIT) {
+      direct_split_kernel<<<(unsigned)((npairs + 255) / 256), 256, 0, s>>>(
+          nf, S, nphi, (const float4 *)mesh->d_pr4, mesh->d_frame, hz, sun, disk, ndisk, E,
+          direct_exact_only() ? 1 : 0, mesh->d_body, per_body ? 1 : 0, step, flux, mesh->d_dq,
+          mesh->dq_cap);
+      ILLUM_LAUNCH_CHECK();
+      direct_queue_kernel<<<DIRECT_QUEUE_BLOCKS, 256, 0, s>>>(
+          nf, S, nphi, (const float4 *)mesh->d_pr4, mesh->d_frame, hz, sun, disk, ndisk, E,
+          mesh->d_body, per_body ? 1 : 0, step, flux, mesh->d_dq, mesh->dq_cap);
+      ILLUM_LAUNCH_CHECK();
+      return;
+    }
     const int64_t lanes = npairs * (L > 0 ? L : L == 0 ? 1 : -L);
     auto kern = L == 0 ? direct_wave_kernel<1> : L == -4 ? direct_wave_kernel<4>
               : L == -8 ? direct_wave_kernel<8> : L == -16 ? direct_wave_kernel<16>
@@ -546,13 +655,18 @@ static void direct_launch(illum_mesh *mesh, int nphi, const double *hz, const do
         direct_exact_only() ? 1 : 0, mesh->d_body, per_body ? 1 : 0, step, flux);
     ILLUM_LAUNCH_CHECK();
   };
+  const bool split_ok = pairs <= INT32_MAX && mesh->d_dq;
   int L = direct_lanes(pairs);
+  if (L == DIRECT_SPLIT && !split_ok) L = 16;
   if (!getenv("ILLUM_DIRECT_LANES")) {
     int b = 0;
     while (b < 62 && ((int64_t)1 << (b + 1)) <= pairs) ++b;
     const int64_t sample = std::min(pairs, DIRECT_TUNE_PAIRS);
-    L = autotune(mesh->ctx, "direct_lanes[pairs~2^" + std::to_string(b) + "]",
-                 {-16, -8, -4, 0, 1, 2, 4, 8, 16}, L, [&](int c) { go_n(c, sample); });
+    std::vector<int> cands = {-16, -8, -4, 0, 1, 2, 4, 8, 16};
+    if (split_ok) cands.push_back(DIRECT_SPLIT);
+    L = autotune(mesh->ctx, "direct_lanes[pairs~2^" + std::to_string(b) + "]", cands, L,
+                 [&](int c) { go_n(c, sample); });
+    if (L == DIRECT_SPLIT && !split_ok) L = 16;
   }
   go_n(L, pairs);
 }

@@ -154,6 +154,12 @@ struct illum_mesh {
   int nbody = 1;
   int32_t *d_boff = nullptr;   // nbody + 1
   int32_t *d_body = nullptr;   // nf, null for one body
+  // Direct insolation's queue of open (facet, sun) pairs for its two-pass
+  // shape: [0] count, [1] finished workgroups of the second pass, then
+  // dq_cap pair indices.  Zeroed at creation; the second pass leaves it
+  // zeroed again.
+  int32_t *d_dq = nullptr;
+  int64_t dq_cap = 0;
 };
 
 struct illum_bvh {

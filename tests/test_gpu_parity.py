@@ -298,15 +298,18 @@ def test_direct_whole_disk_decision_is_exact(api, monkeypatch):
     sun = rng.normal(size=(3, 48))
     sun = sun / np.linalg.norm(sun, axis=0) * 227390024.0
     E = ctx.get_direct_radiosity(sun, disk)
-    # the disk samples split over 1..16 lanes per (facet, sun) pair, or (0)
-    # the open pairs of a wavefront sampled by all its lanes: an integer
-    # count, so the same E whatever the split (one sun and all 48 suns)
-    for lanes in ("-16", "-8", "-4", "0", "1", "2", "4", "8", "16"):
+    # the disk samples split over 1..16 lanes per (facet, sun) pair, (0) the
+    # open pairs of a wavefront sampled by all its lanes, or (64) the open
+    # pairs queued and each sampled by a whole wavefront: an integer count,
+    # so the same E whatever the split (one sun and all 48 suns)
+    for lanes in ("-16", "-8", "-4", "0", "1", "2", "4", "8", "16", "64"):
         monkeypatch.setenv("ILLUM_DIRECT_LANES", lanes)
         assert np.array_equal(ctx.get_direct_radiosity(sun, disk), E), lanes
         assert np.array_equal(ctx.get_direct_radiosity(sun[:, 7:8], disk)[:, 0], E[:, 7]), lanes
+    # every pair open: 245,760 of them overflow the two-pass queue (40,960
+    # entries), whose overflow is sampled by each pair's own lane
     monkeypatch.setenv("ILLUM_DIRECT_EXACT", "1")
-    for lanes in ("-4", "0", "4"):
+    for lanes in ("-4", "0", "4", "64"):
         monkeypatch.setenv("ILLUM_DIRECT_LANES", lanes)
         assert np.array_equal(ctx.get_direct_radiosity(sun, disk), E), lanes
     monkeypatch.delenv("ILLUM_DIRECT_LANES")

@@ -174,7 +174,7 @@ def test_autotuned_launch_shapes_do_not_change_results(api, monkeypatch):
     disk = api.fib_spiral(100)
     sun = configs.hemisphere_suns(136, 4)
     E = ic.get_direct_radiosity(sun, disk)
-    for L in (-16, -8, -4, 0, 1, 2, 4, 8, 16):
+    for L in (-16, -8, -4, 0, 1, 2, 4, 8, 16, 64):
         monkeypatch.setenv("ILLUM_DIRECT_LANES", str(L))
         assert np.array_equal(ic.get_direct_radiosity(sun, disk), E), L
     monkeypatch.delenv("ILLUM_DIRECT_LANES")
