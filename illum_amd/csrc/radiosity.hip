@@ -608,6 +608,38 @@ __global__ __launch_bounds__(256) void colnorm_final_kernel(int G, int S, int cw
   }
 }
 
+// One right-hand side without FP32 remainders (S = 1, B mode): the G
+// partials summed by 1,024 threads -- each a strided run in order, then a
+// fixed shuffle tree per wavefront and the 16 wavefront sums in order -- and
+// the stop decision taken by thread 0, as decide does for one column.  Four
+// times the threads of colnorm_final_kernel and no block-wide reductions in
+// the decision: the small solves of a time loop take one after every product.
+__global__ __launch_bounds__(1024) void colnorm_final1_kernel(int G,
+                                                              const double *__restrict__ part,
+                                                              double *__restrict__ col,
+                                                              const double *__restrict__ e2,
+                                                              SolveCtl *__restrict__ ctl,
+                                                              double *__restrict__ hist,
+                                                              double tol, int *__restrict__ ex) {
+  if (ctl->done) return;
+  __shared__ double wsum[16];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < G; i += 1024) acc += part[i];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double r2 = 0.0;
+  for (int w = 0; w < 16; ++w) r2 += wsum[w];
+  col[0] = r2;
+  const int k = ctl->nprod;
+  const double rel = max_nan(0.0, e2[0] > 0 ? sqrt(r2 / e2[0]) : sqrt(r2));
+  if (ex) ex[k + 2] = half_scale_exp(r2);
+  hist[k] = rel;
+  ctl->nprod = k + 1;
+  if (rel <= tol) ctl->done = 1;
+}
+
 __global__ __launch_bounds__(256) void decide_kernel(int S, const double *__restrict__ col,
                                                      const double *__restrict__ e2,
                                                      SolveCtl *__restrict__ ctl,
@@ -1187,6 +1219,11 @@ void radiosity_solve(illum_sparse *F, const double *E, int S, int method, double
       ILLUM_LAUNCH_CHECK();
       nparts = WIDE_SUM_P;
       psum = part2;
+    }
+    if (S == 1 && !tb && !cole) {
+      colnorm_final1_kernel<<<1, 1024, 0, s>>>(nparts, psum, col, e2, ctl, hist, rel_tol, ex);
+      ILLUM_LAUNCH_CHECK();
+      return;
     }
     colnorm_final_kernel<<<fb, 256, 0, s>>>(nparts, S, cw, psum, col, e2, ctl, hist, rel_tol, 1,
                                             ex, tb, cole, sqrt_nf);
