@@ -17,10 +17,16 @@ def api():
     return A
 
 
-@pytest.mark.parametrize("method", ["fp64", "mixed"])
-@pytest.mark.parametrize("eager", [False, True])
-def test_timeloop_equals_call_by_call_steps(api, method, eager):
+@pytest.mark.parametrize("method,eager,direct", [
+    ("fp64", False, "tuned"), ("mixed", False, "tuned"), ("fp64", True, "tuned"),
+    ("mixed", True, "tuned"),
+    # the two-pass direct insolation in replayed graphs: its queue must be
+    # left empty by every launch for the next replay
+    ("mixed", False, "two_pass")])
+def test_timeloop_equals_call_by_call_steps(api, monkeypatch, method, eager, direct):
     from illum_amd import _lib, configs
+    if direct == "two_pass":
+        monkeypatch.setenv("ILLUM_DIRECT_LANES", "64")
     V, F = api.synth_ellipsoid(4, ncraters=40, seed=41)
     ic = api.IllumContext(V=V, F=F)
     Fm = ic.compute_F(0.1)
