@@ -39,6 +39,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFS = 157.3
+PEAK_FP64_TFS = 78.6
 # Crank-Nicolson work per cell-step and the vector peak it is priced against:
 # FP64 Thomas step (right-hand side 9, elimination 6 + 2 divisions, back
 # substitution 2) on FP64 vector 78.6 TFLOP/s; mixed step (FP64 difference and
@@ -178,6 +179,21 @@ def bench_radiosity(api, illum, ctx, Fm, hz, S=512, tol=1e-8):
         out[method] = {"seconds": best, "products": iters.value,
                        "final_rel_resid": float(resid[max(0, nres - 1)]),
                        "kernels_ms": {k: v[0] for k, v in ctx.profile().items()}}
+        # per product against the rooflines: algorithmic FLOP 2 nnz S (FP64
+        # FMAs in both methods) and the unique bytes streamed -- F once (12 B
+        # per nonzero) plus per element the FP64 method's X, R and Y (24 B)
+        # or the mixed method's FP16 correction in and out and FP32 remainder
+        # in and out (10 B: S >= 128 at tolerances >= 1e-9, as here)
+        sweep_ms = out[method]["kernels_ms"].get(f"radiosity_{method}_sweep")
+        if sweep_ms and iters.value:
+            pms = sweep_ms / iters.value
+            flop = 2.0 * Fm.nnz * S
+            ubytes = 12.0 * Fm.nnz + nf * S * (24 if method == "fp64" else 10)
+            out[method]["per_product"] = {
+                "ms": pms, "flop": flop, "tflops": flop / (pms * 1e-3) / 1e12,
+                "frac_fp64_peak": flop / (pms * 1e-3) / 1e12 / PEAK_FP64_TFS,
+                "unique_bytes": ubytes, "gbs": ubytes / (pms * 1e-3) / 1e9,
+                "frac_hbm": ubytes / (pms * 1e-3) / 1e9 / PEAK_HBM_GBS}
         Bh[method] = B.to_host()
         B.close()
         log(f"radiosity {method}: {best:.3f}s, {iters.value} products, "
